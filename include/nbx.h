@@ -1,0 +1,156 @@
+/*
+ * nbx.h -- C ABI of the MI355X (gfx950) KDTree hot path.
+ *
+ * Plain C types only (pointers, sizes, status codes).  Every entry point
+ * names the NBodylib interface it replaces (file:line in pelahi/NBodylib).
+ * Host-pointer entry points copy in/out; the *_dev variants take device
+ * pointers and a hipStream_t passed as void* and leave results in HBM.
+ *
+ * Index spaces (as in the reference):
+ *   ID    = position of a particle in the caller's input array (the reference
+ *           overwrites Particle::id with it, KDTree.cxx:1291).
+ *   slot  = position in this library's tree order (Morton order).  It plays
+ *           the role of the reference's "tree index" (KDTree.h:295).
+ *
+ * All functions return NBX_OK (0) or a negative status; nbx_last_error()
+ * gives the message (thread-local).  The reference prints and exit()s in
+ * these cases (KDCalcSmoothQuantities.cxx:205-207, :273-276,
+ * PriorityQueue.h:49-53, KDTree.cxx:1096-1105).
+ */
+#ifndef NBX_H
+#define NBX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NBX_OK 0
+#define NBX_ERR_ARG (-1)         /* invalid argument */
+#define NBX_ERR_HIP (-2)         /* HIP runtime error */
+#define NBX_ERR_UNSUPPORTED (-3) /* option the reference has but this path does not implement */
+#define NBX_ERR_NOMEM (-4)       /* device allocation failed */
+#define NBX_ERR_SEARCH (-5)      /* fewer than k neighbours ("CalcDensity failed") */
+
+/* KDTree::TPHYS/TPHS (KDTree.h:90-91) */
+#define NBX_TPHYS 0
+#define NBX_TPHS 3
+/* KDTree::KSPH/KGAUSS/KEPAN/KTH (KDTree.h:97) */
+#define NBX_KSPH 0
+#define NBX_KGAUSS 1
+#define NBX_KEPAN 2
+#define NBX_KTH 3
+/* arithmetic forms */
+#define NBX_FORM_PLAIN 0 /* FP64 throughout: the drop-in */
+#define NBX_FORM_MIXED 1 /* FP32 node-relative search + FP64 re-check band */
+
+typedef struct nbx_tree nbx_tree;
+
+/* Mirrors the KDTree constructor options (KDTree.h:229-245). */
+typedef struct nbx_params {
+    int32_t bucket_size; /* bucket_size (default 16); only the N<=b edge case depends on it */
+    int32_t treetype;    /* NBX_TPHYS or NBX_TPHS */
+    int32_t kerntype;    /* NBX_KSPH ... (reference default KEPAN) */
+    int32_t kernres;     /* kernel table size (default 1000; <100 clamps to 100) */
+    int32_t splitting;   /* only KDTREE_SPLIT_SPREAD (0) */
+    int32_t aniso;       /* TPHS: -1 only (metric factor 1 on caller-prescaled x/sx, v/sv) */
+    int32_t scalespace;  /* must be 0 (the reference's ScaleSpace reads uninitialised xmean) */
+    int32_t periodic;    /* 0 or 1 */
+    double period[3];    /* Period[3] when periodic */
+    int32_t form;        /* NBX_FORM_PLAIN or NBX_FORM_MIXED */
+    double recheck_rel;  /* mixed form: relative FP64 re-check band (default 1e-5) */
+} nbx_params;
+
+/* Defaults of KDTree::KDTree (KDTree.h:229-245). */
+void nbx_params_default(nbx_params *p);
+
+/* Builds the tree.  Replaces KDTree::KDTree(Particle*,N,...) KDTree.cxx:1238-1306
+ * (BuildNodes :970-1055, KernelConstruction :1144-1183).
+ * pos: 3*n SoA (x[0..n), y[..], z[..]); vel: 3*n SoA or NULL (needed for TPHS
+ * and velocity density); mass: n or NULL (then 1/n each). */
+int nbx_tree_create(const nbx_params *p, const double *pos, const double *vel, const double *mass, int64_t n,
+                    nbx_tree **out);
+int nbx_tree_create_dev(const nbx_params *p, const double *pos_dev, const double *vel_dev, const double *mass_dev,
+                        int64_t n, void *stream, nbx_tree **out);
+/* Replaces KDTree::~KDTree (KDTree.cxx:1340-1356); caller-side reorder is the shim's job. */
+int nbx_tree_destroy(nbx_tree *t);
+/* slot -> ID map (host array of n).  The C++ shim uses it to reorder the
+ * caller's Particle array as the reference does in BuildNodes. */
+int nbx_tree_order(nbx_tree *t, int32_t *slot_to_id);
+int64_t nbx_tree_size(const nbx_tree *t);
+/* Kernel normalisation and table (KDTree::GetKernNorm, KDTree.h:283). */
+int nbx_tree_kernel(const nbx_tree *t, double *kernnorm, double *table, int32_t cap);
+
+/* kNN of every particle.  Replaces the batch loops
+ * KDTree::FindNearest(Int_t**,Double_t**,k) KDFindNearest.cxx:444-459 and
+ * the per-particle FindNearest(tt) :247-318, FindNearestPos(tt) :320-334,
+ * FindNearestPhase(tt) :347-361. */
+#define NBX_KNN_FINDNEAREST 0      /* self and d2==0 excluded; periodic: k+1 search, drop smallest */
+#define NBX_KNN_FINDNEARESTPOS 1   /* periodic: k+1 search, drop largest (returns self first) */
+#define NBX_KNN_FINDNEARESTPHASE 2 /* 6-D phase metric (TPHS trees) */
+#define NBX_KNN_NONPERIODIC 3      /* the search CalcDensity makes (KDCalcSmoothQuantities.cxx:227) */
+#define NBX_LAYOUT_BY_ID 0   /* row r = particle with ID r; entries are IDs */
+#define NBX_LAYOUT_BY_SLOT 1 /* row r = slot r; entries are slots ("tree indices") */
+/* nn: n*k int32, d2: n*k double or NULL; rows ascending in d2 (LoadNN,
+ * KDFindNearest.cxx:11-24); missing neighbours are -1 with d2 = 1e32. */
+int nbx_knn(nbx_tree *t, int32_t k, int32_t variant, int32_t layout, int32_t *nn, double *d2);
+int nbx_knn_dev(nbx_tree *t, int32_t k, int32_t variant, int32_t layout, int32_t *nn_dev, double *d2_dev,
+                void *stream);
+/* kNN of arbitrary positions (q: 3*m SoA).  Replaces FindNearestPos(Double_t*x,...)
+ * KDFindNearest.cxx:462-564 (period-aware like the reference).  Entries are IDs. */
+int nbx_knn_points(nbx_tree *t, const double *q, int64_t m, int32_t k, int32_t *nn, double *d2);
+
+/* SPH densities, results keyed by ID.
+ * SYMMETRIC replaces KDTree::CalcDensity KDCalcSmoothQuantities.cxx:203-305;
+ * GATHER replaces CalcDensityParticle :768-844 for every particle;
+ * VEL replaces CalcVelDensity :309-389 (k = Nsmooth, nsearch = Nsearch).
+ * h_by_id (nullable) receives h_i = r_k/2 (the reference computes but never stores it). */
+#define NBX_DENS_SYMMETRIC 0
+#define NBX_DENS_GATHER 1
+#define NBX_DENS_VEL 2
+int nbx_density(nbx_tree *t, int32_t k, int32_t variant, int32_t nsearch, double *rho_by_id, double *h_by_id);
+int nbx_density_dev(nbx_tree *t, int32_t k, int32_t variant, int32_t nsearch, double *rho_by_id_dev,
+                    double *h_by_id_dev, void *stream);
+/* Fused: kNN (NBX_KNN_NONPERIODIC semantics) + symmetric density in one pass,
+ * neighbour IDs by slot.  The batch form of FindNearest + CalcDensity. */
+int nbx_knn_density_dev(nbx_tree *t, int32_t k, int32_t *nn_by_slot_dev, double *rho_by_id_dev,
+                        double *h_by_id_dev, void *stream);
+
+/* Ball search around a position: IDs of all particles with d2 < r2 (strict).
+ * Replaces SearchBallPosTagged(Double_t*x, fdist2, Int_t*tagged)
+ * KDFindNearest.cxx:618-688 (period-aware).  out: capacity cap; *nout = total found. */
+int nbx_ball_points(nbx_tree *t, const double *x, double r2, int32_t *out, int64_t cap, int64_t *nout);
+
+/* Friends-of-friends.  Replaces KDTree::FOF(fdist, numgroup, minnum, order)
+ * KDFOF.cxx:29-153 (periodic when the tree is).  labels_by_id: canonical labels
+ * -- 0 = in no group of >= minnum members; groups 1..G by size descending,
+ * ties by smaller minimum member ID (the reference's order=1 numbering up to
+ * its heap tie order). */
+int nbx_fof(nbx_tree *t, double fdist, int32_t minnum, int32_t *labels_by_id, int64_t *ngroups);
+int nbx_fof_dev(nbx_tree *t, double fdist, int32_t minnum, int32_t *labels_by_id_dev, int64_t *ngroups, void *stream);
+
+/* Differentiation form (no reference counterpart: the reference allocates
+ * derKernel but never fills it, KDTree.cxx:1150).  Symmetric SPH density with
+ * the analytic C1 cubic spline (SmoothingKernels.h:32-37, 3-D norm 1/pi) over a
+ * frozen neighbour list nbr (n*k indices, last column = k-th neighbour that
+ * sets h_i = |x_i - x_K|/2).  pos: 3*n SoA.  All device pointers. */
+int nbx_sph_density_fwd_dev(const double *pos_dev, const double *mass_dev, const int32_t *nbr_dev, int64_t n,
+                            int32_t k, double *rho_dev, double *h_dev, void *stream);
+/* Adjoint: given g = dL/drho, accumulates dL/dpos (3*n SoA) and dL/dmass (n). */
+int nbx_sph_density_adjoint_dev(const double *pos_dev, const double *mass_dev, const int32_t *nbr_dev, int64_t n,
+                                int32_t k, const double *g_dev, double *dpos_dev, double *dmass_dev, void *stream);
+
+/* Per-kernel timing with HIP events recorded on the launch stream.  When on,
+ * every library kernel launch is bracketed; nbx_prof_get returns the summed
+ * milliseconds and launch count of kernels whose name contains `name`. */
+void nbx_prof_enable(int on);
+void nbx_prof_reset(void);
+int nbx_prof_get(const char *name, double *ms, int64_t *launches);
+
+const char *nbx_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NBX_H */

@@ -1,0 +1,264 @@
+"""nbodylib_amd -- MI355X-native KDTree hot path with NBodylib's call surface.
+
+Python mirror of NBody::KDTree (src/KDTree/KDTree.h:81-657) over the C ABI in
+include/nbx.h (libnbx.so, hand-written HIP for gfx950).  Method names and
+argument meanings follow the reference; results are keyed by particle ID
+(= input index, KDTree.cxx:1291) unless a method says otherwise.
+
+There is no CPU fallback: every call runs the HIP kernels, and importing the
+module fails if libnbx.so has not been built.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, "libnbx.so")
+
+# KDTree.h:90-97 constants
+TPHYS, TPROJ, TVEL, TPHS, TMETRIC = 0, 1, 2, 3, 4
+KSPH, KGAUSS, KEPAN, KTH = 0, 1, 2, 3
+FORM_PLAIN, FORM_MIXED = 0, 1
+KNN_FINDNEAREST, KNN_FINDNEARESTPOS, KNN_FINDNEARESTPHASE, KNN_NONPERIODIC = 0, 1, 2, 3
+LAYOUT_BY_ID, LAYOUT_BY_SLOT = 0, 1
+DENS_SYMMETRIC, DENS_GATHER, DENS_VEL = 0, 1, 2
+
+NBX_OK = 0
+NBX_ERR_ARG, NBX_ERR_HIP, NBX_ERR_UNSUPPORTED, NBX_ERR_NOMEM, NBX_ERR_SEARCH = -1, -2, -3, -4, -5
+
+EXPORTS = [
+    "nbx_params_default", "nbx_tree_create", "nbx_tree_create_dev", "nbx_tree_destroy", "nbx_tree_order",
+    "nbx_tree_size", "nbx_tree_kernel", "nbx_knn", "nbx_knn_dev", "nbx_knn_points", "nbx_density",
+    "nbx_density_dev", "nbx_knn_density_dev", "nbx_ball_points", "nbx_fof", "nbx_fof_dev",
+    "nbx_sph_density_fwd_dev", "nbx_sph_density_adjoint_dev", "nbx_prof_enable", "nbx_prof_reset", "nbx_prof_get",
+    "nbx_last_error",
+]
+
+
+class NbxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"nbx error {code}: {msg}")
+        self.code = code
+
+
+class Params(ctypes.Structure):
+    """nbx_params (include/nbx.h), mirroring the KDTree constructor options."""
+    _fields_ = [
+        ("bucket_size", ctypes.c_int32), ("treetype", ctypes.c_int32), ("kerntype", ctypes.c_int32),
+        ("kernres", ctypes.c_int32), ("splitting", ctypes.c_int32), ("aniso", ctypes.c_int32),
+        ("scalespace", ctypes.c_int32), ("periodic", ctypes.c_int32), ("period", ctypes.c_double * 3),
+        ("form", ctypes.c_int32), ("recheck_rel", ctypes.c_double),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """Load libnbx.so (raises if it is missing -- no fallback path exists)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBPATH):
+        raise ImportError(f"{LIBPATH} is not built; run __graft_entry__.build()")
+    L = ctypes.CDLL(LIBPATH)
+    vp, i32, i64, dbl = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_double
+    P = ctypes.POINTER
+    L.nbx_params_default.argtypes = [P(Params)]
+    L.nbx_params_default.restype = None
+    L.nbx_tree_create.argtypes = [P(Params), vp, vp, vp, i64, P(vp)]
+    L.nbx_tree_create_dev.argtypes = [P(Params), vp, vp, vp, i64, vp, P(vp)]
+    L.nbx_tree_destroy.argtypes = [vp]
+    L.nbx_tree_order.argtypes = [vp, vp]
+    L.nbx_tree_size.argtypes = [vp]
+    L.nbx_tree_size.restype = i64
+    L.nbx_tree_kernel.argtypes = [vp, P(dbl), vp, i32]
+    L.nbx_knn.argtypes = [vp, i32, i32, i32, vp, vp]
+    L.nbx_knn_dev.argtypes = [vp, i32, i32, i32, vp, vp, vp]
+    L.nbx_knn_points.argtypes = [vp, vp, i64, i32, vp, vp]
+    L.nbx_density.argtypes = [vp, i32, i32, i32, vp, vp]
+    L.nbx_density_dev.argtypes = [vp, i32, i32, i32, vp, vp, vp]
+    L.nbx_knn_density_dev.argtypes = [vp, i32, vp, vp, vp, vp]
+    L.nbx_ball_points.argtypes = [vp, vp, dbl, vp, i64, P(i64)]
+    L.nbx_fof.argtypes = [vp, dbl, i32, vp, P(i64)]
+    L.nbx_fof_dev.argtypes = [vp, dbl, i32, vp, P(i64), vp]
+    L.nbx_sph_density_fwd_dev.argtypes = [vp, vp, vp, i64, i32, vp, vp, vp]
+    L.nbx_sph_density_adjoint_dev.argtypes = [vp, vp, vp, i64, i32, vp, vp, vp, vp]
+    L.nbx_prof_enable.argtypes = [ctypes.c_int]
+    L.nbx_prof_enable.restype = None
+    L.nbx_prof_reset.argtypes = []
+    L.nbx_prof_reset.restype = None
+    L.nbx_prof_get.argtypes = [ctypes.c_char_p, P(dbl), P(i64)]
+    L.nbx_last_error.argtypes = []
+    L.nbx_last_error.restype = ctypes.c_char_p
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc < 0:
+        raise NbxError(rc, lib().nbx_last_error().decode())
+    return rc
+
+
+def _p(a):
+    return None if a is None else ctypes.c_void_p(a.ctypes.data)
+
+
+def _soa(a, n, name):
+    a = np.asarray(a, dtype=np.float64)
+    if a.shape == (n, 3):
+        a = a.T
+    if a.shape != (3, n):
+        raise ValueError(f"{name} must have shape (n, 3) or (3, n)")
+    return np.ascontiguousarray(a)
+
+
+def params(bucket_size=16, treetype=TPHYS, kerntype=KEPAN, kernres=1000, splitting=0, aniso=0, scalespace=0,
+           period=None, form=FORM_PLAIN, recheck_rel=1e-5):
+    """Build nbx_params with the reference constructor's defaults (KDTree.h:229-245)."""
+    p = Params()
+    lib().nbx_params_default(ctypes.byref(p))
+    p.bucket_size, p.treetype, p.kerntype, p.kernres = bucket_size, treetype, kerntype, kernres
+    p.splitting, p.aniso, p.scalespace = splitting, aniso, scalespace
+    if period is not None:
+        per = np.broadcast_to(np.asarray(period, dtype=np.float64), (3,))
+        p.periodic = 1
+        for i in range(3):
+            p.period[i] = float(per[i])
+    p.form, p.recheck_rel = form, recheck_rel
+    return p
+
+
+class KDTree:
+    """NBody::KDTree over host numpy arrays (the constructor of KDTree.h:229-245).
+
+    pos, vel: (n, 3) or (3, n) float64; mass: (n,) or None (equal masses 1/n).
+    """
+
+    def __init__(self, pos, mass=None, vel=None, bucket_size=16, treetype=TPHYS, kerntype=KEPAN, kernres=1000,
+                 splitting=0, aniso=0, scalespace=0, period=None, form=FORM_PLAIN, recheck_rel=1e-5):
+        pos = np.asarray(pos, dtype=np.float64)
+        n = pos.shape[0] if pos.ndim == 2 and pos.shape[1] == 3 else pos.shape[-1]
+        self.n = int(n)
+        self.period = period
+        self._params = params(bucket_size, treetype, kerntype, kernres, splitting, aniso, scalespace, period, form,
+                              recheck_rel)
+        spos = _soa(pos, n, "pos")
+        svel = None if vel is None else _soa(vel, n, "vel")
+        m = None if mass is None else np.ascontiguousarray(mass, dtype=np.float64)
+        h = ctypes.c_void_p()
+        _check(lib().nbx_tree_create(ctypes.byref(self._params), _p(spos), _p(svel), _p(m), self.n, ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().nbx_tree_destroy(h)
+            self._h = None
+
+    # -------------------------------------------------------------- info
+    def order(self):
+        """slot -> ID (the tree order the shim imposes on Particle arrays)."""
+        o = np.zeros(self.n, dtype=np.int32)
+        _check(lib().nbx_tree_order(self._h, _p(o)))
+        return o
+
+    def GetKernNorm(self):
+        kn = ctypes.c_double()
+        _check(lib().nbx_tree_kernel(self._h, ctypes.byref(kn), None, 0))
+        return kn.value
+
+    def kernel_table(self):
+        kn = ctypes.c_double()
+        res = _check(lib().nbx_tree_kernel(self._h, ctypes.byref(kn), None, 0))
+        tab = np.zeros(res, dtype=np.float64)
+        lib().nbx_tree_kernel(self._h, None, _p(tab), res)
+        return tab
+
+    # -------------------------------------------------------------- kNN
+    def _knn(self, k, variant, layout=LAYOUT_BY_ID, with_d2=True):
+        nn = np.zeros((self.n, k), dtype=np.int32)
+        d2 = np.zeros((self.n, k), dtype=np.float64) if with_d2 else None
+        _check(lib().nbx_knn(self._h, k, variant, layout, _p(nn), _p(d2)))
+        return nn, d2
+
+    def FindNearest(self, Nsearch=64, layout=LAYOUT_BY_ID):
+        """FindNearest(Int_t**,Double_t**,Nsearch) for every particle (KDFindNearest.cxx:444-446)."""
+        return self._knn(Nsearch, KNN_FINDNEAREST, layout)
+
+    def FindNearestPos(self, Nsearch=64, layout=LAYOUT_BY_ID):
+        """FindNearestPos(Int_t**,...) (KDFindNearest.cxx:448-450)."""
+        return self._knn(Nsearch, KNN_FINDNEARESTPOS, layout)
+
+    def FindNearestPhase(self, Nsearch=64, layout=LAYOUT_BY_ID):
+        """FindNearestPhase(Int_t**,...) (KDFindNearest.cxx:454-456)."""
+        return self._knn(Nsearch, KNN_FINDNEARESTPHASE, layout)
+
+    def FindNearestNonPeriodic(self, Nsearch=64, layout=LAYOUT_BY_ID):
+        """The search CalcDensity makes (KDCalcSmoothQuantities.cxx:227)."""
+        return self._knn(Nsearch, KNN_NONPERIODIC, layout)
+
+    def FindNearestPosPoints(self, x, Nsearch=64):
+        """FindNearestPos(Double_t *x, ...) for many positions x (m, 3)."""
+        x = np.asarray(x, dtype=np.float64)
+        m = x.shape[0]
+        q = np.ascontiguousarray(x.T)
+        nn = np.zeros((m, Nsearch), dtype=np.int32)
+        d2 = np.zeros((m, Nsearch), dtype=np.float64)
+        _check(lib().nbx_knn_points(self._h, _p(q), m, Nsearch, _p(nn), _p(d2)))
+        return nn, d2
+
+    # -------------------------------------------------------------- densities
+    def CalcDensity(self, Nsmooth=64, return_h=False):
+        """KDTree::CalcDensity (KDCalcSmoothQuantities.cxx:203-305); rho by ID."""
+        rho = np.zeros(self.n, dtype=np.float64)
+        h = np.zeros(self.n, dtype=np.float64) if return_h else None
+        _check(lib().nbx_density(self._h, Nsmooth, DENS_SYMMETRIC, 0, _p(rho), _p(h)))
+        return (rho, h) if return_h else rho
+
+    def CalcDensityParticleAll(self, Nsmooth=64):
+        """CalcDensityParticle(i, Nsmooth) for every i (KDCalcSmoothQuantities.cxx:768-844)."""
+        rho = np.zeros(self.n, dtype=np.float64)
+        h = np.zeros(self.n, dtype=np.float64)
+        _check(lib().nbx_density(self._h, Nsmooth, DENS_GATHER, 0, _p(rho), _p(h)))
+        return rho, h
+
+    def CalcVelDensity(self, Nsmooth=64, Nsearch=64):
+        """KDTree::CalcVelDensity (KDCalcSmoothQuantities.cxx:309-389); result by ID."""
+        rho = np.zeros(self.n, dtype=np.float64)
+        _check(lib().nbx_density(self._h, Nsmooth, DENS_VEL, Nsearch, _p(rho), None))
+        return rho
+
+    # -------------------------------------------------------------- ball / FOF
+    def SearchBallPosTagged(self, x, fdist2, cap=None):
+        """SearchBallPosTagged(Double_t *x, fdist2, tagged) (KDFindNearest.cxx:628-636); IDs, unordered."""
+        x = np.ascontiguousarray(np.asarray(x, dtype=np.float64).reshape(3))
+        cap = self.n * 8 if cap is None else cap
+        out = np.zeros(max(cap, 1), dtype=np.int32)
+        nout = ctypes.c_int64()
+        _check(lib().nbx_ball_points(self._h, _p(x), float(fdist2), _p(out), cap, ctypes.byref(nout)))
+        return out[: min(nout.value, cap)].copy()
+
+    def FOF(self, fdist, minnum=8):
+        """KDTree::FOF (KDFOF.cxx:29-153): canonical labels by ID and the group count."""
+        lab = np.zeros(self.n, dtype=np.int32)
+        ng = ctypes.c_int64()
+        _check(lib().nbx_fof(self._h, float(fdist), minnum, _p(lab), ctypes.byref(ng)))
+        return lab, ng.value
+
+
+def prof_enable(on=True):
+    lib().nbx_prof_enable(1 if on else 0)
+
+
+def prof_reset():
+    lib().nbx_prof_reset()
+
+
+def prof_get(name):
+    ms = ctypes.c_double()
+    n = ctypes.c_int64()
+    lib().nbx_prof_get(name.encode(), ctypes.byref(ms), ctypes.byref(n))
+    return ms.value, n.value

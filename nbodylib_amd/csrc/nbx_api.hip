@@ -1,0 +1,685 @@
+// nbx_api.hip -- host side of the C ABI declared in include/nbx.h.
+//
+// Owns the device-resident tree (sorted SoA particles, node boxes, kernel
+// table) and turns each NBodylib call into kernel launches on the caller's
+// stream.  No CPU fallback exists: every result comes from the HIP kernels.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/nbx.h"
+#include "nbx_kernels.h"
+
+hipError_t nbx_fof_canonical(const DevTree &T, int32_t *P, int32_t minnum, int32_t *labels_by_id, int64_t *ngroups,
+                             hipStream_t st);
+__global__ void k_ball(DevTree T, double qx, double qy, double qz, double r2, int32_t *out, int64_t cap,
+                       unsigned long long *count);
+
+// ------------------------------------------------------------------ errors
+static thread_local std::string g_err;
+static int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+static int hipfail(hipError_t e, const char *where) {
+    return fail(e == hipErrorOutOfMemory ? NBX_ERR_NOMEM : NBX_ERR_HIP,
+                std::string(where) + ": " + hipGetErrorString(e));
+}
+#define HIPCK(x)                                  \
+    do {                                          \
+        hipError_t _e = (x);                      \
+        if (_e != hipSuccess) return hipfail(_e, #x); \
+    } while (0)
+
+extern "C" const char *nbx_last_error(void) { return g_err.c_str(); }
+
+// ------------------------------------------------------------------ profiling
+struct ProfRec {
+    std::string name;
+    hipEvent_t e0, e1;
+};
+static std::mutex g_prof_mu;
+static bool g_prof_on = false;
+static std::vector<ProfRec> g_prof_pending;
+struct ProfAcc {
+    std::string name;
+    double ms;
+    int64_t n;
+};
+static std::vector<ProfAcc> g_prof_acc;
+
+ProfScope::ProfScope(const char *nm, hipStream_t s) : st(s), e0(nullptr), e1(nullptr), name(nm), on(g_prof_on) {
+    if (!on) return;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    hipEventRecord(e0, st);
+}
+ProfScope::~ProfScope() {
+    if (!on) return;
+    hipEventRecord(e1, st);
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    g_prof_pending.push_back({name, e0, e1});
+}
+static void prof_drain() {
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    for (auto &r : g_prof_pending) {
+        hipEventSynchronize(r.e1);
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, r.e0, r.e1);
+        bool found = false;
+        for (auto &a : g_prof_acc)
+            if (a.name == r.name) { a.ms += ms; a.n += 1; found = true; break; }
+        if (!found) g_prof_acc.push_back({r.name, (double)ms, 1});
+        hipEventDestroy(r.e0);
+        hipEventDestroy(r.e1);
+    }
+    g_prof_pending.clear();
+}
+extern "C" void nbx_prof_enable(int on) { g_prof_on = on != 0; }
+extern "C" void nbx_prof_reset(void) {
+    prof_drain();
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    g_prof_acc.clear();
+}
+extern "C" int nbx_prof_get(const char *name, double *ms, int64_t *launches) {
+    prof_drain();
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    double t = 0;
+    int64_t c = 0;
+    for (auto &a : g_prof_acc)
+        if (strstr(a.name.c_str(), name)) { t += a.ms; c += a.n; }
+    if (ms) *ms = t;
+    if (launches) *launches = c;
+    return NBX_OK;
+}
+
+// ------------------------------------------------------------------ tree
+struct nbx_tree {
+    nbx_params p;
+    int64_t n;
+    int nd;        // tree dimensionality (3 or 6)
+    bool has_vel;  // TPHYS tree that also keeps sorted velocities (x[3..5])
+    double *xs[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    double *mass = nullptr;
+    int32_t *perm = nullptr;
+    double *box = nullptr;
+    double *org = nullptr;
+    float *xf[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    double *kern = nullptr;
+    int kernres;
+    double kernnorm;
+    std::vector<double> hkern;
+    LevelInfo lv;
+    int32_t nleaf;
+    int64_t nnodes;
+    DevTree view() const {
+        DevTree T;
+        memset(&T, 0, sizeof(T));
+        T.n = (int32_t)n;
+        T.nleaf = nleaf;
+        T.nd = nd;
+        for (int d = 0; d < 6; ++d) { T.x[d] = xs[d]; T.xf[d] = xf[d]; }
+        T.mass = mass;
+        T.perm = perm;
+        T.box = box;
+        T.org = org;
+        T.kern = kern;
+        T.kernres = kernres;
+        T.lv = lv;
+        return T;
+    }
+};
+
+extern "C" void nbx_params_default(nbx_params *p) {
+    memset(p, 0, sizeof(*p));
+    p->bucket_size = 16;
+    p->treetype = NBX_TPHYS;
+    p->kerntype = NBX_KEPAN;
+    p->kernres = 1000;
+    p->splitting = 0;
+    p->aniso = 0;
+    p->scalespace = 0;
+    p->periodic = 0;
+    p->form = NBX_FORM_PLAIN;
+    p->recheck_rel = 1e-5;
+}
+
+// SmoothingKernels.h:32-56 and KernelConstruction (KDTree.cxx:1144-1183)
+static double kw_sph(double r, double h) {
+    double rh = r / h;
+    if (rh <= 1.0) return (1.0 - 0.75 * (2.0 - rh) * (rh * rh));
+    else if (rh > 1.0 && rh <= 2.0) return 0.25 * (2.0 - rh) * (2.0 - rh) * (2.0 - rh);
+    return 0;
+}
+static double kw_gauss(double r, double h) { double rh = r / h * 2.0; return exp(-0.5 * pow(rh, 2.0)); }
+static double kw_epan(double r, double h) { double rh = r / h * 0.5; return rh <= 1.0 ? (1.0 - rh * rh) : 0.; }
+static double kw_th(double r, double h) { double rh = r / h * 0.5; return (rh <= 1.0); }
+static void make_kernel(int nd, int type, int &res, double &norm, std::vector<double> &tab) {
+    if (res < 100) res = 100;
+    norm = 1.0 / nd * pow(M_1_PI, nd / 2.) * tgamma(nd / 2. + 1.);
+    double (*f)(double, double) = kw_sph;
+    if (type == NBX_KGAUSS) { f = kw_gauss; norm *= pow(0.5 * M_1_PI, nd / 2.); }
+    else if (type == NBX_KEPAN) { f = kw_epan; norm *= nd * (nd + 2.) * pow(0.5, nd + 1.); }
+    else if (type == NBX_KTH) { f = kw_th; }
+    else { norm *= nd * (nd + 1.) * (nd + 2.) * (nd + 3.) / (6 * (pow(2., nd + 1) - 1.)); }
+    double delta = 2.0 / (double)(res - 1);
+    tab.resize(res);
+    for (int i = 0; i < res; i++) tab[i] = norm * f(i * delta, 1.0);
+}
+
+static inline int nblk(int64_t n, int b = 256) { return (int)((n + b - 1) / b); }
+
+static void free_tree(nbx_tree *t) {
+    if (!t) return;
+    for (int d = 0; d < 6; ++d) { if (t->xs[d]) hipFree(t->xs[d]); if (t->xf[d]) hipFree(t->xf[d]); }
+    if (t->mass) hipFree(t->mass);
+    if (t->perm) hipFree(t->perm);
+    if (t->box) hipFree(t->box);
+    if (t->org) hipFree(t->org);
+    if (t->kern) hipFree(t->kern);
+    delete t;
+}
+
+static int build_tree(nbx_tree *t, const double *pos, const double *vel, const double *mass, hipStream_t st) {
+    const int64_t n = t->n;
+    const int nd = t->nd;
+    const int ncoord = (nd == 6 || t->has_vel) ? 6 : 3;
+    const double *c[6];
+    for (int d = 0; d < 3; ++d) c[d] = pos + d * n;
+    for (int d = 0; d < 3; ++d) c[3 + d] = vel ? vel + d * n : nullptr;
+
+    // bounding box of the key coordinates
+    double lo[6], scale[6];
+    {
+        const int nb = 1024;
+        double *part = nullptr;
+        HIPCK(hipMallocAsync((void **)&part, sizeof(double) * 2 * nb * nd, st));
+        for (int d = 0; d < nd; ++d)
+            hipLaunchKernelGGL(k_bbox, dim3(nb), dim3(256), 0, st, c[d], n, part + 2 * nb * d);
+        std::vector<double> hp(2 * nb * nd);
+        HIPCK(hipMemcpyAsync(hp.data(), part, sizeof(double) * hp.size(), hipMemcpyDeviceToHost, st));
+        HIPCK(hipStreamSynchronize(st));
+        HIPCK(hipFreeAsync(part, st));
+        const double qmax = nd == 3 ? (double)((1 << 21) - 1) : (double)((1 << 10) - 1);
+        for (int d = 0; d < nd; ++d) {
+            double mn = INFINITY, mx = -INFINITY;
+            for (int b = 0; b < nb; ++b) { mn = std::min(mn, hp[2 * nb * d + 2 * b]); mx = std::max(mx, hp[2 * nb * d + 2 * b + 1]); }
+            if (!std::isfinite(mn) || !std::isfinite(mx)) return fail(NBX_ERR_ARG, "non-finite coordinates");
+            lo[d] = mn;
+            scale[d] = mx > mn ? qmax / (mx - mn) : 0.0;
+        }
+    }
+    uint64_t *keys = nullptr, *keys2 = nullptr;
+    int32_t *vals = nullptr, *vals2 = nullptr;
+    void *tmp = nullptr;
+    const int bits = nd == 3 ? 63 : 60;
+    HIPCK(hipMallocAsync((void **)&keys, sizeof(uint64_t) * n, st));
+    HIPCK(hipMallocAsync((void **)&keys2, sizeof(uint64_t) * n, st));
+    HIPCK(hipMallocAsync((void **)&vals, sizeof(int32_t) * n, st));
+    HIPCK(hipMallocAsync((void **)&vals2, sizeof(int32_t) * n, st));
+    {
+        ProfScope ps("k_build_morton", st);
+        MortonArgs a;
+        memset(&a, 0, sizeof(a));
+        a.n = n;
+        a.nd = nd;
+        for (int d = 0; d < nd; ++d) { a.c[d] = c[d]; a.lo[d] = lo[d]; a.scale[d] = scale[d]; }
+        a.keys = keys;
+        a.vals = vals;
+        hipLaunchKernelGGL(k_morton, dim3(nblk(n)), dim3(256), 0, st, a);
+    }
+    size_t tb = nbx_sort_temp_bytes(n, bits);
+    HIPCK(hipMallocAsync(&tmp, tb, st));
+    {
+        ProfScope ps("k_build_sort", st);
+        HIPCK(nbx_sort_pairs(tmp, tb, keys, keys2, vals, vals2, n, bits, st));
+    }
+    for (int d = 0; d < ncoord; ++d) HIPCK(hipMalloc((void **)&t->xs[d], sizeof(double) * n));
+    HIPCK(hipMalloc((void **)&t->mass, sizeof(double) * n));
+    HIPCK(hipMalloc((void **)&t->perm, sizeof(int32_t) * n));
+    {
+        ProfScope ps("k_build_gather", st);
+        GatherArgs g;
+        memset(&g, 0, sizeof(g));
+        g.n = n;
+        g.nd = ncoord;
+        g.order = vals2;
+        for (int d = 0; d < ncoord; ++d) { g.in[d] = c[d]; g.out[d] = t->xs[d]; }
+        g.mass_in = mass;
+        g.mass_default = 1.0 / (double)n;
+        g.mass_out = t->mass;
+        g.perm = t->perm;
+        g.inv = nullptr;
+        hipLaunchKernelGGL(k_gather, dim3(nblk(n)), dim3(256), 0, st, g);
+    }
+    HIPCK(hipFreeAsync(keys, st));
+    HIPCK(hipFreeAsync(keys2, st));
+    HIPCK(hipFreeAsync(vals, st));
+    HIPCK(hipFreeAsync(vals2, st));
+    HIPCK(hipFreeAsync(tmp, st));
+
+    // implicit tree levels
+    t->nleaf = (int32_t)((n + NBX_LEAF - 1) / NBX_LEAF);
+    LevelInfo &lv = t->lv;
+    memset(&lv, 0, sizeof(lv));
+    int32_t cnt = t->nleaf, off = 0, L = 0;
+    for (;;) {
+        if (L >= NBX_MAXLEV) return fail(NBX_ERR_ARG, "tree too deep");
+        lv.off[L] = off;
+        lv.cnt[L] = cnt;
+        off += cnt;
+        L++;
+        if (cnt == 1) break;
+        cnt = (cnt + 1) / 2;
+    }
+    lv.nlev = L;
+    t->nnodes = off;
+    HIPCK(hipMalloc((void **)&t->box, sizeof(double) * 2 * nd * (size_t)t->nnodes));
+    if (t->p.form == NBX_FORM_MIXED) HIPCK(hipMalloc((void **)&t->org, sizeof(double) * nd * (size_t)t->nleaf));
+    {
+        ProfScope ps("k_build_boxes", st);
+        LeafBoxArgs a;
+        memset(&a, 0, sizeof(a));
+        a.n = n;
+        a.nleaf = t->nleaf;
+        a.nd = nd;
+        for (int d = 0; d < nd; ++d) a.c[d] = t->xs[d];
+        a.box = t->box;
+        a.org = t->org;
+        hipLaunchKernelGGL(k_leafbox, dim3(nblk(t->nleaf)), dim3(256), 0, st, a);
+        for (int l = 1; l < lv.nlev; ++l)
+            hipLaunchKernelGGL(k_levelbox, dim3(nblk(lv.cnt[l])), dim3(256), 0, st, t->box, lv.off[l - 1],
+                               lv.cnt[l - 1], lv.off[l], lv.cnt[l], nd);
+    }
+    if (t->p.form == NBX_FORM_MIXED) {
+        for (int d = 0; d < nd; ++d) HIPCK(hipMalloc((void **)&t->xf[d], sizeof(float) * n));
+        MixedArgs m;
+        memset(&m, 0, sizeof(m));
+        m.n = n;
+        m.nd = nd;
+        for (int d = 0; d < nd; ++d) { m.c[d] = t->xs[d]; m.xf[d] = t->xf[d]; }
+        m.org = t->org;
+        hipLaunchKernelGGL(k_mixed_offsets, dim3(nblk(n)), dim3(256), 0, st, m);
+    }
+    HIPCK(hipGetLastError());
+    // kernel table
+    make_kernel(nd, t->p.kerntype, t->kernres, t->kernnorm, t->hkern);
+    HIPCK(hipMalloc((void **)&t->kern, sizeof(double) * t->kernres));
+    HIPCK(hipMemcpyAsync(t->kern, t->hkern.data(), sizeof(double) * t->kernres, hipMemcpyHostToDevice, st));
+    HIPCK(hipStreamSynchronize(st));
+    return NBX_OK;
+}
+
+static int check_params(const nbx_params *p, int64_t n, bool has_vel) {
+    if (!p) return fail(NBX_ERR_ARG, "params is NULL");
+    if (n < 1 || n > 0x7fffffffLL - 4096) return fail(NBX_ERR_ARG, "number of particles out of range");
+    if (p->treetype != NBX_TPHYS && p->treetype != NBX_TPHS)
+        return fail(NBX_ERR_UNSUPPORTED, "Error in type of tree specified (TPHYS or TPHS supported)");
+    if (p->splitting != 0) return fail(NBX_ERR_UNSUPPORTED, "only the spread splitting criterion is supported");
+    if (p->scalespace != 0) return fail(NBX_ERR_UNSUPPORTED, "ScaleSpace is not supported; prescale coordinates");
+    if (p->treetype == NBX_TPHS && p->aniso != -1)
+        return fail(NBX_ERR_UNSUPPORTED, "TPHS trees need aniso = -1 (metric factor 1 on prescaled coordinates)");
+    if (p->treetype == NBX_TPHS && !has_vel) return fail(NBX_ERR_ARG, "TPHS tree needs velocities");
+    if (p->periodic)
+        for (int k = 0; k < 3; ++k)
+            if (!(p->period[k] > 0)) return fail(NBX_ERR_ARG, "period must be positive");
+    if (p->form != NBX_FORM_PLAIN && p->form != NBX_FORM_MIXED) return fail(NBX_ERR_ARG, "unknown form");
+    return NBX_OK;
+}
+
+extern "C" int nbx_tree_create_dev(const nbx_params *p, const double *pos, const double *vel, const double *mass,
+                                   int64_t n, void *stream, nbx_tree **out) {
+    if (!out || !pos) return fail(NBX_ERR_ARG, "NULL argument");
+    int rc = check_params(p, n, vel != nullptr);
+    if (rc) return rc;
+    nbx_tree *t = new nbx_tree();
+    t->p = *p;
+    if (t->p.recheck_rel <= 0) t->p.recheck_rel = 1e-5;
+    t->n = n;
+    t->nd = p->treetype == NBX_TPHS ? 6 : 3;
+    t->has_vel = (t->nd == 3 && vel != nullptr);
+    t->kernres = p->kernres;
+    rc = build_tree(t, pos, vel, mass, (hipStream_t)stream);
+    if (rc) { free_tree(t); return rc; }
+    *out = t;
+    return NBX_OK;
+}
+
+extern "C" int nbx_tree_create(const nbx_params *p, const double *pos, const double *vel, const double *mass, int64_t n,
+                               nbx_tree **out) {
+    if (!out || !pos) return fail(NBX_ERR_ARG, "NULL argument");
+    if (n < 1) return fail(NBX_ERR_ARG, "number of particles out of range");
+    double *dpos = nullptr, *dvel = nullptr, *dm = nullptr;
+    HIPCK(hipMalloc((void **)&dpos, sizeof(double) * 3 * n));
+    HIPCK(hipMemcpy(dpos, pos, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+    if (vel) {
+        HIPCK(hipMalloc((void **)&dvel, sizeof(double) * 3 * n));
+        HIPCK(hipMemcpy(dvel, vel, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+    }
+    if (mass) {
+        HIPCK(hipMalloc((void **)&dm, sizeof(double) * n));
+        HIPCK(hipMemcpy(dm, mass, sizeof(double) * n, hipMemcpyHostToDevice));
+    }
+    int rc = nbx_tree_create_dev(p, dpos, dvel, dm, n, nullptr, out);
+    hipFree(dpos);
+    if (dvel) hipFree(dvel);
+    if (dm) hipFree(dm);
+    return rc;
+}
+
+extern "C" int nbx_tree_destroy(nbx_tree *t) {
+    if (!t) return NBX_OK;
+    hipDeviceSynchronize();
+    free_tree(t);
+    return NBX_OK;
+}
+extern "C" int64_t nbx_tree_size(const nbx_tree *t) { return t ? t->n : 0; }
+extern "C" int nbx_tree_order(nbx_tree *t, int32_t *slot_to_id) {
+    if (!t || !slot_to_id) return fail(NBX_ERR_ARG, "NULL argument");
+    HIPCK(hipMemcpy(slot_to_id, t->perm, sizeof(int32_t) * t->n, hipMemcpyDeviceToHost));
+    return NBX_OK;
+}
+extern "C" int nbx_tree_kernel(const nbx_tree *t, double *kernnorm, double *table, int32_t cap) {
+    if (!t) return fail(NBX_ERR_ARG, "NULL tree");
+    if (kernnorm) *kernnorm = t->kernnorm;
+    if (table) memcpy(table, t->hkern.data(), sizeof(double) * std::min<int64_t>(cap, t->kernres));
+    return t->kernres;
+}
+
+// ------------------------------------------------------------------ kNN
+static int heap_cap(const nbx_tree *t) { return t->nd == 6 ? 65 : 65; }
+
+static int setup_knn(nbx_tree *t, int32_t k, int32_t variant, KnnArgs &A, int &nq) {
+    memset(&A, 0, sizeof(A));
+    if (!t) return fail(NBX_ERR_ARG, "NULL tree");
+    if (k < 1) return fail(NBX_ERR_ARG, "k must be >= 1");
+    const bool per = t->p.periodic != 0;
+    const bool rootleaf = t->n <= t->p.bucket_size;  // the reference's root is a leaf
+    A.excl_zero = 1;
+    A.kout = k;
+    nq = (int)t->n;
+    switch (variant) {
+        case NBX_KNN_FINDNEAREST:
+        case NBX_KNN_FINDNEARESTPHASE:
+            if (variant == NBX_KNN_FINDNEARESTPHASE && t->nd != 6)
+                return fail(NBX_ERR_UNSUPPORTED, "FindNearestPhase needs a TPHS tree");
+            if (!per) { A.qsrc = NBX_Q_SLOTS; A.kh = k; A.drop = NBX_DROP_NONE; }
+            else if (rootleaf && t->nd == 3) { A.qsrc = NBX_Q_SLOTS; A.kh = k + 1; A.drop = NBX_DROP_SMALLEST; }
+            else { A.qsrc = NBX_Q_PERIODIC; A.kh = k + 1; A.drop = NBX_DROP_SMALLEST; A.periodic = !rootleaf; }
+            break;
+        case NBX_KNN_FINDNEARESTPOS:
+            if (t->nd != 3) return fail(NBX_ERR_UNSUPPORTED, "FindNearestPos needs a TPHYS tree");
+            if (!per) { A.qsrc = NBX_Q_SLOTS; A.kh = k; A.drop = NBX_DROP_NONE; }
+            else if (rootleaf) { A.qsrc = NBX_Q_SLOTS; A.kh = k + 1; A.drop = NBX_DROP_LARGEST; }
+            else { A.qsrc = NBX_Q_PERIODIC; A.kh = k + 1; A.drop = NBX_DROP_LARGEST; A.periodic = 1; }
+            break;
+        case NBX_KNN_NONPERIODIC:
+            A.qsrc = NBX_Q_SLOTS; A.kh = k; A.drop = NBX_DROP_NONE;
+            break;
+        default:
+            return fail(NBX_ERR_ARG, "unknown kNN variant");
+    }
+    if (A.kh > heap_cap(t)) return fail(NBX_ERR_UNSUPPORTED, "k too large (k <= 64 supported)");
+    if (A.periodic) for (int d = 0; d < 3; ++d) A.period[d] = t->p.period[d];
+    return NBX_OK;
+}
+
+extern "C" int nbx_knn_dev(nbx_tree *t, int32_t k, int32_t variant, int32_t layout, int32_t *nn_dev, double *d2_dev,
+                           void *stream) {
+    KnnArgs A;
+    int nq;
+    int rc = setup_knn(t, k, variant, A, nq);
+    if (rc) return rc;
+    if (layout != NBX_LAYOUT_BY_ID && layout != NBX_LAYOUT_BY_SLOT) return fail(NBX_ERR_ARG, "unknown layout");
+    A.layout = layout;
+    A.nn = nn_dev;
+    A.d2 = d2_dev;
+    hipStream_t st = (hipStream_t)stream;
+    HIPCK(nbx_launch_knn(t->view(), A, nq, st));
+    return NBX_OK;
+}
+
+extern "C" int nbx_knn(nbx_tree *t, int32_t k, int32_t variant, int32_t layout, int32_t *nn, double *d2) {
+    if (!t) return fail(NBX_ERR_ARG, "NULL tree");
+    int32_t *dnn = nullptr;
+    double *dd2 = nullptr;
+    size_t cnt = (size_t)t->n * (size_t)k;
+    if (nn) HIPCK(hipMalloc((void **)&dnn, sizeof(int32_t) * cnt));
+    if (d2) HIPCK(hipMalloc((void **)&dd2, sizeof(double) * cnt));
+    int rc = nbx_knn_dev(t, k, variant, layout, dnn, dd2, nullptr);
+    if (rc == NBX_OK) {
+        hipError_t e = hipDeviceSynchronize();
+        if (e == hipSuccess && nn) e = hipMemcpy(nn, dnn, sizeof(int32_t) * cnt, hipMemcpyDeviceToHost);
+        if (e == hipSuccess && d2) e = hipMemcpy(d2, dd2, sizeof(double) * cnt, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = hipfail(e, "nbx_knn");
+    }
+    if (dnn) hipFree(dnn);
+    if (dd2) hipFree(dd2);
+    return rc;
+}
+
+extern "C" int nbx_knn_points(nbx_tree *t, const double *q, int64_t m, int32_t k, int32_t *nn, double *d2) {
+    if (!t || !q || !nn) return fail(NBX_ERR_ARG, "NULL argument");
+    if (t->nd != 3) return fail(NBX_ERR_UNSUPPORTED, "position queries need a TPHYS tree");
+    if (k < 1 || k > 64) return fail(NBX_ERR_UNSUPPORTED, "k out of range (1..64)");
+    if (m <= 0) return NBX_OK;
+    double *dq = nullptr, *dd2 = nullptr;
+    int32_t *dnn = nullptr;
+    HIPCK(hipMalloc((void **)&dq, sizeof(double) * 3 * m));
+    HIPCK(hipMemcpy(dq, q, sizeof(double) * 3 * m, hipMemcpyHostToDevice));
+    HIPCK(hipMalloc((void **)&dnn, sizeof(int32_t) * m * k));
+    if (d2) HIPCK(hipMalloc((void **)&dd2, sizeof(double) * m * k));
+    KnnArgs A;
+    memset(&A, 0, sizeof(A));
+    A.qsrc = NBX_Q_POINTS;
+    A.kh = k;
+    A.kout = k;
+    A.m = (int32_t)m;
+    A.periodic = t->p.periodic && t->n > t->p.bucket_size;
+    for (int d = 0; d < 3; ++d) { A.period[d] = t->p.period[d]; A.q[d] = dq + d * m; }
+    A.nn = dnn;
+    A.d2 = dd2;
+    hipError_t e = nbx_launch_knn(t->view(), A, (int)m, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(nn, dnn, sizeof(int32_t) * m * k, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && d2) e = hipMemcpy(d2, dd2, sizeof(double) * m * k, hipMemcpyDeviceToHost);
+    hipFree(dq);
+    hipFree(dnn);
+    if (dd2) hipFree(dd2);
+    if (e != hipSuccess) return hipfail(e, "nbx_knn_points");
+    return NBX_OK;
+}
+
+// ------------------------------------------------------------------ density
+static int density_impl(nbx_tree *t, int32_t k, int32_t variant, int32_t nsearch, double *rho_id, double *h_id,
+                        int32_t *nn_slot, hipStream_t st) {
+    if (!t) return fail(NBX_ERR_ARG, "NULL tree");
+    if (k < 1) return fail(NBX_ERR_ARG, "k must be >= 1");
+    KnnArgs A;
+    memset(&A, 0, sizeof(A));
+    A.qsrc = NBX_Q_SLOTS;
+    A.excl_zero = 1;
+    A.kh = k;
+    A.kout = k;
+    A.layout = NBX_LAYOUT_BY_SLOT;
+    if (variant == NBX_DENS_SYMMETRIC) A.dens = NBX_DO_SYM;
+    else if (variant == NBX_DENS_GATHER) A.dens = NBX_DO_GATHER;
+    else if (variant == NBX_DENS_VEL) {
+        if (t->nd != 3 || !t->has_vel)
+            return fail(NBX_ERR_UNSUPPORTED, "CalcVelDensity needs a TPHYS tree built with velocities");
+        A.dens = NBX_DO_VEL;
+        if (nsearch < 1) return fail(NBX_ERR_ARG, "Nsearch must be >= 1");
+        A.kh = nsearch;
+        A.nsmooth = k > nsearch ? nsearch : k;
+    } else
+        return fail(NBX_ERR_ARG, "unknown density variant");
+    if (A.kh > heap_cap(t)) return fail(NBX_ERR_UNSUPPORTED, "k too large (k <= 64 supported)");
+    const int64_t n = t->n;
+    double *rho = nullptr, *h = nullptr;
+    int32_t *err = nullptr;
+    HIPCK(hipMallocAsync((void **)&rho, sizeof(double) * n, st));
+    if (h_id) HIPCK(hipMallocAsync((void **)&h, sizeof(double) * n, st));
+    HIPCK(hipMallocAsync((void **)&err, sizeof(int32_t), st));
+    hipLaunchKernelGGL(k_fill_f64, dim3(nblk(n)), dim3(256), 0, st, rho, 0.0, n);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, st, err, 0, (int64_t)1);
+    A.rho = rho;
+    A.hout = h;
+    A.err = err;
+    A.nn = nn_slot;
+    hipError_t e = nbx_launch_knn(t->view(), A, (int)n, st);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_scatter_by_id, dim3(nblk(n)), dim3(256), 0, st, rho, t->perm, rho_id, n);
+        if (h_id) hipLaunchKernelGGL(k_scatter_by_id, dim3(nblk(n)), dim3(256), 0, st, h, t->perm, h_id, n);
+        e = hipGetLastError();
+    }
+    int32_t herr = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&herr, err, sizeof(int32_t), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    hipFreeAsync(rho, st);
+    if (h) hipFreeAsync(h, st);
+    hipFreeAsync(err, st);
+    if (e != hipSuccess) return hipfail(e, "density");
+    if (herr) return fail(NBX_ERR_SEARCH, "CalcDensity failed: fewer than k neighbours");
+    return NBX_OK;
+}
+
+extern "C" int nbx_density_dev(nbx_tree *t, int32_t k, int32_t variant, int32_t nsearch, double *rho_by_id_dev,
+                               double *h_by_id_dev, void *stream) {
+    if (!rho_by_id_dev) return fail(NBX_ERR_ARG, "NULL output");
+    return density_impl(t, k, variant, nsearch, rho_by_id_dev, h_by_id_dev, nullptr, (hipStream_t)stream);
+}
+
+extern "C" int nbx_knn_density_dev(nbx_tree *t, int32_t k, int32_t *nn_by_slot_dev, double *rho_by_id_dev,
+                                   double *h_by_id_dev, void *stream) {
+    if (!rho_by_id_dev) return fail(NBX_ERR_ARG, "NULL output");
+    return density_impl(t, k, NBX_DENS_SYMMETRIC, 0, rho_by_id_dev, h_by_id_dev, nn_by_slot_dev, (hipStream_t)stream);
+}
+
+extern "C" int nbx_density(nbx_tree *t, int32_t k, int32_t variant, int32_t nsearch, double *rho_by_id,
+                           double *h_by_id) {
+    if (!t || !rho_by_id) return fail(NBX_ERR_ARG, "NULL argument");
+    double *dr = nullptr, *dh = nullptr;
+    HIPCK(hipMalloc((void **)&dr, sizeof(double) * t->n));
+    if (h_by_id) HIPCK(hipMalloc((void **)&dh, sizeof(double) * t->n));
+    int rc = density_impl(t, k, variant, nsearch, dr, dh, nullptr, nullptr);
+    if (rc == NBX_OK) {
+        hipError_t e = hipMemcpy(rho_by_id, dr, sizeof(double) * t->n, hipMemcpyDeviceToHost);
+        if (e == hipSuccess && h_by_id) e = hipMemcpy(h_by_id, dh, sizeof(double) * t->n, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = hipfail(e, "nbx_density");
+    }
+    hipFree(dr);
+    if (dh) hipFree(dh);
+    return rc;
+}
+
+// ------------------------------------------------------------------ FOF
+extern "C" int nbx_fof_dev(nbx_tree *t, double fdist, int32_t minnum, int32_t *labels_by_id_dev, int64_t *ngroups,
+                           void *stream) {
+    if (!t || !labels_by_id_dev || !ngroups) return fail(NBX_ERR_ARG, "NULL argument");
+    if (t->nd != 3) return fail(NBX_ERR_UNSUPPORTED, "FOF on TPHS trees is not supported (use a TPHYS tree)");
+    if (!(fdist >= 0)) return fail(NBX_ERR_ARG, "linking length must be >= 0");
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t n = t->n;
+    int32_t *P = nullptr;
+    size_t bytes = sizeof(int32_t) * n + (size_t)t->nnodes + 16;
+    HIPCK(hipMallocAsync((void **)&P, bytes, st));
+    FofArgs A;
+    memset(&A, 0, sizeof(A));
+    A.b2 = fdist * fdist;
+    A.periodic = t->p.periodic && n > t->p.bucket_size;
+    for (int d = 0; d < 3; ++d) A.period[d] = t->p.period[d];
+    A.parent = P;
+    DevTree T = t->view();
+    hipError_t e = nbx_launch_fof(T, A, st);
+    if (e == hipSuccess) e = nbx_fof_canonical(T, P, minnum, labels_by_id_dev, ngroups, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    hipFreeAsync(P, st);
+    if (e != hipSuccess) return hipfail(e, "fof");
+    return NBX_OK;
+}
+
+extern "C" int nbx_fof(nbx_tree *t, double fdist, int32_t minnum, int32_t *labels_by_id, int64_t *ngroups) {
+    if (!t || !labels_by_id || !ngroups) return fail(NBX_ERR_ARG, "NULL argument");
+    int32_t *d = nullptr;
+    HIPCK(hipMalloc((void **)&d, sizeof(int32_t) * t->n));
+    int rc = nbx_fof_dev(t, fdist, minnum, d, ngroups, nullptr);
+    if (rc == NBX_OK) {
+        hipError_t e = hipMemcpy(labels_by_id, d, sizeof(int32_t) * t->n, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = hipfail(e, "nbx_fof");
+    }
+    hipFree(d);
+    return rc;
+}
+
+// ------------------------------------------------------------------ ball search
+extern "C" int nbx_ball_points(nbx_tree *t, const double *x, double r2, int32_t *out, int64_t cap, int64_t *nout) {
+    if (!t || !x || !nout) return fail(NBX_ERR_ARG, "NULL argument");
+    if (t->nd != 3) return fail(NBX_ERR_UNSUPPORTED, "ball search needs a TPHYS tree");
+    int32_t *dout = nullptr;
+    unsigned long long *dc = nullptr;
+    int64_t dcap = cap > 0 ? cap : 1;
+    HIPCK(hipMalloc((void **)&dout, sizeof(int32_t) * dcap));
+    HIPCK(hipMalloc((void **)&dc, sizeof(unsigned long long)));
+    HIPCK(hipMemset(dc, 0, sizeof(unsigned long long)));
+    DevTree T = t->view();
+    // primary position and the images of SearchBallPosPeriodicTagged
+    // (KDSplitNode.cxx:1388-1414): image when r2 > s^2
+    double pts[8][3];
+    int np = 0;
+    pts[np][0] = x[0]; pts[np][1] = x[1]; pts[np][2] = x[2]; np++;
+    if (t->p.periodic && t->n > t->p.bucket_size) {
+        const double *p = t->p.period;
+        for (int k = 0; k < 3; ++k) {
+            double xp[3] = {x[0], x[1], x[2]}, s;
+            if (x[k] < p[k] / 2.0) { xp[k] = x[k] + p[k]; s = x[k]; } else { xp[k] = x[k] - p[k]; s = -xp[k]; }
+            if (r2 > s * s) { memcpy(pts[np], xp, sizeof(xp)); np++; }
+        }
+        const int pr[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+        for (int c = 0; c < 4; ++c) {
+            double xp[3] = {x[0], x[1], x[2]}, s = 0;
+            for (int k = 0; k < 3; ++k) {
+                bool use = c == 3 || k == pr[c][0] || k == pr[c][1];
+                if (!use) continue;
+                if (x[k] < p[k] / 2.0) { xp[k] = x[k] + p[k]; s += x[k] * x[k]; } else { xp[k] = x[k] - p[k]; s += xp[k] * xp[k]; }
+            }
+            s = sqrt(s);
+            if (r2 > s * s) { memcpy(pts[np], xp, sizeof(xp)); np++; }
+        }
+    }
+    for (int i = 0; i < np; ++i)
+        hipLaunchKernelGGL(k_ball, dim3(1), dim3(NBX_WAVE), 0, 0, T, pts[i][0], pts[i][1], pts[i][2], r2, dout, cap, dc);
+    unsigned long long hc = 0;
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpy(&hc, dc, sizeof(hc), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && cap > 0) e = hipMemcpy(out, dout, sizeof(int32_t) * std::min<int64_t>(cap, (int64_t)hc), hipMemcpyDeviceToHost);
+    hipFree(dout);
+    hipFree(dc);
+    if (e != hipSuccess) return hipfail(e, "ball");
+    *nout = (int64_t)hc;
+    return NBX_OK;
+}
+
+// ------------------------------------------------------------------ differentiation form
+extern "C" int nbx_sph_density_fwd_dev(const double *pos, const double *mass, const int32_t *nbr, int64_t n, int32_t k,
+                                       double *rho, double *h, void *stream) {
+    if (!pos || !mass || !nbr || !rho) return fail(NBX_ERR_ARG, "NULL argument");
+    if (n < 1 || k < 1) return fail(NBX_ERR_ARG, "bad sizes");
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_fill_f64, dim3(nblk(n)), dim3(256), 0, st, rho, 0.0, n);
+    HIPCK(nbx_launch_sph_fwd(pos, mass, nbr, n, k, rho, h, st));
+    return NBX_OK;
+}
+extern "C" int nbx_sph_density_adjoint_dev(const double *pos, const double *mass, const int32_t *nbr, int64_t n,
+                                           int32_t k, const double *g, double *dpos, double *dmass, void *stream) {
+    if (!pos || !mass || !nbr || !g || !dpos || !dmass) return fail(NBX_ERR_ARG, "NULL argument");
+    if (n < 1 || k < 1) return fail(NBX_ERR_ARG, "bad sizes");
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_fill_f64, dim3(nblk(3 * n)), dim3(256), 0, st, dpos, 0.0, 3 * n);
+    hipLaunchKernelGGL(k_fill_f64, dim3(nblk(n)), dim3(256), 0, st, dmass, 0.0, n);
+    HIPCK(nbx_launch_sph_adj(pos, mass, nbr, n, k, g, dpos, dmass, st));
+    return NBX_OK;
+}

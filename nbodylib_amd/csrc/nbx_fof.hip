@@ -1,0 +1,336 @@
+// nbx_fof.hip -- friends-of-friends on gfx950.
+//
+// Replaces KDTree::FOF (KDFOF.cxx:29-153) with its node routines
+// SplitNode/LeafNode::FOFSearchBall (KDSplitNode.cxx:921-988,
+// KDLeafNode.cxx:527-590) and the periodic image search
+// (KDSplitNode.cxx:1602-1646).  The reference grows one group at a time by a
+// serial breadth-first search; here every particle links concurrently into a
+// lock-free union-find over slots, which yields the same partition (the
+// transitive closure of d2 < b^2), then groups are ranked canonically.
+//
+// Exact link shortcuts (the reference's enclosure test, KDNode.h:279-303):
+//  * a leaf whose box diagonal is < b is internally connected (pre-pass);
+//  * a query whose farthest-corner distance to such a leaf is < b links to
+//    the whole leaf with one union;
+//  * when the farthest distance between the wave's 64 particles and a node
+//    is < b, all cross pairs link, so one union joins the node to the wave.
+// All bounds are monotone in the roundings (nbx_internal.h), so a shortcut
+// is taken only when every computed particle distance is below b^2.
+#include "nbx_kernels.h"
+
+__global__ __launch_bounds__(256) void k_fof_leafinit(DevTree T, double b2, int32_t *P, uint8_t *conn) {
+    int32_t L = blockIdx.x * 256 + threadIdx.x;
+    if (L >= T.nleaf) return;
+    cdptr bx = (cdptr)node_box(T, 0, L, 3);
+    int32_t s0 = L * NBX_LEAF;
+    int32_t s1 = s0 + NBX_LEAF < T.n ? s0 + NBX_LEAF : T.n;
+    bool c = boxbox_ub<3>(bx, bx) < b2;
+    conn[L] = c ? 1 : 0;
+    for (int32_t s = s0; s < s1; ++s) P[s] = c ? s0 : s;
+}
+
+// Internal node whose diagonal is < b: both children are connected (their
+// diagonals are smaller) and every cross pair links, so one union connects it.
+__global__ __launch_bounds__(256) void k_fof_nodeinit(DevTree T, double b2, int32_t *P, uint8_t *conn, int level) {
+    int32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= T.lv.cnt[level]) return;
+    cdptr bx = (cdptr)node_box(T, level, i, 3);
+    bool c = boxbox_ub<3>(bx, bx) < b2;
+    conn[T.lv.off[level] + i] = c ? 1 : 0;
+    int32_t c1 = 2 * i + 1;
+    if (c && c1 < T.lv.cnt[level - 1]) uf_union(P, (i << level) * NBX_LEAF, (c1 << (level - 1)) * NBX_LEAF);
+}
+
+// link test from the (possibly image-shifted) position q to candidate slot cs
+template <int PASS0>
+__device__ __forceinline__ void fof_leaf(const DevTree &T, int leaf, const double *q, bool act, int32_t qi,
+                                         double b2, int32_t *P, const uint8_t *conn) {
+    const int base = leaf * NBX_LEAF;
+    int cnt = T.n - base;
+    cnt = cnt < NBX_LEAF ? cnt : NBX_LEAF;
+    cdptr bx = (cdptr)node_box(T, 0, leaf, 3);
+    bool encl = act && conn[leaf] && box_ub<3>(q, bx) < b2;
+    if (encl) uf_union(P, qi, base);
+    bool pair = act && !encl;
+    if (!ballot(pair)) return;
+    uint32_t mask = 0;
+    for (int j = 0; j < cnt; ++j) {
+        double d2 = 0.0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            double t = q[d] - ((cdptr)T.x[d])[base + j];
+            d2 += t * t;
+        }
+        bool l = pair && d2 < b2 && (PASS0 ? (base + j > qi) : (base + j != qi));
+        mask |= l ? (1u << j) : 0u;
+    }
+    while (mask) {
+        int j = __builtin_ctz(mask);
+        mask &= mask - 1;
+        uf_union(P, qi, base + j);
+    }
+}
+
+template <int PASS0>
+__device__ __forceinline__ void fof_walk(const DevTree &T, const double *q, bool act, int32_t qi, int g, double b2,
+                                         int32_t *P, uint8_t *conn) {
+    const int top = T.lv.nlev - 1;
+    // the wave's own particles form node g of level 1 (or leaf 0 when there is one leaf)
+    const int glev = top >= 1 ? 1 : 0;
+    cdptr gb = (cdptr)node_box(T, glev, g, 3);
+    int level = top, idx = 0;
+    bool down = true;
+    for (;;) {
+        if (down) {
+            cdptr bx = (cdptr)node_box(T, level, idx, 3);
+            bool want = act && box_lb<3>(q, bx) < b2;
+            if (ballot(want)) {
+                if (PASS0 && boxbox_ub<3>(gb, bx) < b2) {
+                    // every wave particle links to every particle of the node:
+                    // wave and node form one component
+                    const int32_t nfirst = (idx << level) * NBX_LEAF;
+                    if (act) uf_union(P, qi, nfirst);
+                    const int32_t gi = T.lv.off[level] + idx;
+                    if (!conn[gi]) {
+                        int32_t nlast = ((idx + 1) << level) * NBX_LEAF;
+                        nlast = nlast < T.n ? nlast : T.n;
+                        for (int32_t s = nfirst + 1 + threadIdx.x; s < nlast; s += NBX_WAVE) uf_union(P, s, nfirst);
+                        conn[gi] = 1;
+                    }
+                    down = false;
+                } else if (level == 0) {
+                    fof_leaf<PASS0>(T, idx, q, act, qi, b2, P, conn);
+                    down = false;
+                } else {
+                    level -= 1;
+                    idx = 2 * idx;
+                    continue;
+                }
+            } else {
+                down = false;
+            }
+        }
+        if (level == top) break;
+        int sib = idx + 1;
+        if (!(idx & 1) && sib < T.lv.cnt[level]) {
+            idx = sib;
+            down = true;
+        } else {
+            idx >>= 1;
+            level += 1;
+        }
+    }
+}
+
+__global__ __launch_bounds__(64) void k_fof(DevTree T, FofArgs A, uint8_t *conn) {
+    const int lane = threadIdx.x;
+    const int g = blockIdx.x;
+    const int32_t qi = g * NBX_WAVE + lane;
+    const bool valid = qi < T.n;
+    double q0[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) q0[d] = valid ? T.x[d][qi] : 0.0;
+    fof_walk<1>(T, q0, valid, qi, g, A.b2, A.parent, conn);
+    if (!A.periodic) return;
+    // FOFSearchBallPeriodic: image toward the far wall when b^2 > s^2
+    const double *per = A.period;
+    for (int pass = 1; pass < 8; ++pass) {
+        double xp[3] = {q0[0], q0[1], q0[2]};
+        double sval = 0.0;
+        if (pass <= 3) {
+            int k = pass - 1;
+            if (q0[k] < per[k] / 2.0) { xp[k] = q0[k] + per[k]; sval = q0[k]; }
+            else { xp[k] = q0[k] - per[k]; sval = -xp[k]; }
+        } else {
+            int k1 = (pass == 6) ? 1 : 0, k2 = (pass == 4) ? 1 : 2;
+            for (int k = 0; k < 3; ++k) {
+                bool use = (pass == 7) || k == k1 || k == k2;
+                if (!use) continue;
+                if (q0[k] < per[k] / 2.0) { xp[k] = q0[k] + per[k]; sval += q0[k] * q0[k]; }
+                else { xp[k] = q0[k] - per[k]; sval += xp[k] * xp[k]; }
+            }
+            sval = sqrt(sval);
+        }
+        bool act = valid && A.b2 > sval * sval;
+        if (!ballot(act)) continue;
+        fof_walk<0>(T, xp, act, qi, g, A.b2, A.parent, conn);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_fof_compress(int32_t *P, int64_t n, const int32_t *perm, int32_t *cnt,
+                                                        int32_t *minid) {
+    int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    int32_t r = uf_find(P, (int32_t)i);
+    atomicAdd(cnt + r, 1);
+    atomicMin(minid + r, perm[i]);
+}
+__global__ __launch_bounds__(256) void k_fof_store_root(int32_t *P, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    P[i] = uf_find(P, (int32_t)i);
+}
+__global__ __launch_bounds__(256) void k_fof_collect(const int32_t *P, const int32_t *cnt, const int32_t *minid,
+                                                       int64_t n, int32_t minnum, uint64_t *keys, int32_t *vals,
+                                                       int32_t *ng) {
+    int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    if (P[i] != i || cnt[i] < minnum) return;
+    int32_t at = atomicAdd(ng, 1);
+    keys[at] = ((uint64_t)(0xffffffffu - (uint32_t)cnt[i]) << 32) | (uint32_t)minid[i];
+    vals[at] = (int32_t)i;
+}
+__global__ __launch_bounds__(256) void k_fof_rank(const int32_t *vals, int64_t ng, int32_t *lab) {
+    int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (r >= ng) return;
+    lab[vals[r]] = (int32_t)(r + 1);
+}
+__global__ __launch_bounds__(256) void k_fof_out(const int32_t *P, const int32_t *lab, const int32_t *perm, int64_t n,
+                                                   int32_t *out) {
+    int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    out[perm[i]] = lab[P[i]];
+}
+
+static inline int nblk(int64_t n) { return (int)((n + 255) / 256); }
+
+hipError_t nbx_launch_fof(const DevTree &T, const FofArgs &A, hipStream_t st) {
+    // per-node connectivity flags live right after the parent array (allocated by the caller)
+    uint8_t *conn = (uint8_t *)(A.parent + T.n);
+    {
+        ProfScope ps("k_fof_init", st);
+        hipLaunchKernelGGL(k_fof_leafinit, dim3(nblk(T.nleaf)), dim3(256), 0, st, T, A.b2, A.parent, conn);
+        for (int l = 1; l < T.lv.nlev; ++l)
+            hipLaunchKernelGGL(k_fof_nodeinit, dim3(nblk(T.lv.cnt[l])), dim3(256), 0, st, T, A.b2, A.parent, conn, l);
+    }
+    int groups = (T.n + NBX_WAVE - 1) / NBX_WAVE;
+    ProfScope ps("k_fof_link", st);
+    hipLaunchKernelGGL(k_fof, dim3(groups), dim3(NBX_WAVE), 0, st, T, A, conn);
+    return hipGetLastError();
+}
+
+// Canonical labels: groups of >= minnum members numbered 1..G by size
+// descending, ties by smaller minimum member ID; everything else 0.
+hipError_t nbx_fof_canonical(const DevTree &T, int32_t *P, int32_t minnum, int32_t *labels_by_id, int64_t *ngroups,
+                             hipStream_t st) {
+    int64_t n = T.n;
+    int32_t *cnt = nullptr, *minid = nullptr, *vals = nullptr, *vals2 = nullptr, *ng = nullptr, *lab = nullptr;
+    uint64_t *keys = nullptr, *keys2 = nullptr;
+    void *tmp = nullptr;
+    hipError_t e = hipSuccess;
+#define CK(x) do { e = (x); if (e != hipSuccess) goto done; } while (0)
+    CK(hipMallocAsync((void **)&cnt, sizeof(int32_t) * n, st));
+    CK(hipMallocAsync((void **)&minid, sizeof(int32_t) * n, st));
+    CK(hipMallocAsync((void **)&lab, sizeof(int32_t) * n, st));
+    CK(hipMallocAsync((void **)&ng, sizeof(int32_t), st));
+    hipLaunchKernelGGL(k_fill_i32, dim3(nblk(n)), dim3(256), 0, st, cnt, 0, n);
+    hipLaunchKernelGGL(k_fill_i32, dim3(nblk(n)), dim3(256), 0, st, minid, 0x7fffffff, n);
+    hipLaunchKernelGGL(k_fill_i32, dim3(nblk(n)), dim3(256), 0, st, lab, 0, n);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, st, ng, 0, (int64_t)1);
+    {
+        ProfScope ps("k_fof_finish", st);
+        hipLaunchKernelGGL(k_fof_compress, dim3(nblk(n)), dim3(256), 0, st, P, n, T.perm, cnt, minid);
+        hipLaunchKernelGGL(k_fof_store_root, dim3(nblk(n)), dim3(256), 0, st, P, n);
+    }
+    {
+        int64_t cap = n / (minnum > 1 ? minnum : 1) + 1;
+        CK(hipMallocAsync((void **)&keys, sizeof(uint64_t) * cap, st));
+        CK(hipMallocAsync((void **)&keys2, sizeof(uint64_t) * cap, st));
+        CK(hipMallocAsync((void **)&vals, sizeof(int32_t) * cap, st));
+        CK(hipMallocAsync((void **)&vals2, sizeof(int32_t) * cap, st));
+        hipLaunchKernelGGL(k_fof_collect, dim3(nblk(n)), dim3(256), 0, st, P, cnt, minid, n, minnum > 1 ? minnum : 1,
+                           keys, vals, ng);
+        int32_t hng = 0;
+        CK(hipMemcpyAsync(&hng, ng, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        CK(hipStreamSynchronize(st));
+        *ngroups = hng;
+        if (hng > 0) {
+            size_t tb = 0;
+            CK(nbx_sort_pairs_u64(nullptr, tb, keys, keys2, vals, vals2, hng, st));
+            CK(hipMallocAsync(&tmp, tb, st));
+            CK(nbx_sort_pairs_u64(tmp, tb, keys, keys2, vals, vals2, hng, st));
+            hipLaunchKernelGGL(k_fof_rank, dim3(nblk(hng)), dim3(256), 0, st, vals2, (int64_t)hng, lab);
+        }
+        hipLaunchKernelGGL(k_fof_out, dim3(nblk(n)), dim3(256), 0, st, P, lab, T.perm, n, labels_by_id);
+        CK(hipGetLastError());
+    }
+done:
+    if (cnt) hipFreeAsync(cnt, st);
+    if (minid) hipFreeAsync(minid, st);
+    if (lab) hipFreeAsync(lab, st);
+    if (ng) hipFreeAsync(ng, st);
+    if (keys) hipFreeAsync(keys, st);
+    if (keys2) hipFreeAsync(keys2, st);
+    if (vals) hipFreeAsync(vals, st);
+    if (vals2) hipFreeAsync(vals2, st);
+    if (tmp) hipFreeAsync(tmp, st);
+#undef CK
+    return e;
+}
+
+// ------------------------------------------------------------------ small kernels
+__global__ __launch_bounds__(256) void k_scatter_by_id(const double *src, const int32_t *perm, double *dst, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) dst[perm[i]] = src[i];
+}
+__global__ __launch_bounds__(256) void k_fill_i32(int32_t *p, int32_t v, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+__global__ __launch_bounds__(256) void k_fill_f64(double *p, double v, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+__global__ __launch_bounds__(256) void k_iota(int32_t *p, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = (int32_t)i;
+}
+
+// ------------------------------------------------------------------ ball search
+// SearchBallPosTagged(x) (KDFindNearest.cxx:618-688, KDLeafNode.cxx:336-356):
+// every particle with d2 < r2 around one position; one wave walks the tree
+// and each lane tests one candidate of a leaf.
+__global__ __launch_bounds__(64) void k_ball(DevTree T, double qx, double qy, double qz, double r2, int32_t *out,
+                                             int64_t cap, unsigned long long *count) {
+    const double q[3] = {qx, qy, qz};
+    const int top = T.lv.nlev - 1;
+    int level = top, idx = 0;
+    bool down = true;
+    for (;;) {
+        if (down) {
+            cdptr bx = (cdptr)node_box(T, level, idx, 3);
+            if (box_lb<3>(q, bx) < r2) {
+                if (level == 0) {
+                    int32_t cs = idx * NBX_LEAF + (int)threadIdx.x;
+                    if (threadIdx.x < NBX_LEAF && cs < T.n) {
+                        double d2 = 0.0;
+                        for (int d = 0; d < 3; ++d) {
+                            double t = q[d] - T.x[d][cs];
+                            d2 += t * t;
+                        }
+                        if (d2 < r2) {
+                            unsigned long long at = atomicAdd(count, 1ULL);
+                            if ((int64_t)at < cap) out[at] = T.perm[cs];
+                        }
+                    }
+                    down = false;
+                } else {
+                    level -= 1;
+                    idx = 2 * idx;
+                    continue;
+                }
+            } else {
+                down = false;
+            }
+        }
+        if (level == top) break;
+        int sib = idx + 1;
+        if (!(idx & 1) && sib < T.lv.cnt[level]) {
+            idx = sib;
+            down = true;
+        } else {
+            idx >>= 1;
+            level += 1;
+        }
+    }
+}

@@ -1,0 +1,153 @@
+// nbx_internal.h -- device-side layout of the KDTree hot path on gfx950.
+//
+// Particles live in HBM as SoA arrays sorted by Morton key ("slots").  Leaves
+// are fixed buckets of NBX_LEAF consecutive slots; internal nodes form an
+// implicit binary tree over the leaves (level 0 = leaves, node i of level l
+// has children 2i and 2i+1 of level l-1).  Every node stores its tight FP64
+// bounding box, so all pruning bounds are computed with the same roundings as
+// the particle distances (see box_lb / box_ub).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define NBX_LEAF 32
+#define NBX_WAVE 64
+#define NBX_MAXLEV 40
+#define NBX_MAXVALUE 1e32  // MAXVALUE, Precision.h:48
+
+typedef const __attribute__((address_space(4))) double *cdptr;  // scalar-cache (read-only) view
+typedef const __attribute__((address_space(4))) float *cfptr;
+typedef const __attribute__((address_space(4))) int32_t *ciptr;
+
+struct LevelInfo {
+    int32_t nlev;
+    int32_t off[NBX_MAXLEV];
+    int32_t cnt[NBX_MAXLEV];
+};
+
+// Kernel-side view of a built tree (passed by value).
+struct DevTree {
+    int32_t n;      // particles
+    int32_t nleaf;  // leaves = ceil(n / NBX_LEAF)
+    int32_t nd;     // 3 or 6
+    int32_t pad;
+    const double *x[6];    // sorted coordinates (x y z [vx vy vz])
+    const double *mass;    // sorted masses
+    const int32_t *perm;   // slot -> ID
+    const double *box;     // node boxes, [node][lo[nd], hi[nd]]
+    const double *org;     // mixed form: per-leaf FP64 origin [leaf][nd]
+    const float *xf[6];    // mixed form: FP32 offsets from the leaf origin
+    const double *kern;    // kernel table
+    int32_t kernres;
+    int32_t pad2;
+    LevelInfo lv;
+};
+
+__device__ __forceinline__ const double *node_box(const DevTree &T, int level, int idx, int nd) {
+    return T.box + (size_t)(T.lv.off[level] + idx) * 2 * nd;
+}
+
+// Lower bound on the squared distance from q to any point of box b, summed
+// in coordinate order from 0.  Because IEEE subtraction, squaring and
+// addition are monotone, the computed bound never exceeds the computed
+// distance to a particle inside the box (reference: DistFunc.h:14-19 sums
+// (p1-p2)^2 in the same order), so a search pruned by it is exact.
+template <int ND>
+__device__ __forceinline__ double box_lb(const double *q, cdptr b) {
+    double s = 0.0;
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        double lo = b[d], hi = b[ND + d];
+        double t = 0.0;
+        if (q[d] < lo) t = lo - q[d];
+        else if (q[d] > hi) t = q[d] - hi;
+        s += t * t;
+    }
+    return s;
+}
+// Upper bound on the squared distance from q to any point of box b (farthest
+// corner, KDNode.h:279-291 computes the same for whole-node links).
+template <int ND>
+__device__ __forceinline__ double box_ub(const double *q, cdptr b) {
+    double s = 0.0;
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        double a = q[d] - b[d], c = b[ND + d] - q[d];
+        double a2 = a * a, c2 = c * c;
+        s += (a2 > c2) ? a2 : c2;
+    }
+    return s;
+}
+// Upper bound on the squared distance between any point of box a and any
+// point of box b (monotone for the same reason).
+template <int ND>
+__device__ __forceinline__ double boxbox_ub(cdptr a, cdptr b) {
+    double s = 0.0;
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        double u = b[ND + d] - a[d], v = a[ND + d] - b[d];
+        double m = u > v ? u : v;
+        s += m * m;
+    }
+    return s;
+}
+template <int ND>
+__device__ __forceinline__ double boxbox_lb(cdptr a, cdptr b) {
+    double s = 0.0;
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        double t = 0.0;
+        if (a[ND + d] < b[d]) t = b[d] - a[ND + d];
+        else if (b[ND + d] < a[d]) t = a[d] - b[ND + d];
+        s += t * t;
+    }
+    return s;
+}
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p ? 1 : 0); }
+
+__device__ __forceinline__ double readfirst_d(double v) {
+    union { double d; int32_t i[2]; } u;
+    u.d = v;
+    u.i[0] = __builtin_amdgcn_readfirstlane(u.i[0]);
+    u.i[1] = __builtin_amdgcn_readfirstlane(u.i[1]);
+    return u.d;
+}
+
+// Wsm, KDCalcSmoothQuantities.cxx:12-15: linear interpolation in the table,
+// indexed as (int)(r/h*0.5*(kernres-1)).
+__device__ __forceinline__ double wsm_at(double rij, double hi, const double *kern, int kernres) {
+    double r = rij / hi;
+    int i = (int)(rij / hi * 0.5 * (kernres - 1));
+    if (i < 0) i = 0;
+    double delta = 2.0 / (double)(kernres - 1);
+    if (i < kernres - 1) return kern[i] + (kern[i + 1] - kern[i]) * (r - delta * i) / delta;
+    return kern[kernres - 1];
+}
+
+// Lock-free union-find over slots (hook the larger root under the smaller).
+__device__ __forceinline__ int32_t uf_load(int32_t *P, int32_t x) {
+    return __hip_atomic_load(P + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int32_t uf_find(int32_t *P, int32_t x) {
+    int32_t p = uf_load(P, x);
+    while (p != x) {
+        int32_t gp = uf_load(P, p);
+        if (gp != p) __hip_atomic_store(P + x, gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        x = p;
+        p = gp;
+    }
+    return x;
+}
+__device__ __forceinline__ void uf_union(int32_t *P, int32_t a, int32_t b) {
+    for (;;) {
+        a = uf_find(P, a);
+        b = uf_find(P, b);
+        if (a == b) return;
+        if (a < b) { int32_t t = a; a = b; b = t; }
+        int32_t expected = a;
+        if (__hip_atomic_compare_exchange_strong(P + a, &expected, b, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT))
+            return;
+    }
+}

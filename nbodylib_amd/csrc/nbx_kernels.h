@@ -1,0 +1,127 @@
+// nbx_kernels.h -- argument blocks and host-side launchers shared between the
+// HIP translation units and the C ABI (nbx_api.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "nbx_internal.h"
+
+struct MortonArgs {
+    int64_t n;
+    int nd;
+    const double *c[6];
+    double lo[6], scale[6];
+    uint64_t *keys;
+    int32_t *vals;
+};
+struct GatherArgs {
+    int64_t n;
+    int nd;
+    const int32_t *order;
+    const double *in[6];
+    double *out[6];
+    const double *mass_in;
+    double mass_default;
+    double *mass_out;
+    int32_t *perm, *inv;
+};
+struct LeafBoxArgs {
+    int64_t n;
+    int32_t nleaf;
+    int nd;
+    const double *c[6];
+    double *box;
+    double *org;  // nullable: leaf centre (mixed-form origin)
+};
+struct MixedArgs {
+    int64_t n;
+    int nd;
+    const double *c[6];
+    const double *org;
+    float *xf[6];
+};
+
+__global__ void k_bbox(const double *c, int64_t n, double *part);
+__global__ void k_morton(MortonArgs a);
+__global__ void k_gather(GatherArgs a);
+__global__ void k_leafbox(LeafBoxArgs a);
+__global__ void k_levelbox(double *box, int32_t off_child, int32_t cnt_child, int32_t off_par, int32_t cnt_par, int nd);
+__global__ void k_mixed_offsets(MixedArgs a);
+
+size_t nbx_sort_temp_bytes(int64_t n, int bits);
+hipError_t nbx_sort_pairs(void *temp, size_t temp_bytes, uint64_t *kin, uint64_t *kout, int32_t *vin, int32_t *vout,
+                          int64_t n, int bits, hipStream_t st);
+hipError_t nbx_sort_pairs_u64(void *temp, size_t &temp_bytes, uint64_t *kin, uint64_t *kout, int32_t *vin,
+                              int32_t *vout, int64_t n, hipStream_t st);
+
+// ------------------------------------------------------------------ kNN
+// Query source
+#define NBX_Q_SLOTS 0   // every particle, target-based (self / d2==0 excluded)
+#define NBX_Q_PERIODIC 1  // every particle, position-based + sequential periodic images
+#define NBX_Q_POINTS 2  // external points, position-based (+ images if periodic)
+// Density fused into the search
+#define NBX_DO_NONE 0
+#define NBX_DO_SYM 1     // CalcDensity: 1/2 W scattered to both ends
+#define NBX_DO_GATHER 2  // CalcDensityParticle: W m_j gathered
+#define NBX_DO_VEL 3     // CalcVelDensity
+// Output drop rule for the k+1 periodic search
+#define NBX_DROP_NONE 0
+#define NBX_DROP_SMALLEST 1
+#define NBX_DROP_LARGEST 2
+
+struct KnnArgs {
+    int32_t qsrc;       // NBX_Q_*
+    int32_t kh;         // heap size (k, or k+1 for periodic)
+    int32_t kout;       // entries written per row
+    int32_t drop;       // NBX_DROP_*
+    int32_t excl_zero;  // target rule: skip d2 == 0
+    int32_t layout;     // NBX_LAYOUT_BY_ID / BY_SLOT
+    int32_t dens;       // NBX_DO_*
+    int32_t nsmooth;    // velocity density
+    int32_t m;          // external points
+    int32_t periodic;
+    double period[3];
+    const double *q[3];  // external points (SoA)
+    int32_t *nn;        // nullable
+    double *d2;         // nullable
+    double *rho;        // by slot (density)
+    double *hout;       // by slot (nullable)
+    int32_t *err;       // set to 1 if a row has fewer than kh neighbours
+};
+
+hipError_t nbx_launch_knn(const DevTree &T, const KnnArgs &A, int nqueries, hipStream_t st);
+hipError_t nbx_launch_knn_mixed(const DevTree &T, const KnnArgs &A, double band, int32_t *fallback_list,
+                                int32_t *fallback_count, hipStream_t st);
+
+// ------------------------------------------------------------------ FOF
+struct FofArgs {
+    double b2;
+    int32_t periodic;
+    double period[3];
+    int32_t *parent;
+};
+hipError_t nbx_launch_fof(const DevTree &T, const FofArgs &A, hipStream_t st);
+hipError_t nbx_launch_fof_finish(const DevTree &T, int32_t *parent, int32_t *count, int32_t *minid, int64_t n,
+                                 hipStream_t st);
+
+// ------------------------------------------------------------------ small kernels
+__global__ void k_scatter_by_id(const double *src, const int32_t *perm, double *dst, int64_t n);
+__global__ void k_fill_i32(int32_t *p, int32_t v, int64_t n);
+__global__ void k_fill_f64(double *p, double v, int64_t n);
+__global__ void k_iota(int32_t *p, int64_t n);
+
+// ------------------------------------------------------------------ adjoint
+hipError_t nbx_launch_sph_fwd(const double *pos, const double *mass, const int32_t *nbr, int64_t n, int k, double *rho,
+                              double *h, hipStream_t st);
+hipError_t nbx_launch_sph_adj(const double *pos, const double *mass, const int32_t *nbr, int64_t n, int k,
+                              const double *g, double *dpos, double *dmass, hipStream_t st);
+
+// ------------------------------------------------------------------ profiling
+struct ProfScope {
+    hipStream_t st;
+    hipEvent_t e0, e1;
+    const char *name;
+    bool on;
+    ProfScope(const char *name, hipStream_t st);
+    ~ProfScope();
+};

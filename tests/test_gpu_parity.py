@@ -1,0 +1,158 @@
+"""GPU parity of the plain (FP64) form against the CPU oracle.
+
+Each case prints its name, sizes and seed before touching the GPU.
+Bar: neighbour ID sets and FOF labels bit-identical, kNN squared distances
+bit-identical, densities within 1e-12 relative.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from nbodylib_amd import datagen
+
+pytestmark = pytest.mark.gpu
+DENS_TOL = 1e-12
+
+
+def _nb():
+    import nbodylib_amd as nb
+
+    return nb
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+@pytest.mark.parametrize("n,k,seed", [(4096, 32, 1), (5000, 16, 2), (777, 32, 3)])
+def test_knn_nonperiodic_vs_brute(gpu, n, k, seed):
+    nb = _nb()
+    say(f"knn nonperiodic n={n} k={k} seed={seed}")
+    pos = datagen.uniform(n, 1.0, seed)
+    t = nb.KDTree(pos, kerntype=nb.KSPH)
+    nn, d2 = t.FindNearest(k)
+    bi, bd = oracle.knn_brute(pos, k, mode=0)
+    assert np.array_equal(nn, bi)
+    assert np.array_equal(d2, bd)
+
+
+@pytest.mark.parametrize("variant,mode", [(0, 1), (1, 2)])
+def test_knn_periodic_vs_brute(gpu, variant, mode):
+    nb = _nb()
+    n, k = 4096, 32
+    say(f"knn periodic variant={variant} n={n} k={k}")
+    pos = datagen.uniform(n, 1.0, 11)
+    t = nb.KDTree(pos, kerntype=nb.KSPH, period=1.0)
+    nn, d2 = t._knn(k, variant)
+    bi, bd = oracle.knn_brute(pos, k, mode=mode, period=1.0)
+    assert np.array_equal(np.sort(nn, 1), np.sort(bi, 1))
+    assert np.array_equal(d2, bd)
+
+
+def test_density_c0(gpu):
+    """C0: N=32^3 uniform periodic unit box, b=16, k=32, KSPH."""
+    nb = _nb()
+    n, k = 32768, 32
+    say(f"density C0 n={n} k={k}")
+    pos = datagen.uniform(n, 1.0, 20261202)
+    t = nb.KDTree(pos, bucket_size=16, kerntype=nb.KSPH, period=1.0)
+    rho, h = t.CalcDensity(k, return_h=True)
+    ot = oracle.Tree(pos, b=16, kerntype=oracle.KSPH, period=1.0)
+    ro = ot.calc_density(k)
+    assert np.max(np.abs(rho / ro - 1.0)) < DENS_TOL
+    ids, d2 = ot.knn(k, 3)
+    assert np.allclose(h, 0.5 * np.sqrt(d2[:, -1]), rtol=0, atol=0)
+    nn, _ = t.FindNearestNonPeriodic(k)
+    assert np.array_equal(np.sort(nn, 1), np.sort(ids, 1))
+
+
+def test_density_gather(gpu):
+    nb = _nb()
+    n, k = 6000, 32
+    say(f"density gather n={n} k={k}")
+    pos = datagen.uniform(n, 1.0, 5)
+    m = datagen.lognormal_masses(n, 5)
+    t = nb.KDTree(pos, mass=m, kerntype=nb.KSPH)
+    rho, h = t.CalcDensityParticleAll(k)
+    ot = oracle.Tree(pos, mass=m, b=16, kerntype=oracle.KSPH)
+    ro, ho = ot.calc_density_gather(k)
+    assert np.max(np.abs(rho / ro - 1.0)) < DENS_TOL
+    assert np.array_equal(h, ho)
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_fof_vs_brute(gpu, periodic):
+    nb = _nb()
+    n = 8000
+    pos = datagen.uniform(n, 1.0, 7)
+    b = 0.5 * datagen.mean_spacing(n, 1.0)
+    say(f"fof n={n} b={b} periodic={periodic}")
+    per = 1.0 if periodic else None
+    t = nb.KDTree(pos, kerntype=nb.KSPH, period=per)
+    lab, ng = t.FOF(b, 8)
+    ol, og = oracle.fof_brute(pos, b, 8, period=per)
+    assert ng == og
+    assert np.array_equal(lab, ol)
+
+
+def test_fof_clustered_vs_tree_oracle(gpu):
+    nb = _nb()
+    n = 65536
+    pos, _ = datagen.clustered(n, 100.0, nhalo=50, seed=3)
+    b = 0.2 * datagen.mean_spacing(n, 100.0)
+    say(f"fof clustered n={n} b={b}")
+    t = nb.KDTree(pos, kerntype=nb.KSPH, period=100.0)
+    lab, ng = t.FOF(b, 8)
+    ot = oracle.Tree(pos, b=16, kerntype=oracle.KSPH, period=100.0)
+    raw, og = ot.fof(b, 8)
+    can, _ = oracle.fof_canonical(raw, 8)
+    assert ng == og
+    assert np.array_equal(lab, can)
+
+
+def test_clustered_knn_density(gpu):
+    nb = _nb()
+    n, k = 65536, 32
+    pos, _ = datagen.clustered(n, 100.0, nhalo=50, seed=4)
+    say(f"clustered knn+density n={n} k={k}")
+    t = nb.KDTree(pos, kerntype=nb.KSPH, period=100.0)
+    rho = t.CalcDensity(k)
+    ot = oracle.Tree(pos, b=16, kerntype=oracle.KSPH, period=100.0)
+    ro = ot.calc_density(k)
+    assert np.max(np.abs(rho / ro - 1.0)) < DENS_TOL
+    nn, d2 = t.FindNearest(k)
+    oi, od = ot.knn(k, 0)
+    assert np.array_equal(np.sort(nn, 1), np.sort(oi, 1))
+    assert np.array_equal(d2, od)
+
+
+def test_ball_search(gpu):
+    nb = _nb()
+    n = 5000
+    pos = datagen.uniform(n, 1.0, 9)
+    say(f"ball n={n}")
+    t = nb.KDTree(pos, kerntype=nb.KSPH, period=1.0)
+    for x in ([0.5, 0.5, 0.5], [0.01, 0.99, 0.02]):
+        got = np.sort(t.SearchBallPosTagged(np.array(x), 0.01))
+        # brute force with the reference's image rule (images when r2 > s^2)
+        exp = []
+        xs = [np.array(x)]
+        x0 = np.array(x)
+        for ax in ([0], [1], [2], [0, 1], [0, 2], [1, 2], [0, 1, 2]):
+            xp = x0.copy()
+            s = 0.0
+            for a in ax:
+                if x0[a] < 0.5:
+                    xp[a] = x0[a] + 1.0
+                    s += x0[a] * x0[a]
+                else:
+                    xp[a] = x0[a] - 1.0
+                    s += xp[a] * xp[a]
+            s = np.sqrt(s) if len(ax) > 1 else (x0[ax[0]] if x0[ax[0]] < 0.5 else -xp[ax[0]])
+            if 0.01 > s * s:
+                xs.append(xp)
+        for q in xs:
+            d = q[None, :] - pos
+            d2 = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
+            exp.extend(np.nonzero(d2 < 0.01)[0].tolist())
+        assert np.array_equal(got, np.sort(np.array(exp, dtype=np.int32)))
