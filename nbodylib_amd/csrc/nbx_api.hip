@@ -238,14 +238,16 @@ static int build_tree(nbx_tree *t, const double *pos, const double *vel, const d
         ProfScope ps("k_build_sort", st);
         HIPCK(nbx_sort_pairs(tmp, tb, keys, keys2, vals, vals2, n, bits, st));
     }
-    for (int d = 0; d < ncoord; ++d) HIPCK(hipMalloc((void **)&t->xs[d], sizeof(double) * n));
-    HIPCK(hipMalloc((void **)&t->mass, sizeof(double) * n));
-    HIPCK(hipMalloc((void **)&t->perm, sizeof(int32_t) * n));
+    const int64_t npad = (n + NBX_LEAF - 1) / NBX_LEAF * NBX_LEAF;
+    for (int d = 0; d < ncoord; ++d) HIPCK(hipMalloc((void **)&t->xs[d], sizeof(double) * npad));
+    HIPCK(hipMalloc((void **)&t->mass, sizeof(double) * npad));
+    HIPCK(hipMalloc((void **)&t->perm, sizeof(int32_t) * npad));
     {
         ProfScope ps("k_build_gather", st);
         GatherArgs g;
         memset(&g, 0, sizeof(g));
         g.n = n;
+        g.npad = npad;
         g.nd = ncoord;
         g.order = vals2;
         for (int d = 0; d < ncoord; ++d) { g.in[d] = c[d]; g.out[d] = t->xs[d]; }
@@ -254,7 +256,7 @@ static int build_tree(nbx_tree *t, const double *pos, const double *vel, const d
         g.mass_out = t->mass;
         g.perm = t->perm;
         g.inv = nullptr;
-        hipLaunchKernelGGL(k_gather, dim3(nblk(n)), dim3(256), 0, st, g);
+        hipLaunchKernelGGL(k_gather, dim3(nblk(npad)), dim3(256), 0, st, g);
     }
     HIPCK(hipFreeAsync(keys, st));
     HIPCK(hipFreeAsync(keys2, st));
@@ -296,14 +298,15 @@ static int build_tree(nbx_tree *t, const double *pos, const double *vel, const d
                                lv.cnt[l - 1], lv.off[l], lv.cnt[l], nd);
     }
     if (t->p.form == NBX_FORM_MIXED) {
-        for (int d = 0; d < nd; ++d) HIPCK(hipMalloc((void **)&t->xf[d], sizeof(float) * n));
+        for (int d = 0; d < nd; ++d) HIPCK(hipMalloc((void **)&t->xf[d], sizeof(float) * npad));
         MixedArgs m;
         memset(&m, 0, sizeof(m));
         m.n = n;
+        m.npad = npad;
         m.nd = nd;
         for (int d = 0; d < nd; ++d) { m.c[d] = t->xs[d]; m.xf[d] = t->xf[d]; }
         m.org = t->org;
-        hipLaunchKernelGGL(k_mixed_offsets, dim3(nblk(n)), dim3(256), 0, st, m);
+        hipLaunchKernelGGL(k_mixed_offsets, dim3(nblk(npad)), dim3(256), 0, st, m);
     }
     HIPCK(hipGetLastError());
     // kernel table
@@ -518,6 +521,7 @@ static int density_impl(nbx_tree *t, int32_t k, int32_t variant, int32_t nsearch
     } else
         return fail(NBX_ERR_ARG, "unknown density variant");
     if (A.kh > heap_cap(t)) return fail(NBX_ERR_UNSUPPORTED, "k too large (k <= 64 supported)");
+    if (t->p.form == NBX_FORM_MIXED) return fail(NBX_ERR_UNSUPPORTED, "mixed-form density is not available yet");
     const int64_t n = t->n;
     double *rho = nullptr, *h = nullptr;
     int32_t *err = nullptr;

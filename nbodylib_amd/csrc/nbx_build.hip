@@ -78,7 +78,13 @@ __global__ __launch_bounds__(256) void k_morton(MortonArgs a) {
 // ---------------------------------------------------------------- gather
 __global__ __launch_bounds__(256) void k_gather(GatherArgs a) {
     int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (s >= a.n) return;
+    if (s >= a.npad) return;
+    if (s >= a.n) {  // padding up to a whole leaf
+        for (int d = 0; d < a.nd; ++d) a.out[d][s] = NBX_PAD_COORD;
+        a.mass_out[s] = 0.0;
+        a.perm[s] = -1;
+        return;
+    }
     int32_t id = a.order[s];
     for (int d = 0; d < a.nd; ++d) a.out[d][s] = a.in[d][id];
     a.mass_out[s] = a.mass_in ? a.mass_in[id] : a.mass_default;
@@ -128,9 +134,10 @@ __global__ __launch_bounds__(256) void k_levelbox(double *__restrict__ box, int3
 // Mixed form: FP32 offsets of each particle from its leaf's FP64 origin.
 __global__ __launch_bounds__(256) void k_mixed_offsets(MixedArgs a) {
     int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (s >= a.n) return;
+    if (s >= a.npad) return;
     int64_t L = s / NBX_LEAF;
-    for (int d = 0; d < a.nd; ++d) a.xf[d][s] = (float)(a.c[d][s] - a.org[L * a.nd + d]);
+    for (int d = 0; d < a.nd; ++d)
+        a.xf[d][s] = s < a.n ? (float)(a.c[d][s] - a.org[L * a.nd + d]) : NBX_PAD_COORD_F;
 }
 
 // ---------------------------------------------------------------- sort

@@ -14,6 +14,13 @@
 #define NBX_WAVE 64
 #define NBX_MAXLEV 40
 #define NBX_MAXVALUE 1e32  // MAXVALUE, Precision.h:48
+// Coordinate arrays are padded to whole leaves with this far sentinel: any
+// squared distance to it overflows to +inf, so padding never passes a test.
+#define NBX_PAD_COORD 1e300
+#define NBX_PAD_COORD_F 1e30f
+
+typedef double dx8 __attribute__((ext_vector_type(8)));
+typedef const __attribute__((address_space(4))) dx8 *cd8ptr;
 
 typedef const __attribute__((address_space(4))) double *cdptr;  // scalar-cache (read-only) view
 typedef const __attribute__((address_space(4))) float *cfptr;
@@ -57,10 +64,8 @@ __device__ __forceinline__ double box_lb(const double *q, cdptr b) {
     double s = 0.0;
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
-        double lo = b[d], hi = b[ND + d];
-        double t = 0.0;
-        if (q[d] < lo) t = lo - q[d];
-        else if (q[d] > hi) t = q[d] - hi;
+        // at most one of (lo - q), (q - hi) is positive; max() is exact
+        double t = fmax(fmax(b[d] - q[d], q[d] - b[ND + d]), 0.0);
         s += t * t;
     }
     return s;

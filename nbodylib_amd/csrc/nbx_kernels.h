@@ -15,7 +15,7 @@ struct MortonArgs {
     int32_t *vals;
 };
 struct GatherArgs {
-    int64_t n;
+    int64_t n, npad;
     int nd;
     const int32_t *order;
     const double *in[6];
@@ -34,7 +34,7 @@ struct LeafBoxArgs {
     double *org;  // nullable: leaf centre (mixed-form origin)
 };
 struct MixedArgs {
-    int64_t n;
+    int64_t n, npad;
     int nd;
     const double *c[6];
     const double *org;

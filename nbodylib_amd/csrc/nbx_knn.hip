@@ -9,14 +9,16 @@
 // Mapping to CDNA4: one wavefront owns 64 Morton-consecutive queries (two
 // sibling leaves).  The wave walks the implicit tree once for all 64 lanes
 // (a node is opened when any lane's bound reaches it), so node boxes and
-// leaf particles arrive through the scalar cache and are shared by the
-// whole wave.  Each lane keeps its own max-heap of (d2, slot) in LDS -- the
-// reference's PriorityQueue (PriorityQueue.h:47-84) -- laid out
-// [position][lane] so same-position accesses are conflict-free.  A leaf is
-// consumed in two phases: (1) all lanes test every candidate against their
-// current k-th distance, building a 32-bit accept mask; (2) lanes insert
-// their accepted candidates one at a time, so the wave pays for as many heap
-// insertions as its busiest lane, not as the sum over lanes.
+// leaf particles arrive through the scalar cache as 64-byte s_load bursts
+// and are shared by the whole wave.  Each lane keeps its own 4-ary max-heap
+// of (d2, slot) in LDS -- the reference's PriorityQueue
+// (PriorityQueue.h:47-84), with three dependent levels instead of five for
+// k = 32 -- laid out [position][lane] so same-position accesses are
+// conflict-free.  A leaf is consumed in two phases: (1) all lanes test every
+// candidate against their current k-th distance, building a 32-bit accept
+// mask; (2) lanes insert their accepted candidates one at a time from an LDS
+// copy of the leaf, so the wave pays for as many heap insertions as its
+// busiest lane, not the sum over lanes.
 //
 // Exactness: distances are summed in the reference's order with no FMA
 // (this file is built with -ffp-contract=off); pruning uses box_lb, which is
@@ -35,62 +37,88 @@ __device__ __forceinline__ bool key_gt(double da, int32_t sa, double db, int32_t
     return sid(perm, sa) > sid(perm, sb);
 }
 
-// Place (dv, sv) at heap position p of a max-heap of size kh and sift down.
+#define HD(p) hd[(p) * NBX_WAVE + lane]
+#define HS(p) hs[(p) * NBX_WAVE + lane]
+
+// 4-ary max-heap, 0-based: children of p are 4p+1 .. 4p+4.  Place (dv, sv)
+// at position p of a heap of size kh and sift it down.
 __device__ __forceinline__ void heap_sift(double *hd, int32_t *hs, int lane, int kh, int p, double dv, int32_t sv,
                                           const int32_t *perm) {
     for (;;) {
-        int l = 2 * p;
-        if (l > kh) break;
-        double dc = hd[(l - 1) * NBX_WAVE + lane];
-        int32_t sc = hs[(l - 1) * NBX_WAVE + lane];
-        int c = l;
-        if (l + 1 <= kh) {
-            double dr = hd[l * NBX_WAVE + lane];
-            int32_t sr = hs[l * NBX_WAVE + lane];
-            if (key_gt(dr, sr, dc, sc, perm)) { c = l + 1; dc = dr; sc = sr; }
+        int c = 4 * p + 1;
+        if (c >= kh) break;
+        double dc = HD(c);
+        int32_t sc = HS(c);
+        int best = c;
+#pragma unroll
+        for (int u = 1; u < 4; ++u) {
+            if (c + u < kh) {
+                double du = HD(c + u);
+                int32_t su = HS(c + u);
+                if (key_gt(du, su, dc, sc, perm)) { best = c + u; dc = du; sc = su; }
+            }
         }
         if (!key_gt(dc, sc, dv, sv, perm)) break;
-        hd[(p - 1) * NBX_WAVE + lane] = dc;
-        hs[(p - 1) * NBX_WAVE + lane] = sc;
-        p = c;
+        HD(p) = dc;
+        HS(p) = sc;
+        p = best;
     }
-    hd[(p - 1) * NBX_WAVE + lane] = dv;
-    hs[(p - 1) * NBX_WAVE + lane] = sv;
+    HD(p) = dv;
+    HS(p) = sv;
 }
 
-// Heap sort in place: positions 1..kh ascending afterwards.
+// Heap sort in place: positions 0..kh-1 ascending afterwards.
 __device__ __forceinline__ void heap_sort(double *hd, int32_t *hs, int lane, int kh, const int32_t *perm) {
-    for (int m = kh; m > 1; --m) {
-        double dt = hd[lane];
-        int32_t st = hs[lane];
-        double dm = hd[(m - 1) * NBX_WAVE + lane];
-        int32_t sm = hs[(m - 1) * NBX_WAVE + lane];
-        hd[(m - 1) * NBX_WAVE + lane] = dt;
-        hs[(m - 1) * NBX_WAVE + lane] = st;
-        heap_sift(hd, hs, lane, m - 1, 1, dm, sm, perm);
+    for (int m = kh - 1; m > 0; --m) {
+        double dt = HD(0);
+        int32_t st = HS(0);
+        double dm = HD(m);
+        int32_t sm = HS(m);
+        HD(m) = dt;
+        HS(m) = st;
+        heap_sift(hd, hs, lane, m, 0, dm, sm, perm);
     }
 }
+
+template <int ND>
+struct LeafCtx {
+    double q[ND];
+    bool act;
+    int32_t self;
+    bool exz;
+    double kth;
+    int32_t kslot;
+};
 
 // Consume one leaf for every active lane (two-phase insertion, see header).
 template <int ND>
-__device__ __forceinline__ void knn_leaf(const DevTree &T, int leaf, const double *q, bool act, int32_t self,
-                                         bool exz, double &kth, int32_t &kslot, double *hd, int32_t *hs, int lane,
-                                         int kh) {
+__device__ __forceinline__ void knn_leaf(const DevTree &T, int leaf, LeafCtx<ND> &c, double *hd, int32_t *hs,
+                                         double *cx, int lane, int kh) {
     const int base = leaf * NBX_LEAF;
-    int cnt = T.n - base;
-    cnt = cnt < NBX_LEAF ? cnt : NBX_LEAF;
-    uint32_t mask = 0;
-    for (int j = 0; j < cnt; ++j) {
-        double d2 = 0.0;
+    if (lane < NBX_LEAF) {
 #pragma unroll
-        for (int d = 0; d < ND; ++d) {
-            double c = ((cdptr)T.x[d])[base + j];
-            double t = q[d] - c;
-            d2 += t * t;
-        }
-        bool ok = act && (d2 <= kth) && (!exz || d2 > 0.0) && (base + j != self);
-        mask |= ok ? (1u << j) : 0u;
+        for (int d = 0; d < ND; ++d) cx[d * NBX_LEAF + lane] = T.x[d][base + lane];
     }
+    uint32_t mask = 0;
+#pragma unroll
+    for (int jj = 0; jj < NBX_LEAF; jj += 8) {
+        dx8 cv[ND];
+#pragma unroll
+        for (int d = 0; d < ND; ++d) cv[d] = *(cd8ptr)(T.x[d] + base + jj);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            double d2 = 0.0;
+#pragma unroll
+            for (int d = 0; d < ND; ++d) {
+                double t = c.q[d] - cv[d][u];
+                d2 += t * t;
+            }
+            bool ok = (d2 <= c.kth) && (!c.exz || d2 > 0.0) && (base + jj + u != c.self);
+            mask |= ok ? (1u << (jj + u)) : 0u;
+        }
+    }
+    if (!c.act) mask = 0;
+    __syncthreads();
     while (ballot(mask != 0)) {
         if (mask) {
             int j = __builtin_ctz(mask);
@@ -99,25 +127,46 @@ __device__ __forceinline__ void knn_leaf(const DevTree &T, int leaf, const doubl
             double d2 = 0.0;
 #pragma unroll
             for (int d = 0; d < ND; ++d) {
-                double t = q[d] - T.x[d][cs];
+                double t = c.q[d] - cx[d * NBX_LEAF + j];
                 d2 += t * t;
             }
-            if (key_gt(kth, kslot, d2, cs, T.perm)) {
-                heap_sift(hd, hs, lane, kh, 1, d2, cs, T.perm);
-                kth = hd[lane];
-                kslot = hs[lane];
+            if (key_gt(c.kth, c.kslot, d2, cs, T.perm)) {
+                heap_sift(hd, hs, lane, kh, 0, d2, cs, T.perm);
+                c.kth = HD(0);
+                c.kslot = HS(0);
             }
         }
     }
+    __syncthreads();
 }
 
-// Uniform stackless depth-first walk of the implicit tree, near child first
-// (ordered by the uniform point qc).  Nodes are opened when any active lane
-// has box_lb <= its current k-th distance.
+// Order of the two children of a node as seen from the uniform point qc:
+// nearer box first, ties (overlapping sibling boxes) by box-centre distance.
 template <int ND>
-__device__ __forceinline__ void knn_walk(const DevTree &T, const double *q, const double *qc, bool act, int32_t self,
-                                         bool exz, double &kth, int32_t &kslot, double *hd, int32_t *hs, int lane,
-                                         int kh) {
+__device__ __forceinline__ bool first_is_right(const DevTree &T, int level, int c0, const double *qc) {
+    cdptr b0 = (cdptr)node_box(T, level, c0, ND);
+    cdptr b1 = (cdptr)node_box(T, level, c0 + 1, ND);
+    double d0 = box_lb<ND>(qc, b0), d1 = box_lb<ND>(qc, b1);
+    if (d1 != d0) return d1 < d0;
+    double e0 = 0.0, e1 = 0.0;
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        double u = qc[d] - 0.5 * (b0[d] + b0[ND + d]), v = qc[d] - 0.5 * (b1[d] + b1[ND + d]);
+        e0 += u * u;
+        e1 += v * v;
+    }
+    return e1 < e0;
+}
+
+// Uniform stackless depth-first walk of the implicit tree.  The first
+// descent follows the ancestors of the wave's own leaf (anc_leaf; -1 when
+// the queries are not tree particles or are image-shifted), so every heap
+// is filled from the query's own neighbourhood before anything else is
+// opened; below that children are taken nearest first from qc.  A node is
+// opened when any active lane has box_lb <= its current k-th distance.
+template <int ND>
+__device__ __forceinline__ void knn_walk(const DevTree &T, LeafCtx<ND> &c, const double *qc, int anc_leaf, double *hd,
+                                         int32_t *hs, double *cx, int lane, int kh) {
     const int top = T.lv.nlev - 1;
     int level = top, idx = 0;
     uint64_t firstright = 0;
@@ -125,19 +174,16 @@ __device__ __forceinline__ void knn_walk(const DevTree &T, const double *q, cons
     for (;;) {
         if (down) {
             cdptr bx = (cdptr)node_box(T, level, idx, ND);
-            bool want = act && box_lb<ND>(q, bx) <= kth;
+            bool want = c.act && box_lb<ND>(c.q, bx) <= c.kth;
             if (ballot(want)) {
                 if (level == 0) {
-                    knn_leaf<ND>(T, idx, q, act, self, exz, kth, kslot, hd, hs, lane, kh);
+                    knn_leaf<ND>(T, idx, c, hd, hs, cx, lane, kh);
                     down = false;
                 } else {
                     int c0 = 2 * idx, c1 = c0 + 1;
                     int first = c0;
-                    if (c1 < T.lv.cnt[level - 1]) {
-                        double d0 = box_lb<ND>(qc, (cdptr)node_box(T, level - 1, c0, ND));
-                        double d1 = box_lb<ND>(qc, (cdptr)node_box(T, level - 1, c1, ND));
-                        if (d1 < d0) first = c1;
-                    }
+                    if (anc_leaf >= 0 && (anc_leaf >> level) == idx) first = anc_leaf >> (level - 1);
+                    else if (c1 < T.lv.cnt[level - 1] && first_is_right<ND>(T, level - 1, c0, qc)) first = c1;
                     if (first & 1) firstright |= (1ULL << (level - 1));
                     else firstright &= ~(1ULL << (level - 1));
                     level -= 1;
@@ -163,106 +209,105 @@ __device__ __forceinline__ void knn_walk(const DevTree &T, const double *q, cons
 
 // Image point and test for periodic pass p (1..7), DistFunc.h:326-355 and
 // KDSplitNode.cxx:1119-1148: 1-D images need sqrt(Top) > s, the 2-D and 3-D
-// ones compare Top (a squared distance) with s (a distance), as the reference does.
+// ones compare Top (a squared distance) with s (a distance), as the
+// reference does.  Phase-space searches take every image (:1153-1179).
 __device__ __forceinline__ bool image_pass(int pass, const double *x0, const double *per, double kth, bool uncond,
                                            double *xp) {
-    xp[0] = x0[0]; xp[1] = x0[1]; xp[2] = x0[2];
-    if (pass <= 3) {
-        int k = pass - 1;
-        double sval;
-        if (x0[k] < per[k] / 2.0) { xp[k] = x0[k] + per[k]; sval = x0[k]; }
-        else { xp[k] = x0[k] - per[k]; sval = -xp[k]; }
-        return uncond || sqrt(kth) > sval;
-    }
-    int k1, k2;
-    double sval = 0.0;
-    if (pass < 7) {
-        k1 = (pass == 6) ? 1 : 0;
-        k2 = (pass == 4) ? 1 : 2;
-        if (x0[k1] < per[k1] / 2.0) { xp[k1] = x0[k1] + per[k1]; sval += x0[k1] * x0[k1]; }
-        else { xp[k1] = x0[k1] - per[k1]; sval += xp[k1] * xp[k1]; }
-        if (x0[k2] < per[k2] / 2.0) { xp[k2] = x0[k2] + per[k2]; sval += x0[k2] * x0[k2]; }
-        else { xp[k2] = x0[k2] - per[k2]; sval += xp[k2] * xp[k2]; }
-    } else {
-        for (int k = 0; k < 3; ++k) {
-            if (x0[k] < per[k] / 2.0) { xp[k] = x0[k] + per[k]; sval += x0[k] * x0[k]; }
-            else { xp[k] = x0[k] - per[k]; sval += xp[k] * xp[k]; }
+    const bool u0 = pass == 1 || pass == 4 || pass == 5 || pass == 7;
+    const bool u1 = pass == 2 || pass == 4 || pass == 6 || pass == 7;
+    const bool u2 = pass == 3 || pass == 5 || pass == 6 || pass == 7;
+    const bool use[3] = {u0, u1, u2};
+    double sval = 0.0, s1 = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        xp[k] = x0[k];
+        if (use[k]) {
+            if (x0[k] < per[k] / 2.0) {
+                xp[k] = x0[k] + per[k];
+                sval += x0[k] * x0[k];
+                s1 = x0[k];
+            } else {
+                xp[k] = x0[k] - per[k];
+                sval += xp[k] * xp[k];
+                s1 = -xp[k];
+            }
         }
     }
-    sval = sqrt(sval);
-    return uncond || kth > sval;
+    if (pass <= 3) return uncond || sqrt(kth) > s1;
+    return uncond || kth > sqrt(sval);
 }
 
-template <int ND, int KMAX>
+template <int ND, int KMAX, int QSRC>
 __global__ __launch_bounds__(64) void k_knn(DevTree T, KnnArgs A) {
     __shared__ double hd[KMAX * NBX_WAVE];
     __shared__ int32_t hs[KMAX * NBX_WAVE];
+    __shared__ double cx[ND * NBX_LEAF];
     const int lane = threadIdx.x;
     const int g = blockIdx.x;
     const int kh = A.kh;
-    const int nq = (A.qsrc == NBX_Q_POINTS) ? A.m : T.n;
+    const int nq = (QSRC == NBX_Q_POINTS) ? A.m : T.n;
     const int qi = g * NBX_WAVE + lane;
     const bool valid = qi < nq;
     double q0[ND];
-    int32_t self = -1;
-    if (A.qsrc == NBX_Q_POINTS) {
+    if (QSRC == NBX_Q_POINTS) {
 #pragma unroll
         for (int d = 0; d < ND; ++d) q0[d] = valid ? A.q[d][qi] : 0.0;
     } else {
 #pragma unroll
         for (int d = 0; d < ND; ++d) q0[d] = valid ? T.x[d][qi] : 0.0;
-        if (A.qsrc == NBX_Q_SLOTS) self = qi;
     }
-    const bool exz = (A.qsrc == NBX_Q_SLOTS) && A.excl_zero;
     for (int p = 0; p < kh; ++p) {
-        hd[p * NBX_WAVE + lane] = NBX_MAXVALUE;
-        hs[p * NBX_WAVE + lane] = -1;
+        HD(p) = NBX_MAXVALUE;
+        HS(p) = -1;
     }
-    double kth = NBX_MAXVALUE;
-    int32_t kslot = -1;
-    const int npass = (A.periodic && A.qsrc != NBX_Q_SLOTS) ? 8 : 1;
+    LeafCtx<ND> c;
+    c.self = (QSRC == NBX_Q_SLOTS) ? qi : -1;
+    c.exz = (QSRC == NBX_Q_SLOTS) && A.excl_zero;
+    c.kth = NBX_MAXVALUE;
+    c.kslot = -1;
+    const int npass = (QSRC != NBX_Q_SLOTS && A.periodic) ? 8 : 1;
     for (int pass = 0; pass < npass; ++pass) {
-        double q[ND];
 #pragma unroll
-        for (int d = 0; d < ND; ++d) q[d] = q0[d];
-        bool act = valid;
+        for (int d = 0; d < ND; ++d) c.q[d] = q0[d];
+        c.act = valid;
         if (pass > 0) {
             double xp[3];
-            act = valid && image_pass(pass, q0, A.period, kth, ND == 6, xp);
-            q[0] = xp[0]; q[1] = xp[1]; q[2] = xp[2];
+            c.act = valid && image_pass(pass, q0, A.period, c.kth, ND == 6, xp);
+            c.q[0] = xp[0]; c.q[1] = xp[1]; c.q[2] = xp[2];
         }
-        if (!ballot(act)) continue;
+        if (!ballot(c.act)) continue;
         double qc[ND];
 #pragma unroll
-        for (int d = 0; d < ND; ++d) qc[d] = readfirst_d(q[d]);
-        knn_walk<ND>(T, q, qc, act, self, exz, kth, kslot, hd, hs, lane, kh);
+        for (int d = 0; d < ND; ++d) qc[d] = readfirst_d(c.q[d]);
+        const int anc = (pass == 0 && QSRC != NBX_Q_POINTS) ? 2 * g : -1;
+        knn_walk<ND>(T, c, qc, anc, hd, hs, cx, lane, kh);
     }
+    const double kth = c.kth;
 
     // ------------------------------------------------------------ densities
-    if (A.dens != NBX_DO_NONE && valid) {
+    if (QSRC == NBX_Q_SLOTS && A.dens != NBX_DO_NONE && valid) {
         bool bad = false;
         if (A.dens == NBX_DO_VEL) {
             // CalcVelDensity: |dv| of the Nsearch physical neighbours, the
             // Nsmooth smallest of them, h_v = max/2, sum W(|dv|/h_v) h_v^-3.
             const int ns = A.nsmooth;
-            for (int p = 1; p <= kh; ++p) {
-                int32_t s = hs[(p - 1) * NBX_WAVE + lane];
+            for (int p = 0; p < kh; ++p) {
+                int32_t s = HS(p);
                 if (s < 0) { bad = true; break; }
                 double vx = T.x[3][qi] - T.x[3][s], vy = T.x[4][qi] - T.x[4][s], vz = T.x[5][qi] - T.x[5][s];
-                hd[(p - 1) * NBX_WAVE + lane] = sqrt(vx * vx + vy * vy + vz * vz);
+                HD(p) = sqrt(vx * vx + vy * vy + vz * vz);
             }
             if (!bad) {
-                for (int p = ns / 2; p >= 1; --p)
-                    heap_sift(hd, hs, lane, ns, p, hd[(p - 1) * NBX_WAVE + lane], hs[(p - 1) * NBX_WAVE + lane], T.perm);
-                for (int p = ns + 1; p <= kh; ++p) {
-                    double dv = hd[(p - 1) * NBX_WAVE + lane];
-                    int32_t sv = hs[(p - 1) * NBX_WAVE + lane];
-                    if (key_gt(hd[lane], hs[lane], dv, sv, T.perm)) heap_sift(hd, hs, lane, ns, 1, dv, sv, T.perm);
+                for (int p = (ns - 2) / 4; p >= 0; --p) heap_sift(hd, hs, lane, ns, p, HD(p), HS(p), T.perm);
+                for (int p = ns; p < kh; ++p) {
+                    double dv = HD(p);
+                    int32_t sv = HS(p);
+                    if (key_gt(HD(0), HS(0), dv, sv, T.perm)) heap_sift(hd, hs, lane, ns, 0, dv, sv, T.perm);
                 }
-                double hv = 0.5 * hd[lane];
+                double hv = 0.5 * HD(0);
                 double norm = 1.0 / pow(hv, 3.0);
                 double den = 0.0;
-                for (int p = 1; p <= ns; ++p) den = den + wsm_at(hd[(p - 1) * NBX_WAVE + lane], hv, T.kern, T.kernres) * norm;
+                for (int p = 0; p < ns; ++p) den = den + wsm_at(HD(p), hv, T.kern, T.kernres) * norm;
                 A.rho[qi] = den;
                 if (A.hout) A.hout[qi] = hv;
             }
@@ -271,10 +316,10 @@ __global__ __launch_bounds__(64) void k_knn(DevTree T, KnnArgs A) {
             double norm = 1.0 / pow(hi, (double)ND);
             double mi = T.mass[qi];
             double acc = 0.0;
-            for (int p = 1; p <= kh; ++p) {
-                int32_t s = hs[(p - 1) * NBX_WAVE + lane];
+            for (int p = 0; p < kh; ++p) {
+                int32_t s = HS(p);
                 if (s < 0) { bad = true; continue; }
-                double rij = sqrt(hd[(p - 1) * NBX_WAVE + lane]);
+                double rij = sqrt(HD(p));
                 double W = wsm_at(rij, hi, T.kern, T.kernres);
                 if (A.dens == NBX_DO_SYM) {
                     double Wij = 0.5 * W * norm;
@@ -304,16 +349,26 @@ __global__ __launch_bounds__(64) void k_knn(DevTree T, KnnArgs A) {
         int qr = g * NBX_WAVE + r;
         if (qr >= nq) break;
         int64_t row;
-        if (A.qsrc == NBX_Q_POINTS || A.layout == 1) row = qr;
+        if (QSRC == NBX_Q_POINTS || A.layout == 1) row = qr;
         else row = T.perm[qr];
         int32_t s = hs[(t + off) * NBX_WAVE + r];
         if (A.nn) {
             int32_t v = s;
-            if (s >= 0 && (A.qsrc == NBX_Q_POINTS || A.layout == 0)) v = T.perm[s];
+            if (s >= 0 && (QSRC == NBX_Q_POINTS || A.layout == 0)) v = T.perm[s];
             A.nn[row * kout + t] = v;
         }
         if (A.d2) A.d2[row * kout + t] = hd[(t + off) * NBX_WAVE + r];
     }
+}
+
+template <int ND, int KMAX>
+static void launch_q(const DevTree &T, const KnnArgs &A, int blocks, hipStream_t st) {
+    if (A.qsrc == NBX_Q_SLOTS)
+        hipLaunchKernelGGL((k_knn<ND, KMAX, NBX_Q_SLOTS>), dim3(blocks), dim3(NBX_WAVE), 0, st, T, A);
+    else if (A.qsrc == NBX_Q_PERIODIC)
+        hipLaunchKernelGGL((k_knn<ND, KMAX, NBX_Q_PERIODIC>), dim3(blocks), dim3(NBX_WAVE), 0, st, T, A);
+    else
+        hipLaunchKernelGGL((k_knn<ND, KMAX, NBX_Q_POINTS>), dim3(blocks), dim3(NBX_WAVE), 0, st, T, A);
 }
 
 hipError_t nbx_launch_knn(const DevTree &T, const KnnArgs &A, int nqueries, hipStream_t st) {
@@ -321,11 +376,12 @@ hipError_t nbx_launch_knn(const DevTree &T, const KnnArgs &A, int nqueries, hipS
     if (blocks == 0) return hipSuccess;
     ProfScope ps("k_knn", st);
     if (T.nd == 6) {
-        if (A.kh <= 65) hipLaunchKernelGGL((k_knn<6, 65>), dim3(blocks), dim3(NBX_WAVE), 0, st, T, A);
+        if (A.kh <= 65) launch_q<6, 65>(T, A, blocks, st);
         else return hipErrorInvalidValue;
     } else {
-        if (A.kh <= 33) hipLaunchKernelGGL((k_knn<3, 33>), dim3(blocks), dim3(NBX_WAVE), 0, st, T, A);
-        else if (A.kh <= 65) hipLaunchKernelGGL((k_knn<3, 65>), dim3(blocks), dim3(NBX_WAVE), 0, st, T, A);
+        if (A.kh <= 32) launch_q<3, 32>(T, A, blocks, st);
+        else if (A.kh <= 33) launch_q<3, 33>(T, A, blocks, st);
+        else if (A.kh <= 65) launch_q<3, 65>(T, A, blocks, st);
         else return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -70,6 +70,11 @@ def lib():
         L.orc_fof_brute.restype = _i64
         L.orc_density_analytic.argtypes = [_dp, _dp, _i64, _ip, _ci, _dp, ctypes.c_void_p]
         L.orc_density_adjoint.argtypes = [_dp, _dp, _i64, _ip, _ci, _dp, _dp, _dp]
+        L.orc_knn_density_sample.argtypes = [_vp, _ci, np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"),
+                                             _i64, _dp, _ci]
+        L.orc_knn_density_sample.restype = _i64
+        L.orc_num_threads.argtypes = []
+        L.orc_num_threads.restype = _ci
         _lib = L
     return _lib
 
@@ -170,10 +175,21 @@ class Tree:
             raise RuntimeError("CalcVelDensity failed")
         return rho
 
+    def knn_density_sample(self, k, slots, nthreads=0):
+        """kNN + gather density for the given tree slots (CPU baseline sample)."""
+        slots = np.ascontiguousarray(slots, dtype=np.int64)
+        rho = np.zeros(slots.shape[0], dtype=np.float64)
+        lib().orc_knn_density_sample(self.h, k, slots, slots.shape[0], rho, nthreads)
+        return rho
+
     def fof(self, b, minnum=8, order=0):
         lab = np.zeros(self.n, dtype=np.int32)
         ng = lib().orc_fof_tree(self.h, b, minnum, order, lab)
         return lab, ng
+
+
+def num_threads():
+    return lib().orc_num_threads()
 
 
 def knn_brute(pos, k, mode=0, period=None, vel=None, nd=3, nthreads=0):

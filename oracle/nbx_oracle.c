@@ -1190,3 +1190,47 @@ void orc_density_adjoint(const double *pos, const double *mass, int64_t n, const
         }
     }
 }
+
+/* ============================================================ CPU baseline */
+/* kNN + gather density (CalcDensityParticle, KDCalcSmoothQuantities.cxx:768-844)
+ * for m sampled tree slots on all host cores: the bounded CPU sample bench.py
+ * times.  Returns the number of queries done. */
+int64_t orc_knn_density_sample(void *h, int k, const int64_t *slots, int64_t m, double *rho, int nthreads) {
+    otree *t = (otree *)h;
+    int nd = t->nd;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel
+    {
+        pq_t pq;
+        pq_init(&pq, k);
+#pragma omp for schedule(guided)
+        for (int64_t q = 0; q < m; q++) {
+            int64_t i = slots[q];
+            double off[6] = {0, 0, 0, 0, 0, 0}, den = 0.;
+            pq.n = 0;
+            for (int j = 0; j < k; j++) pq_push(&pq, -1, ORC_MAXVALUE);
+            nn_target(t, t->root, 0.0, &pq, off, i, t->treetype == ORC_TPHS, nd);
+            double hi = 0.5 * sqrt(pq_top(&pq));
+            double norm = 1.0 / pow(hi, (double)(nd * 1.));
+            for (int j = 0; j < k; j++) {
+                if (pq_topq(&pq) < 0) break;
+                double rij = sqrt(pq_top(&pq));
+                den += wsm_at(rij, hi, t->kernres, t->kern) * norm * t->p[pq_topq(&pq)].m;
+                pq_pop(&pq);
+            }
+            rho[q] = den;
+        }
+        pq_free(&pq);
+    }
+    return m;
+}
+
+int orc_num_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
