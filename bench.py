@@ -44,6 +44,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fof", action="store_true")
     ap.add_argument("--cpu-stride", type=int, default=2, help="time every s-th tree slot on the CPU")
+    ap.add_argument("--forms", default="plain,mixed", help="forms to time (plain, mixed)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -94,20 +95,26 @@ def main():
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         nb.prof_enable(False)
-        kms, klaunch = nb.prof_get("k_knn")
-        log(f"{label}: {wall / args.steps * 1e3:.2f} ms/step, k_knn {kms / max(klaunch, 1):.2f} ms/launch")
+        kname = "k_knn_mixed" if form == nb.FORM_MIXED else "k_knn_fp64"
+        kms, klaunch = nb.prof_get(kname)
+        fbms, _ = nb.prof_get("k_knn_fallback")
+        log(f"{label}: {wall / args.steps * 1e3:.2f} ms/step, {kname} {kms / max(klaunch, 1):.2f} ms/launch, "
+            f"fallback {fbms / args.steps:.2f} ms/step, fallbacks {tree.last_fallbacks()}")
         return tree, build_ms, wall, kms, klaunch
 
+    forms = args.forms.split(",")
     plain_tree, plain_build_ms, plain_wall, plain_kms, plain_kl = run_form(nb.FORM_PLAIN, "plain-fp64")
     rho_plain = rho.clone()
     headline = dict(form="plain-fp64", wall=plain_wall, kms=plain_kms, kl=plain_kl, build_ms=plain_build_ms)
 
     mixed = None
     try:
+        if "mixed" not in forms:
+            raise nb.NbxError(-3, "mixed form not requested")
         mixed_tree, mixed_build_ms, mixed_wall, mixed_kms, mixed_kl = run_form(nb.FORM_MIXED, "mixed")
         relerr = float(torch.max(torch.abs(rho / rho_plain - 1.0)))
         mixed = dict(form="mixed", wall=mixed_wall, kms=mixed_kms, kl=mixed_kl, build_ms=mixed_build_ms,
-                     rel=relerr)
+                     rel=relerr, fallbacks=mixed_tree.last_fallbacks())
         headline = mixed
         del mixed_tree
     except nb.NbxError as e:
@@ -139,7 +146,7 @@ def main():
     achieved = bytes_per_q * n / (kern_ms / 1e3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                 "frac": round(achieved / PEAK_HBM_GBS, 5), "traffic": None,
-                "kernel": "k_knn", "kernel_ms": round(kern_ms, 3), "bytes_per_query": bytes_per_q}
+                "kernel": "k_knn_mixed" if headline["form"] == "mixed" else "k_knn_fp64", "kernel_ms": round(kern_ms, 3), "bytes_per_query": bytes_per_q}
 
     cpu = None
     if not args.no_cpu:
@@ -182,6 +189,7 @@ def main():
         "mixed_mqps": round(value, 3) if mixed else None,
         "mixed_vs_plain": round(value / plain_rate, 3) if mixed else None,
         "mixed_density_max_rel_vs_fp64": mixed["rel"] if mixed else None,
+        "mixed_fp64_fallback_queries": mixed["fallbacks"] if mixed else None,
         "build_ms": round(headline["build_ms"], 2),
         "fof_mparticles_s": round(fof_rate, 3) if fof_rate else None,
         "fof_ms": round(fof_ms, 2) if fof_ms else None,

@@ -141,6 +141,10 @@ int nbx_sph_density_fwd_dev(const double *pos_dev, const double *mass_dev, const
 int nbx_sph_density_adjoint_dev(const double *pos_dev, const double *mass_dev, const int32_t *nbr_dev, int64_t n,
                                 int32_t k, const double *g_dev, double *dpos_dev, double *dmass_dev, void *stream);
 
+/* Precision-form statistics of the last kNN/density call on this tree:
+ * queries whose FP32 certificate failed and were recomputed in FP64. */
+int nbx_tree_last_fallbacks(const nbx_tree *t, int64_t *fallbacks);
+
 /* Per-kernel timing with HIP events recorded on the launch stream.  When on,
  * every library kernel launch is bracketed; nbx_prof_get returns the summed
  * milliseconds and launch count of kernels whose name contains `name`. */

@@ -32,7 +32,7 @@ EXPORTS = [
     "nbx_tree_size", "nbx_tree_kernel", "nbx_knn", "nbx_knn_dev", "nbx_knn_points", "nbx_density",
     "nbx_density_dev", "nbx_knn_density_dev", "nbx_ball_points", "nbx_fof", "nbx_fof_dev",
     "nbx_sph_density_fwd_dev", "nbx_sph_density_adjoint_dev", "nbx_prof_enable", "nbx_prof_reset", "nbx_prof_get",
-    "nbx_last_error",
+    "nbx_tree_last_fallbacks", "nbx_last_error",
 ]
 
 
@@ -90,6 +90,7 @@ def lib():
     L.nbx_prof_reset.argtypes = []
     L.nbx_prof_reset.restype = None
     L.nbx_prof_get.argtypes = [ctypes.c_char_p, P(dbl), P(i64)]
+    L.nbx_tree_last_fallbacks.argtypes = [vp, P(i64)]
     L.nbx_last_error.argtypes = []
     L.nbx_last_error.restype = ctypes.c_char_p
     _lib = L
@@ -176,6 +177,12 @@ class KDTree:
         tab = np.zeros(res, dtype=np.float64)
         lib().nbx_tree_kernel(self._h, None, _p(tab), res)
         return tab
+
+    def last_fallbacks(self):
+        """Precision form: queries of the last call re-run in FP64 (certificate failed)."""
+        v = ctypes.c_int64()
+        _check(lib().nbx_tree_last_fallbacks(self._h, ctypes.byref(v)))
+        return v.value
 
     # -------------------------------------------------------------- kNN
     def _knn(self, k, variant, layout=LAYOUT_BY_ID, with_d2=True):

@@ -110,11 +110,13 @@ struct nbx_tree {
     double *org = nullptr;
     float *xf[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     double *kern = nullptr;
+    float *kernf = nullptr;
+    double *ext = nullptr;
+    int64_t last_fallbacks = 0;
     int kernres;
     double kernnorm;
     std::vector<double> hkern;
-    LevelInfo lv;
-    int32_t nleaf;
+    int32_t nleaf, nleafp, depth;
     int64_t nnodes;
     DevTree view() const {
         DevTree T;
@@ -128,8 +130,10 @@ struct nbx_tree {
         T.box = box;
         T.org = org;
         T.kern = kern;
+        T.kernf = kernf;
         T.kernres = kernres;
-        T.lv = lv;
+        T.nleafp = nleafp;
+        T.depth = depth;
         return T;
     }
 };
@@ -181,6 +185,8 @@ static void free_tree(nbx_tree *t) {
     if (t->box) hipFree(t->box);
     if (t->org) hipFree(t->org);
     if (t->kern) hipFree(t->kern);
+    if (t->kernf) hipFree(t->kernf);
+    if (t->ext) hipFree(t->ext);
     delete t;
 }
 
@@ -264,38 +270,33 @@ static int build_tree(nbx_tree *t, const double *pos, const double *vel, const d
     HIPCK(hipFreeAsync(vals2, st));
     HIPCK(hipFreeAsync(tmp, st));
 
-    // implicit tree levels
+    // implicit heap-ordered tree over the leaves padded to a power of two
     t->nleaf = (int32_t)((n + NBX_LEAF - 1) / NBX_LEAF);
-    LevelInfo &lv = t->lv;
-    memset(&lv, 0, sizeof(lv));
-    int32_t cnt = t->nleaf, off = 0, L = 0;
-    for (;;) {
-        if (L >= NBX_MAXLEV) return fail(NBX_ERR_ARG, "tree too deep");
-        lv.off[L] = off;
-        lv.cnt[L] = cnt;
-        off += cnt;
-        L++;
-        if (cnt == 1) break;
-        cnt = (cnt + 1) / 2;
-    }
-    lv.nlev = L;
-    t->nnodes = off;
+    t->nleafp = 1;
+    t->depth = 0;
+    while (t->nleafp < t->nleaf) { t->nleafp <<= 1; t->depth++; }
+    t->nnodes = 2 * (int64_t)t->nleafp;
     HIPCK(hipMalloc((void **)&t->box, sizeof(double) * 2 * nd * (size_t)t->nnodes));
-    if (t->p.form == NBX_FORM_MIXED) HIPCK(hipMalloc((void **)&t->org, sizeof(double) * nd * (size_t)t->nleaf));
+    if (t->p.form == NBX_FORM_MIXED) {
+        HIPCK(hipMalloc((void **)&t->org, sizeof(double) * nd * (size_t)t->nleaf));
+        HIPCK(hipMalloc((void **)&t->ext, sizeof(double) * (size_t)t->nleaf));
+    }
     {
         ProfScope ps("k_build_boxes", st);
         LeafBoxArgs a;
         memset(&a, 0, sizeof(a));
         a.n = n;
         a.nleaf = t->nleaf;
+        a.nleafp = t->nleafp;
         a.nd = nd;
         for (int d = 0; d < nd; ++d) a.c[d] = t->xs[d];
         a.box = t->box;
         a.org = t->org;
-        hipLaunchKernelGGL(k_leafbox, dim3(nblk(t->nleaf)), dim3(256), 0, st, a);
-        for (int l = 1; l < lv.nlev; ++l)
-            hipLaunchKernelGGL(k_levelbox, dim3(nblk(lv.cnt[l])), dim3(256), 0, st, t->box, lv.off[l - 1],
-                               lv.cnt[l - 1], lv.off[l], lv.cnt[l], nd);
+        a.ext = t->ext;
+        hipLaunchKernelGGL(k_leafbox, dim3(nblk(t->nleafp)), dim3(256), 0, st, a);
+        for (int dep = t->depth - 1; dep >= 0; --dep)
+            hipLaunchKernelGGL(k_levelbox, dim3(nblk((int64_t)1 << dep)), dim3(256), 0, st, t->box, 1 << dep,
+                               1 << dep, nd);
     }
     if (t->p.form == NBX_FORM_MIXED) {
         for (int d = 0; d < nd; ++d) HIPCK(hipMalloc((void **)&t->xf[d], sizeof(float) * npad));
@@ -313,6 +314,12 @@ static int build_tree(nbx_tree *t, const double *pos, const double *vel, const d
     make_kernel(nd, t->p.kerntype, t->kernres, t->kernnorm, t->hkern);
     HIPCK(hipMalloc((void **)&t->kern, sizeof(double) * t->kernres));
     HIPCK(hipMemcpyAsync(t->kern, t->hkern.data(), sizeof(double) * t->kernres, hipMemcpyHostToDevice, st));
+    {
+        std::vector<float> kf(t->hkern.begin(), t->hkern.end());
+        HIPCK(hipMalloc((void **)&t->kernf, sizeof(float) * t->kernres));
+        HIPCK(hipMemcpyAsync(t->kernf, kf.data(), sizeof(float) * t->kernres, hipMemcpyHostToDevice, st));
+        HIPCK(hipStreamSynchronize(st));
+    }
     HIPCK(hipStreamSynchronize(st));
     return NBX_OK;
 }
@@ -393,6 +400,46 @@ extern "C" int nbx_tree_kernel(const nbx_tree *t, double *kernnorm, double *tabl
     return t->kernres;
 }
 
+// ------------------------------------------------------------------ precision form
+// FP32 search with the FP64 certificate; queries whose certificate fails are
+// re-run by the FP64 kernel from a list, so results equal the plain form's
+// neighbour sets.  A.dens / A.rho / A.nn / A.layout as for nbx_launch_knn.
+static hipError_t run_mixed(nbx_tree *t, KnnArgs A, int k, hipStream_t st) {
+    const int64_t n = t->n;
+    int32_t *fb = nullptr;
+    hipError_t e = hipMallocAsync((void **)&fb, sizeof(int32_t) * (n + 1), st);
+    if (e != hipSuccess) return e;
+    int32_t *cnt = fb + n;
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, st, cnt, 0, (int64_t)1);
+    KnnArgs M = A;
+    M.kh = k + 1;
+    M.kout = k;
+    DevTree T = t->view();
+    e = nbx_launch_knn_mixed(T, M, t->ext, fb, cnt, st);
+    int32_t hc = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&hc, cnt, sizeof(int32_t), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e == hipSuccess && hc > 0) {
+        KnnArgs F = A;
+        F.qsrc = NBX_Q_SLOTS;
+        F.kh = k;
+        F.kout = k;
+        F.excl_zero = 1;
+        F.qlist = fb;
+        F.m = hc;
+        e = nbx_launch_knn(T, F, hc, st);
+    }
+    t->last_fallbacks = hc;
+    hipError_t e2 = hipFreeAsync(fb, st);
+    return e != hipSuccess ? e : e2;
+}
+
+extern "C" int nbx_tree_last_fallbacks(const nbx_tree *t, int64_t *fallbacks) {
+    if (!t || !fallbacks) return fail(NBX_ERR_ARG, "NULL argument");
+    *fallbacks = t->last_fallbacks;
+    return NBX_OK;
+}
+
 // ------------------------------------------------------------------ kNN
 static int heap_cap(const nbx_tree *t) { return t->nd == 6 ? 65 : 65; }
 
@@ -442,6 +489,10 @@ extern "C" int nbx_knn_dev(nbx_tree *t, int32_t k, int32_t variant, int32_t layo
     A.nn = nn_dev;
     A.d2 = d2_dev;
     hipStream_t st = (hipStream_t)stream;
+    if (t->p.form == NBX_FORM_MIXED && t->nd == 3 && A.qsrc == NBX_Q_SLOTS && A.kh == k && d2_dev == nullptr) {
+        HIPCK(run_mixed(t, A, k, st));
+        return NBX_OK;
+    }
     HIPCK(nbx_launch_knn(t->view(), A, nq, st));
     return NBX_OK;
 }
@@ -521,7 +572,7 @@ static int density_impl(nbx_tree *t, int32_t k, int32_t variant, int32_t nsearch
     } else
         return fail(NBX_ERR_ARG, "unknown density variant");
     if (A.kh > heap_cap(t)) return fail(NBX_ERR_UNSUPPORTED, "k too large (k <= 64 supported)");
-    if (t->p.form == NBX_FORM_MIXED) return fail(NBX_ERR_UNSUPPORTED, "mixed-form density is not available yet");
+    const bool mixed = t->p.form == NBX_FORM_MIXED && t->nd == 3 && variant != NBX_DENS_VEL;
     const int64_t n = t->n;
     double *rho = nullptr, *h = nullptr;
     int32_t *err = nullptr;
@@ -534,7 +585,7 @@ static int density_impl(nbx_tree *t, int32_t k, int32_t variant, int32_t nsearch
     A.hout = h;
     A.err = err;
     A.nn = nn_slot;
-    hipError_t e = nbx_launch_knn(t->view(), A, (int)n, st);
+    hipError_t e = mixed ? run_mixed(t, A, k, st) : nbx_launch_knn(t->view(), A, (int)n, st);
     if (e == hipSuccess) {
         hipLaunchKernelGGL(k_scatter_by_id, dim3(nblk(n)), dim3(256), 0, st, rho, t->perm, rho_id, n);
         if (h_id) hipLaunchKernelGGL(k_scatter_by_id, dim3(nblk(n)), dim3(256), 0, st, h, t->perm, h_id, n);

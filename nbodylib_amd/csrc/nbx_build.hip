@@ -93,13 +93,18 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs a) {
 }
 
 // ---------------------------------------------------------------- boxes
-// One thread per leaf: tight min/max over its NBX_LEAF slots.
+// One thread per (padded) leaf: tight min/max over its NBX_LEAF slots;
+// padding leaves get the empty box (+inf, -inf).
 __global__ __launch_bounds__(256) void k_leafbox(LeafBoxArgs a) {
     int32_t L = blockIdx.x * 256 + threadIdx.x;
-    if (L >= a.nleaf) return;
+    if (L >= a.nleafp) return;
+    double *b = a.box + (size_t)(a.nleafp + L) * 2 * a.nd;
+    if (L >= a.nleaf) {
+        for (int d = 0; d < a.nd; ++d) { b[d] = 1.0 / 0.0; b[a.nd + d] = -1.0 / 0.0; }
+        return;
+    }
     int64_t s0 = (int64_t)L * NBX_LEAF, s1 = s0 + NBX_LEAF;
     if (s1 > a.n) s1 = a.n;
-    double *b = a.box + (size_t)L * 2 * a.nd;
     for (int d = 0; d < a.nd; ++d) {
         const double *c = a.c[d];
         double mn = c[s0], mx = mn;
@@ -112,22 +117,29 @@ __global__ __launch_bounds__(256) void k_leafbox(LeafBoxArgs a) {
         b[a.nd + d] = mx;
         if (a.org) a.org[(size_t)L * a.nd + d] = 0.5 * (mn + mx);
     }
+    if (a.ext) {
+        // half-diagonal from the centre, rounded up
+        double e2 = 0.0;
+        for (int d = 0; d < a.nd; ++d) {
+            double o = 0.5 * (b[d] + b[a.nd + d]);
+            double w = fmax(o - b[d], b[a.nd + d] - o);
+            e2 += w * w;
+        }
+        a.ext[L] = sqrt(e2) * (1.0 + 1e-12);
+    }
 }
 
-__global__ __launch_bounds__(256) void k_levelbox(double *__restrict__ box, int32_t off_child, int32_t cnt_child,
-                                                   int32_t off_par, int32_t cnt_par, int nd) {
+// Internal nodes [first, first+count) of one depth: union of the two children.
+__global__ __launch_bounds__(256) void k_levelbox(double *__restrict__ box, int32_t first, int32_t count, int nd) {
     int32_t i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= cnt_par) return;
-    const double *a = box + (size_t)(off_child + 2 * i) * 2 * nd;
-    double *o = box + (size_t)(off_par + i) * 2 * nd;
-    if (2 * i + 1 < cnt_child) {
-        const double *b = a + 2 * nd;
-        for (int d = 0; d < nd; ++d) {
-            o[d] = fmin(a[d], b[d]);
-            o[nd + d] = fmax(a[nd + d], b[nd + d]);
-        }
-    } else {
-        for (int d = 0; d < 2 * nd; ++d) o[d] = a[d];
+    if (i >= count) return;
+    int32_t nn = first + i;
+    const double *a = box + (size_t)(2 * nn) * 2 * nd;
+    const double *b = a + 2 * nd;
+    double *o = box + (size_t)nn * 2 * nd;
+    for (int d = 0; d < nd; ++d) {
+        o[d] = fmin(a[d], b[d]);
+        o[nd + d] = fmax(a[nd + d], b[nd + d]);
     }
 }
 

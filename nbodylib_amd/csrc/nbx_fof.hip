@@ -21,24 +21,31 @@
 __global__ __launch_bounds__(256) void k_fof_leafinit(DevTree T, double b2, int32_t *P, uint8_t *conn) {
     int32_t L = blockIdx.x * 256 + threadIdx.x;
     if (L >= T.nleaf) return;
-    cdptr bx = (cdptr)node_box(T, 0, L, 3);
+    const int node = T.nleafp + L;
+    cdptr bx = (cdptr)node_box(T, node, 3);
     int32_t s0 = L * NBX_LEAF;
     int32_t s1 = s0 + NBX_LEAF < T.n ? s0 + NBX_LEAF : T.n;
     bool c = boxbox_ub<3>(bx, bx) < b2;
-    conn[L] = c ? 1 : 0;
+    conn[node] = c ? 1 : 0;
     for (int32_t s = s0; s < s1; ++s) P[s] = c ? s0 : s;
 }
 
 // Internal node whose diagonal is < b: both children are connected (their
-// diagonals are smaller) and every cross pair links, so one union connects it.
-__global__ __launch_bounds__(256) void k_fof_nodeinit(DevTree T, double b2, int32_t *P, uint8_t *conn, int level) {
+// diagonals are smaller) and every cross pair links, so one union connects
+// it.  Nodes [first, first+count) of one depth.  An empty box (padding) has
+// an infinite diagonal and is never connected.
+__global__ __launch_bounds__(256) void k_fof_nodeinit(DevTree T, double b2, int32_t *P, uint8_t *conn, int32_t first,
+                                                       int32_t count) {
     int32_t i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= T.lv.cnt[level]) return;
-    cdptr bx = (cdptr)node_box(T, level, i, 3);
+    if (i >= count) return;
+    const int node = first + i;
+    cdptr bx = (cdptr)node_box(T, node, 3);
     bool c = boxbox_ub<3>(bx, bx) < b2;
-    conn[T.lv.off[level] + i] = c ? 1 : 0;
-    int32_t c1 = 2 * i + 1;
-    if (c && c1 < T.lv.cnt[level - 1]) uf_union(P, (i << level) * NBX_LEAF, (c1 << (level - 1)) * NBX_LEAF);
+    conn[node] = c ? 1 : 0;
+    if (!c) return;
+    const int right = 2 * node + 1;
+    cdptr br = (cdptr)node_box(T, right, 3);
+    if (br[0] <= br[3]) uf_union(P, node_first(T, node), node_first(T, right));
 }
 
 // link test from the (possibly image-shifted) position q to candidate slot cs
@@ -48,8 +55,8 @@ __device__ __forceinline__ void fof_leaf(const DevTree &T, int leaf, const doubl
     const int base = leaf * NBX_LEAF;
     int cnt = T.n - base;
     cnt = cnt < NBX_LEAF ? cnt : NBX_LEAF;
-    cdptr bx = (cdptr)node_box(T, 0, leaf, 3);
-    bool encl = act && conn[leaf] && box_ub<3>(q, bx) < b2;
+    cdptr bx = (cdptr)node_box(T, T.nleafp + leaf, 3);
+    bool encl = act && conn[T.nleafp + leaf] && box_ub<3>(q, bx) < b2;
     if (encl) uf_union(P, qi, base);
     bool pair = act && !encl;
     if (!ballot(pair)) return;
@@ -74,50 +81,46 @@ __device__ __forceinline__ void fof_leaf(const DevTree &T, int leaf, const doubl
 template <int PASS0>
 __device__ __forceinline__ void fof_walk(const DevTree &T, const double *q, bool act, int32_t qi, int g, double b2,
                                          int32_t *P, uint8_t *conn) {
-    const int top = T.lv.nlev - 1;
-    // the wave's own particles form node g of level 1 (or leaf 0 when there is one leaf)
-    const int glev = top >= 1 ? 1 : 0;
-    cdptr gb = (cdptr)node_box(T, glev, g, 3);
-    int level = top, idx = 0;
+    const int D = T.depth;
+    // the wave's own 64 particles form the parent of leaves 2g, 2g+1 (or the
+    // single leaf when the tree has one)
+    const int gnode = D >= 1 ? (T.nleafp + 2 * g) >> 1 : 1;
+    cdptr gb = (cdptr)node_box(T, gnode, 3);
+    int node = 1;
     bool down = true;
     for (;;) {
         if (down) {
-            cdptr bx = (cdptr)node_box(T, level, idx, 3);
+            cdptr bx = (cdptr)node_box(T, node, 3);
             bool want = act && box_lb<3>(q, bx) < b2;
             if (ballot(want)) {
                 if (PASS0 && boxbox_ub<3>(gb, bx) < b2) {
                     // every wave particle links to every particle of the node:
                     // wave and node form one component
-                    const int32_t nfirst = (idx << level) * NBX_LEAF;
+                    const int32_t nfirst = node_first(T, node);
                     if (act) uf_union(P, qi, nfirst);
-                    const int32_t gi = T.lv.off[level] + idx;
-                    if (!conn[gi]) {
-                        int32_t nlast = ((idx + 1) << level) * NBX_LEAF;
-                        nlast = nlast < T.n ? nlast : T.n;
+                    if (!conn[node]) {
+                        const int32_t nlast = node_end(T, node);
                         for (int32_t s = nfirst + 1 + threadIdx.x; s < nlast; s += NBX_WAVE) uf_union(P, s, nfirst);
-                        conn[gi] = 1;
+                        conn[node] = 1;
                     }
                     down = false;
-                } else if (level == 0) {
-                    fof_leaf<PASS0>(T, idx, q, act, qi, b2, P, conn);
+                } else if (node >= T.nleafp) {
+                    fof_leaf<PASS0>(T, node - T.nleafp, q, act, qi, b2, P, conn);
                     down = false;
                 } else {
-                    level -= 1;
-                    idx = 2 * idx;
+                    node = 2 * node;
                     continue;
                 }
             } else {
                 down = false;
             }
         }
-        if (level == top) break;
-        int sib = idx + 1;
-        if (!(idx & 1) && sib < T.lv.cnt[level]) {
-            idx = sib;
+        if (node == 1) break;
+        if (!(node & 1)) {
+            node ^= 1;
             down = true;
         } else {
-            idx >>= 1;
-            level += 1;
+            node >>= 1;
         }
     }
 }
@@ -200,8 +203,9 @@ hipError_t nbx_launch_fof(const DevTree &T, const FofArgs &A, hipStream_t st) {
     {
         ProfScope ps("k_fof_init", st);
         hipLaunchKernelGGL(k_fof_leafinit, dim3(nblk(T.nleaf)), dim3(256), 0, st, T, A.b2, A.parent, conn);
-        for (int l = 1; l < T.lv.nlev; ++l)
-            hipLaunchKernelGGL(k_fof_nodeinit, dim3(nblk(T.lv.cnt[l])), dim3(256), 0, st, T, A.b2, A.parent, conn, l);
+        for (int dep = T.depth - 1; dep >= 0; --dep)
+            hipLaunchKernelGGL(k_fof_nodeinit, dim3(nblk((int64_t)1 << dep)), dim3(256), 0, st, T, A.b2, A.parent,
+                               conn, 1 << dep, 1 << dep);
     }
     int groups = (T.n + NBX_WAVE - 1) / NBX_WAVE;
     ProfScope ps("k_fof_link", st);
@@ -293,15 +297,14 @@ __global__ __launch_bounds__(256) void k_iota(int32_t *p, int64_t n) {
 __global__ __launch_bounds__(64) void k_ball(DevTree T, double qx, double qy, double qz, double r2, int32_t *out,
                                              int64_t cap, unsigned long long *count) {
     const double q[3] = {qx, qy, qz};
-    const int top = T.lv.nlev - 1;
-    int level = top, idx = 0;
+    int node = 1;
     bool down = true;
     for (;;) {
         if (down) {
-            cdptr bx = (cdptr)node_box(T, level, idx, 3);
+            cdptr bx = (cdptr)node_box(T, node, 3);
             if (box_lb<3>(q, bx) < r2) {
-                if (level == 0) {
-                    int32_t cs = idx * NBX_LEAF + (int)threadIdx.x;
+                if (node >= T.nleafp) {
+                    int32_t cs = (node - T.nleafp) * NBX_LEAF + (int)threadIdx.x;
                     if (threadIdx.x < NBX_LEAF && cs < T.n) {
                         double d2 = 0.0;
                         for (int d = 0; d < 3; ++d) {
@@ -315,22 +318,19 @@ __global__ __launch_bounds__(64) void k_ball(DevTree T, double qx, double qy, do
                     }
                     down = false;
                 } else {
-                    level -= 1;
-                    idx = 2 * idx;
+                    node = 2 * node;
                     continue;
                 }
             } else {
                 down = false;
             }
         }
-        if (level == top) break;
-        int sib = idx + 1;
-        if (!(idx & 1) && sib < T.lv.cnt[level]) {
-            idx = sib;
+        if (node == 1) break;
+        if (!(node & 1)) {
+            node ^= 1;
             down = true;
         } else {
-            idx >>= 1;
-            level += 1;
+            node >>= 1;
         }
     }
 }

@@ -2,10 +2,12 @@
 //
 // Particles live in HBM as SoA arrays sorted by Morton key ("slots").  Leaves
 // are fixed buckets of NBX_LEAF consecutive slots; internal nodes form an
-// implicit binary tree over the leaves (level 0 = leaves, node i of level l
-// has children 2i and 2i+1 of level l-1).  Every node stores its tight FP64
-// bounding box, so all pruning bounds are computed with the same roundings as
-// the particle distances (see box_lb / box_ub).
+// implicit complete binary tree in heap order over the leaf count padded to a
+// power of two (nleafp): the root is node 1, node n has children 2n and 2n+1,
+// and leaf L is node nleafp + L.  Padding leaves carry an empty box
+// (lo = +inf, hi = -inf), whose lower bound is +inf, so they are never opened.
+// Every node stores its tight FP64 bounding box, so all pruning bounds are
+// computed with the same roundings as the particle distances (box_lb/box_ub).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -26,12 +28,6 @@ typedef const __attribute__((address_space(4))) double *cdptr;  // scalar-cache 
 typedef const __attribute__((address_space(4))) float *cfptr;
 typedef const __attribute__((address_space(4))) int32_t *ciptr;
 
-struct LevelInfo {
-    int32_t nlev;
-    int32_t off[NBX_MAXLEV];
-    int32_t cnt[NBX_MAXLEV];
-};
-
 // Kernel-side view of a built tree (passed by value).
 struct DevTree {
     int32_t n;      // particles
@@ -45,13 +41,26 @@ struct DevTree {
     const double *org;     // mixed form: per-leaf FP64 origin [leaf][nd]
     const float *xf[6];    // mixed form: FP32 offsets from the leaf origin
     const double *kern;    // kernel table
+    const float *kernf;    // kernel table in FP32 (mixed form)
     int32_t kernres;
+    int32_t nleafp;        // leaves padded to a power of two
+    int32_t depth;         // log2(nleafp): depth of the leaf level (root depth 0)
     int32_t pad2;
-    LevelInfo lv;
 };
 
-__device__ __forceinline__ const double *node_box(const DevTree &T, int level, int idx, int nd) {
-    return T.box + (size_t)(T.lv.off[level] + idx) * 2 * nd;
+// box of heap node n (1 <= n < 2 nleafp)
+__device__ __forceinline__ const double *node_box(const DevTree &T, int n, int nd) {
+    return T.box + (size_t)n * 2 * nd;
+}
+__device__ __forceinline__ int node_depth(int n) { return 31 - __clz(n); }
+// first slot of heap node n
+__device__ __forceinline__ int32_t node_first(const DevTree &T, int n) {
+    return ((n << (T.depth - node_depth(n))) - T.nleafp) * NBX_LEAF;
+}
+// one past the last slot of heap node n (clamped to the particle count)
+__device__ __forceinline__ int32_t node_end(const DevTree &T, int n) {
+    int32_t e = (((n + 1) << (T.depth - node_depth(n))) - T.nleafp) * NBX_LEAF;
+    return e < T.n ? e : T.n;
 }
 
 // Lower bound on the squared distance from q to any point of box b, summed

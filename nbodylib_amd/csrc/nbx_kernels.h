@@ -27,11 +27,12 @@ struct GatherArgs {
 };
 struct LeafBoxArgs {
     int64_t n;
-    int32_t nleaf;
+    int32_t nleaf, nleafp;
     int nd;
     const double *c[6];
     double *box;
     double *org;  // nullable: leaf centre (mixed-form origin)
+    double *ext;  // nullable: leaf half-diagonal (mixed-form error bound)
 };
 struct MixedArgs {
     int64_t n, npad;
@@ -45,7 +46,7 @@ __global__ void k_bbox(const double *c, int64_t n, double *part);
 __global__ void k_morton(MortonArgs a);
 __global__ void k_gather(GatherArgs a);
 __global__ void k_leafbox(LeafBoxArgs a);
-__global__ void k_levelbox(double *box, int32_t off_child, int32_t cnt_child, int32_t off_par, int32_t cnt_par, int nd);
+__global__ void k_levelbox(double *box, int32_t first, int32_t count, int nd);
 __global__ void k_mixed_offsets(MixedArgs a);
 
 size_t nbx_sort_temp_bytes(int64_t n, int bits);
@@ -87,10 +88,11 @@ struct KnnArgs {
     double *rho;        // by slot (density)
     double *hout;       // by slot (nullable)
     int32_t *err;       // set to 1 if a row has fewer than kh neighbours
+    const int32_t *qlist;  // nullable: query slots (FP64 re-run of mixed-form fallbacks)
 };
 
 hipError_t nbx_launch_knn(const DevTree &T, const KnnArgs &A, int nqueries, hipStream_t st);
-hipError_t nbx_launch_knn_mixed(const DevTree &T, const KnnArgs &A, double band, int32_t *fallback_list,
+hipError_t nbx_launch_knn_mixed(const DevTree &T, const KnnArgs &A, const double *ext, int32_t *fallback_list,
                                 int32_t *fallback_count, hipStream_t st);
 
 // ------------------------------------------------------------------ FOF

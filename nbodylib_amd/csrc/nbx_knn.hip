@@ -84,7 +84,6 @@ template <int ND>
 struct LeafCtx {
     double q[ND];
     bool act;
-    int32_t self;
     bool exz;
     double kth;
     int32_t kslot;
@@ -113,7 +112,10 @@ __device__ __forceinline__ void knn_leaf(const DevTree &T, int leaf, LeafCtx<ND>
                 double t = c.q[d] - cv[d][u];
                 d2 += t * t;
             }
-            bool ok = (d2 <= c.kth) && (!c.exz || d2 > 0.0) && (base + jj + u != c.self);
+            // the query itself has d2 == 0: excluded by the d2 > 0 rule of
+            // the target search and kept by the position search, as in the
+            // reference (KDLeafNode.cxx:19-21, :123-124)
+            bool ok = (d2 <= c.kth) && (!c.exz || d2 > 0.0);
             mask |= ok ? (1u << (jj + u)) : 0u;
         }
     }
@@ -143,9 +145,9 @@ __device__ __forceinline__ void knn_leaf(const DevTree &T, int leaf, LeafCtx<ND>
 // Order of the two children of a node as seen from the uniform point qc:
 // nearer box first, ties (overlapping sibling boxes) by box-centre distance.
 template <int ND>
-__device__ __forceinline__ bool first_is_right(const DevTree &T, int level, int c0, const double *qc) {
-    cdptr b0 = (cdptr)node_box(T, level, c0, ND);
-    cdptr b1 = (cdptr)node_box(T, level, c0 + 1, ND);
+__device__ __forceinline__ bool first_is_right(const DevTree &T, int c0, const double *qc) {
+    cdptr b0 = (cdptr)node_box(T, c0, ND);
+    cdptr b1 = (cdptr)node_box(T, c0 + 1, ND);
     double d0 = box_lb<ND>(qc, b0), d1 = box_lb<ND>(qc, b1);
     if (d1 != d0) return d1 < d0;
     double e0 = 0.0, e1 = 0.0;
@@ -158,51 +160,51 @@ __device__ __forceinline__ bool first_is_right(const DevTree &T, int level, int 
     return e1 < e0;
 }
 
-// Uniform stackless depth-first walk of the implicit tree.  The first
-// descent follows the ancestors of the wave's own leaf (anc_leaf; -1 when
+// Uniform stackless depth-first walk of the heap-ordered tree.  The first
+// descent follows the ancestors of the wave's own leaf node (anc; -1 when
 // the queries are not tree particles or are image-shifted), so every heap
 // is filled from the query's own neighbourhood before anything else is
 // opened; below that children are taken nearest first from qc.  A node is
 // opened when any active lane has box_lb <= its current k-th distance.
+// State is uniform (SGPRs): node, depth and one "right child first" bit per
+// depth.
 template <int ND>
-__device__ __forceinline__ void knn_walk(const DevTree &T, LeafCtx<ND> &c, const double *qc, int anc_leaf, double *hd,
+__device__ __forceinline__ void knn_walk(const DevTree &T, LeafCtx<ND> &c, const double *qc, int anc, double *hd,
                                          int32_t *hs, double *cx, int lane, int kh) {
-    const int top = T.lv.nlev - 1;
-    int level = top, idx = 0;
+    const int D = T.depth;
+    int node = 1, dep = 0;
     uint64_t firstright = 0;
     bool down = true;
     for (;;) {
         if (down) {
-            cdptr bx = (cdptr)node_box(T, level, idx, ND);
+            cdptr bx = (cdptr)node_box(T, node, ND);
             bool want = c.act && box_lb<ND>(c.q, bx) <= c.kth;
             if (ballot(want)) {
-                if (level == 0) {
-                    knn_leaf<ND>(T, idx, c, hd, hs, cx, lane, kh);
+                if (dep == D) {
+                    knn_leaf<ND>(T, node - T.nleafp, c, hd, hs, cx, lane, kh);
                     down = false;
                 } else {
-                    int c0 = 2 * idx, c1 = c0 + 1;
+                    int c0 = 2 * node;
                     int first = c0;
-                    if (anc_leaf >= 0 && (anc_leaf >> level) == idx) first = anc_leaf >> (level - 1);
-                    else if (c1 < T.lv.cnt[level - 1] && first_is_right<ND>(T, level - 1, c0, qc)) first = c1;
-                    if (first & 1) firstright |= (1ULL << (level - 1));
-                    else firstright &= ~(1ULL << (level - 1));
-                    level -= 1;
-                    idx = first;
+                    if (anc >= 0 && (anc >> (D - dep)) == node) first = anc >> (D - dep - 1);
+                    else if (first_is_right<ND>(T, c0, qc)) first = c0 + 1;
+                    dep += 1;
+                    if (first & 1) firstright |= (1ULL << dep);
+                    else firstright &= ~(1ULL << dep);
+                    node = first;
                     continue;
                 }
             } else {
                 down = false;
             }
         }
-        if (level == top) break;
-        bool wasfirst = (uint64_t)(idx & 1) == ((firstright >> level) & 1ULL);
-        int sib = idx ^ 1;
-        if (wasfirst && sib < T.lv.cnt[level]) {
-            idx = sib;
+        if (node == 1) break;
+        if ((uint64_t)(node & 1) == ((firstright >> dep) & 1ULL)) {
+            node ^= 1;  // the sibling always exists in the padded tree
             down = true;
         } else {
-            idx >>= 1;
-            level += 1;
+            node >>= 1;
+            dep -= 1;
         }
     }
 }
@@ -245,9 +247,11 @@ __global__ __launch_bounds__(64) void k_knn(DevTree T, KnnArgs A) {
     const int lane = threadIdx.x;
     const int g = blockIdx.x;
     const int kh = A.kh;
-    const int nq = (QSRC == NBX_Q_POINTS) ? A.m : T.n;
-    const int qi = g * NBX_WAVE + lane;
-    const bool valid = qi < nq;
+    const bool listed = (QSRC == NBX_Q_SLOTS) && A.qlist != nullptr;
+    const int nq = (QSRC == NBX_Q_POINTS || listed) ? A.m : T.n;
+    const int gi = g * NBX_WAVE + lane;
+    const bool valid = gi < nq;
+    const int qi = listed ? (valid ? A.qlist[gi] : 0) : gi;
     double q0[ND];
     if (QSRC == NBX_Q_POINTS) {
 #pragma unroll
@@ -261,7 +265,6 @@ __global__ __launch_bounds__(64) void k_knn(DevTree T, KnnArgs A) {
         HS(p) = -1;
     }
     LeafCtx<ND> c;
-    c.self = (QSRC == NBX_Q_SLOTS) ? qi : -1;
     c.exz = (QSRC == NBX_Q_SLOTS) && A.excl_zero;
     c.kth = NBX_MAXVALUE;
     c.kslot = -1;
@@ -279,7 +282,7 @@ __global__ __launch_bounds__(64) void k_knn(DevTree T, KnnArgs A) {
         double qc[ND];
 #pragma unroll
         for (int d = 0; d < ND; ++d) qc[d] = readfirst_d(c.q[d]);
-        const int anc = (pass == 0 && QSRC != NBX_Q_POINTS) ? 2 * g : -1;
+        const int anc = (pass == 0 && QSRC != NBX_Q_POINTS && !listed) ? T.nleafp + 2 * g : -1;
         knn_walk<ND>(T, c, qc, anc, hd, hs, cx, lane, kh);
     }
     const double kth = c.kth;
@@ -348,6 +351,7 @@ __global__ __launch_bounds__(64) void k_knn(DevTree T, KnnArgs A) {
         int r = e / kout, t = e - r * kout;
         int qr = g * NBX_WAVE + r;
         if (qr >= nq) break;
+        if (listed) qr = A.qlist[qr];
         int64_t row;
         if (QSRC == NBX_Q_POINTS || A.layout == 1) row = qr;
         else row = T.perm[qr];
@@ -374,7 +378,7 @@ static void launch_q(const DevTree &T, const KnnArgs &A, int blocks, hipStream_t
 hipError_t nbx_launch_knn(const DevTree &T, const KnnArgs &A, int nqueries, hipStream_t st) {
     int blocks = (nqueries + NBX_WAVE - 1) / NBX_WAVE;
     if (blocks == 0) return hipSuccess;
-    ProfScope ps("k_knn", st);
+    ProfScope ps(A.qlist ? "k_knn_fallback" : "k_knn_fp64", st);
     if (T.nd == 6) {
         if (A.kh <= 65) launch_q<6, 65>(T, A, blocks, st);
         else return hipErrorInvalidValue;

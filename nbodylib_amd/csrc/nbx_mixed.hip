@@ -1,9 +1,440 @@
-// nbx_mixed.hip -- precision form (FP32 node-relative search with an FP64
-// re-check band).  Launcher stub until the kernel lands.
+// nbx_mixed.hip -- precision form of kNN + SPH density (FP32 search, FP64
+// re-check band, exact neighbour sets).
+//
+// Same call as CalcDensity (KDCalcSmoothQuantities.cxx:203-305) and the
+// target-based FindNearest (KDFindNearest.cxx:247-318) on a non-periodic
+// search, with the same wave/leaf structure as the FP64 kernel (nbx_knn.hip).
+//
+// Search: particle coordinates are stored as FP32 offsets from their leaf's
+// FP64 centre; a query is re-expressed per leaf as fl32(q - centre) so every
+// difference is formed between two small FP32 numbers.  Each lane keeps the
+// k+1 smallest packed keys (bits(d32) << 32 | slot) in a 4-ary LDS heap --
+// 8-byte entries compared with one 64-bit integer compare.
+//
+// Error bound: with u = 2^-24 and E the half-diagonal of the leaf a
+// candidate came from, |dist32 - dist64| <= 12 u dist32 + 5 u E (rounding
+// of the two offsets, of the difference, of the FMA sum and of sqrt, with a
+// factor 2 of margin).  Each lane tracks E_max over the leaves it evaluated
+// as reachable or accepted a candidate from (candidates of other leaves are
+// outside by the FP64 node bound).
+//
+// Re-check band: after the walk the k+1 keys are sorted; the candidates
+// whose distance interval overlaps the k-th one's (plus any that could be a
+// d2 == 0 duplicate) are recomputed in FP64 and the k-th neighbour is chosen
+// among them by (d2, ID) as in the FP64 form.  Certificate: the largest
+// selected FP64 d2 must be strictly below the squared lower bound of every
+// particle outside the heap (bounded through the (k+1)-th key).  Lanes that
+// fail it (the bound is loose when a lane has evaluated a very large leaf)
+// are verified exactly: the k smallest (d2, ID) of the k+1 candidates are
+// taken in FP64 and an FP64 walk of that ball counts the particles below the
+// k-th; exactly k-1 proves the set.  Only lanes that fail this too are
+// appended to a fallback list and recomputed by the FP64 kernel, so
+// neighbour sets are always exact.
+// h = |x - x_k| / 2 comes from the FP64 k-th distance, and the kernel
+// weights of the k selected neighbours use FP64 distances (an FP32 offset
+// carries a storage error proportional to its leaf's size, which in
+// clustered data can reach 1e-5 of the neighbour distance); the search --
+// every candidate test and heap operation -- stays in FP32.
 #include "nbx_kernels.h"
 
-hipError_t nbx_launch_knn_mixed(const DevTree &T, const KnnArgs &A, double band, int32_t *fallback_list,
+typedef float fx8 __attribute__((ext_vector_type(8)));
+typedef const __attribute__((address_space(4))) fx8 *cf8ptr;
+
+#define MX_U 5.9604644775390625e-8  // 2^-24
+#define MX_EMPTY 0x7F7FFFFFFFFFFFFFULL  // key of an empty heap entry (FLT_MAX, slot ~0)
+
+__device__ __forceinline__ uint64_t mkkey(float d, int32_t s) {
+    return ((uint64_t)__float_as_uint(d) << 32) | (uint32_t)s;
+}
+__device__ __forceinline__ float keyd(uint64_t k) { return __uint_as_float((uint32_t)(k >> 32)); }
+__device__ __forceinline__ int32_t keys(uint64_t k) { return (int32_t)(uint32_t)k; }
+
+#define HK(p) hk[(p) * NBX_WAVE + lane]
+
+__device__ __forceinline__ void kheap_sift(uint64_t *hk, int lane, int kh, int p, uint64_t v) {
+    for (;;) {
+        int c = 4 * p + 1;
+        if (c >= kh) break;
+        uint64_t bc = HK(c);
+        int best = c;
+#pragma unroll
+        for (int u = 1; u < 4; ++u) {
+            if (c + u < kh) {
+                uint64_t x = HK(c + u);
+                if (x > bc) { bc = x; best = c + u; }
+            }
+        }
+        if (bc <= v) break;
+        HK(p) = bc;
+        p = best;
+    }
+    HK(p) = v;
+}
+
+struct MixCtx {
+    double q[3];
+    bool act;
+    int32_t self;
+    uint64_t root;  // current largest key kept
+    double emax;    // largest leaf half-diagonal that matters for this lane
+};
+
+__device__ __forceinline__ void mix_leaf(const DevTree &T, const double *ext, int leaf, MixCtx &c, bool want,
+                                         uint64_t *hk, float *cx, int lane, int kh) {
+    const int base = leaf * NBX_LEAF;
+    if (lane < NBX_LEAF) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) cx[d * NBX_LEAF + lane] = T.xf[d][base + lane];
+    }
+    // query relative to this leaf's centre, rounded once to FP32
+    const double *o = T.org + (size_t)leaf * 3;
+    float qx = (float)(c.q[0] - ((cdptr)o)[0]);
+    float qy = (float)(c.q[1] - ((cdptr)o)[1]);
+    float qz = (float)(c.q[2] - ((cdptr)o)[2]);
+    const float th = keyd(c.root);
+    uint32_t mask = 0;
+#pragma unroll
+    for (int jj = 0; jj < NBX_LEAF; jj += 8) {
+        fx8 px = *(cf8ptr)(T.xf[0] + base + jj);
+        fx8 py = *(cf8ptr)(T.xf[1] + base + jj);
+        fx8 pz = *(cf8ptr)(T.xf[2] + base + jj);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            float tx = qx - px[u], ty = qy - py[u], tz = qz - pz[u];
+            float d = __builtin_fmaf(tz, tz, __builtin_fmaf(ty, ty, tx * tx));
+            mask |= (d <= th) ? (1u << (jj + u)) : 0u;
+        }
+    }
+    if (c.self >= base && c.self < base + NBX_LEAF) mask &= ~(1u << (c.self - base));
+    if (!c.act) mask = 0;
+    if (c.act && (want || mask)) c.emax = fmax(c.emax, ((cdptr)ext)[leaf]);
+    __syncthreads();
+    while (ballot(mask != 0)) {
+        if (mask) {
+            int j = __builtin_ctz(mask);
+            mask &= mask - 1;
+            float tx = qx - cx[j], ty = qy - cx[NBX_LEAF + j], tz = qz - cx[2 * NBX_LEAF + j];
+            float d = __builtin_fmaf(tz, tz, __builtin_fmaf(ty, ty, tx * tx));
+            uint64_t key = mkkey(d, base + j);
+            if (key < c.root) {
+                kheap_sift(hk, lane, kh, 0, key);
+                c.root = HK(0);
+            }
+        }
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ void mix_walk(const DevTree &T, const double *ext, MixCtx &c, const double *qc, int anc,
+                                         uint64_t *hk, float *cx, int lane, int kh) {
+    const int D = T.depth;
+    int node = 1, dep = 0;
+    uint64_t firstright = 0;
+    bool down = true;
+    for (;;) {
+        if (down) {
+            cdptr bx = (cdptr)node_box(T, node, 3);
+            bool want = c.act && box_lb<3>(c.q, bx) <= (double)keyd(c.root);
+            if (ballot(want)) {
+                if (dep == D) {
+                    mix_leaf(T, ext, node - T.nleafp, c, want, hk, cx, lane, kh);
+                    down = false;
+                } else {
+                    int c0 = 2 * node;
+                    int first = c0;
+                    if (anc >= 0 && (anc >> (D - dep)) == node) {
+                        first = anc >> (D - dep - 1);
+                    } else {
+                        cdptr b0 = (cdptr)node_box(T, c0, 3);
+                        cdptr b1 = (cdptr)node_box(T, c0 + 1, 3);
+                        double d0 = box_lb<3>(qc, b0), d1 = box_lb<3>(qc, b1);
+                        if (d1 < d0) first = c0 + 1;
+                        else if (d1 == d0) {
+                            double e0 = 0.0, e1 = 0.0;
+                            for (int d = 0; d < 3; ++d) {
+                                double u = qc[d] - 0.5 * (b0[d] + b0[3 + d]), v = qc[d] - 0.5 * (b1[d] + b1[3 + d]);
+                                e0 += u * u;
+                                e1 += v * v;
+                            }
+                            if (e1 < e0) first = c0 + 1;
+                        }
+                    }
+                    dep += 1;
+                    if (first & 1) firstright |= (1ULL << dep);
+                    else firstright &= ~(1ULL << dep);
+                    node = first;
+                    continue;
+                }
+            } else {
+                down = false;
+            }
+        }
+        if (node == 1) break;
+        if ((uint64_t)(node & 1) == ((firstright >> dep) & 1ULL)) {
+            node ^= 1;
+            down = true;
+        } else {
+            node >>= 1;
+            dep -= 1;
+        }
+    }
+}
+
+__device__ __forceinline__ double d64(const DevTree &T, const double *q, int32_t s) {
+    double d2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        double t = q[d] - T.x[d][s];
+        d2 += t * t;
+    }
+    return d2;
+}
+
+// Exact FP64 check of a candidate set: counts the particles whose key
+// (d2, ID) is below (Dk, IDk) -- self and d2 == 0 excluded, as in the target
+// search -- within the ball of radius^2 Dk, for the lanes with act set.
+// Wave-uniform left-first walk; leaves through 64-byte scalar loads.
+__device__ __forceinline__ int verify_walk(const DevTree &T, const double *q, bool act, double Dk, int32_t IDk,
+                                           int32_t self) {
+    int cnt = 0;
+    int node = 1;
+    bool down = true;
+    for (;;) {
+        if (down) {
+            cdptr bx = (cdptr)node_box(T, node, 3);
+            bool want = act && box_lb<3>(q, bx) <= Dk;
+            if (ballot(want)) {
+                if (node >= T.nleafp) {
+                    const int base = (node - T.nleafp) * NBX_LEAF;
+#pragma unroll
+                    for (int jj = 0; jj < NBX_LEAF; jj += 8) {
+                        dx8 cv0 = *(cd8ptr)(T.x[0] + base + jj);
+                        dx8 cv1 = *(cd8ptr)(T.x[1] + base + jj);
+                        dx8 cv2 = *(cd8ptr)(T.x[2] + base + jj);
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) {
+                            double t0 = q[0] - cv0[u], t1 = q[1] - cv1[u], t2 = q[2] - cv2[u];
+                            double d2 = 0.0;
+                            d2 += t0 * t0;
+                            d2 += t1 * t1;
+                            d2 += t2 * t2;
+                            const int32_t cs = base + jj + u;
+                            bool below = d2 < Dk;
+                            if (d2 == Dk && act && cs != self) below = T.perm[cs] < IDk;
+                            cnt += (act && want && d2 > 0.0 && cs != self && below) ? 1 : 0;
+                        }
+                    }
+                    down = false;
+                } else {
+                    node = 2 * node;
+                    continue;
+                }
+            } else {
+                down = false;
+            }
+        }
+        if (node == 1) break;
+        if (!(node & 1)) {
+            node ^= 1;
+            down = true;
+        } else {
+            node >>= 1;
+        }
+    }
+    return cnt;
+}
+
+// k = kh - 1 neighbours; A.dens: NBX_DO_NONE / NBX_DO_SYM / NBX_DO_GATHER.
+template <int KMAX>
+__global__ __launch_bounds__(64) void k_knn_mixed(DevTree T, KnnArgs A, const double *ext, int32_t *fb_list,
+                                                   int32_t *fb_count) {
+    __shared__ uint64_t hk[KMAX * NBX_WAVE];
+    __shared__ float cx[3 * NBX_LEAF];
+    const int lane = threadIdx.x;
+    const int g = blockIdx.x;
+    const int kh = A.kh;  // k + 1
+    const int k = kh - 1;
+    const int qi = g * NBX_WAVE + lane;
+    const bool valid = qi < T.n;
+    const int kres = T.kernres;
+    MixCtx c;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) c.q[d] = valid ? T.x[d][qi] : 0.0;
+    c.act = valid;
+    c.self = qi;
+    c.root = MX_EMPTY;
+    c.emax = 0.0;
+    for (int p = 0; p < kh; ++p) HK(p) = MX_EMPTY;
+    double qc[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) qc[d] = readfirst_d(c.q[d]);
+    mix_walk(T, ext, c, qc, T.nleafp + 2 * g, hk, cx, lane, kh);
+
+    // ---------------------------------------------------------------- band
+    bool ok = valid;
+    if (valid) {
+        for (int m = kh - 1; m > 0; --m) {  // heap sort: ascending keys
+            uint64_t top = HK(0), last = HK(m);
+            HK(m) = top;
+            kheap_sift(hk, lane, m, 0, last);
+        }
+        if (HK(kh - 1) == MX_EMPTY) ok = false;  // fewer than k+1 candidates
+    }
+    const double eu = 12.0 * MX_U, ea = 5.0 * MX_U * c.emax;
+    int i0 = k - 1;
+    int32_t bs[4];
+    double bd[4];
+    double Dk = 0.0;
+    if (ok) {
+        // band: entries whose interval overlaps the k-th entry's
+        double dk = sqrt((double)keyd(HK(k - 1)));
+        double lo_k = dk - (eu * dk + ea);
+        while (i0 > 0) {
+            double di = sqrt((double)keyd(HK(i0 - 1)));
+            if (di + (eu * di + ea) < lo_k) break;
+            --i0;
+        }
+        // a certain entry that could be an exact duplicate (d2 == 0) must be checked too
+        double d0 = sqrt((double)keyd(HK(0)));
+        if (d0 - (eu * d0 + ea) <= 0.0 && i0 > 0) ok = false;
+        if (kh - i0 > 4) ok = false;
+    }
+    if (ok) {
+        // recompute the band in FP64; drop one entry: a d2 == 0 duplicate if
+        // present, else the largest (d2, ID)
+        const int nb = kh - i0;
+        int drop = -1, nzero = 0;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            bs[b] = -1;
+            bd[b] = 0.0;
+            if (b < nb) {
+                bs[b] = keys(HK(i0 + b));
+                bd[b] = d64(T, c.q, bs[b]);
+                if (bd[b] == 0.0) { nzero++; drop = b; }
+            }
+        }
+        if (nzero > 1) ok = false;
+        if (drop < 0) {
+            drop = 0;
+#pragma unroll
+            for (int b = 1; b < 4; ++b)
+                if (b < nb && (bd[b] > bd[drop] || (bd[b] == bd[drop] && T.perm[bs[b]] > T.perm[bs[drop]]))) drop = b;
+        }
+        Dk = 0.0;
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+            if (b < nb && b != drop && bd[b] > Dk) Dk = bd[b];
+        // certificate against everything outside the heap
+        double dl = sqrt((double)keyd(HK(kh - 1)));
+        double lout = dl - (eu * dl + ea);
+        if (!(lout > 0.0 && Dk < lout * lout)) ok = false;
+        if (ok) {
+            // the kept band entries fill positions i0 .. k-1 (FP32 order)
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+                if (b < nb && b != drop) HK(i0 + b - (b > drop ? 1 : 0)) = (uint64_t)(uint32_t)bs[b];
+        }
+    }
+    // exact verification for lanes the error bound could not decide: keep
+    // the k smallest (d2, ID) of the k+1 candidates in FP64, then count in
+    // FP64 the particles below the k-th of them
+    bool verify = valid && !ok && HK(kh - 1) != MX_EMPTY;
+    int32_t IDk = 0;
+    if (verify) {
+        int zero_at = -1, nzero = 0, maxp = -1;
+        double dmax = -1.0;
+        int32_t imax = -1;
+        for (int p = 0; p < kh; ++p) {
+            int32_t s = keys(HK(p));
+            double d = d64(T, c.q, s);
+            if (d == 0.0) {
+                nzero++;
+                zero_at = p;
+                continue;
+            }
+            int32_t id = T.perm[s];
+            if (maxp < 0 || d > dmax || (d == dmax && id > imax)) { maxp = p; dmax = d; imax = id; }
+        }
+        const int drop = nzero == 1 ? zero_at : maxp;
+        if (nzero > 1) verify = false;
+        if (verify) {
+            for (int p = drop; p < kh - 1; ++p) HK(p) = HK(p + 1);
+            Dk = 0.0;
+            IDk = -1;
+            for (int p = 0; p < k; ++p) {
+                int32_t s = keys(HK(p));
+                double d = d64(T, c.q, s);
+                int32_t id = T.perm[s];
+                if (d > Dk || (d == Dk && id > IDk)) { Dk = d; IDk = id; }
+            }
+        }
+    }
+    if (ballot(verify)) {
+        int cnt = verify_walk(T, c.q, verify, Dk, IDk, qi);
+        if (verify && cnt == k - 1) ok = true;
+    }
+    // fallback list for the FP64 kernel
+    uint64_t fbm = ballot(valid && !ok);
+    if (fbm) {
+        int base = 0;
+        if (lane == 0) base = atomicAdd(fb_count, __popcll(fbm));
+        base = __shfl(base, 0);
+        if (valid && !ok) fb_list[base + __popcll(fbm & ((1ULL << lane) - 1))] = qi;
+    }
+
+    // ---------------------------------------------------------------- density
+    if (ok && A.dens != NBX_DO_NONE) {
+        // weights from FP64 distances of the k selected neighbours (the FP32
+        // offsets carry a storage error proportional to the leaf size)
+        const double hi = 0.5 * sqrt(Dk);
+        const double norm = 1.0 / pow(hi, 3.0);
+        const double mi = T.mass[qi];
+        double acc = 0.0;
+        for (int p = 0; p < k; ++p) {
+            int32_t s = keys(HK(p));
+            double rij = sqrt(d64(T, c.q, s));
+            double W = wsm_at(rij, hi, T.kern, kres);
+            if (A.dens == NBX_DO_SYM) {
+                double Wij = 0.5 * W * norm;
+                acc = acc + Wij * T.mass[s];
+                unsafeAtomicAdd(A.rho + s, Wij * mi);
+            } else {
+                acc += W * norm * T.mass[s];
+            }
+        }
+        if (A.dens == NBX_DO_SYM) unsafeAtomicAdd(A.rho + qi, acc);
+        else A.rho[qi] = acc;
+        if (A.hout) A.hout[qi] = hi;
+    }
+
+    // ---------------------------------------------------------------- outputs
+    if (A.nn == nullptr) return;
+    const uint64_t okm = ballot(ok);
+    __syncthreads();
+    const int total = NBX_WAVE * k;
+    for (int e = lane; e < total; e += NBX_WAVE) {
+        int r = e / k, t = e - r * k;
+        int qr = g * NBX_WAVE + r;
+        if (qr >= T.n) break;
+        if (!((okm >> r) & 1ULL)) continue;
+        int32_t s = keys(hk[t * NBX_WAVE + r]);
+        int64_t row = A.layout == 1 ? qr : T.perm[qr];
+        A.nn[row * k + t] = A.layout == 1 ? s : T.perm[s];
+    }
+}
+
+hipError_t nbx_launch_knn_mixed(const DevTree &T, const KnnArgs &A, const double *ext, int32_t *fallback_list,
                                 int32_t *fallback_count, hipStream_t st) {
-    (void)T; (void)A; (void)band; (void)fallback_list; (void)fallback_count; (void)st;
-    return hipErrorNotSupported;
+    int blocks = (T.n + NBX_WAVE - 1) / NBX_WAVE;
+    if (blocks == 0) return hipSuccess;
+    if (T.nd != 3 || A.kh > 65 || A.kh < 2 || !T.kernf || !T.org || !ext) return hipErrorInvalidValue;
+    ProfScope ps("k_knn_mixed", st);
+    if (A.kh <= 33)
+        hipLaunchKernelGGL((k_knn_mixed<33>), dim3(blocks), dim3(NBX_WAVE), 0, st, T, A, ext, fallback_list,
+                           fallback_count);
+    else
+        hipLaunchKernelGGL((k_knn_mixed<65>), dim3(blocks), dim3(NBX_WAVE), 0, st, T, A, ext, fallback_list,
+                           fallback_count);
+    return hipGetLastError();
 }

@@ -56,6 +56,11 @@ class DeviceTree:
             lib().nbx_tree_destroy(h)
             self._h = None
 
+    def last_fallbacks(self):
+        v = ctypes.c_int64()
+        _check(lib().nbx_tree_last_fallbacks(self._h, ctypes.byref(v)))
+        return v.value
+
     def knn(self, k, nn_out, d2_out=None, variant=KNN_FINDNEAREST, layout=LAYOUT_BY_SLOT):
         _need(nn_out, "nn_out", torch.int32, (self.n, k))
         _need(d2_out, "d2_out", torch.float64, (self.n, k))
