@@ -362,7 +362,8 @@ extern "C" int nbx_tree_create_dev(const nbx_params *p, const double *pos, const
 extern "C" int nbx_tree_create(const nbx_params *p, const double *pos, const double *vel, const double *mass, int64_t n,
                                nbx_tree **out) {
     if (!out || !pos) return fail(NBX_ERR_ARG, "NULL argument");
-    if (n < 1) return fail(NBX_ERR_ARG, "number of particles out of range");
+    int rc0 = check_params(p, n, vel != nullptr);
+    if (rc0) return rc0;
     double *dpos = nullptr, *dvel = nullptr, *dm = nullptr;
     HIPCK(hipMalloc((void **)&dpos, sizeof(double) * 3 * n));
     HIPCK(hipMemcpy(dpos, pos, sizeof(double) * 3 * n, hipMemcpyHostToDevice));

@@ -1,0 +1,87 @@
+"""C ABI checks that need no GPU: the library loads, exports every function
+include/nbx.h declares, keeps the reference constructor defaults, and rejects
+invalid options before touching the device."""
+import ctypes
+import os
+import re
+
+import pytest
+
+import nbodylib_amd as nb
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    src = open(os.path.join(ROOT, "include", "nbx.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(nbx_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_library_exports_header():
+    lib = nb.lib()
+    names = _declared()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(nb.EXPORTS) <= set(names)
+
+
+def test_params_defaults_match_reference():
+    """KDTree.h:229-245: bucket 16, TPHYS, KEPAN, kernres 1000, spread split,
+    aniso 0, no scaling, no period."""
+    p = nb.Params()
+    nb.lib().nbx_params_default(ctypes.byref(p))
+    assert (p.bucket_size, p.treetype, p.kerntype, p.kernres) == (16, nb.TPHYS, nb.KEPAN, 1000)
+    assert (p.splitting, p.aniso, p.scalespace, p.periodic) == (0, 0, 0, 0)
+    assert p.form == nb.FORM_PLAIN
+
+
+def _create(p, n=10, pos=True, vel=False):
+    import numpy as np
+
+    a = np.zeros(3 * n)
+    h = ctypes.c_void_p()
+    ptr = ctypes.c_void_p(a.ctypes.data) if pos else None
+    vptr = ctypes.c_void_p(a.ctypes.data) if vel else None
+    return nb.lib().nbx_tree_create(ctypes.byref(p), ptr, vptr, None, n, ctypes.byref(h))
+
+
+@pytest.mark.parametrize("field,value,code", [
+    ("treetype", 7, nb.NBX_ERR_UNSUPPORTED),   # KDTree.cxx:1096-1099 "Error in type of tree specified"
+    ("treetype", nb.TVEL, nb.NBX_ERR_UNSUPPORTED),
+    ("splitting", 1, nb.NBX_ERR_UNSUPPORTED),
+    ("scalespace", 1, nb.NBX_ERR_UNSUPPORTED),
+])
+def test_invalid_options_rejected(field, value, code):
+    p = nb.params()
+    setattr(p, field, value)
+    assert _create(p) == code
+    assert len(nb.lib().nbx_last_error()) > 0
+
+
+def test_phase_tree_needs_velocities_and_aniso():
+    p = nb.params(treetype=nb.TPHS, aniso=-1)
+    assert _create(p, vel=False) == nb.NBX_ERR_ARG
+    p = nb.params(treetype=nb.TPHS, aniso=0)
+    assert _create(p, vel=True) == nb.NBX_ERR_UNSUPPORTED
+
+
+def test_null_and_size_arguments():
+    p = nb.params()
+    assert _create(p, pos=False) == nb.NBX_ERR_ARG
+    assert _create(p, n=0) == nb.NBX_ERR_ARG
+    bad = nb.params(period=0.0)
+    assert _create(bad) == nb.NBX_ERR_ARG
+
+
+def test_python_mirror_fails_loudly_without_library(tmp_path, monkeypatch):
+    import importlib
+
+    monkeypatch.setattr(nb, "LIBPATH", str(tmp_path / "missing.so"))
+    monkeypatch.setattr(nb, "_lib", None)
+    with pytest.raises(ImportError):
+        nb.lib()
+    monkeypatch.undo()
+    importlib.reload(nb)
+    nb.lib()

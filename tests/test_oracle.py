@@ -1,0 +1,315 @@
+"""Pins for the CPU oracle (no GPU).
+
+The reference ships no golden vectors and cannot be built here (GSL is
+absent), so every oracle function is pinned by something independent of its
+own formula: a closed form, an all-pairs brute force that shares no code with
+the tree restatement, an invariant or a statistical property -- and each pin
+is shown to catch a planted mistake.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from nbodylib_amd import datagen
+
+
+# ---------------------------------------------------------------- kernel
+def test_kernel_closed_forms():
+    kn3, tab3 = oracle.kernel(3, oracle.KSPH, 1000)
+    kn6, tab6 = oracle.kernel(6, oracle.KSPH, 1000)
+    assert abs(kn3 - 1.0 / math.pi) < 1e-15
+    assert abs(kn6 - 3024.0 / (762.0 * math.pi ** 3)) < 1e-15
+    assert tab3[0] == kn3 and abs(tab3[-1]) < 1e-12
+    assert len(tab3) == 1000
+    assert len(oracle.kernel(3, oracle.KSPH, 50)[1]) == 100  # kernres < 100 clamps (KDTree.cxx:1145-1148)
+
+
+@pytest.mark.parametrize("nd,kt", [(3, oracle.KSPH), (6, oracle.KSPH), (3, oracle.KEPAN), (2, oracle.KSPH)])
+def test_kernel_normalisation_integral(nd, kt):
+    """int_0^2 S_{nd-1} r^{nd-1} W(r) dr = 1 (unit-h kernel, support 2h)."""
+    kn, _ = oracle.kernel(nd, kt, 1000)
+    r = np.linspace(0.0, 2.0, 200001)
+    if kt == oracle.KSPH:
+        w = np.where(r <= 1, 1 - 0.75 * (2 - r) * r * r, 0.25 * (2 - r) ** 3)
+    else:
+        w = np.clip(1 - (r / 2) ** 2, 0, None)
+    surf = 2 * math.pi ** (nd / 2) / math.gamma(nd / 2)
+    integral = np.trapezoid(surf * r ** (nd - 1) * kn * w, r)
+    assert abs(integral - 1.0) < 1e-6
+
+
+def test_kernel_mutation_caught():
+    oracle.set_mutation(1)
+    try:
+        kn3, _ = oracle.kernel(3, oracle.KSPH, 1000)
+        assert abs(kn3 - 1.0 / math.pi) > 1e-3
+    finally:
+        oracle.set_mutation(0)
+
+
+# ---------------------------------------------------------------- kNN
+@pytest.mark.parametrize("n,k,seed", [(2000, 32, 1), (3001, 16, 2), (40, 32, 3), (17, 16, 4)])
+def test_tree_knn_vs_brute(n, k, seed):
+    pos = datagen.uniform(n, 1.0, seed)
+    t = oracle.Tree(pos, b=16)
+    ids, d2 = t.knn(min(k, n - 1), 0)
+    bi, bd = oracle.knn_brute(pos, min(k, n - 1), mode=0)
+    assert np.array_equal(np.sort(ids, 1), np.sort(bi, 1))
+    assert np.array_equal(d2, bd)
+
+
+def test_tree_knn_clustered_vs_brute():
+    pos, _ = datagen.clustered(6000, 100.0, nhalo=20, seed=5)
+    t = oracle.Tree(pos, b=16, period=100.0)
+    ids, d2 = t.knn(32, 3)
+    bi, bd = oracle.knn_brute(pos, 32, mode=0)
+    assert np.array_equal(np.sort(ids, 1), np.sort(bi, 1))
+    assert np.array_equal(d2, bd)
+
+
+def test_knn_excludes_duplicates_and_self():
+    """Target search skips i == tt and every d2 == 0 (KDLeafNode.cxx:19-21)."""
+    pos = datagen.uniform(500, 1.0, 6)
+    pos[1] = pos[0]
+    t = oracle.Tree(pos, b=16)
+    ids, d2 = t.knn(8, 0)
+    assert 1 not in ids[0] and 0 not in ids[1] and np.all(d2 > 0)
+    bi, _ = oracle.knn_brute(pos, 8, mode=0)
+    assert np.array_equal(np.sort(ids, 1), np.sort(bi, 1))
+
+
+def test_knn_zero_rule_mutation_caught():
+    pos = datagen.uniform(500, 1.0, 6)
+    pos[1] = pos[0]
+    oracle.set_mutation(2)
+    try:
+        ids, _ = oracle.Tree(pos, b=16).knn(8, 0)
+    finally:
+        oracle.set_mutation(0)
+    bi, _ = oracle.knn_brute(pos, 8, mode=0)
+    assert not np.array_equal(np.sort(ids, 1), np.sort(bi, 1))
+
+
+def _min_image_knn(pos, k, L):
+    d = pos[:, None, :] - pos[None, :, :]
+    d -= L * np.round(d / L)
+    d2 = (d * d).sum(-1)
+    np.fill_diagonal(d2, np.inf)
+    return np.sort(np.argsort(d2, axis=1, kind="stable")[:, :k], 1)
+
+
+@pytest.mark.parametrize("variant,mode", [(0, 1), (1, 2)])
+def test_tree_periodic_vs_brute(variant, mode):
+    pos = datagen.uniform(3000, 1.0, 7)
+    t = oracle.Tree(pos, b=16, period=1.0)
+    ids, d2 = t.knn(32, variant)
+    bi, bd = oracle.knn_brute(pos, 32, mode=mode, period=1.0)
+    assert np.array_equal(np.sort(ids, 1), np.sort(bi, 1))
+    assert np.array_equal(d2, bd)
+
+
+def test_periodic_image_rule_quirk():
+    """KDSplitNode.cxx:1134-1146 compares Top (a squared distance) with s (a
+    distance) for the 2-D and 3-D images: in a unit box that skips diagonal
+    images a minimum-image search would use, so some rows differ from the
+    exact minimum image; in a box of side 100 it only over-searches and every
+    row matches the minimum image."""
+    pos = datagen.uniform(3000, 1.0, 8)
+    ids, _ = oracle.Tree(pos, b=16, period=1.0).knn(32, 0)
+    mi = _min_image_knn(pos, 32, 1.0)
+    diff = np.any(np.sort(ids, 1) != mi, axis=1)
+    assert 0 < diff.sum() < 0.2 * len(pos)
+    pos100 = pos * 100.0
+    ids100, _ = oracle.Tree(pos100, b=16, period=100.0).knn(32, 0)
+    assert np.array_equal(np.sort(ids100, 1), _min_image_knn(pos100, 32, 100.0))
+
+
+def test_root_leaf_periodic_edge():
+    """N <= b: the root is a leaf and the periodic search degrades to the
+    plain target search (KDLeafNode.cxx:658-661)."""
+    pos = datagen.uniform(12, 1.0, 9)
+    ids, _ = oracle.Tree(pos, b=16, period=1.0).knn(4, 0)
+    bi, _ = oracle.knn_brute(pos, 5, mode=0)
+    assert np.array_equal(np.sort(ids, 1), np.sort(bi[:, 1:], 1))
+
+
+def test_phase_knn_vs_brute():
+    pos, vel, _ = datagen.clustered(3000, 100.0, nhalo=10, seed=10, with_vel=True)
+    ph_pos = pos / pos.std(0)
+    ph_vel = vel / vel.std(0)
+    t = oracle.Tree(ph_pos, vel=ph_vel, b=16, treetype=oracle.TPHS)
+    ids, d2 = t.knn(32, 2)
+    bi, bd = oracle.knn_brute(ph_pos, 32, mode=0, vel=ph_vel, nd=6)
+    assert np.array_equal(np.sort(ids, 1), np.sort(bi, 1))
+    assert np.array_equal(d2, bd)
+
+
+# ---------------------------------------------------------------- density
+def test_calc_density_vs_lists():
+    pos = datagen.uniform(4000, 1.0, 11)
+    m = datagen.lognormal_masses(4000, 11)
+    t = oracle.Tree(pos, mass=m, b=16, period=1.0)
+    rho = t.calc_density(32)
+    bi, _ = oracle.knn_brute(pos, 32, mode=0)
+    rb, _ = oracle.density_from_lists(pos, m, bi)
+    assert np.max(np.abs(rho / rb - 1)) < 1e-12
+
+
+def test_density_mean_uniform_box():
+    """Mean density of a uniform unit box with total mass 1, away from the
+    faces (CalcDensity ignores the period), is 1 to a few per cent; a dropped
+    factor 1/2 or a dropped scatter term fails it."""
+    n = 32768
+    pos = datagen.uniform(n, 1.0, 20261202)
+    inner = np.all((pos > 0.15) & (pos < 0.85), axis=1)
+    rho = oracle.Tree(pos, b=16, period=1.0).calc_density(32)
+    assert abs(rho[inner].mean() - 1.0) < 0.03
+    for mut in (1, 3):
+        oracle.set_mutation(mut)
+        try:
+            bad = oracle.Tree(pos, b=16, period=1.0).calc_density(32)
+        finally:
+            oracle.set_mutation(0)
+        assert abs(bad[inner].mean() - 1.0) > 0.3
+
+
+def test_gather_density_vs_lists():
+    pos = datagen.uniform(3000, 1.0, 12)
+    t = oracle.Tree(pos, b=16)
+    rho, h = t.calc_density_gather(32)
+    bi, bd = oracle.knn_brute(pos, 32, mode=0)
+    rb, hb = oracle.density_from_lists(pos, None, bi, symmetric=False)
+    assert np.max(np.abs(rho / rb - 1)) < 1e-12
+    assert np.array_equal(h, 0.5 * np.sqrt(bd[:, -1]))
+    big = datagen.uniform(32768, 1.0, 20261202)
+    rg, _ = oracle.Tree(big, b=16).calc_density_gather(32)
+    inner = np.all((big > 0.15) & (big < 0.85), axis=1)
+    assert abs(rg[inner].mean() - 1.0) < 0.03
+
+
+def _veldens_numpy(pos, vel, nsmooth, nsearch):
+    ids, _ = oracle.knn_brute(pos, nsearch, mode=0)
+    kn, tab = oracle.kernel(3, oracle.KSPH, 1000)
+    out = np.zeros(len(pos))
+    for i in range(len(pos)):
+        dv = np.sort(np.sqrt(((vel[ids[i]] - vel[i]) ** 2).sum(1)))[:nsmooth]
+        h = 0.5 * dv[-1]
+        q = dv / h
+        j = (q * 0.5 * 999).astype(int)
+        w = np.where(j < 999, tab[np.minimum(j, 998)] + (tab[np.minimum(j + 1, 999)] - tab[np.minimum(j, 998)]) *
+                     (q - (2 / 999) * j) / (2 / 999), tab[999])
+        out[i] = (w / h ** 3).sum()
+    return out
+
+
+def test_vel_density_vs_numpy_and_homogeneity():
+    n = 1500
+    pos, vel, _ = datagen.clustered(n, 100.0, nhalo=8, seed=13, with_vel=True)
+    t = oracle.Tree(pos, vel=vel, b=16)
+    rv = t.calc_veldensity(16, 48)
+    ref = _veldens_numpy(pos, vel, 16, 48)
+    assert np.max(np.abs(rv / ref - 1)) < 1e-12
+    # scaling velocities by s scales the velocity density by s^-3
+    rv2 = oracle.Tree(pos, vel=2.0 * vel, b=16).calc_veldensity(16, 48)
+    assert np.max(np.abs(rv2 * 8.0 / rv - 1)) < 1e-9
+
+
+# ---------------------------------------------------------------- FOF
+@pytest.mark.parametrize("periodic", [False, True])
+def test_fof_tree_vs_brute(periodic):
+    n = 6000
+    pos = datagen.uniform(n, 1.0, 14)
+    b = 0.6 * datagen.mean_spacing(n, 1.0)
+    per = 1.0 if periodic else None
+    raw, g = oracle.Tree(pos, b=16, period=per).fof(b, 8)
+    can, g2 = oracle.fof_canonical(raw, 8)
+    bl, bg = oracle.fof_brute(pos, b, 8, period=per)
+    assert g == g2 == bg
+    assert np.array_equal(can, bl)
+
+
+def test_fof_clustered_vs_brute():
+    pos, _ = datagen.clustered(5000, 100.0, nhalo=15, seed=15)
+    b = 0.2 * datagen.mean_spacing(5000, 100.0)
+    raw, g = oracle.Tree(pos, b=16, period=100.0).fof(b, 8)
+    can, _ = oracle.fof_canonical(raw, 8)
+    bl, bg = oracle.fof_brute(pos, b, 8, period=100.0)
+    assert g == bg and np.array_equal(can, bl)
+
+
+def test_fof_chains():
+    """Particles on a line: spacing 0.9 b links into one group, 1.1 b does
+    not; groups below minnum are dropped; a chain across the periodic face
+    links through the image."""
+    b = 0.1
+    line = np.zeros((20, 3))
+    line[:, 0] = 0.2 + 0.9 * b * np.arange(20)
+    line[:, 1:] = 0.5
+    raw, g = oracle.Tree(line, b=4).fof(b, 8)
+    assert g == 1 and len(set(raw)) == 1
+    far = line.copy()
+    far[:, 0] = 0.1 + 1.1 * b * np.arange(20) / 3.0 * 3.0
+    raw, g = oracle.Tree(far, b=4).fof(b, 2)
+    assert g == 0 and np.all(raw == 0)
+    wrap = np.zeros((10, 3)) + 0.5
+    wrap[:, 0] = (0.95 + 0.9 * b * np.arange(10)) % 1.0
+    raw, g = oracle.Tree(wrap, b=4, period=1.0).fof(b, 10)
+    assert g == 1 and np.all(raw == 1)
+    raw, g = oracle.Tree(wrap, b=4).fof(b, 10)
+    assert g == 0  # without the period the chain is cut at the face (9 < minnum)
+
+
+def test_fof_order_and_canonical():
+    pos, _ = datagen.clustered(4000, 100.0, nhalo=12, seed=16)
+    b = 0.3 * datagen.mean_spacing(4000, 100.0)
+    t = oracle.Tree(pos, b=16)
+    raw0, g0 = t.fof(b, 8, order=0)
+    raw1, g1 = t.fof(b, 8, order=1)
+    assert g0 == g1
+    sizes = np.bincount(raw1)[1:]
+    assert np.all(np.diff(sizes) <= 0)  # order=1: labels by size descending
+    c0, _ = oracle.fof_canonical(raw0, 8)
+    c1, _ = oracle.fof_canonical(raw1, 8)
+    assert np.array_equal(c0, c1)
+
+
+# ---------------------------------------------------------------- adjoint
+def test_adjoint_vs_finite_differences():
+    n, k = 300, 16
+    pos = datagen.uniform(n, 1.0, 17)
+    mass = datagen.lognormal_masses(n, 17)
+    nbr, _ = oracle.knn_brute(pos, k, mode=0)
+    rho, _ = oracle.density_analytic(pos, mass, nbr)
+    g = 1.0 / rho  # L = sum log rho
+    dpos, dm = oracle.density_adjoint(pos, mass, nbr, g)
+    rng = np.random.default_rng(0)
+    eps = 1e-6
+    for i in rng.choice(n, 12, replace=False):
+        for c in range(3):
+            p1, p2 = pos.copy(), pos.copy()
+            p1[i, c] += eps
+            p2[i, c] -= eps
+            f1 = np.log(oracle.density_analytic(p1, mass, nbr)[0]).sum()
+            f2 = np.log(oracle.density_analytic(p2, mass, nbr)[0]).sum()
+            fd = (f1 - f2) / (2 * eps)
+            assert abs(fd - dpos[i, c]) < 1e-5 * max(1.0, abs(fd))
+        m1, m2 = mass.copy(), mass.copy()
+        m1[i] += eps * mass[i]
+        m2[i] -= eps * mass[i]
+        fd = (np.log(oracle.density_analytic(pos, m1, nbr)[0]).sum() -
+              np.log(oracle.density_analytic(pos, m2, nbr)[0]).sum()) / (2 * eps * mass[i])
+        assert abs(fd - dm[i]) < 1e-5 * max(1.0, abs(fd))
+
+
+def test_analytic_density_matches_table_density():
+    """The analytic cubic spline and the 1000-entry table agree to the
+    table's interpolation error."""
+    n = 2000
+    pos = datagen.uniform(n, 1.0, 18)
+    mass = np.full(n, 1.0 / n)
+    nbr, _ = oracle.knn_brute(pos, 32, mode=0)
+    ra, _ = oracle.density_analytic(pos, mass, nbr)
+    rt, _ = oracle.density_from_lists(pos, mass, nbr)
+    assert np.max(np.abs(ra / rt - 1)) < 1e-5
