@@ -1,0 +1,170 @@
+/*
+ * nbx_kdtree.h -- drop-in NBody::KDTree / NBody::Particle for the KDTree hot
+ * path, backed by the MI355X C ABI (nbx.h, libnbx.so).
+ *
+ * Mirrors the reference's call surface for the hot path:
+ *   Particle      src/NBody/Particle.h:264-293 (88-byte layout, default build)
+ *   KDTree ctor   src/KDTree/KDTree.h:229-274
+ *   FindNearest*  KDTree.h:300-331      SearchBallPosTagged  KDTree.h:363-365
+ *   CalcDensity*  KDTree.h:411-416      FOF                  KDTree.h:480-482
+ *   OverWriteInputOrder / SetResetOrder KDTree.h:515-518
+ *
+ * Semantics kept from the reference: the constructor overwrites Particle::id
+ * with the input index and reorders the caller's array into tree order (here
+ * Morton order); "tree indices" returned by the kNN calls index that array;
+ * Group/FOF arrays are keyed by ID; ~KDTree sorts the array back by ID.
+ * Errors print a message and exit(1) as the reference does, unless
+ * KDTree::SetThrowOnError(true) selects std::runtime_error instead.
+ *
+ * Single-particle queries (FindNearest(tt,...), CalcDensityParticle) are
+ * answered from one batched GPU call per (variant, Nsearch) whose result is
+ * cached, so OpenMP loops over particles (the reference's usage,
+ * test_kdtree.cxx:278-301) run at batch speed; the cache is dropped when the
+ * tree is destroyed.
+ */
+#ifndef NBX_KDTREE_H
+#define NBX_KDTREE_H
+
+#include <mutex>
+#include <vector>
+
+#include "nbx.h"
+
+namespace NBody {
+
+typedef double Double_t;  // Precision.h:38-52 default build
+typedef float Real_t;
+typedef int Int_t;
+typedef int Int_tree_t;
+
+class Particle {
+  protected:
+    Double_t mass;
+    Double_t position[3];
+    Double_t velocity[3];
+    Int_t pid;
+    Int_t id;
+    int type;
+    Double_t rho;
+    Double_t phi;
+
+  public:
+    Particle(Double_t Mass = 0, Double_t x = 0, Double_t y = 0, Double_t z = 0, Double_t vx = 0, Double_t vy = 0,
+             Double_t vz = 0, Int_t ID = 0, int Type = 0, Double_t Rho = 0, Double_t Phi = 0, Int_t PID = 0)
+        : mass(Mass), position{x, y, z}, velocity{vx, vy, vz}, pid(PID), id(ID), type(Type), rho(Rho), phi(Phi) {}
+    Particle(Double_t Mass, const Double_t *NewPos, const Double_t *NewVel, Int_t ID = 0, int Type = 0,
+             Double_t Rho = 0, Double_t Phi = 0, Int_t PID = 0)
+        : mass(Mass), position{NewPos[0], NewPos[1], NewPos[2]}, velocity{NewVel[0], NewVel[1], NewVel[2]},
+          pid(PID), id(ID), type(Type), rho(Rho), phi(Phi) {}
+
+    Double_t GetMass() const { return mass; }
+    void SetMass(const Double_t &m) { mass = m; }
+    const Double_t *GetPosition() const { return position; }
+    Double_t GetPosition(const int &i) const { return position[i]; }
+    void SetPosition(const int &i, const Double_t &x) { position[i] = x; }
+    void SetPosition(const Double_t *p) { position[0] = p[0]; position[1] = p[1]; position[2] = p[2]; }
+    const Double_t *GetVelocity() const { return velocity; }
+    Double_t GetVelocity(const int &i) const { return velocity[i]; }
+    void SetVelocity(const int &i, const Double_t &v) { velocity[i] = v; }
+    Double_t GetPhase(const int &i) const { return i < 3 ? position[i] : velocity[i - 3]; }
+    void SetPhase(const int &i, const Double_t &x) { if (i < 3) position[i] = x; else velocity[i - 3] = x; }
+    Int_t GetPID() const { return pid; }
+    void SetPID(const Int_t &i) { pid = i; }
+    Int_t GetID() const { return id; }
+    void SetID(const Int_t &i) { id = i; }
+    int GetType() const { return type; }
+    void SetType(int t) { type = t; }
+    Double_t GetDensity() const { return rho; }
+    void SetDensity(const Double_t &r) { rho = r; }
+    Double_t GetPotential() const { return phi; }
+    void SetPotential(const Double_t &p) { phi = p; }
+};
+
+class KDTree {
+  public:
+    const static int TPHYS = 0, TPROJ = 1, TVEL = 2, TPHS = 3, TMETRIC = 4;
+    const static int KSPH = 0, KGAUSS = 1, KEPAN = 2, KTH = 3;
+    const static int KDTREE_SPLIT_ENTROPY = 1, KDTREE_SPLIT_DISPERSION = 2, KDTREE_SPLIT_MAXINTERPARTICLESPACING = 3,
+                     KDTREE_SPLIT_SPREAD = 0;
+
+    KDTree(Particle *p, Int_t numparts, Int_t bucket_size = 16, int TreeType = TPHYS, int KernType = KEPAN,
+           int KernRes = 1000, int SplittingCriterion = KDTREE_SPLIT_SPREAD, int Aniso = 0, int ScaleSpace = 0,
+           Double_t *Period = nullptr, Double_t **metric = nullptr, bool iBuildInParallel = true,
+           bool iKeepInputOrder = false, Double_t Rdistadapt = -1, Double_t AdaptiveMedianFac = 0.0,
+           Int_t min_bucket_size = 16);
+    KDTree(std::vector<Particle> &p, Int_t bucket_size = 16, int TreeType = TPHYS, int KernType = KEPAN,
+           int KernRes = 1000, int SplittingCriterion = KDTREE_SPLIT_SPREAD, int Aniso = 0, int ScaleSpace = 0,
+           Double_t *Period = nullptr, Double_t **metric = nullptr, bool iBuildInParallel = true,
+           bool iKeepInputOrder = false, Double_t Rdistadapt = -1, Double_t AdaptiveMedianFac = 0.0,
+           Int_t min_bucket_size = 16);
+    ~KDTree();
+    KDTree(const KDTree &) = delete;
+    KDTree &operator=(const KDTree &) = delete;
+
+    Int_t GetNumNodes() { return numnodes; }
+    Int_t GetNumLeafNodes() { return numleafnodes; }
+    Int_t GetBucketSize() { return b; }
+    Int_t GetTreeType() { return treetype; }
+    Int_t GetKernType() { return kernfunctype; }
+    Double_t GetKernNorm();
+    Double_t GetPeriod(int j) { return period[j]; }
+
+    // kNN for every particle: nn[i][*] are tree indices, ascending in dist2
+    void FindNearest(Int_t **nn, Double_t **dist2, Int_t Nsearch = 64);
+    void FindNearestPos(Int_t **nn, Double_t **dist2, Int_t Nsearch = 64);
+    void FindNearestPhase(Int_t **nn, Double_t **dist2, Int_t Nsearch = 64);
+    // one particle (tree index tt)
+    void FindNearest(Int_t tt, Int_t *nn, Double_t *dist2, Int_t Nsearch = 64);
+    void FindNearestPos(Int_t tt, Int_t *nn, Double_t *dist2, Int_t Nsearch = 64);
+    void FindNearestPhase(Int_t tt, Int_t *nn, Double_t *dist2, Int_t Nsearch = 64);
+    // one position
+    void FindNearestPos(Double_t *x, Int_t *nn, Double_t *dist2, Int_t Nsearch = 64);
+
+    // tree indices of all particles with d2 < fdist2 around a position / particle
+    Int_t SearchBallPosTagged(Double_t *x, Double_t fdist2, Int_t *tagged);
+    Int_t SearchBallPosTagged(Int_t tt, Double_t fdist2, Int_t *tagged);
+    std::vector<Int_t> SearchBallPosTagged(Double_t *x, Double_t fdist2);
+
+    void CalcDensity(Int_t Nsmooth = 64);
+    void CalcVelDensity(Int_t Nsmooth = 64, Int_t Nsearch = 64);
+    Double_t CalcDensityParticle(Int_t target, Int_t Nsmooth = 64);
+
+    Int_t *FOF(Double_t fdist, Int_t &numgroup, Int_t minnum = 8, int order = 0, Int_tree_t *pHead = nullptr,
+               Int_tree_t *pNext = nullptr, Int_tree_t *pTail = nullptr, Int_tree_t *pLen = nullptr);
+
+    void OverWriteInputOrder();
+    void SetResetOrder(bool);
+
+    static void SetThrowOnError(bool on);
+    // Precision form for the batched calls (NBX_FORM_MIXED): exact neighbour
+    // sets, densities within 2e-6 of FP64 (see nbx.h).  Off by default.
+    static void SetPrecisionForm(bool mixed);
+
+  private:
+    Particle *bucket;
+    Int_t numparts;
+    Int_t b;
+    int treetype, kernfunctype;
+    Int_t numnodes, numleafnodes;
+    Double_t period[3];
+    bool periodic;
+    bool iresetorder;
+    nbx_tree *tree;
+    std::vector<Int_t> slot_of_id;  // ID -> tree index
+    std::mutex mu;
+    // cached batched single-particle answers
+    int cache_variant, cache_k;
+    std::vector<Int_t> cache_nn;
+    std::vector<Double_t> cache_d2;
+    int dcache_k;
+    std::vector<Double_t> dcache;
+
+    void init(Particle *p, Int_t n, Int_t bucket_size, int TreeType, int KernType, int KernRes, int Splitting,
+              int Aniso, int ScaleSpace, Double_t *Period);
+    void batch_knn(int variant, Int_t k, Int_t *nn, Double_t *d2);
+    void ensure_cache(int variant, Int_t k);
+};
+
+}  // namespace NBody
+
+#endif

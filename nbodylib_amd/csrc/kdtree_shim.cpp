@@ -1,0 +1,240 @@
+// kdtree_shim.cpp -- NBody::KDTree drop-in over the C ABI (include/nbx_kdtree.h).
+//
+// Reproduces the reference's host-side contract around the hot path:
+//  * KDTree::KDTree (KDTree.cxx:1238-1306): ids overwritten with the input
+//    index (:1291) and the caller's array reordered into tree order;
+//  * ~KDTree (:1340-1356): array sorted back by id unless SetResetOrder(false);
+//  * OverWriteInputOrder (:1358-1361): ids rewritten to the current order;
+//  * CalcDensity / CalcVelDensity write Particle::rho (SetDensity);
+//  * FOF returns new Int_t[N] keyed by id (KDFOF.cxx:35, :152).
+// All computation is in libnbx.so; this file only moves data.
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/nbx_kdtree.h"
+
+namespace NBody {
+
+static bool g_throw = false;
+static bool g_mixed = false;
+
+void KDTree::SetThrowOnError(bool on) { g_throw = on; }
+void KDTree::SetPrecisionForm(bool mixed) { g_mixed = mixed; }
+
+static void check(int rc, const char *what) {
+    if (rc >= 0) return;
+    std::string msg = std::string(what) + ": " + nbx_last_error();
+    if (g_throw) throw std::runtime_error(msg);
+    std::fprintf(stderr, "%s\n", msg.c_str());
+    std::exit(1);
+}
+
+void KDTree::init(Particle *p, Int_t n, Int_t bucket_size, int TreeType, int KernType, int KernRes, int Splitting,
+                  int Aniso, int ScaleSpace, Double_t *Period) {
+    bucket = p;
+    numparts = n;
+    b = bucket_size;
+    treetype = TreeType;
+    kernfunctype = KernType;
+    iresetorder = true;
+    tree = nullptr;
+    cache_variant = cache_k = dcache_k = -1;
+    periodic = Period != nullptr;
+    for (int k = 0; k < 3; ++k) period[k] = periodic ? Period[k] : 0.0;
+    nbx_params prm;
+    nbx_params_default(&prm);
+    prm.bucket_size = bucket_size;
+    prm.treetype = TreeType;
+    prm.kerntype = KernType;
+    prm.kernres = KernRes;
+    prm.splitting = Splitting;
+    // the phase-space tree is supported with the metric factor taken as 1
+    prm.aniso = (TreeType == TPHS && Aniso == 0) ? -1 : Aniso;
+    prm.scalespace = ScaleSpace;
+    prm.periodic = periodic ? 1 : 0;
+    for (int k = 0; k < 3; ++k) prm.period[k] = period[k];
+    prm.form = g_mixed ? NBX_FORM_MIXED : NBX_FORM_PLAIN;
+    // ids := input index (KDTree.cxx:1291); SoA copies for the device
+    std::vector<double> pos(3 * (size_t)n), vel(3 * (size_t)n), mass(n);
+    for (Int_t i = 0; i < n; ++i) {
+        p[i].SetID(i);
+        for (int d = 0; d < 3; ++d) {
+            pos[d * (size_t)n + i] = p[i].GetPosition(d);
+            vel[d * (size_t)n + i] = p[i].GetVelocity(d);
+        }
+        mass[i] = p[i].GetMass();
+    }
+    check(nbx_tree_create(&prm, pos.data(), vel.data(), mass.data(), n, &tree), "KDTree");
+    // reorder the caller's array into tree order, as BuildNodes does in place
+    std::vector<int32_t> order(n);
+    check(nbx_tree_order(tree, order.data()), "KDTree");
+    std::vector<Particle> copy(p, p + n);
+    slot_of_id.assign(n, 0);
+    for (Int_t s = 0; s < n; ++s) {
+        p[s] = copy[order[s]];
+        slot_of_id[order[s]] = s;
+    }
+    numleafnodes = (n + 31) / 32;
+    int lp = 1;
+    while (lp < numleafnodes) lp <<= 1;
+    numnodes = 2 * lp - 1;
+}
+
+KDTree::KDTree(Particle *p, Int_t n, Int_t bucket_size, int TreeType, int KernType, int KernRes, int Splitting,
+               int Aniso, int ScaleSpace, Double_t *Period, Double_t **, bool, bool, Double_t, Double_t, Int_t) {
+    init(p, n, bucket_size, TreeType, KernType, KernRes, Splitting, Aniso, ScaleSpace, Period);
+}
+
+KDTree::KDTree(std::vector<Particle> &p, Int_t bucket_size, int TreeType, int KernType, int KernRes, int Splitting,
+               int Aniso, int ScaleSpace, Double_t *Period, Double_t **, bool, bool, Double_t, Double_t, Int_t) {
+    init(p.data(), (Int_t)p.size(), bucket_size, TreeType, KernType, KernRes, Splitting, Aniso, ScaleSpace, Period);
+}
+
+KDTree::~KDTree() {
+    if (tree) nbx_tree_destroy(tree);
+    tree = nullptr;
+    if (iresetorder)
+        std::sort(bucket, bucket + numparts, [](const Particle &a, const Particle &c) { return a.GetID() < c.GetID(); });
+}
+
+void KDTree::OverWriteInputOrder() {
+    iresetorder = false;
+    for (Int_t i = 0; i < numparts; ++i) bucket[i].SetID(i);
+}
+void KDTree::SetResetOrder(bool a) { iresetorder = a; }
+
+Double_t KDTree::GetKernNorm() {
+    double kn = 0;
+    check(nbx_tree_kernel(tree, &kn, nullptr, 0), "GetKernNorm");
+    return kn;
+}
+
+// ------------------------------------------------------------------ kNN
+void KDTree::batch_knn(int variant, Int_t k, Int_t *nn, Double_t *d2) {
+    check(nbx_knn(tree, k, variant, NBX_LAYOUT_BY_SLOT, nn, d2), "FindNearest");
+}
+
+static void scatter_rows(const std::vector<Int_t> &nn, const std::vector<Double_t> &d2, Int_t n, Int_t k, Int_t **out,
+                         Double_t **dout) {
+    for (Int_t i = 0; i < n; ++i)
+        for (Int_t j = 0; j < k; ++j) {
+            out[i][j] = nn[(size_t)i * k + j];
+            dout[i][j] = d2[(size_t)i * k + j];
+        }
+}
+
+void KDTree::FindNearest(Int_t **nn, Double_t **dist2, Int_t Nsearch) {
+    std::lock_guard<std::mutex> lk(mu);
+    ensure_cache(treetype == TPHS ? NBX_KNN_FINDNEARESTPHASE : NBX_KNN_FINDNEAREST, Nsearch);
+    scatter_rows(cache_nn, cache_d2, numparts, Nsearch, nn, dist2);
+}
+void KDTree::FindNearestPos(Int_t **nn, Double_t **dist2, Int_t Nsearch) {
+    std::lock_guard<std::mutex> lk(mu);
+    ensure_cache(NBX_KNN_FINDNEARESTPOS, Nsearch);
+    scatter_rows(cache_nn, cache_d2, numparts, Nsearch, nn, dist2);
+}
+void KDTree::FindNearestPhase(Int_t **nn, Double_t **dist2, Int_t Nsearch) {
+    std::lock_guard<std::mutex> lk(mu);
+    ensure_cache(NBX_KNN_FINDNEARESTPHASE, Nsearch);
+    scatter_rows(cache_nn, cache_d2, numparts, Nsearch, nn, dist2);
+}
+
+void KDTree::ensure_cache(int variant, Int_t k) {
+    if (cache_variant == variant && cache_k == k) return;
+    cache_nn.assign((size_t)numparts * k, -1);
+    cache_d2.assign((size_t)numparts * k, 0.0);
+    batch_knn(variant, k, cache_nn.data(), cache_d2.data());
+    cache_variant = variant;
+    cache_k = k;
+}
+
+void KDTree::FindNearest(Int_t tt, Int_t *nn, Double_t *dist2, Int_t Nsearch) {
+    std::lock_guard<std::mutex> lk(mu);
+    ensure_cache(treetype == TPHS ? NBX_KNN_FINDNEARESTPHASE : NBX_KNN_FINDNEAREST, Nsearch);
+    std::copy(cache_nn.begin() + (size_t)tt * Nsearch, cache_nn.begin() + (size_t)(tt + 1) * Nsearch, nn);
+    std::copy(cache_d2.begin() + (size_t)tt * Nsearch, cache_d2.begin() + (size_t)(tt + 1) * Nsearch, dist2);
+}
+void KDTree::FindNearestPos(Int_t tt, Int_t *nn, Double_t *dist2, Int_t Nsearch) {
+    std::lock_guard<std::mutex> lk(mu);
+    ensure_cache(NBX_KNN_FINDNEARESTPOS, Nsearch);
+    std::copy(cache_nn.begin() + (size_t)tt * Nsearch, cache_nn.begin() + (size_t)(tt + 1) * Nsearch, nn);
+    std::copy(cache_d2.begin() + (size_t)tt * Nsearch, cache_d2.begin() + (size_t)(tt + 1) * Nsearch, dist2);
+}
+void KDTree::FindNearestPhase(Int_t tt, Int_t *nn, Double_t *dist2, Int_t Nsearch) {
+    std::lock_guard<std::mutex> lk(mu);
+    ensure_cache(NBX_KNN_FINDNEARESTPHASE, Nsearch);
+    std::copy(cache_nn.begin() + (size_t)tt * Nsearch, cache_nn.begin() + (size_t)(tt + 1) * Nsearch, nn);
+    std::copy(cache_d2.begin() + (size_t)tt * Nsearch, cache_d2.begin() + (size_t)(tt + 1) * Nsearch, dist2);
+}
+
+void KDTree::FindNearestPos(Double_t *x, Int_t *nn, Double_t *dist2, Int_t Nsearch) {
+    std::lock_guard<std::mutex> lk(mu);
+    double q[3] = {x[0], x[1], x[2]};
+    std::vector<int32_t> ids(Nsearch);
+    check(nbx_knn_points(tree, q, 1, Nsearch, ids.data(), dist2), "FindNearestPos");
+    for (Int_t j = 0; j < Nsearch; ++j) nn[j] = ids[j] >= 0 ? slot_of_id[ids[j]] : -1;
+}
+
+// ------------------------------------------------------------------ ball search
+std::vector<Int_t> KDTree::SearchBallPosTagged(Double_t *x, Double_t fdist2) {
+    std::lock_guard<std::mutex> lk(mu);
+    int64_t cap = numparts, nout = 0;
+    std::vector<int32_t> ids;
+    for (;;) {
+        ids.resize(cap);
+        check(nbx_ball_points(tree, x, fdist2, ids.data(), cap, &nout), "SearchBallPosTagged");
+        if (nout <= cap) break;
+        cap = nout;
+    }
+    std::vector<Int_t> out(nout);
+    for (int64_t i = 0; i < nout; ++i) out[i] = slot_of_id[ids[i]];
+    return out;
+}
+Int_t KDTree::SearchBallPosTagged(Double_t *x, Double_t fdist2, Int_t *tagged) {
+    std::vector<Int_t> v = SearchBallPosTagged(x, fdist2);
+    std::copy(v.begin(), v.end(), tagged);
+    return (Int_t)v.size();
+}
+Int_t KDTree::SearchBallPosTagged(Int_t tt, Double_t fdist2, Int_t *tagged) {
+    Double_t x[3] = {bucket[tt].GetPosition(0), bucket[tt].GetPosition(1), bucket[tt].GetPosition(2)};
+    return SearchBallPosTagged(x, fdist2, tagged);
+}
+
+// ------------------------------------------------------------------ densities
+void KDTree::CalcDensity(Int_t Nsmooth) {
+    std::lock_guard<std::mutex> lk(mu);
+    std::vector<double> rho(numparts);
+    check(nbx_density(tree, Nsmooth, NBX_DENS_SYMMETRIC, 0, rho.data(), nullptr), "CalcDensity failed");
+    for (Int_t s = 0; s < numparts; ++s) bucket[s].SetDensity(rho[bucket[s].GetID()]);
+}
+void KDTree::CalcVelDensity(Int_t Nsmooth, Int_t Nsearch) {
+    std::lock_guard<std::mutex> lk(mu);
+    std::vector<double> rho(numparts);
+    check(nbx_density(tree, Nsmooth, NBX_DENS_VEL, Nsearch, rho.data(), nullptr), "CalcVelDensity failed");
+    for (Int_t s = 0; s < numparts; ++s) bucket[s].SetDensity(rho[bucket[s].GetID()]);
+}
+Double_t KDTree::CalcDensityParticle(Int_t target, Int_t Nsmooth) {
+    std::lock_guard<std::mutex> lk(mu);
+    if (dcache_k != Nsmooth) {
+        dcache.assign(numparts, 0.0);
+        check(nbx_density(tree, Nsmooth, NBX_DENS_GATHER, 0, dcache.data(), nullptr), "CalcDensity failed");
+        dcache_k = Nsmooth;
+    }
+    return dcache[bucket[target].GetID()];
+}
+
+// ------------------------------------------------------------------ FOF
+Int_t *KDTree::FOF(Double_t fdist, Int_t &numgroup, Int_t minnum, int, Int_tree_t *, Int_tree_t *, Int_tree_t *,
+                   Int_tree_t *) {
+    std::lock_guard<std::mutex> lk(mu);
+    Int_t *pGroup = new Int_t[numparts];
+    int64_t ng = 0;
+    check(nbx_fof(tree, fdist, minnum, pGroup, &ng), "FOF");
+    numgroup = (Int_t)ng;
+    return pGroup;
+}
+
+}  // namespace NBody

@@ -1,0 +1,122 @@
+"""GPU parity: 6-D phase space, velocity density, position queries and the
+differentiation form (adjoint)."""
+import numpy as np
+import pytest
+
+import oracle
+from nbodylib_amd import datagen
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def _phase_set(n, seed):
+    pos, vel, _ = datagen.clustered(n, 100.0, nhalo=max(4, n // 2000), seed=seed, with_vel=True)
+    return pos / pos.std(0), vel / vel.std(0)
+
+
+@pytest.mark.parametrize("k", [32, 64])
+def test_phase_knn_vs_brute(gpu, k):
+    import nbodylib_amd as nb
+
+    n = 4000
+    say(f"phase knn n={n} k={k}")
+    x, v = _phase_set(n, 31)
+    t = nb.KDTree(x, vel=v, treetype=nb.TPHS, aniso=-1, kerntype=nb.KSPH)
+    nn, d2 = t.FindNearestPhase(k)
+    bi, bd = oracle.knn_brute(x, k, mode=0, vel=v, nd=6)
+    assert np.array_equal(np.sort(nn, 1), np.sort(bi, 1))
+    assert np.array_equal(d2, bd)
+
+
+def test_phase_density_vs_oracle(gpu):
+    """TPHS CalcDensity with the metric factor taken as 1 (6-D norm 3024/(762 pi^3))."""
+    import nbodylib_amd as nb
+
+    n, k = 20000, 64
+    say(f"phase density n={n} k={k}")
+    x, v = _phase_set(n, 32)
+    m = datagen.lognormal_masses(n, 32)
+    t = nb.KDTree(x, mass=m, vel=v, treetype=nb.TPHS, aniso=-1, kerntype=nb.KSPH)
+    rho = t.CalcDensity(k)
+    ro = oracle.Tree(x, mass=m, vel=v, b=16, treetype=oracle.TPHS, kerntype=oracle.KSPH).calc_density(k)
+    assert np.max(np.abs(rho / ro - 1.0)) < 1e-12
+
+
+def test_vel_density_vs_oracle(gpu):
+    import nbodylib_amd as nb
+
+    n = 20000
+    say(f"vel density n={n}")
+    pos, vel, _ = datagen.clustered(n, 100.0, nhalo=10, seed=33, with_vel=True)
+    t = nb.KDTree(pos, vel=vel, kerntype=nb.KSPH)
+    rv = t.CalcVelDensity(32, 64)
+    ro = oracle.Tree(pos, vel=vel, b=16, kerntype=oracle.KSPH).calc_veldensity(32, 64)
+    assert np.max(np.abs(rv / ro - 1.0)) < 1e-12
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_knn_points_vs_oracle(gpu, periodic):
+    import nbodylib_amd as nb
+
+    n, m, k = 5000, 777, 16
+    say(f"knn points periodic={periodic}")
+    pos = datagen.uniform(n, 1.0, 34)
+    q = datagen.uniform(m, 1.0, 35)
+    per = 1.0 if periodic else None
+    t = nb.KDTree(pos, kerntype=nb.KSPH, period=per)
+    nn, d2 = t.FindNearestPosPoints(q, k)
+    oi, od = oracle.Tree(pos, b=16, period=per).knn_points(q, k)
+    assert np.array_equal(np.sort(nn, 1), np.sort(oi, 1))
+    assert np.array_equal(d2, od)
+
+
+def test_adjoint_gradcheck_and_reference(gpu):
+    """Differentiation form at N=4096 in FP64: torch.autograd.gradcheck with
+    central differences (eps 1e-6) on a subset of coordinates, and the full
+    gradient against the oracle's hand-derived FP64 adjoint."""
+    import torch
+
+    from nbodylib_amd.device import SPHDensity, sph_density
+
+    n, k = 4096, 32
+    say(f"adjoint n={n} k={k}")
+    pos = datagen.uniform(n, 1.0, 36)
+    mass = datagen.lognormal_masses(n, 36)
+    nbr, _ = oracle.knn_brute(pos, k, mode=0)
+    dev = torch.device("cuda:0")
+    P = torch.tensor(np.ascontiguousarray(pos.T), device=dev, requires_grad=True)
+    M = torch.tensor(mass, device=dev, requires_grad=True)
+    N = torch.tensor(nbr, device=dev)
+    rho = SPHDensity.apply(P, M, N)
+    ro, _ = oracle.density_analytic(pos, mass, nbr)
+    assert np.max(np.abs(rho.detach().cpu().numpy() / ro - 1)) < 1e-12
+    loss = torch.log(rho).sum()
+    loss.backward()
+    dpos_ref, dm_ref = oracle.density_adjoint(pos, mass, nbr, 1.0 / ro)
+    gp = P.grad.cpu().numpy().T
+    gm = M.grad.cpu().numpy()
+    scale = np.max(np.abs(dpos_ref))
+    assert np.max(np.abs(gp - dpos_ref)) < 1e-9 * scale
+    assert np.max(np.abs(gm - dm_ref)) < 1e-9 * np.max(np.abs(dm_ref))
+
+    # gradcheck on a small slice: perturb 24 particles' coordinates and masses
+    idx = torch.tensor(np.random.default_rng(0).choice(n, 24, replace=False), device=dev)
+    base_p = P.detach().clone()
+    base_m = M.detach().clone()
+
+    def f(sub_p, sub_m):
+        p = base_p.clone()
+        p[:, idx] = sub_p
+        mm = base_m.clone()
+        mm[idx] = sub_m
+        return torch.log(SPHDensity.apply(p.contiguous(), mm.contiguous(), N)).sum().reshape(1)
+
+    sp = base_p[:, idx].clone().requires_grad_(True)
+    sm = base_m[idx].clone().requires_grad_(True)
+    # the adjoint accumulates with FP64 atomics, so repeated backward passes
+    # differ in the last bits (summation order): allow that, not more
+    assert torch.autograd.gradcheck(f, (sp, sm), eps=1e-6, atol=1e-5, rtol=1e-4, nondet_tol=1e-12)

@@ -1,0 +1,51 @@
+"""The C++ drop-in (NBody::KDTree over libnbx.so) driven by a compiled
+program, checked against the CPU oracle."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+from nbodylib_amd import datagen
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "tests", "cpp", "test_kdtree_shim")
+
+
+@pytest.mark.parametrize("periodic", [0, 1])
+def test_cpp_shim(gpu, tmp_path, periodic):
+    if not os.path.exists(EXE):
+        pytest.fail("tests/cpp/test_kdtree_shim is not built (run __graft_entry__.build())")
+    n, k = 20000, 32
+    pos = datagen.uniform(n, 1.0, 41)
+    vel = np.random.default_rng(41).normal(size=(n, 3))
+    m = datagen.lognormal_masses(n, 41)
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    with open(fin, "wb") as f:
+        np.array([n], dtype=np.int32).tofile(f)
+        np.concatenate([pos, vel, m[:, None]], axis=1).astype(np.float64).tofile(f)
+    b = 0.6 * datagen.mean_spacing(n, 1.0)
+    print(f"shim periodic={periodic}", flush=True)
+    r = subprocess.run([EXE, str(fin), str(fout), str(k), repr(b), str(periodic), "1.0"], capture_output=True,
+                       text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    with open(fout, "rb") as f:
+        nn_, kk = np.fromfile(f, np.int32, 2)
+        ids = np.fromfile(f, np.int32, n * k).reshape(n, k)
+        rho = np.fromfile(f, np.float64, n)
+        lab = np.fromfile(f, np.int32, n)
+        flags = np.fromfile(f, np.int32, 4)
+    assert flags[0] == 0, "per-particle FindNearest differs from the batch"
+    assert flags[1] == 0, "particle fields lost in the reorder"
+    assert flags[2] == 0, "input order not restored by ~KDTree"
+    per = 1.0 if periodic else None
+    ot = oracle.Tree(pos, mass=m, b=16, kerntype=oracle.KSPH, period=per)
+    oi, _ = ot.knn(k, 0)
+    assert np.array_equal(np.sort(ids, 1), np.sort(oi, 1))
+    ro = ot.calc_density(k)
+    assert np.max(np.abs(rho / ro - 1)) < 1e-12
+    ol, og = oracle.fof_brute(pos, b, 8, period=per)
+    assert flags[3] == og and np.array_equal(lab, ol)
