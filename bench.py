@@ -45,7 +45,22 @@ def main():
     ap.add_argument("--no-fof", action="store_true")
     ap.add_argument("--cpu-stride", type=int, default=2, help="time every s-th tree slot on the CPU")
     ap.add_argument("--forms", default="plain,mixed", help="forms to time (plain, mixed)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB (calibrates rocprof byte counters) and exit")
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic.json"),
+                    help="measured HBM bytes per launch of the dominant kernel (from rocprofv3 --pmc runs)")
     args = ap.parse_args()
+
+    if args.copy_probe:
+        import torch
+
+        src = torch.ones(args.copy_probe * (1 << 20) // 4, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_mib": args.copy_probe, "launches": 3}), flush=True)
+        return
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -144,9 +159,16 @@ def main():
     bytes_per_q = 48 + 4 * k
     kern_ms = headline["kms"] / max(headline["kl"], 1)
     achieved = bytes_per_q * n / (kern_ms / 1e3) / 1e9
+    kname = "k_knn_mixed" if headline["form"] == "mixed" else "k_knn_fp64"
+    traffic = None
+    if os.path.exists(args.traffic):
+        tj = json.load(open(args.traffic))
+        ent = tj.get(kname)
+        if ent and ent.get("n") == n and ent.get("k") == k:
+            traffic = ent["bytes_per_launch"]
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                "frac": round(achieved / PEAK_HBM_GBS, 5), "traffic": None,
-                "kernel": "k_knn_mixed" if headline["form"] == "mixed" else "k_knn_fp64", "kernel_ms": round(kern_ms, 3), "bytes_per_query": bytes_per_q}
+                "frac": round(achieved / PEAK_HBM_GBS, 5), "traffic": traffic,
+                "kernel": kname, "kernel_ms": round(kern_ms, 3), "bytes_per_query": bytes_per_q}
 
     cpu = None
     if not args.no_cpu:
