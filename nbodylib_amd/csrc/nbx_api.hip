@@ -106,6 +106,7 @@ struct nbx_tree {
     double *xs[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     double *mass = nullptr;
     int32_t *perm = nullptr;
+    int32_t *inv = nullptr;
     double *box = nullptr;
     double *org = nullptr;
     float *xf[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -127,6 +128,7 @@ struct nbx_tree {
         for (int d = 0; d < 6; ++d) { T.x[d] = xs[d]; T.xf[d] = xf[d]; }
         T.mass = mass;
         T.perm = perm;
+        T.inv = inv;
         T.box = box;
         T.org = org;
         T.kern = kern;
@@ -182,6 +184,7 @@ static void free_tree(nbx_tree *t) {
     for (int d = 0; d < 6; ++d) { if (t->xs[d]) hipFree(t->xs[d]); if (t->xf[d]) hipFree(t->xf[d]); }
     if (t->mass) hipFree(t->mass);
     if (t->perm) hipFree(t->perm);
+    if (t->inv) hipFree(t->inv);
     if (t->box) hipFree(t->box);
     if (t->org) hipFree(t->org);
     if (t->kern) hipFree(t->kern);
@@ -248,6 +251,7 @@ static int build_tree(nbx_tree *t, const double *pos, const double *vel, const d
     for (int d = 0; d < ncoord; ++d) HIPCK(hipMalloc((void **)&t->xs[d], sizeof(double) * npad));
     HIPCK(hipMalloc((void **)&t->mass, sizeof(double) * npad));
     HIPCK(hipMalloc((void **)&t->perm, sizeof(int32_t) * npad));
+    HIPCK(hipMalloc((void **)&t->inv, sizeof(int32_t) * n));
     {
         ProfScope ps("k_build_gather", st);
         GatherArgs g;
@@ -261,7 +265,7 @@ static int build_tree(nbx_tree *t, const double *pos, const double *vel, const d
         g.mass_default = 1.0 / (double)n;
         g.mass_out = t->mass;
         g.perm = t->perm;
-        g.inv = nullptr;
+        g.inv = t->inv;
         hipLaunchKernelGGL(k_gather, dim3(nblk(npad)), dim3(256), 0, st, g);
     }
     HIPCK(hipFreeAsync(keys, st));

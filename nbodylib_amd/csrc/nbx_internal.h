@@ -23,6 +23,8 @@
 
 typedef double dx8 __attribute__((ext_vector_type(8)));
 typedef const __attribute__((address_space(4))) dx8 *cd8ptr;
+typedef double dx4 __attribute__((ext_vector_type(4)));
+typedef const __attribute__((address_space(4))) dx4 *cd4ptr;
 
 typedef const __attribute__((address_space(4))) double *cdptr;  // scalar-cache (read-only) view
 typedef const __attribute__((address_space(4))) float *cfptr;
@@ -37,6 +39,7 @@ struct DevTree {
     const double *x[6];    // sorted coordinates (x y z [vx vy vz])
     const double *mass;    // sorted masses
     const int32_t *perm;   // slot -> ID
+    const int32_t *inv;    // ID -> slot
     const double *box;     // node boxes, [node][lo[nd], hi[nd]]
     const double *org;     // mixed form: per-leaf FP64 origin [leaf][nd]
     const float *xf[6];    // mixed form: FP32 offsets from the leaf origin
@@ -116,6 +119,16 @@ __device__ __forceinline__ double boxbox_lb(cdptr a, cdptr b) {
         s += t * t;
     }
     return s;
+}
+
+// Workgroups of a launch are dealt round-robin to the 8 XCDs (workgroup b
+// runs on XCD b % 8), each with its own L2.  Renumber them so that XCD x
+// walks one contiguous run of Morton-ordered waves: concurrently resident
+// waves on an XCD then share tree nodes and leaves in that XCD's L2 instead
+// of every L2 fetching every neighbourhood.  A bijection for any grid size.
+__device__ __forceinline__ int xcd_wave(int b, int nb) {
+    const int q = nb >> 3, r = nb & 7, x = b & 7, i = b >> 3;
+    return x * q + (x < r ? x : r) + i;
 }
 
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p ? 1 : 0); }

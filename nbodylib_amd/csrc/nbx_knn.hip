@@ -27,56 +27,136 @@
 // (the reference's choice there depends on its traversal order).
 #include "nbx_kernels.h"
 
-__device__ __forceinline__ int32_t sid(const int32_t *perm, int32_t s) { return s < 0 ? 0x7fffffff : perm[s]; }
+// Heap entries are (d2, particle ID) pairs ordered by d2, then ID; the ID is
+// carried in the heap so ties never need a lookup.  Empty entries are
+// (NBX_MAXVALUE, NBX_NOID); filler entries (-1, -1) pad the heap to a
+// capacity of 1 (mod 4) and, being smaller than every key, are never evicted.
+#define NBX_NOID 0x7fffffff
 
-// (da, ID(sa)) > (db, ID(sb))
-__device__ __forceinline__ bool key_gt(double da, int32_t sa, double db, int32_t sb, const int32_t *perm) {
-    if (da > db) return true;
-    if (da < db) return false;
-    if (sa == sb) return false;
-    return sid(perm, sa) > sid(perm, sb);
+// (da, ia) > (db, ib)
+__device__ __forceinline__ bool kgt(double da, int32_t ia, double db, int32_t ib) {
+    return da > db || (da == db && ia > ib);
 }
 
 #define HD(p) hd[(p) * NBX_WAVE + lane]
-#define HS(p) hs[(p) * NBX_WAVE + lane]
+#define HI(p) hid[(p) * NBX_WAVE + lane]
 
-// 4-ary max-heap, 0-based: children of p are 4p+1 .. 4p+4.  Place (dv, sv)
-// at position p of a heap of size kh and sift it down.
-__device__ __forceinline__ void heap_sift(double *hd, int32_t *hs, int lane, int kh, int p, double dv, int32_t sv,
-                                          const int32_t *perm) {
+// Heap capacity for kh kept entries: kh rounded up to 1 (mod 4), so every
+// parent of a full 4-ary heap has four children.
+__host__ __device__ __forceinline__ int heap_cap4(int kh) { return ((kh + 2) & ~3) + 1; }
+
+// Largest of four children (and its index 0..3).
+__device__ __forceinline__ void max4(const double *d, const int32_t *id, double &bd, int32_t &bi, int &bx) {
+    const bool s01 = kgt(d[1], id[1], d[0], id[0]), s23 = kgt(d[3], id[3], d[2], id[2]);
+    const double m01 = s01 ? d[1] : d[0], m23 = s23 ? d[3] : d[2];
+    const int32_t j01 = s01 ? id[1] : id[0], j23 = s23 ? id[3] : id[2];
+    const bool s = kgt(m23, j23, m01, j01);
+    bd = s ? m23 : m01;
+    bi = s ? j23 : j01;
+    bx = s ? (s23 ? 3 : 2) : (s01 ? 1 : 0);
+}
+
+// 4-ary max-heap, 0-based, children of p at 4p+1 .. 4p+4 (the reference's
+// PriorityQueue, PriorityQueue.h:47-84, with three dependent levels instead
+// of five for k = 32).  Replace the root of the full heap of hc entries by
+// (dv, iv) for the lanes with ins set and sift it down; the other lanes
+// rewrite the root with itself.  Three unrolled levels cover hc <= 85 and
+// nothing branches.  (kth, kid) is the root afterwards.
+__device__ __forceinline__ void heap_replace_root(double *hd, int32_t *hid, int lane, int hc, bool ins, double dn,
+                                                  int32_t in, double &kth, int32_t &kid) {
+    const double dv = ins ? dn : kth;
+    const int32_t iv = ins ? in : kid;
+    double td = dv;
+    int32_t ti = iv;
+    int p = 0;
+    bool go = ins;
+#pragma unroll
+    for (int lv = 0; lv < 3; ++lv) {
+        const int c = 4 * p + 1;
+        go = go && c < hc;
+        const int cc = go ? c : 1;
+        double d[4];
+        int32_t id[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) { d[u] = HD(cc + u); id[u] = HI(cc + u); }
+        double bd;
+        int32_t bi;
+        int bx;
+        max4(d, id, bd, bi, bx);
+        go = go && kgt(bd, bi, dv, iv);
+        HD(p) = go ? bd : dv;
+        HI(p) = go ? bi : iv;
+        if (lv == 0) { td = go ? bd : dv; ti = go ? bi : iv; }
+        p = go ? cc + bx : p;
+    }
+    HD(p) = dv;
+    HI(p) = iv;
+    kth = td;
+    kid = ti;
+}
+
+// Sift (dv, iv) down from the root of a heap of m <= hc entries; children
+// at positions >= m read as (-inf, min) and never outrank an entry.
+__device__ __forceinline__ void heap_sift_bounded(double *hd, int32_t *hid, int lane, int m, double dv, int32_t iv) {
+    int p = 0;
+    bool go = true;
+#pragma unroll
+    for (int lv = 0; lv < 3; ++lv) {
+        const int c = 4 * p + 1;
+        go = go && c < m;
+        const int cc = go ? c : 1;
+        double d[4];
+        int32_t id[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const double t = HD(cc + u);
+            const int32_t ti = HI(cc + u);
+            const bool in = cc + u < m;
+            d[u] = in ? t : -INFINITY;
+            id[u] = in ? ti : (int32_t)0x80000000;
+        }
+        double bd;
+        int32_t bi;
+        int bx;
+        max4(d, id, bd, bi, bx);
+        go = go && kgt(bd, bi, dv, iv);
+        HD(p) = go ? bd : dv;
+        HI(p) = go ? bi : iv;
+        p = go ? cc + bx : p;
+    }
+    HD(p) = dv;
+    HI(p) = iv;
+}
+
+// General sift from position p in a heap of size m (short selection of the
+// velocity density only).
+__device__ __forceinline__ void heap_sift(double *hd, int32_t *hid, int lane, int m, int p, double dv, int32_t iv) {
     for (;;) {
         int c = 4 * p + 1;
-        if (c >= kh) break;
+        if (c >= m) break;
         double dc = HD(c);
-        int32_t sc = HS(c);
+        int32_t ic = HI(c);
         int best = c;
-#pragma unroll
         for (int u = 1; u < 4; ++u) {
-            if (c + u < kh) {
-                double du = HD(c + u);
-                int32_t su = HS(c + u);
-                if (key_gt(du, su, dc, sc, perm)) { best = c + u; dc = du; sc = su; }
-            }
+            if (c + u < m && kgt(HD(c + u), HI(c + u), dc, ic)) { best = c + u; dc = HD(c + u); ic = HI(c + u); }
         }
-        if (!key_gt(dc, sc, dv, sv, perm)) break;
+        if (!kgt(dc, ic, dv, iv)) break;
         HD(p) = dc;
-        HS(p) = sc;
+        HI(p) = ic;
         p = best;
     }
     HD(p) = dv;
-    HS(p) = sv;
+    HI(p) = iv;
 }
 
-// Heap sort in place: positions 0..kh-1 ascending afterwards.
-__device__ __forceinline__ void heap_sort(double *hd, int32_t *hs, int lane, int kh, const int32_t *perm) {
-    for (int m = kh - 1; m > 0; --m) {
-        double dt = HD(0);
-        int32_t st = HS(0);
-        double dm = HD(m);
-        int32_t sm = HS(m);
+// Heap sort in place: positions 0..hc-1 ascending afterwards (fillers first).
+__device__ __forceinline__ void heap_sort(double *hd, int32_t *hid, int lane, int hc) {
+    for (int m = hc - 1; m > 0; --m) {
+        const double dt = HD(0), dm = HD(m);
+        const int32_t it = HI(0), im = HI(m);
         HD(m) = dt;
-        HS(m) = st;
-        heap_sift(hd, hs, lane, m, 0, dm, sm, perm);
+        HI(m) = it;
+        heap_sift_bounded(hd, hid, lane, m, dm, im);
     }
 }
 
@@ -85,27 +165,33 @@ struct LeafCtx {
     double q[ND];
     bool act;
     bool exz;
-    double kth;
-    int32_t kslot;
+    double kth;   // root of the heap: the kh-th smallest (d2, ID) so far
+    int32_t kid;
 };
 
 // Consume one leaf for every active lane (two-phase insertion, see header).
 template <int ND>
-__device__ __forceinline__ void knn_leaf(const DevTree &T, int leaf, LeafCtx<ND> &c, double *hd, int32_t *hs,
-                                         double *cx, int lane, int kh) {
+__device__ __forceinline__ void knn_leaf(const DevTree &T, int leaf, LeafCtx<ND> &c, double *hd, int32_t *hid,
+                                         double *cx, int32_t *cid, int lane, int hc) {
     const int base = leaf * NBX_LEAF;
     if (lane < NBX_LEAF) {
 #pragma unroll
         for (int d = 0; d < ND; ++d) cx[d * NBX_LEAF + lane] = T.x[d][base + lane];
+        cid[lane] = T.perm[base + lane];
     }
     uint32_t mask = 0;
+    // candidates arrive in chunks through the scalar cache: 8 per dimension
+    // (24 SGPR pairs) in 3-D, 4 in 6-D
+    constexpr int CH = ND == 6 ? 4 : 8;
+    typedef double dxc __attribute__((ext_vector_type(CH)));
+    typedef const __attribute__((address_space(4))) dxc *cdcptr;
 #pragma unroll
-    for (int jj = 0; jj < NBX_LEAF; jj += 8) {
-        dx8 cv[ND];
+    for (int jj = 0; jj < NBX_LEAF; jj += CH) {
+        dxc cv[ND];
 #pragma unroll
-        for (int d = 0; d < ND; ++d) cv[d] = *(cd8ptr)(T.x[d] + base + jj);
+        for (int d = 0; d < ND; ++d) cv[d] = *(cdcptr)(T.x[d] + base + jj);
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
+        for (int u = 0; u < CH; ++u) {
             double d2 = 0.0;
 #pragma unroll
             for (int d = 0; d < ND; ++d) {
@@ -122,22 +208,17 @@ __device__ __forceinline__ void knn_leaf(const DevTree &T, int leaf, LeafCtx<ND>
     if (!c.act) mask = 0;
     __syncthreads();
     while (ballot(mask != 0)) {
-        if (mask) {
-            int j = __builtin_ctz(mask);
-            mask &= mask - 1;
-            int32_t cs = base + j;
-            double d2 = 0.0;
+        const bool has = mask != 0;
+        const int j = __builtin_ctz(mask | 0x80000000u);
+        mask &= mask - 1;
+        double d2 = 0.0;
 #pragma unroll
-            for (int d = 0; d < ND; ++d) {
-                double t = c.q[d] - cx[d * NBX_LEAF + j];
-                d2 += t * t;
-            }
-            if (key_gt(c.kth, c.kslot, d2, cs, T.perm)) {
-                heap_sift(hd, hs, lane, kh, 0, d2, cs, T.perm);
-                c.kth = HD(0);
-                c.kslot = HS(0);
-            }
+        for (int d = 0; d < ND; ++d) {
+            double t = c.q[d] - cx[d * NBX_LEAF + j];
+            d2 += t * t;
         }
+        const int32_t id = cid[j];
+        heap_replace_root(hd, hid, lane, hc, has && kgt(c.kth, c.kid, d2, id), d2, id, c.kth, c.kid);
     }
     __syncthreads();
 }
@@ -170,7 +251,7 @@ __device__ __forceinline__ bool first_is_right(const DevTree &T, int c0, const d
 // depth.
 template <int ND>
 __device__ __forceinline__ void knn_walk(const DevTree &T, LeafCtx<ND> &c, const double *qc, int anc, double *hd,
-                                         int32_t *hs, double *cx, int lane, int kh) {
+                                         int32_t *hid, double *cx, int32_t *cid, int lane, int hc) {
     const int D = T.depth;
     int node = 1, dep = 0;
     uint64_t firstright = 0;
@@ -181,7 +262,7 @@ __device__ __forceinline__ void knn_walk(const DevTree &T, LeafCtx<ND> &c, const
             bool want = c.act && box_lb<ND>(c.q, bx) <= c.kth;
             if (ballot(want)) {
                 if (dep == D) {
-                    knn_leaf<ND>(T, node - T.nleafp, c, hd, hs, cx, lane, kh);
+                    knn_leaf<ND>(T, node - T.nleafp, c, hd, hid, cx, cid, lane, hc);
                     down = false;
                 } else {
                     int c0 = 2 * node;
@@ -242,11 +323,14 @@ __device__ __forceinline__ bool image_pass(int pass, const double *x0, const dou
 template <int ND, int KMAX, int QSRC>
 __global__ __launch_bounds__(64) void k_knn(DevTree T, KnnArgs A) {
     __shared__ double hd[KMAX * NBX_WAVE];
-    __shared__ int32_t hs[KMAX * NBX_WAVE];
+    __shared__ int32_t hid[KMAX * NBX_WAVE];
     __shared__ double cx[ND * NBX_LEAF];
+    __shared__ int32_t cid[NBX_LEAF];
     const int lane = threadIdx.x;
-    const int g = blockIdx.x;
+    const int g = xcd_wave(blockIdx.x, gridDim.x);
     const int kh = A.kh;
+    const int hc = heap_cap4(kh);
+    const int nfill = hc - kh;  // filler entries, first after sorting
     const bool listed = (QSRC == NBX_Q_SLOTS) && A.qlist != nullptr;
     const int nq = (QSRC == NBX_Q_POINTS || listed) ? A.m : T.n;
     const int gi = g * NBX_WAVE + lane;
@@ -260,14 +344,17 @@ __global__ __launch_bounds__(64) void k_knn(DevTree T, KnnArgs A) {
 #pragma unroll
         for (int d = 0; d < ND; ++d) q0[d] = valid ? T.x[d][qi] : 0.0;
     }
-    for (int p = 0; p < kh; ++p) {
-        HD(p) = NBX_MAXVALUE;
-        HS(p) = -1;
+    // the fillers sit at the bottom of the heap array, where a 4-ary heap of
+    // empty entries stays valid with them as leaves
+    for (int p = 0; p < hc; ++p) {
+        const bool fill = p >= kh;
+        HD(p) = fill ? -1.0 : NBX_MAXVALUE;
+        HI(p) = fill ? -1 : NBX_NOID;
     }
     LeafCtx<ND> c;
     c.exz = (QSRC == NBX_Q_SLOTS) && A.excl_zero;
     c.kth = NBX_MAXVALUE;
-    c.kslot = -1;
+    c.kid = NBX_NOID;
     const int npass = (QSRC != NBX_Q_SLOTS && A.periodic) ? 8 : 1;
     for (int pass = 0; pass < npass; ++pass) {
 #pragma unroll
@@ -283,7 +370,7 @@ __global__ __launch_bounds__(64) void k_knn(DevTree T, KnnArgs A) {
 #pragma unroll
         for (int d = 0; d < ND; ++d) qc[d] = readfirst_d(c.q[d]);
         const int anc = (pass == 0 && QSRC != NBX_Q_POINTS && !listed) ? T.nleafp + 2 * g : -1;
-        knn_walk<ND>(T, c, qc, anc, hd, hs, cx, lane, kh);
+        knn_walk<ND>(T, c, qc, anc, hd, hid, cx, cid, lane, hc);
     }
     const double kth = c.kth;
 
@@ -293,19 +380,23 @@ __global__ __launch_bounds__(64) void k_knn(DevTree T, KnnArgs A) {
         if (A.dens == NBX_DO_VEL) {
             // CalcVelDensity: |dv| of the Nsearch physical neighbours, the
             // Nsmooth smallest of them, h_v = max/2, sum W(|dv|/h_v) h_v^-3.
+            // The kh entries move to positions 0..kh-1 first (sorted order).
             const int ns = A.nsmooth;
+            heap_sort(hd, hid, lane, hc);
             for (int p = 0; p < kh; ++p) {
-                int32_t s = HS(p);
-                if (s < 0) { bad = true; break; }
+                const int32_t id = HI(p + nfill);
+                if (id == NBX_NOID) { bad = true; break; }
+                const int32_t s = T.inv[id];
                 double vx = T.x[3][qi] - T.x[3][s], vy = T.x[4][qi] - T.x[4][s], vz = T.x[5][qi] - T.x[5][s];
                 HD(p) = sqrt(vx * vx + vy * vy + vz * vz);
+                HI(p) = id;
             }
             if (!bad) {
-                for (int p = (ns - 2) / 4; p >= 0; --p) heap_sift(hd, hs, lane, ns, p, HD(p), HS(p), T.perm);
+                for (int p = (ns - 2) / 4; p >= 0; --p) heap_sift(hd, hid, lane, ns, p, HD(p), HI(p));
                 for (int p = ns; p < kh; ++p) {
-                    double dv = HD(p);
-                    int32_t sv = HS(p);
-                    if (key_gt(HD(0), HS(0), dv, sv, T.perm)) heap_sift(hd, hs, lane, ns, 0, dv, sv, T.perm);
+                    const double dv = HD(p);
+                    const int32_t iv = HI(p);
+                    if (kgt(HD(0), HI(0), dv, iv)) heap_sift(hd, hid, lane, ns, 0, dv, iv);
                 }
                 double hv = 0.5 * HD(0);
                 double norm = 1.0 / pow(hv, 3.0);
@@ -319,9 +410,11 @@ __global__ __launch_bounds__(64) void k_knn(DevTree T, KnnArgs A) {
             double norm = 1.0 / pow(hi, (double)ND);
             double mi = T.mass[qi];
             double acc = 0.0;
-            for (int p = 0; p < kh; ++p) {
-                int32_t s = HS(p);
-                if (s < 0) { bad = true; continue; }
+            for (int p = 0; p < hc; ++p) {
+                const int32_t id = HI(p);
+                if (id < 0) continue;  // filler
+                if (id == NBX_NOID) { bad = true; continue; }
+                const int32_t s = T.inv[id];
                 double rij = sqrt(HD(p));
                 double W = wsm_at(rij, hi, T.kern, T.kernres);
                 if (A.dens == NBX_DO_SYM) {
@@ -342,10 +435,11 @@ __global__ __launch_bounds__(64) void k_knn(DevTree T, KnnArgs A) {
 
     // ------------------------------------------------------------ outputs
     if (A.nn == nullptr && A.d2 == nullptr) return;
-    if (valid) heap_sort(hd, hs, lane, kh, T.perm);
+    // (the velocity density sorted already and reused the entries)
+    if (valid && A.dens != NBX_DO_VEL) heap_sort(hd, hid, lane, hc);
     __syncthreads();
     const int kout = A.kout;
-    const int off = (A.drop == NBX_DROP_SMALLEST) ? 1 : 0;
+    const int off = ((A.drop == NBX_DROP_SMALLEST) ? 1 : 0) + (A.dens == NBX_DO_VEL ? 0 : nfill);
     const int total = NBX_WAVE * kout;
     for (int e = lane; e < total; e += NBX_WAVE) {
         int r = e / kout, t = e - r * kout;
@@ -355,10 +449,10 @@ __global__ __launch_bounds__(64) void k_knn(DevTree T, KnnArgs A) {
         int64_t row;
         if (QSRC == NBX_Q_POINTS || A.layout == 1) row = qr;
         else row = T.perm[qr];
-        int32_t s = hs[(t + off) * NBX_WAVE + r];
+        const int32_t id = hid[(t + off) * NBX_WAVE + r];
         if (A.nn) {
-            int32_t v = s;
-            if (s >= 0 && (QSRC == NBX_Q_POINTS || A.layout == 0)) v = T.perm[s];
+            int32_t v = -1;
+            if (id != NBX_NOID) v = (QSRC == NBX_Q_POINTS || A.layout == 0) ? id : T.inv[id];
             A.nn[row * kout + t] = v;
         }
         if (A.d2) A.d2[row * kout + t] = hd[(t + off) * NBX_WAVE + r];
@@ -383,8 +477,7 @@ hipError_t nbx_launch_knn(const DevTree &T, const KnnArgs &A, int nqueries, hipS
         if (A.kh <= 65) launch_q<6, 65>(T, A, blocks, st);
         else return hipErrorInvalidValue;
     } else {
-        if (A.kh <= 32) launch_q<3, 32>(T, A, blocks, st);
-        else if (A.kh <= 33) launch_q<3, 33>(T, A, blocks, st);
+        if (A.kh <= 33) launch_q<3, 33>(T, A, blocks, st);
         else if (A.kh <= 65) launch_q<3, 65>(T, A, blocks, st);
         else return hipErrorInvalidValue;
     }

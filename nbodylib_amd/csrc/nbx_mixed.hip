@@ -51,25 +51,67 @@ __device__ __forceinline__ int32_t keys(uint64_t k) { return (int32_t)(uint32_t)
 
 #define HK(p) hk[(p) * NBX_WAVE + lane]
 
-__device__ __forceinline__ void kheap_sift(uint64_t *hk, int lane, int kh, int p, uint64_t v) {
-    for (;;) {
-        int c = 4 * p + 1;
-        if (c >= kh) break;
-        uint64_t bc = HK(c);
-        int best = c;
+// Replace the root of a full 4-ary max-heap of hc entries, hc = 1 (mod 4)
+// so every parent has four children, and sift the new key down -- for the
+// lanes with ins set; the others rewrite the root with its own value.  Three
+// unrolled levels cover hc <= 85, and nothing branches, so the wave never
+// splits on how deep a key sinks.  Returns the new root.
+__device__ __forceinline__ uint64_t kheap_replace_root(uint64_t *hk, int lane, int hc, bool ins, uint64_t key,
+                                                       uint64_t root) {
+    const uint64_t v = ins ? key : root;
+    uint64_t top = v;
+    int p = 0;
+    bool go = ins;
 #pragma unroll
-        for (int u = 1; u < 4; ++u) {
-            if (c + u < kh) {
-                uint64_t x = HK(c + u);
-                if (x > bc) { bc = x; best = c + u; }
-            }
+    for (int lv = 0; lv < 3; ++lv) {
+        const int c = 4 * p + 1;
+        go = go && c < hc;
+        const int cc = go ? c : 1;
+        const uint64_t x0 = HK(cc), x1 = HK(cc + 1), x2 = HK(cc + 2), x3 = HK(cc + 3);
+        const bool s01 = x1 > x0, s23 = x3 > x2;
+        const uint64_t m01 = s01 ? x1 : x0, m23 = s23 ? x3 : x2;
+        const bool s = m23 > m01;
+        const uint64_t bc = s ? m23 : m01;
+        const int bi = s ? (s23 ? 3 : 2) : (s01 ? 1 : 0);
+        go = go && bc > v;
+        HK(p) = go ? bc : v;
+        if (lv == 0) top = go ? bc : v;
+        p = go ? cc + bi : p;
+    }
+    HK(p) = v;
+    return top;
+}
+
+// Sift v down from the root of a heap of m entries (any m <= hc); children
+// at positions >= m read as 0, which never outranks a key.
+__device__ __forceinline__ void kheap_sift_bounded(uint64_t *hk, int lane, int m, uint64_t v) {
+    int p = 0;
+    bool go = true;
+#pragma unroll
+    for (int lv = 0; lv < 3; ++lv) {
+        const int c = 4 * p + 1;
+        go = go && c < m;
+        const int cc = go ? c : 1;
+        uint64_t x[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint64_t t = HK(cc + u);
+            x[u] = cc + u < m ? t : 0ULL;
         }
-        if (bc <= v) break;
-        HK(p) = bc;
-        p = best;
+        const bool s01 = x[1] > x[0], s23 = x[3] > x[2];
+        const uint64_t m01 = s01 ? x[1] : x[0], m23 = s23 ? x[3] : x[2];
+        const bool s = m23 > m01;
+        const uint64_t bc = s ? m23 : m01;
+        const int bi = s ? (s23 ? 3 : 2) : (s01 ? 1 : 0);
+        go = go && bc > v;
+        HK(p) = go ? bc : v;
+        p = go ? cc + bi : p;
     }
     HK(p) = v;
 }
+
+// Heap capacity for k+1 = kh kept keys: kh rounded up to 1 (mod 4).
+__host__ __device__ __forceinline__ int kheap_cap(int kh) { return ((kh + 2) & ~3) + 1; }
 
 struct MixCtx {
     double q[3];
@@ -80,7 +122,7 @@ struct MixCtx {
 };
 
 __device__ __forceinline__ void mix_leaf(const DevTree &T, const double *ext, int leaf, MixCtx &c, bool want,
-                                         uint64_t *hk, float *cx, int lane, int kh) {
+                                         uint64_t *hk, float *cx, int lane, int hc) {
     const int base = leaf * NBX_LEAF;
     if (lane < NBX_LEAF) {
 #pragma unroll
@@ -110,23 +152,19 @@ __device__ __forceinline__ void mix_leaf(const DevTree &T, const double *ext, in
     if (c.act && (want || mask)) c.emax = fmax(c.emax, ((cdptr)ext)[leaf]);
     __syncthreads();
     while (ballot(mask != 0)) {
-        if (mask) {
-            int j = __builtin_ctz(mask);
-            mask &= mask - 1;
-            float tx = qx - cx[j], ty = qy - cx[NBX_LEAF + j], tz = qz - cx[2 * NBX_LEAF + j];
-            float d = __builtin_fmaf(tz, tz, __builtin_fmaf(ty, ty, tx * tx));
-            uint64_t key = mkkey(d, base + j);
-            if (key < c.root) {
-                kheap_sift(hk, lane, kh, 0, key);
-                c.root = HK(0);
-            }
-        }
+        const bool has = mask != 0;
+        const int j = __builtin_ctz(mask | 0x80000000u);
+        mask &= mask - 1;
+        float tx = qx - cx[j], ty = qy - cx[NBX_LEAF + j], tz = qz - cx[2 * NBX_LEAF + j];
+        float d = __builtin_fmaf(tz, tz, __builtin_fmaf(ty, ty, tx * tx));
+        uint64_t key = mkkey(d, base + j);
+        c.root = kheap_replace_root(hk, lane, hc, has && key < c.root, key, c.root);
     }
     __syncthreads();
 }
 
 __device__ __forceinline__ void mix_walk(const DevTree &T, const double *ext, MixCtx &c, const double *qc, int anc,
-                                         uint64_t *hk, float *cx, int lane, int kh) {
+                                         uint64_t *hk, float *cx, int lane, int hc) {
     const int D = T.depth;
     int node = 1, dep = 0;
     uint64_t firstright = 0;
@@ -137,7 +175,7 @@ __device__ __forceinline__ void mix_walk(const DevTree &T, const double *ext, Mi
             bool want = c.act && box_lb<3>(c.q, bx) <= (double)keyd(c.root);
             if (ballot(want)) {
                 if (dep == D) {
-                    mix_leaf(T, ext, node - T.nleafp, c, want, hk, cx, lane, kh);
+                    mix_leaf(T, ext, node - T.nleafp, c, want, hk, cx, lane, hc);
                     down = false;
                 } else {
                     int c0 = 2 * node;
@@ -251,7 +289,7 @@ __global__ __launch_bounds__(64) void k_knn_mixed(DevTree T, KnnArgs A, const do
     __shared__ uint64_t hk[KMAX * NBX_WAVE];
     __shared__ float cx[3 * NBX_LEAF];
     const int lane = threadIdx.x;
-    const int g = blockIdx.x;
+    const int g = xcd_wave(blockIdx.x, gridDim.x);
     const int kh = A.kh;  // k + 1
     const int k = kh - 1;
     const int qi = g * NBX_WAVE + lane;
@@ -264,19 +302,20 @@ __global__ __launch_bounds__(64) void k_knn_mixed(DevTree T, KnnArgs A, const do
     c.self = qi;
     c.root = MX_EMPTY;
     c.emax = 0.0;
-    for (int p = 0; p < kh; ++p) HK(p) = MX_EMPTY;
+    const int hc = kheap_cap(kh);
+    for (int p = 0; p < hc; ++p) HK(p) = MX_EMPTY;
     double qc[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) qc[d] = readfirst_d(c.q[d]);
-    mix_walk(T, ext, c, qc, T.nleafp + 2 * g, hk, cx, lane, kh);
+    mix_walk(T, ext, c, qc, T.nleafp + 2 * g, hk, cx, lane, hc);
 
     // ---------------------------------------------------------------- band
     bool ok = valid;
     if (valid) {
-        for (int m = kh - 1; m > 0; --m) {  // heap sort: ascending keys
+        for (int m = hc - 1; m > 0; --m) {  // heap sort: ascending keys
             uint64_t top = HK(0), last = HK(m);
             HK(m) = top;
-            kheap_sift(hk, lane, m, 0, last);
+            kheap_sift_bounded(hk, lane, m, last);
         }
         if (HK(kh - 1) == MX_EMPTY) ok = false;  // fewer than k+1 candidates
     }
