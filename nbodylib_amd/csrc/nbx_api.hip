@@ -108,6 +108,7 @@ struct nbx_tree {
     int32_t *perm = nullptr;
     int32_t *inv = nullptr;
     double *box = nullptr;
+    float *plane = nullptr;
     double *org = nullptr;
     float *xf[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     double *kern = nullptr;
@@ -130,6 +131,7 @@ struct nbx_tree {
         T.perm = perm;
         T.inv = inv;
         T.box = box;
+        T.plane = plane;
         T.org = org;
         T.kern = kern;
         T.kernf = kernf;
@@ -186,6 +188,7 @@ static void free_tree(nbx_tree *t) {
     if (t->perm) hipFree(t->perm);
     if (t->inv) hipFree(t->inv);
     if (t->box) hipFree(t->box);
+    if (t->plane) hipFree(t->plane);
     if (t->org) hipFree(t->org);
     if (t->kern) hipFree(t->kern);
     if (t->kernf) hipFree(t->kernf);
@@ -281,6 +284,7 @@ static int build_tree(nbx_tree *t, const double *pos, const double *vel, const d
     while (t->nleafp < t->nleaf) { t->nleafp <<= 1; t->depth++; }
     t->nnodes = 2 * (int64_t)t->nleafp;
     HIPCK(hipMalloc((void **)&t->box, sizeof(double) * 2 * nd * (size_t)t->nnodes));
+    HIPCK(hipMalloc((void **)&t->plane, sizeof(float) * (nd == 6 ? 8 : 4) * (size_t)t->nleafp));
     if (t->p.form == NBX_FORM_MIXED) {
         HIPCK(hipMalloc((void **)&t->org, sizeof(double) * nd * (size_t)t->nleaf));
         HIPCK(hipMalloc((void **)&t->ext, sizeof(double) * (size_t)t->nleaf));
@@ -299,8 +303,8 @@ static int build_tree(nbx_tree *t, const double *pos, const double *vel, const d
         a.ext = t->ext;
         hipLaunchKernelGGL(k_leafbox, dim3(nblk(t->nleafp)), dim3(256), 0, st, a);
         for (int dep = t->depth - 1; dep >= 0; --dep)
-            hipLaunchKernelGGL(k_levelbox, dim3(nblk((int64_t)1 << dep)), dim3(256), 0, st, t->box, 1 << dep,
-                               1 << dep, nd);
+            hipLaunchKernelGGL(k_levelbox, dim3(nblk((int64_t)1 << dep)), dim3(256), 0, st, t->box, t->plane,
+                               1 << dep, 1 << dep, nd);
     }
     if (t->p.form == NBX_FORM_MIXED) {
         for (int d = 0; d < nd; ++d) HIPCK(hipMalloc((void **)&t->xf[d], sizeof(float) * npad));

@@ -130,7 +130,8 @@ __global__ __launch_bounds__(256) void k_leafbox(LeafBoxArgs a) {
 }
 
 // Internal nodes [first, first+count) of one depth: union of the two children.
-__global__ __launch_bounds__(256) void k_levelbox(double *__restrict__ box, int32_t first, int32_t count, int nd) {
+__global__ __launch_bounds__(256) void k_levelbox(double *__restrict__ box, float *__restrict__ plane, int32_t first,
+                                                  int32_t count, int nd) {
     int32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i >= count) return;
     int32_t nn = first + i;
@@ -141,6 +142,21 @@ __global__ __launch_bounds__(256) void k_levelbox(double *__restrict__ box, int3
         o[d] = fmin(a[d], b[d]);
         o[nd + d] = fmax(a[nd + d], b[nd + d]);
     }
+    // Child order plane: the right child is nearer to a point q than the left
+    // when n.q > off, n = c_right - c_left (box centres), off = n.(c_l + c_r)/2.
+    // An empty right child (padding) gives n = 0, off = 1: left first.
+    const int plw = nd == 6 ? 8 : 4;
+    float *pl = plane + (size_t)nn * plw;
+    const bool empty = !(a[0] <= a[nd]) || !(b[0] <= b[nd]);
+    double off = 0.0;
+    for (int d = 0; d < nd; ++d) {
+        const double ca = 0.5 * (a[d] + a[nd + d]), cb = 0.5 * (b[d] + b[nd + d]);
+        const double nv = empty ? 0.0 : cb - ca;
+        pl[d] = (float)nv;
+        off += empty ? 0.0 : nv * 0.5 * (ca + cb);
+    }
+    pl[nd] = empty ? 1.0f : (float)off;
+    for (int d = nd + 1; d < plw; ++d) pl[d] = 0.0f;
 }
 
 // Mixed form: FP32 offsets of each particle from its leaf's FP64 origin.

@@ -41,6 +41,7 @@ struct DevTree {
     const int32_t *perm;   // slot -> ID
     const int32_t *inv;    // ID -> slot
     const double *box;     // node boxes, [node][lo[nd], hi[nd]]
+    const float *plane;    // internal nodes: child-order plane [node][n[nd], off] (4 or 8 floats)
     const double *org;     // mixed form: per-leaf FP64 origin [leaf][nd]
     const float *xf[6];    // mixed form: FP32 offsets from the leaf origin
     const double *kern;    // kernel table
@@ -119,6 +120,18 @@ __device__ __forceinline__ double boxbox_lb(cdptr a, cdptr b) {
         s += t * t;
     }
     return s;
+}
+
+// Which child of internal node p to descend first, seen from the uniform
+// point qc: the one whose box centre is nearer (a heuristic for the visiting
+// order only; pruning never depends on it).
+template <int ND>
+__device__ __forceinline__ bool first_is_right(const DevTree &T, int p, const float *qc) {
+    const cfptr pl = (cfptr)(T.plane + (size_t)p * (ND == 6 ? 8 : 4));
+    float s = 0.0f;
+#pragma unroll
+    for (int d = 0; d < ND; ++d) s = __builtin_fmaf(pl[d], qc[d], s);
+    return s > pl[ND];
 }
 
 // Workgroups of a launch are dealt round-robin to the 8 XCDs (workgroup b

@@ -46,7 +46,7 @@ __global__ void k_bbox(const double *c, int64_t n, double *part);
 __global__ void k_morton(MortonArgs a);
 __global__ void k_gather(GatherArgs a);
 __global__ void k_leafbox(LeafBoxArgs a);
-__global__ void k_levelbox(double *box, int32_t first, int32_t count, int nd);
+__global__ void k_levelbox(double *box, float *plane, int32_t first, int32_t count, int nd);
 __global__ void k_mixed_offsets(MixedArgs a);
 
 size_t nbx_sort_temp_bytes(int64_t n, int bits);

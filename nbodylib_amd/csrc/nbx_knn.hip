@@ -223,24 +223,6 @@ __device__ __forceinline__ void knn_leaf(const DevTree &T, int leaf, LeafCtx<ND>
     __syncthreads();
 }
 
-// Order of the two children of a node as seen from the uniform point qc:
-// nearer box first, ties (overlapping sibling boxes) by box-centre distance.
-template <int ND>
-__device__ __forceinline__ bool first_is_right(const DevTree &T, int c0, const double *qc) {
-    cdptr b0 = (cdptr)node_box(T, c0, ND);
-    cdptr b1 = (cdptr)node_box(T, c0 + 1, ND);
-    double d0 = box_lb<ND>(qc, b0), d1 = box_lb<ND>(qc, b1);
-    if (d1 != d0) return d1 < d0;
-    double e0 = 0.0, e1 = 0.0;
-#pragma unroll
-    for (int d = 0; d < ND; ++d) {
-        double u = qc[d] - 0.5 * (b0[d] + b0[ND + d]), v = qc[d] - 0.5 * (b1[d] + b1[ND + d]);
-        e0 += u * u;
-        e1 += v * v;
-    }
-    return e1 < e0;
-}
-
 // Uniform stackless depth-first walk of the heap-ordered tree.  The first
 // descent follows the ancestors of the wave's own leaf node (anc; -1 when
 // the queries are not tree particles or are image-shifted), so every heap
@@ -250,7 +232,7 @@ __device__ __forceinline__ bool first_is_right(const DevTree &T, int c0, const d
 // State is uniform (SGPRs): node, depth and one "right child first" bit per
 // depth.
 template <int ND>
-__device__ __forceinline__ void knn_walk(const DevTree &T, LeafCtx<ND> &c, const double *qc, int anc, double *hd,
+__device__ __forceinline__ void knn_walk(const DevTree &T, LeafCtx<ND> &c, const float *qc, int anc, double *hd,
                                          int32_t *hid, double *cx, int32_t *cid, int lane, int hc) {
     const int D = T.depth;
     int node = 1, dep = 0;
@@ -268,7 +250,7 @@ __device__ __forceinline__ void knn_walk(const DevTree &T, LeafCtx<ND> &c, const
                     int c0 = 2 * node;
                     int first = c0;
                     if (anc >= 0 && (anc >> (D - dep)) == node) first = anc >> (D - dep - 1);
-                    else if (first_is_right<ND>(T, c0, qc)) first = c0 + 1;
+                    else if (first_is_right<ND>(T, node, qc)) first = c0 + 1;
                     dep += 1;
                     if (first & 1) firstright |= (1ULL << dep);
                     else firstright &= ~(1ULL << dep);
@@ -366,9 +348,9 @@ __global__ __launch_bounds__(64) void k_knn(DevTree T, KnnArgs A) {
             c.q[0] = xp[0]; c.q[1] = xp[1]; c.q[2] = xp[2];
         }
         if (!ballot(c.act)) continue;
-        double qc[ND];
+        float qc[ND];
 #pragma unroll
-        for (int d = 0; d < ND; ++d) qc[d] = readfirst_d(c.q[d]);
+        for (int d = 0; d < ND; ++d) qc[d] = (float)readfirst_d(c.q[d]);
         const int anc = (pass == 0 && QSRC != NBX_Q_POINTS && !listed) ? T.nleafp + 2 * g : -1;
         knn_walk<ND>(T, c, qc, anc, hd, hid, cx, cid, lane, hc);
     }

@@ -163,7 +163,7 @@ __device__ __forceinline__ void mix_leaf(const DevTree &T, const double *ext, in
     __syncthreads();
 }
 
-__device__ __forceinline__ void mix_walk(const DevTree &T, const double *ext, MixCtx &c, const double *qc, int anc,
+__device__ __forceinline__ void mix_walk(const DevTree &T, const double *ext, MixCtx &c, const float *qc, int anc,
                                          uint64_t *hk, float *cx, int lane, int hc) {
     const int D = T.depth;
     int node = 1, dep = 0;
@@ -182,20 +182,8 @@ __device__ __forceinline__ void mix_walk(const DevTree &T, const double *ext, Mi
                     int first = c0;
                     if (anc >= 0 && (anc >> (D - dep)) == node) {
                         first = anc >> (D - dep - 1);
-                    } else {
-                        cdptr b0 = (cdptr)node_box(T, c0, 3);
-                        cdptr b1 = (cdptr)node_box(T, c0 + 1, 3);
-                        double d0 = box_lb<3>(qc, b0), d1 = box_lb<3>(qc, b1);
-                        if (d1 < d0) first = c0 + 1;
-                        else if (d1 == d0) {
-                            double e0 = 0.0, e1 = 0.0;
-                            for (int d = 0; d < 3; ++d) {
-                                double u = qc[d] - 0.5 * (b0[d] + b0[3 + d]), v = qc[d] - 0.5 * (b1[d] + b1[3 + d]);
-                                e0 += u * u;
-                                e1 += v * v;
-                            }
-                            if (e1 < e0) first = c0 + 1;
-                        }
+                    } else if (first_is_right<3>(T, node, qc)) {
+                        first = c0 + 1;
                     }
                     dep += 1;
                     if (first & 1) firstright |= (1ULL << dep);
@@ -304,9 +292,9 @@ __global__ __launch_bounds__(64) void k_knn_mixed(DevTree T, KnnArgs A, const do
     c.emax = 0.0;
     const int hc = kheap_cap(kh);
     for (int p = 0; p < hc; ++p) HK(p) = MX_EMPTY;
-    double qc[3];
+    float qc[3];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) qc[d] = readfirst_d(c.q[d]);
+    for (int d = 0; d < 3; ++d) qc[d] = (float)readfirst_d(c.q[d]);
     mix_walk(T, ext, c, qc, T.nleafp + 2 * g, hk, cx, lane, hc);
 
     // ---------------------------------------------------------------- band
