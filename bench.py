@@ -14,6 +14,16 @@ headline form (mixed precision when available, else plain FP64), plus the
 plain-FP64 rate, FOF Mparticles/s, build time, the roofline of the dominant
 kernel (k_knn, HIP events on its launch stream) and the CPU baseline (the
 oracle's OpenMP restatement of the reference, timed on a bounded sample).
+
+Other measured workloads (--workload), each printing its own JSON line:
+  phase6d  N=16.8M clustered particles with velocities, metric (x/sigma_x,
+           v/sigma_v): 6-D phase-space tree, k=64 kNN + 6-D SPH density,
+           plus 3-D FOF at b=0.2 mean spacing on the same particles.
+  adjoint  N=1M clustered, lognormal masses: SPH density over a frozen k=32
+           list and its hand-written adjoint (d sum log rho / d x, d m), in
+           input (ID) order and in tree order; list memory vs recompute time.
+  large    N=134M clustered, lognormal masses: build, mixed kNN + density
+           (k=32), adjoint over the stored list (tree order) and FOF.
 """
 import argparse
 import json
@@ -49,6 +59,7 @@ def main():
                     help="only run a device-to-device copy of MIB MiB (calibrates rocprof byte counters) and exit")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic.json"),
                     help="measured HBM bytes per launch of the dominant kernel (from rocprofv3 --pmc runs)")
+    ap.add_argument("--workload", default="knn3d", choices=["knn3d", "phase6d", "adjoint", "large"])
     args = ap.parse_args()
 
     if args.copy_probe:
@@ -67,6 +78,9 @@ def main():
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}, WORLD_SIZE {world}); "
               "multi-GPU sharding is designed but not built in this tree", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload != "knn3d":
+        return {"phase6d": run_phase6d, "adjoint": run_adjoint, "large": run_large}[args.workload](args)
 
     import torch
 
@@ -221,6 +235,234 @@ def main():
         "gpu_vs_cpu": round(value / cpu["value"], 2) if cpu else None,
     }
     print(json.dumps(out), flush=True)
+
+
+class Timer:
+    """HIP events on the current stream (the stream the library launches on)."""
+
+    def __init__(self, torch):
+        self.torch = torch
+        self.a = torch.cuda.Event(enable_timing=True)
+        self.b = torch.cuda.Event(enable_timing=True)
+
+    def __enter__(self):
+        self.a.record()
+        return self
+
+    def __exit__(self, *exc):
+        self.b.record()
+        self.b.synchronize()
+        self.ms = self.a.elapsed_time(self.b)
+
+
+def _median(xs):
+    xs = sorted(xs)
+    return xs[len(xs) // 2]
+
+
+def _line(metric, value, unit, args, ms, dtype, workload, extra, higher=True):
+    out = {"metric": metric, "value": round(value, 3), "unit": unit, "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": higher, "scaling": "weak",
+           "vs_baseline": None, "dtype": dtype,
+           "data": "synthetic (seeded NFW-clustered particles generated on the host, resident in HBM)",
+           "config": dict(workload=workload, parallelism="single GPU")}
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+
+
+def run_phase6d(args):
+    import torch
+
+    import nbodylib_amd as nb
+    from nbodylib_amd import datagen
+    from nbodylib_amd.device import DeviceTree
+
+    dev = torch.device("cuda:0")
+    n = args.n
+    k = 64 if args.k == 32 else args.k
+    L = 100.0
+    log(f"generating clustered N={n} with velocities")
+    pos, vel, _ = datagen.clustered(n, L, nhalo=2000, seed=args.seed + 2, with_vel=True)
+    sx, sv = pos.std(axis=0), vel.std(axis=0)
+    xs = torch.from_numpy(np.ascontiguousarray((pos / sx).T)).to(dev)
+    vs = torch.from_numpy(np.ascontiguousarray((vel / sv).T)).to(dev)
+    rho = torch.empty(n, dtype=torch.float64, device=dev)
+    hh = torch.empty(n, dtype=torch.float64, device=dev)
+    builds = []
+    for _ in range(2):
+        torch.cuda.synchronize()
+        with Timer(torch) as tb:
+            t6 = DeviceTree(xs, vel=vs, bucket_size=16, treetype=nb.TPHS, kerntype=nb.KSPH, aniso=-1)
+        builds.append(tb.ms)
+    log(f"6-D tree build {builds[-1]:.1f} ms; warmup {args.warmup}")
+    for _ in range(args.warmup):
+        t6.density(k, rho, hh)
+    nb.prof_reset()
+    nb.prof_enable(True)
+    times = []
+    for _ in range(args.steps):
+        with Timer(torch) as tm:
+            t6.density(k, rho, hh)
+        times.append(tm.ms)
+    nb.prof_enable(False)
+    kms, kl = nb.prof_get("k_knn_fp64")
+    ms = _median(times)
+    log(f"6-D k={k} kNN + density: {ms:.2f} ms (k_knn_fp64 {kms / max(kl, 1):.2f} ms/launch)")
+    del t6
+    # 3-D FOF at b = 0.2 mean spacing on the same particles
+    pd = torch.from_numpy(np.ascontiguousarray(pos.T)).to(dev)
+    t3 = DeviceTree(pd, bucket_size=16, kerntype=nb.KSPH, period=L)
+    b = 0.2 * datagen.mean_spacing(n, L)
+    lab = torch.empty(n, dtype=torch.int32, device=dev)
+    t3.fof(b, 8, lab)
+    fts = []
+    for _ in range(3):
+        with Timer(torch) as tf:
+            ng = t3.fof(b, 8, lab)
+        fts.append(tf.ms)
+    fms = _median(fts)
+    log(f"3-D FOF b={b:.5f}: {fms:.1f} ms, {ng} groups")
+    kern_ms = kms / max(kl, 1)
+    bpq = 48 + 8 + 4 * k + 16
+    ach = bpq * n / (kern_ms / 1e3) / 1e9
+    _line(f"Mqueries/s 6-D kNN (k={k}) + 6-D SPH density, N={n}", n / (ms / 1e3) / 1e6, "Mqueries/s", args, ms,
+          "f64", f"6-D phase space (x/sigma_x, v/sigma_v), clustered NFW N={n}, k={k} CalcDensity on a TPHS tree; "
+                 f"3-D FOF b=0.2 mean spacing", {
+              "build_ms": round(builds[-1], 2), "fof_ms": round(fms, 2), "fof_mparticles_s": round(n / fms / 1e3, 3),
+              "fof_groups": ng,
+              "roofline": {"bound": "hbm", "achieved": round(ach, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                           "frac": round(ach / PEAK_HBM_GBS, 5), "traffic": None, "kernel": "k_knn_fp64",
+                           "kernel_ms": round(kern_ms, 3), "bytes_per_query": bpq},
+              "cpu_baseline": None})
+
+
+def _adjoint_pass(torch, nb, pos, mass, nbr, reps):
+    """Times forward (density over the list) and backward (adjoint) with
+    g = d(sum log rho)/d rho = 1/rho."""
+    from nbodylib_amd.device import sph_density, _ptr, _stream, _check
+
+    n, k = nbr.shape
+    rho, _ = sph_density(pos, mass, nbr)
+    g = (1.0 / rho).contiguous()
+    dpos = torch.empty_like(pos)
+    dmass = torch.empty_like(mass)
+    fw, bw = [], []
+    for _ in range(reps):
+        with Timer(torch) as tf:
+            rho, _ = sph_density(pos, mass, nbr)
+        fw.append(tf.ms)
+        with Timer(torch) as tb:  # the entry point zeroes dpos/dmass itself
+            _check(nb.lib().nbx_sph_density_adjoint_dev(_ptr(pos), _ptr(mass), _ptr(nbr), n, k, _ptr(g), _ptr(dpos),
+                                                        _ptr(dmass), _stream()))
+        bw.append(tb.ms)
+    return _median(fw), _median(bw)
+
+
+def run_adjoint(args):
+    import torch
+
+    import nbodylib_amd as nb
+    from nbodylib_amd import datagen
+    from nbodylib_amd.device import DeviceTree
+
+    dev = torch.device("cuda:0")
+    n = 1 << 20 if args.n == 256 ** 3 else args.n
+    k = args.k
+    L = 100.0
+    log(f"generating clustered N={n}, lognormal masses")
+    pos, _ = datagen.clustered(n, L, nhalo=2000, seed=args.seed + 3)
+    m = datagen.lognormal_masses(n, seed=args.seed + 3)
+    pd = torch.from_numpy(np.ascontiguousarray(pos.T)).to(dev)
+    md = torch.from_numpy(m).to(dev)
+    tree = DeviceTree(pd, mass=md, bucket_size=16, kerntype=nb.KSPH, form=nb.FORM_MIXED)
+    # frozen list in input (ID) order: rows and entries are IDs
+    nn_id = torch.empty((n, k), dtype=torch.int32, device=dev)
+    with Timer(torch) as tk:
+        tree.knn(k, nn_id, layout=nb.LAYOUT_BY_ID)
+    knn_ms = tk.ms
+    fw_id, bw_id = _adjoint_pass(torch, nb, pd, md, nn_id, max(args.steps, 3))
+    # the same list in tree order: positions/masses permuted by the tree
+    order = tree.order().long()
+    inv = torch.empty_like(order)
+    inv[order] = torch.arange(n, device=dev)
+    pt = pd[:, order].contiguous()
+    mt = md[order].contiguous()
+    nn_t = inv[nn_id[order].long()].to(torch.int32).contiguous()
+    fw_t, bw_t = _adjoint_pass(torch, nb, pt, mt, nn_t, max(args.steps, 3))
+    log(f"ID order: fwd {fw_id:.2f} ms bwd {bw_id:.2f} ms; tree order: fwd {fw_t:.2f} ms bwd {bw_t:.2f} ms; "
+        f"kNN to rebuild the list {knn_ms:.2f} ms")
+    bpp = 24 + 8 + 8 + 8 + 4 * k + 32
+    ach = bpp * n / (bw_t / 1e3) / 1e9
+    _line(f"Mparticles/s SPH density adjoint over a frozen k={k} list, N={n}", n / (bw_t / 1e3) / 1e6,
+          "Mparticles/s", args, bw_t, "f64",
+          f"clustered NFW N={n}, lognormal masses, frozen k={k} list; backward = hand-written adjoint of "
+          f"sum log rho", {
+              "fwd_ms_tree_order": round(fw_t, 3), "bwd_ms_tree_order": round(bw_t, 3),
+              "bwd_over_fwd_tree_order": round(bw_t / fw_t, 3),
+              "fwd_ms_id_order": round(fw_id, 3), "bwd_ms_id_order": round(bw_id, 3),
+              "bwd_over_fwd_id_order": round(bw_id / fw_id, 3),
+              "list_bytes_stored": n * k * 4, "list_recompute_ms": round(knn_ms, 3),
+              "roofline": {"bound": "hbm", "achieved": round(ach, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                           "frac": round(ach / PEAK_HBM_GBS, 5), "traffic": None, "kernel": "k_sph_adj",
+                           "kernel_ms": round(bw_t, 3), "bytes_per_particle": bpp},
+              "cpu_baseline": None})
+
+
+def run_large(args):
+    import torch
+
+    import nbodylib_amd as nb
+    from nbodylib_amd import datagen
+    from nbodylib_amd.device import DeviceTree
+
+    dev = torch.device("cuda:0")
+    n = 512 ** 3 if args.n == 256 ** 3 else args.n
+    k = args.k
+    L = 100.0
+    log(f"generating clustered N={n} (host)")
+    pos, _ = datagen.clustered(n, L, nhalo=2000, seed=args.seed + 4)
+    m = datagen.lognormal_masses(n, seed=args.seed + 4)
+    pd = torch.from_numpy(np.ascontiguousarray(pos.T)).to(dev)
+    del pos
+    md = torch.from_numpy(m).to(dev)
+    del m
+    log("inputs resident in HBM")
+    torch.cuda.synchronize()
+    with Timer(torch) as tb:
+        tree = DeviceTree(pd, mass=md, bucket_size=16, kerntype=nb.KSPH, period=L, form=nb.FORM_MIXED)
+    log(f"build {tb.ms:.1f} ms")
+    nn = torch.empty((n, k), dtype=torch.int32, device=dev)
+    rho = torch.empty(n, dtype=torch.float64, device=dev)
+    hh = torch.empty(n, dtype=torch.float64, device=dev)
+    tree.knn_density(k, nn, rho, hh)
+    times = []
+    for _ in range(max(args.steps, 1)):
+        with Timer(torch) as tk:
+            tree.knn_density(k, nn, rho, hh)
+        times.append(tk.ms)
+        log(f"kNN+density {tk.ms:.1f} ms")
+    kms = _median(times)
+    fb = tree.last_fallbacks()
+    # adjoint over the stored list, in tree order (the list is by slot)
+    order = tree.order().long()
+    pt = pd[:, order].contiguous()
+    mt = md[order].contiguous()
+    del order
+    fw, bw = _adjoint_pass(torch, nb, pt, mt, nn, 2)
+    log(f"adjoint fwd {fw:.1f} ms bwd {bw:.1f} ms")
+    del pt, mt
+    b = 0.2 * datagen.mean_spacing(n, L)
+    lab = torch.empty(n, dtype=torch.int32, device=dev)
+    with Timer(torch) as tf:
+        ng = tree.fof(b, 8, lab)
+    log(f"FOF {tf.ms:.1f} ms, {ng} groups")
+    mem = torch.cuda.max_memory_allocated() / 2 ** 30
+    _line(f"Mqueries/s kNN+SPH density (k={k}, N={n}) mixed", n / (kms / 1e3) / 1e6, "Mqueries/s", args, kms,
+          "f32+f64", f"clustered NFW N={n}, lognormal masses, periodic L=100: build, mixed kNN+density k={k}, "
+                     f"adjoint over the stored list, FOF b=0.2 mean spacing", {
+              "build_ms": round(tb.ms, 2), "mixed_fp64_fallback_queries": fb,
+              "adjoint_fwd_ms": round(fw, 2), "adjoint_bwd_ms": round(bw, 2), "fof_ms": round(tf.ms, 2),
+              "fof_groups": ng, "torch_peak_gib": round(mem, 2), "cpu_baseline": None})
 
 
 if __name__ == "__main__":

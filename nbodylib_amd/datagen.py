@@ -28,8 +28,7 @@ def uniform(n, L=1.0, seed=0):
     return rng.random((n, 3)) * L
 
 
-def _nfw_x(u, c, iters=60):
-    """Solve mu(x) = u * mu(c) for x in (0, c] by bisection (vectorised)."""
+def _nfw_x_chunk(u, c, iters):
     mu = lambda x: np.log1p(x) - x / (1.0 + x)
     target = u * mu(c)
     lo = np.zeros_like(target)
@@ -40,6 +39,26 @@ def _nfw_x(u, c, iters=60):
         lo = np.where(f < 0, mid, lo)
         hi = np.where(f < 0, hi, mid)
     return 0.5 * (lo + hi)
+
+
+def _nfw_x(u, c, iters=60):
+    """Solve mu(x) = u * mu(c) for x in (0, c] by bisection (vectorised,
+    elementwise, so chunks run on a thread pool with identical results)."""
+    from concurrent.futures import ThreadPoolExecutor
+    import os
+
+    n = len(u)
+    step = 1 << 20
+    if n <= step:
+        return _nfw_x_chunk(u, c, iters)
+    out = np.empty(n, dtype=np.float64)
+
+    def work(a):
+        out[a:a + step] = _nfw_x_chunk(u[a:a + step], c[a:a + step], iters)
+
+    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        list(ex.map(work, range(0, n, step)))
+    return out
 
 
 def clustered(n, L=100.0, nhalo=2000, seed=0, with_vel=False, alpha=1.9, mrange=1e3):

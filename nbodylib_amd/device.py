@@ -56,6 +56,14 @@ class DeviceTree:
             lib().nbx_tree_destroy(h)
             self._h = None
 
+    def order(self):
+        """slot -> ID permutation (int32 CUDA tensor): tree slot s holds input particle order()[s]."""
+        import numpy as np
+
+        o = np.empty(self.n, dtype=np.int32)
+        _check(lib().nbx_tree_order(self._h, ctypes.c_void_p(o.ctypes.data)))
+        return torch.from_numpy(o).to(torch.device("cuda", torch.cuda.current_device()))
+
     def last_fallbacks(self):
         v = ctypes.c_int64()
         _check(lib().nbx_tree_last_fallbacks(self._h, ctypes.byref(v)))
