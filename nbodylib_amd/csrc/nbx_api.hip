@@ -204,8 +204,7 @@ static int build_tree(nbx_tree *t, const double *pos, const double *vel, const d
     for (int d = 0; d < 3; ++d) c[d] = pos + d * n;
     for (int d = 0; d < 3; ++d) c[3 + d] = vel ? vel + d * n : nullptr;
 
-    // bounding box of the key coordinates
-    double lo[6], scale[6];
+    // coordinates must be finite (a NaN or inf has no place in a split)
     {
         const int nb = 1024;
         double *part = nullptr;
@@ -216,39 +215,23 @@ static int build_tree(nbx_tree *t, const double *pos, const double *vel, const d
         HIPCK(hipMemcpyAsync(hp.data(), part, sizeof(double) * hp.size(), hipMemcpyDeviceToHost, st));
         HIPCK(hipStreamSynchronize(st));
         HIPCK(hipFreeAsync(part, st));
-        const double qmax = nd == 3 ? (double)((1 << 21) - 1) : (double)((1 << 10) - 1);
         for (int d = 0; d < nd; ++d) {
             double mn = INFINITY, mx = -INFINITY;
             for (int b = 0; b < nb; ++b) { mn = std::min(mn, hp[2 * nb * d + 2 * b]); mx = std::max(mx, hp[2 * nb * d + 2 * b + 1]); }
             if (!std::isfinite(mn) || !std::isfinite(mx)) return fail(NBX_ERR_ARG, "non-finite coordinates");
-            lo[d] = mn;
-            scale[d] = mx > mn ? qmax / (mx - mn) : 0.0;
         }
     }
-    uint64_t *keys = nullptr, *keys2 = nullptr;
-    int32_t *vals = nullptr, *vals2 = nullptr;
-    void *tmp = nullptr;
-    const int bits = nd == 3 ? 63 : 60;
-    HIPCK(hipMallocAsync((void **)&keys, sizeof(uint64_t) * n, st));
-    HIPCK(hipMallocAsync((void **)&keys2, sizeof(uint64_t) * n, st));
-    HIPCK(hipMallocAsync((void **)&vals, sizeof(int32_t) * n, st));
-    HIPCK(hipMallocAsync((void **)&vals2, sizeof(int32_t) * n, st));
+    // implicit heap-ordered tree over the leaves padded to a power of two
+    t->nleaf = (int32_t)((n + NBX_LEAF - 1) / NBX_LEAF);
+    t->nleafp = 1;
+    t->depth = 0;
+    while (t->nleafp < t->nleaf) { t->nleafp <<= 1; t->depth++; }
+    t->nnodes = 2 * (int64_t)t->nleafp;
+    int32_t *order = nullptr;
+    HIPCK(hipMallocAsync((void **)&order, sizeof(int32_t) * n, st));
     {
-        ProfScope ps("k_build_morton", st);
-        MortonArgs a;
-        memset(&a, 0, sizeof(a));
-        a.n = n;
-        a.nd = nd;
-        for (int d = 0; d < nd; ++d) { a.c[d] = c[d]; a.lo[d] = lo[d]; a.scale[d] = scale[d]; }
-        a.keys = keys;
-        a.vals = vals;
-        hipLaunchKernelGGL(k_morton, dim3(nblk(n)), dim3(256), 0, st, a);
-    }
-    size_t tb = nbx_sort_temp_bytes(n, bits);
-    HIPCK(hipMallocAsync(&tmp, tb, st));
-    {
-        ProfScope ps("k_build_sort", st);
-        HIPCK(nbx_sort_pairs(tmp, tb, keys, keys2, vals, vals2, n, bits, st));
+        ProfScope ps("k_build_kdorder", st);
+        HIPCK(nbx_kd_order(c, nd, n, t->nleafp, order, st));
     }
     const int64_t npad = (n + NBX_LEAF - 1) / NBX_LEAF * NBX_LEAF;
     for (int d = 0; d < ncoord; ++d) HIPCK(hipMalloc((void **)&t->xs[d], sizeof(double) * npad));
@@ -262,7 +245,7 @@ static int build_tree(nbx_tree *t, const double *pos, const double *vel, const d
         g.n = n;
         g.npad = npad;
         g.nd = ncoord;
-        g.order = vals2;
+        g.order = order;
         for (int d = 0; d < ncoord; ++d) { g.in[d] = c[d]; g.out[d] = t->xs[d]; }
         g.mass_in = mass;
         g.mass_default = 1.0 / (double)n;
@@ -271,18 +254,8 @@ static int build_tree(nbx_tree *t, const double *pos, const double *vel, const d
         g.inv = t->inv;
         hipLaunchKernelGGL(k_gather, dim3(nblk(npad)), dim3(256), 0, st, g);
     }
-    HIPCK(hipFreeAsync(keys, st));
-    HIPCK(hipFreeAsync(keys2, st));
-    HIPCK(hipFreeAsync(vals, st));
-    HIPCK(hipFreeAsync(vals2, st));
-    HIPCK(hipFreeAsync(tmp, st));
+    HIPCK(hipFreeAsync(order, st));
 
-    // implicit heap-ordered tree over the leaves padded to a power of two
-    t->nleaf = (int32_t)((n + NBX_LEAF - 1) / NBX_LEAF);
-    t->nleafp = 1;
-    t->depth = 0;
-    while (t->nleafp < t->nleaf) { t->nleafp <<= 1; t->depth++; }
-    t->nnodes = 2 * (int64_t)t->nleafp;
     HIPCK(hipMalloc((void **)&t->box, sizeof(double) * 2 * nd * (size_t)t->nnodes));
     HIPCK(hipMalloc((void **)&t->plane, sizeof(float) * (nd == 6 ? 8 : 4) * (size_t)t->nleafp));
     if (t->p.form == NBX_FORM_MIXED) {

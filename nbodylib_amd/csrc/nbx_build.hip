@@ -1,10 +1,10 @@
 // nbx_build.hip -- tree construction on gfx950.
 //
 // Replaces KDTree::BuildNodes (KDTree.cxx:970-1055): instead of a recursive
-// median split that swaps 88-byte particles in place, particles are ordered
-// by a Morton key (21 bits per axis in 3-D, 10 in 6-D), gathered into SoA
+// median split that swaps 88-byte particles in place, the split order is
+// computed on index lists (nbx_kdbuild.hip), particles are gathered into SoA
 // arrays in that order, cut into fixed buckets of NBX_LEAF slots and boxed
-// bottom-up.  All passes are streaming and coalesced (HBM-bound).
+// bottom-up.  All passes here are streaming and coalesced (HBM-bound).
 #include <cstring>
 #include <rocprim/rocprim.hpp>
 
@@ -34,45 +34,6 @@ __global__ __launch_bounds__(256) void k_bbox(const double *__restrict__ c, int6
         part[2 * blockIdx.x] = smn[0];
         part[2 * blockIdx.x + 1] = smx[0];
     }
-}
-
-// ---------------------------------------------------------------- morton keys
-__device__ __forceinline__ uint64_t spread3(uint64_t v) {  // 21 bits -> every 3rd bit
-    v &= 0x1fffffULL;
-    v = (v | (v << 32)) & 0x1f00000000ffffULL;
-    v = (v | (v << 16)) & 0x1f0000ff0000ffULL;
-    v = (v | (v << 8)) & 0x100f00f00f00f00fULL;
-    v = (v | (v << 4)) & 0x10c30c30c30c30c3ULL;
-    v = (v | (v << 2)) & 0x1249249249249249ULL;
-    return v;
-}
-
-__device__ __forceinline__ uint64_t quant(double v, double lo, double scale, uint64_t maxq) {
-    double t = (v - lo) * scale;
-    if (!(t > 0.0)) return 0;  // also catches NaN
-    uint64_t u = (uint64_t)t;
-    return u > maxq ? maxq : u;
-}
-
-__global__ __launch_bounds__(256) void k_morton(MortonArgs a) {
-    int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= a.n) return;
-    uint64_t key = 0;
-    if (a.nd == 3) {
-        const uint64_t mq = (1ULL << 21) - 1;
-        uint64_t ix = quant(a.c[0][i], a.lo[0], a.scale[0], mq);
-        uint64_t iy = quant(a.c[1][i], a.lo[1], a.scale[1], mq);
-        uint64_t iz = quant(a.c[2][i], a.lo[2], a.scale[2], mq);
-        key = (spread3(ix) << 2) | (spread3(iy) << 1) | spread3(iz);
-    } else {
-        const uint64_t mq = (1ULL << 10) - 1;
-        uint64_t q[6];
-        for (int d = 0; d < 6; ++d) q[d] = quant(a.c[d][i], a.lo[d], a.scale[d], mq);
-        for (int bit = 9; bit >= 0; --bit)
-            for (int d = 0; d < 6; ++d) key = (key << 1) | ((q[d] >> bit) & 1ULL);
-    }
-    a.keys[i] = key;
-    a.vals[i] = (int32_t)i;
 }
 
 // ---------------------------------------------------------------- gather
@@ -169,18 +130,6 @@ __global__ __launch_bounds__(256) void k_mixed_offsets(MixedArgs a) {
 }
 
 // ---------------------------------------------------------------- sort
-size_t nbx_sort_temp_bytes(int64_t n, int bits) {
-    size_t bytes = 0;
-    rocprim::radix_sort_pairs((void *)nullptr, bytes, (uint64_t *)nullptr, (uint64_t *)nullptr, (int32_t *)nullptr,
-                              (int32_t *)nullptr, (unsigned int)n, 0, bits, (hipStream_t)0, false);
-    return bytes;
-}
-
-hipError_t nbx_sort_pairs(void *temp, size_t temp_bytes, uint64_t *kin, uint64_t *kout, int32_t *vin, int32_t *vout,
-                          int64_t n, int bits, hipStream_t st) {
-    return rocprim::radix_sort_pairs(temp, temp_bytes, kin, kout, vin, vout, (unsigned int)n, 0, bits, st, false);
-}
-
 // sort of 64-bit keys with 32-bit values (FOF group ranking)
 hipError_t nbx_sort_pairs_u64(void *temp, size_t &temp_bytes, uint64_t *kin, uint64_t *kout, int32_t *vin,
                               int32_t *vout, int64_t n, hipStream_t st) {

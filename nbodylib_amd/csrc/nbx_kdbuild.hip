@@ -1,0 +1,271 @@
+// nbx_kdbuild.hip -- particle order of the implicit kd tree on gfx950.
+//
+// Replaces the recursive median split of KDTree::BuildNodes
+// (KDTree.cxx:970-1055) with DetermineSplitDim/SpreadestPos (:459-504,
+// :27-48) and MedianPos (:328-370): every node of the implicit heap-ordered
+// tree over NBX_LEAF-slot leaves is split at its slot midpoint along the
+// dimension of largest extent, with the lower half (by that coordinate) going
+// left -- the reference's "spread" criterion and median split, applied to a
+// balanced tree whose node ranges are fixed powers of two of leaves.
+//
+// Top levels (node ranges longer than KD_LOCAL slots) run over the whole set
+// with one sorted index list per dimension: the split dimension of a node is
+// read from the ends of its lists, the lower half of the chosen list flags
+// the particles that go left, and every list is stably partitioned by that
+// flag (a global exclusive scan gives each element its rank on its side).
+// Lower levels run inside one workgroup per KD_LOCAL-slot range: FP32 copies
+// of the coordinates in LDS, per-range extents by LDS atomics, and bitonic
+// sorts of (coordinate, element) keys within each node range.
+//
+// Split coordinates are compared as FP32 (order-preserving 32-bit keys): a
+// split only decides which particles share a node, and every box is later
+// computed from the FP64 coordinates of its particles, so the tree is exact
+// whatever the rounding of the split.
+#include <cstring>
+#include <rocprim/rocprim.hpp>
+
+#include "nbx_internal.h"
+#include "nbx_kernels.h"
+
+#define KD_LOCAL 2048  // slots per workgroup in the local stage (64 leaves)
+#define KD_EBITS 12    // element index bits in a local sort key
+
+__device__ __forceinline__ uint32_t fkey(float f) {
+    const uint32_t b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float fdekey(uint32_t k) {
+    return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+}
+
+__global__ __launch_bounds__(256) void k_kd_key(const double *__restrict__ c, int64_t n, uint32_t *__restrict__ key,
+                                                int32_t *__restrict__ idx) {
+    int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    key[i] = fkey((float)c[i]);
+    idx[i] = (int32_t)i;
+}
+
+struct KdLists {
+    const double *c[6];
+    int32_t *S[6];   // per-dimension index lists (current)
+    int32_t *S2[6];  // next
+};
+
+// Split dimension of each node at this level: largest extent, read from the
+// ends of the node's sorted lists.
+__global__ __launch_bounds__(256) void k_kd_dims(KdLists L, int nd, int64_t n, int lseg, int64_t nseg,
+                                                 uint8_t *__restrict__ dsel) {
+    int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (s >= nseg) return;
+    const int64_t a = s << lseg;
+    if (a >= n) return;
+    const int64_t b = min(a + ((int64_t)1 << lseg), n) - 1;
+    int best = 0;
+    double bext = -1.0;
+    for (int d = 0; d < nd; ++d) {
+        const double e = L.c[d][L.S[d][b]] - L.c[d][L.S[d][a]];
+        if (e > bext) { bext = e; best = d; }
+    }
+    dsel[s] = (uint8_t)best;
+}
+
+// flag[id] = 1 for the particles of the lower half of each node along its
+// split dimension.
+__global__ __launch_bounds__(256) void k_kd_flag(KdLists L, int64_t n, int lseg, const uint8_t *__restrict__ dsel,
+                                                 uint8_t *__restrict__ flag) {
+    int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int64_t s = i >> lseg;
+    const int64_t a = s << lseg;
+    const int64_t m = min(a + ((int64_t)1 << (lseg - 1)), n);
+    const int d = dsel[s];
+    flag[L.S[d][i]] = i < m ? 1 : 0;
+}
+
+__global__ __launch_bounds__(256) void k_kd_getf(const int32_t *__restrict__ S, int64_t n,
+                                                 const uint8_t *__restrict__ flag, int32_t *__restrict__ f) {
+    int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    f[i] = flag[S[i]];
+}
+
+// Stable partition of one list inside every node: left elements keep their
+// order in [a, m), right ones in [m, b).
+__global__ __launch_bounds__(256) void k_kd_scatter(const int32_t *__restrict__ S, int32_t *__restrict__ S2, int64_t n,
+                                                    int lseg, const int32_t *__restrict__ f,
+                                                    const int32_t *__restrict__ pos) {
+    int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int64_t a = (i >> lseg) << lseg;
+    const int64_t m = min(a + ((int64_t)1 << (lseg - 1)), n);
+    const int64_t nl = (int64_t)pos[i] - pos[a];
+    const int64_t dst = f[i] ? a + nl : m + (i - a - nl);
+    S2[dst] = S[i];
+}
+
+// Local stage: the node ranges of one KD_LOCAL-slot block (or the whole set
+// when it is smaller), split level by level down to single leaves.
+template <int ND>
+__global__ __launch_bounds__(256) void k_kd_local(KdLists L, int64_t n, int lblk, int32_t *__restrict__ order) {
+    __shared__ float cf[ND][KD_LOCAL];
+    __shared__ int32_t ids[KD_LOCAL];
+    __shared__ uint64_t key[KD_LOCAL];
+    __shared__ uint32_t mn[KD_LOCAL / (2 * NBX_LEAF)][ND], mx[KD_LOCAL / (2 * NBX_LEAF)][ND];
+    __shared__ uint8_t dsel[KD_LOCAL / (2 * NBX_LEAF)];
+    const int tid = threadIdx.x;
+    const int ss = 1 << lblk;  // block span in slots (power of two, <= KD_LOCAL)
+    const int64_t a = (int64_t)blockIdx.x << lblk;
+    if (a >= n) return;
+    const int cnt = (int)min((int64_t)ss, n - a);
+    for (int j = tid; j < ss; j += 256) {
+        if (j < cnt) {
+            const int32_t id = L.S[0][a + j];
+            ids[j] = id;
+#pragma unroll
+            for (int d = 0; d < ND; ++d) cf[d][j] = (float)L.c[d][id];
+            key[j] = (uint64_t)j;
+        } else {
+            key[j] = ~0ULL;
+        }
+    }
+    __syncthreads();
+    const uint64_t EM = (1ULL << KD_EBITS) - 1;
+    for (int s = ss; s >= 2 * NBX_LEAF; s >>= 1) {
+        const int nsub = ss / s;
+        for (int t = tid; t < nsub * ND; t += 256) {
+            mn[t / ND][t % ND] = 0xffffffffu;
+            mx[t / ND][t % ND] = 0u;
+        }
+        __syncthreads();
+        for (int j = tid; j < cnt; j += 256) {
+            const int e = (int)(key[j] & EM);
+            const int sub = j / s;
+#pragma unroll
+            for (int d = 0; d < ND; ++d) {
+                const uint32_t k = fkey(cf[d][e]);
+                atomicMin(&mn[sub][d], k);
+                atomicMax(&mx[sub][d], k);
+            }
+        }
+        __syncthreads();
+        for (int t = tid; t < nsub; t += 256) {
+            int best = 0;
+            float bext = -1.0f;
+            for (int d = 0; d < ND; ++d) {
+                const float e = mx[t][d] >= mn[t][d] ? fdekey(mx[t][d]) - fdekey(mn[t][d]) : 0.0f;
+                if (e > bext) { bext = e; best = d; }
+            }
+            dsel[t] = (uint8_t)best;
+        }
+        __syncthreads();
+        for (int j = tid; j < cnt; j += 256) {
+            const int e = (int)(key[j] & EM);
+            const int d = dsel[j / s];
+            key[j] = ((uint64_t)fkey(cf[d][e]) << KD_EBITS) | (uint64_t)e;
+        }
+        __syncthreads();
+        // bitonic sort of every aligned run of s keys, all ascending
+        for (int k = 2; k <= s; k <<= 1) {
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                for (int i = tid; i < ss; i += 256) {
+                    const int ixj = i ^ j;
+                    if (ixj > i) {
+                        const bool up = (k == s) || ((i & k) == 0);
+                        const uint64_t x = key[i], y = key[ixj];
+                        if ((x > y) == up) {
+                            key[i] = y;
+                            key[ixj] = x;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+    }
+    for (int j = tid; j < cnt; j += 256) order[a + j] = ids[(int)(key[j] & EM)];
+}
+
+static inline int nblk256(int64_t n) { return (int)((n + 255) / 256); }
+
+hipError_t nbx_kd_order(const double *const *c, int nd, int64_t n, int32_t nleafp, int32_t *order, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    if (nd != 3 && nd != 6) return hipErrorInvalidValue;
+    int ltot = 0;  // log2 of the padded slot span
+    while (((int64_t)1 << ltot) < (int64_t)nleafp * NBX_LEAF) ++ltot;
+    int lloc = 0;
+    while ((1 << lloc) < KD_LOCAL) ++lloc;
+    const int lblk = ltot < lloc ? ltot : lloc;
+
+    KdLists Lh;
+    memset(&Lh, 0, sizeof(Lh));
+    hipError_t e = hipSuccess;
+    uint32_t *k0 = nullptr, *k1 = nullptr;
+    int32_t *iota = nullptr, *f = nullptr, *pos = nullptr;
+    uint8_t *flag = nullptr, *dsel = nullptr;
+    void *tmp = nullptr;
+    size_t tb = 0, tb2 = 0;
+#define KDCK(x)                          \
+    do {                                 \
+        e = (x);                         \
+        if (e != hipSuccess) goto done;  \
+    } while (0)
+    for (int d = 0; d < nd; ++d) {
+        Lh.c[d] = c[d];
+        KDCK(hipMallocAsync((void **)&Lh.S[d], sizeof(int32_t) * n, st));
+        if (ltot > lblk) KDCK(hipMallocAsync((void **)&Lh.S2[d], sizeof(int32_t) * n, st));
+    }
+    KDCK(hipMallocAsync((void **)&k0, sizeof(uint32_t) * n, st));
+    KDCK(hipMallocAsync((void **)&k1, sizeof(uint32_t) * n, st));
+    KDCK(hipMallocAsync((void **)&iota, sizeof(int32_t) * n, st));
+    KDCK(rocprim::radix_sort_pairs((void *)nullptr, tb, k0, k1, iota, Lh.S[0], (unsigned int)n, 0, 32, st, false));
+    KDCK(rocprim::exclusive_scan((void *)nullptr, tb2, (int32_t *)nullptr, (int32_t *)nullptr, 0, (size_t)n,
+                                 rocprim::plus<int32_t>(), st, false));
+    KDCK(hipMallocAsync(&tmp, tb > tb2 ? tb : tb2, st));
+    // one index list per dimension, sorted by that coordinate (FP32 keys)
+    for (int d = 0; d < (ltot > lblk ? nd : 1); ++d) {
+        hipLaunchKernelGGL(k_kd_key, dim3(nblk256(n)), dim3(256), 0, st, c[d], n, k0, iota);
+        KDCK(rocprim::radix_sort_pairs(tmp, tb, k0, k1, iota, Lh.S[d], (unsigned int)n, 0, 32, st, false));
+    }
+    if (ltot > lblk) {
+        KDCK(hipMallocAsync((void **)&flag, (size_t)n, st));
+        KDCK(hipMallocAsync((void **)&f, sizeof(int32_t) * n, st));
+        KDCK(hipMallocAsync((void **)&pos, sizeof(int32_t) * n, st));
+        KDCK(hipMallocAsync((void **)&dsel, (size_t)1 << (ltot - lblk), st));
+        for (int lseg = ltot; lseg > lblk; --lseg) {
+            const int64_t nseg = (int64_t)1 << (ltot - lseg);
+            hipLaunchKernelGGL(k_kd_dims, dim3(nblk256(nseg)), dim3(256), 0, st, Lh, nd, n, lseg, nseg, dsel);
+            hipLaunchKernelGGL(k_kd_flag, dim3(nblk256(n)), dim3(256), 0, st, Lh, n, lseg, dsel, flag);
+            for (int d = 0; d < nd; ++d) {
+                hipLaunchKernelGGL(k_kd_getf, dim3(nblk256(n)), dim3(256), 0, st, Lh.S[d], n, flag, f);
+                KDCK(rocprim::exclusive_scan(tmp, tb2, f, pos, 0, (size_t)n, rocprim::plus<int32_t>(), st, false));
+                hipLaunchKernelGGL(k_kd_scatter, dim3(nblk256(n)), dim3(256), 0, st, Lh.S[d], Lh.S2[d], n, lseg, f,
+                                   pos);
+            }
+            for (int d = 0; d < nd; ++d) std::swap(Lh.S[d], Lh.S2[d]);
+        }
+    }
+    {
+        const int64_t nb = (n + ((int64_t)1 << lblk) - 1) >> lblk;
+        if (nd == 3)
+            hipLaunchKernelGGL(k_kd_local<3>, dim3((unsigned)nb), dim3(256), 0, st, Lh, n, lblk, order);
+        else
+            hipLaunchKernelGGL(k_kd_local<6>, dim3((unsigned)nb), dim3(256), 0, st, Lh, n, lblk, order);
+    }
+    e = hipGetLastError();
+done:
+    for (int d = 0; d < 6; ++d) {
+        if (Lh.S[d]) hipFreeAsync(Lh.S[d], st);
+        if (Lh.S2[d]) hipFreeAsync(Lh.S2[d], st);
+    }
+    if (k0) hipFreeAsync(k0, st);
+    if (k1) hipFreeAsync(k1, st);
+    if (iota) hipFreeAsync(iota, st);
+    if (f) hipFreeAsync(f, st);
+    if (pos) hipFreeAsync(pos, st);
+    if (flag) hipFreeAsync(flag, st);
+    if (dsel) hipFreeAsync(dsel, st);
+    if (tmp) hipFreeAsync(tmp, st);
+#undef KDCK
+    return e;
+}

@@ -6,14 +6,6 @@
 
 #include "nbx_internal.h"
 
-struct MortonArgs {
-    int64_t n;
-    int nd;
-    const double *c[6];
-    double lo[6], scale[6];
-    uint64_t *keys;
-    int32_t *vals;
-};
 struct GatherArgs {
     int64_t n, npad;
     int nd;
@@ -43,15 +35,14 @@ struct MixedArgs {
 };
 
 __global__ void k_bbox(const double *c, int64_t n, double *part);
-__global__ void k_morton(MortonArgs a);
 __global__ void k_gather(GatherArgs a);
 __global__ void k_leafbox(LeafBoxArgs a);
 __global__ void k_levelbox(double *box, float *plane, int32_t first, int32_t count, int nd);
 __global__ void k_mixed_offsets(MixedArgs a);
 
-size_t nbx_sort_temp_bytes(int64_t n, int bits);
-hipError_t nbx_sort_pairs(void *temp, size_t temp_bytes, uint64_t *kin, uint64_t *kout, int32_t *vin, int32_t *vout,
-                          int64_t n, int bits, hipStream_t st);
+// order (slot -> ID) of the implicit kd tree: midpoint splits along the
+// widest dimension of every node (nbx_kdbuild.hip)
+hipError_t nbx_kd_order(const double *const *c, int nd, int64_t n, int32_t nleafp, int32_t *order, hipStream_t st);
 hipError_t nbx_sort_pairs_u64(void *temp, size_t &temp_bytes, uint64_t *kin, uint64_t *kout, int32_t *vin,
                               int32_t *vout, int64_t n, hipStream_t st);
 
