@@ -9,7 +9,7 @@
  * Index spaces (as in the reference):
  *   ID    = position of a particle in the caller's input array (the reference
  *           overwrites Particle::id with it, KDTree.cxx:1291).
- *   slot  = position in this library's tree order (Morton order).  It plays
+ *   slot  = position in this library's tree order (kd split order).  It plays
  *           the role of the reference's "tree index" (KDTree.h:295).
  *
  * All functions return NBX_OK (0) or a negative status; nbx_last_error()
@@ -137,7 +137,8 @@ int nbx_fof_dev(nbx_tree *t, double fdist, int32_t minnum, int32_t *labels_by_id
  * sets h_i = |x_i - x_K|/2).  pos: 3*n SoA.  All device pointers. */
 int nbx_sph_density_fwd_dev(const double *pos_dev, const double *mass_dev, const int32_t *nbr_dev, int64_t n,
                             int32_t k, double *rho_dev, double *h_dev, void *stream);
-/* Adjoint: given g = dL/drho, accumulates dL/dpos (3*n SoA) and dL/dmass (n). */
+/* Adjoint: given g = dL/drho, writes dL/dpos (3*n SoA) and dL/dmass (n);
+ * both outputs are cleared by the call first. */
 int nbx_sph_density_adjoint_dev(const double *pos_dev, const double *mass_dev, const int32_t *nbr_dev, int64_t n,
                                 int32_t k, const double *g_dev, double *dpos_dev, double *dmass_dev, void *stream);
 

@@ -11,7 +11,7 @@
  *
  * Semantics kept from the reference: the constructor overwrites Particle::id
  * with the input index and reorders the caller's array into tree order (here
- * Morton order); "tree indices" returned by the kNN calls index that array;
+ * kd split order); "tree indices" returned by the kNN calls index that array;
  * Group/FOF arrays are keyed by ID; ~KDTree sorts the array back by ID.
  * Errors print a message and exit(1) as the reference does, unless
  * KDTree::SetThrowOnError(true) selects std::runtime_error instead.

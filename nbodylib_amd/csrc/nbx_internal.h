@@ -1,6 +1,6 @@
 // nbx_internal.h -- device-side layout of the KDTree hot path on gfx950.
 //
-// Particles live in HBM as SoA arrays sorted by Morton key ("slots").  Leaves
+// Particles live in HBM as SoA arrays in kd split order ("slots").  Leaves
 // are fixed buckets of NBX_LEAF consecutive slots; internal nodes form an
 // implicit complete binary tree in heap order over the leaf count padded to a
 // power of two (nleafp): the root is node 1, node n has children 2n and 2n+1,
@@ -136,7 +136,7 @@ __device__ __forceinline__ bool first_is_right(const DevTree &T, int p, const fl
 
 // Workgroups of a launch are dealt round-robin to the 8 XCDs (workgroup b
 // runs on XCD b % 8), each with its own L2.  Renumber them so that XCD x
-// walks one contiguous run of Morton-ordered waves: concurrently resident
+// walks one contiguous run of tree-ordered waves: concurrently resident
 // waves on an XCD then share tree nodes and leaves in that XCD's L2 instead
 // of every L2 fetching every neighbourhood.  A bijection for any grid size.
 __device__ __forceinline__ int xcd_wave(int b, int nb) {

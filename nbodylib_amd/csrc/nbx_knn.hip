@@ -6,19 +6,19 @@
 // (KDSplitNode.cxx:1119-1179), CalcDensity / CalcDensityParticle /
 // CalcVelDensity (KDCalcSmoothQuantities.cxx:203-389, :768-844).
 //
-// Mapping to CDNA4: one wavefront owns 64 Morton-consecutive queries (two
+// Mapping to CDNA4: one wavefront owns 64 slot-consecutive queries (two
 // sibling leaves).  The wave walks the implicit tree once for all 64 lanes
 // (a node is opened when any lane's bound reaches it), so node boxes and
 // leaf particles arrive through the scalar cache as 64-byte s_load bursts
 // and are shared by the whole wave.  Each lane keeps its own 4-ary max-heap
-// of (d2, slot) in LDS -- the reference's PriorityQueue
+// of (d2, particle ID) in LDS -- the reference's PriorityQueue
 // (PriorityQueue.h:47-84), with three dependent levels instead of five for
 // k = 32 -- laid out [position][lane] so same-position accesses are
 // conflict-free.  A leaf is consumed in two phases: (1) all lanes test every
 // candidate against their current k-th distance, building a 32-bit accept
 // mask; (2) lanes insert their accepted candidates one at a time from an LDS
-// copy of the leaf, so the wave pays for as many heap insertions as its
-// busiest lane, not the sum over lanes.
+// copy of the leaf with a branch-free root replacement, so the wave pays for
+// as many heap insertions as its busiest lane, not the sum over lanes.
 //
 // Exactness: distances are summed in the reference's order with no FMA
 // (this file is built with -ffp-contract=off); pruning uses box_lb, which is
