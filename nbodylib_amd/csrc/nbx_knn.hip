@@ -174,11 +174,12 @@ template <int ND>
 __device__ __forceinline__ void knn_leaf(const DevTree &T, int leaf, LeafCtx<ND> &c, double *hd, int32_t *hid,
                                          double *cx, int32_t *cid, int lane, int hc) {
     const int base = leaf * NBX_LEAF;
-    if (lane < NBX_LEAF) {
+    // the LDS copy of the leaf for phase 2: loaded now, stored after phase 1
+    // so the vector-load latency overlaps the candidate tests
+    double st[ND];
 #pragma unroll
-        for (int d = 0; d < ND; ++d) cx[d * NBX_LEAF + lane] = T.x[d][base + lane];
-        cid[lane] = T.perm[base + lane];
-    }
+    for (int d = 0; d < ND; ++d) st[d] = T.x[d][base + (lane & (NBX_LEAF - 1))];
+    const int32_t sid = T.perm[base + (lane & (NBX_LEAF - 1))];
     uint32_t mask = 0;
     // candidates arrive in chunks through the scalar cache: 8 per dimension
     // (24 SGPR pairs) in 3-D, 4 in 6-D
@@ -206,6 +207,11 @@ __device__ __forceinline__ void knn_leaf(const DevTree &T, int leaf, LeafCtx<ND>
         }
     }
     if (!c.act) mask = 0;
+    if (lane < NBX_LEAF) {
+#pragma unroll
+        for (int d = 0; d < ND; ++d) cx[d * NBX_LEAF + lane] = st[d];
+        cid[lane] = sid;
+    }
     __syncthreads();
     while (ballot(mask != 0)) {
         const bool has = mask != 0;
@@ -308,6 +314,7 @@ __global__ __launch_bounds__(64) void k_knn(DevTree T, KnnArgs A) {
     __shared__ int32_t hid[KMAX * NBX_WAVE];
     __shared__ double cx[ND * NBX_LEAF];
     __shared__ int32_t cid[NBX_LEAF];
+    __shared__ double racc[NBX_WAVE];  // symmetric scatter into the wave's own queries
     const int lane = threadIdx.x;
     const int g = xcd_wave(blockIdx.x, gridDim.x);
     const int kh = A.kh;
@@ -357,6 +364,13 @@ __global__ __launch_bounds__(64) void k_knn(DevTree T, KnnArgs A) {
     const double kth = c.kth;
 
     // ------------------------------------------------------------ densities
+    // Symmetric scatter: contributions to the wave's own 64 queries (slot
+    // runs only) are summed in LDS and flushed with one atomic per query.
+    const bool sym = QSRC == NBX_Q_SLOTS && A.dens == NBX_DO_SYM;
+    const bool own = sym && !listed;
+    double racc_own = 0.0;
+    if (own) racc[lane] = 0.0;
+    __syncthreads();
     if (QSRC == NBX_Q_SLOTS && A.dens != NBX_DO_NONE && valid) {
         bool bad = false;
         if (A.dens == NBX_DO_VEL) {
@@ -392,28 +406,48 @@ __global__ __launch_bounds__(64) void k_knn(DevTree T, KnnArgs A) {
             double norm = 1.0 / pow(hi, (double)ND);
             double mi = T.mass[qi];
             double acc = 0.0;
-            for (int p = 0; p < hc; ++p) {
-                const int32_t id = HI(p);
-                if (id < 0) continue;  // filler
-                if (id == NBX_NOID) { bad = true; continue; }
-                const int32_t s = T.inv[id];
-                double rij = sqrt(HD(p));
-                double W = wsm_at(rij, hi, T.kern, T.kernres);
-                if (A.dens == NBX_DO_SYM) {
-                    double Wij = 0.5 * W * norm;
-                    acc = acc + Wij * T.mass[s];
-                    unsafeAtomicAdd(A.rho + s, Wij * mi);
-                } else {
-                    double Wij = W * norm;
-                    acc += Wij * T.mass[s];
+            // entries in groups of 8 so their slot/mass gathers are in flight
+            // together; summed in heap-position order
+            for (int p0 = 0; p0 < hc; p0 += 8) {
+                int32_t idv[8], sv[8];
+                double dv[8], mv[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const bool in = p0 + u < hc;
+                    idv[u] = in ? HI(p0 + u) : -1;
+                    dv[u] = in ? HD(p0 + u) : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) sv[u] = (idv[u] >= 0 && idv[u] != NBX_NOID) ? T.inv[idv[u]] : qi;
+#pragma unroll
+                for (int u = 0; u < 8; ++u) mv[u] = T.mass[sv[u]];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int32_t id = idv[u];
+                    if (id < 0) continue;  // filler or past the end
+                    if (id == NBX_NOID) { bad = true; continue; }
+                    const int32_t s = sv[u];
+                    double W = wsm_at(sqrt(dv[u]), hi, T.kern, T.kernres);
+                    if (A.dens == NBX_DO_SYM) {
+                        double Wij = 0.5 * W * norm;
+                        acc = acc + Wij * mv[u];
+                        const uint32_t r = (uint32_t)(s - g * NBX_WAVE);
+                        if (own && r < NBX_WAVE) atomicAdd(&racc[r], Wij * mi);
+                        else unsafeAtomicAdd(A.rho + s, Wij * mi);
+                    } else {
+                        double Wij = W * norm;
+                        acc += Wij * mv[u];
+                    }
                 }
             }
-            if (A.dens == NBX_DO_SYM) unsafeAtomicAdd(A.rho + qi, acc);
+            if (A.dens == NBX_DO_SYM) racc_own = acc;
             else A.rho[qi] = acc;
             if (A.hout) A.hout[qi] = hi;
         }
         if (bad) atomicOr(A.err, 1);
     }
+    __syncthreads();
+    if (sym && valid) unsafeAtomicAdd(A.rho + qi, racc_own + (own ? racc[lane] : 0.0));
 
     // ------------------------------------------------------------ outputs
     if (A.nn == nullptr && A.d2 == nullptr) return;

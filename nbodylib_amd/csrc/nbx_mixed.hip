@@ -124,10 +124,11 @@ struct MixCtx {
 __device__ __forceinline__ void mix_leaf(const DevTree &T, const double *ext, int leaf, MixCtx &c, bool want,
                                          uint64_t *hk, float *cx, int lane, int hc) {
     const int base = leaf * NBX_LEAF;
-    if (lane < NBX_LEAF) {
+    // the LDS copy of the leaf for phase 2: loaded now, stored after phase 1
+    // so the vector-load latency overlaps the candidate tests
+    float st[3];
 #pragma unroll
-        for (int d = 0; d < 3; ++d) cx[d * NBX_LEAF + lane] = T.xf[d][base + lane];
-    }
+    for (int d = 0; d < 3; ++d) st[d] = T.xf[d][base + (lane & (NBX_LEAF - 1))];
     // query relative to this leaf's centre, rounded once to FP32
     const double *o = T.org + (size_t)leaf * 3;
     float qx = (float)(c.q[0] - ((cdptr)o)[0]);
@@ -150,6 +151,10 @@ __device__ __forceinline__ void mix_leaf(const DevTree &T, const double *ext, in
     if (c.self >= base && c.self < base + NBX_LEAF) mask &= ~(1u << (c.self - base));
     if (!c.act) mask = 0;
     if (c.act && (want || mask)) c.emax = fmax(c.emax, ((cdptr)ext)[leaf]);
+    if (lane < NBX_LEAF) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) cx[d * NBX_LEAF + lane] = st[d];
+    }
     __syncthreads();
     while (ballot(mask != 0)) {
         const bool has = mask != 0;
@@ -276,6 +281,7 @@ __global__ __launch_bounds__(64) void k_knn_mixed(DevTree T, KnnArgs A, const do
                                                    int32_t *fb_count) {
     __shared__ uint64_t hk[KMAX * NBX_WAVE];
     __shared__ float cx[3 * NBX_LEAF];
+    __shared__ double racc[NBX_WAVE];  // symmetric scatter into the wave's own queries
     const int lane = threadIdx.x;
     const int g = xcd_wave(blockIdx.x, gridDim.x);
     const int kh = A.kh;  // k + 1
@@ -411,29 +417,54 @@ __global__ __launch_bounds__(64) void k_knn_mixed(DevTree T, KnnArgs A, const do
     }
 
     // ---------------------------------------------------------------- density
-    if (ok && A.dens != NBX_DO_NONE) {
+    // Symmetric scatter: contributions to the wave's own 64 queries (about
+    // half of all neighbour entries) are summed in LDS and flushed with one
+    // atomic per query; the rest go straight to HBM.
+    const bool sym = A.dens == NBX_DO_SYM;
+    double racc_own = 0.0;
+    const bool dens_here = ok && A.dens != NBX_DO_NONE;
+    if (sym) racc[lane] = 0.0;
+    __syncthreads();
+    if (dens_here) {
         // weights from FP64 distances of the k selected neighbours (the FP32
         // offsets carry a storage error proportional to the leaf size)
         const double hi = 0.5 * sqrt(Dk);
         const double norm = 1.0 / pow(hi, 3.0);
         const double mi = T.mass[qi];
         double acc = 0.0;
-        for (int p = 0; p < k; ++p) {
-            int32_t s = keys(HK(p));
-            double rij = sqrt(d64(T, c.q, s));
-            double W = wsm_at(rij, hi, T.kern, kres);
-            if (A.dens == NBX_DO_SYM) {
-                double Wij = 0.5 * W * norm;
-                acc = acc + Wij * T.mass[s];
-                unsafeAtomicAdd(A.rho + s, Wij * mi);
-            } else {
-                acc += W * norm * T.mass[s];
+        // neighbours in groups of 8 so their coordinate/mass gathers are in
+        // flight together
+        for (int p0 = 0; p0 < k; p0 += 8) {
+            int32_t sv[8];
+            double d2v[8], mv[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                sv[u] = p0 + u < k ? keys(HK(p0 + u)) : qi;
+                d2v[u] = d64(T, c.q, sv[u]);
+                mv[u] = T.mass[sv[u]];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                if (p0 + u >= k) break;
+                const int32_t s = sv[u];
+                double W = wsm_at(sqrt(d2v[u]), hi, T.kern, kres);
+                if (sym) {
+                    double Wij = 0.5 * W * norm;
+                    acc = acc + Wij * mv[u];
+                    const uint32_t r = (uint32_t)(s - g * NBX_WAVE);
+                    if (r < NBX_WAVE) atomicAdd(&racc[r], Wij * mi);
+                    else unsafeAtomicAdd(A.rho + s, Wij * mi);
+                } else {
+                    acc += W * norm * mv[u];
+                }
             }
         }
-        if (A.dens == NBX_DO_SYM) unsafeAtomicAdd(A.rho + qi, acc);
+        if (sym) racc_own = acc;
         else A.rho[qi] = acc;
         if (A.hout) A.hout[qi] = hi;
     }
+    __syncthreads();
+    if (sym && valid) unsafeAtomicAdd(A.rho + qi, racc_own + racc[lane]);
 
     // ---------------------------------------------------------------- outputs
     if (A.nn == nullptr) return;
