@@ -174,12 +174,11 @@ template <int ND>
 __device__ __forceinline__ void knn_leaf(const DevTree &T, int leaf, LeafCtx<ND> &c, double *hd, int32_t *hid,
                                          double *cx, int32_t *cid, int lane, int hc) {
     const int base = leaf * NBX_LEAF;
-    // the LDS copy of the leaf for phase 2: loaded now, stored after phase 1
-    // so the vector-load latency overlaps the candidate tests
-    double st[ND];
+    if (lane < NBX_LEAF) {
 #pragma unroll
-    for (int d = 0; d < ND; ++d) st[d] = T.x[d][base + (lane & (NBX_LEAF - 1))];
-    const int32_t sid = T.perm[base + (lane & (NBX_LEAF - 1))];
+        for (int d = 0; d < ND; ++d) cx[d * NBX_LEAF + lane] = T.x[d][base + lane];
+        cid[lane] = T.perm[base + lane];
+    }
     uint32_t mask = 0;
     // candidates arrive in chunks through the scalar cache: 8 per dimension
     // (24 SGPR pairs) in 3-D, 4 in 6-D
@@ -207,11 +206,6 @@ __device__ __forceinline__ void knn_leaf(const DevTree &T, int leaf, LeafCtx<ND>
         }
     }
     if (!c.act) mask = 0;
-    if (lane < NBX_LEAF) {
-#pragma unroll
-        for (int d = 0; d < ND; ++d) cx[d * NBX_LEAF + lane] = st[d];
-        cid[lane] = sid;
-    }
     __syncthreads();
     while (ballot(mask != 0)) {
         const bool has = mask != 0;
