@@ -41,13 +41,26 @@ typedef float fx8 __attribute__((ext_vector_type(8)));
 typedef const __attribute__((address_space(4))) fx8 *cf8ptr;
 
 #define MX_U 5.9604644775390625e-8  // 2^-24
-#define MX_EMPTY 0x7F7FFFFFFFFFFFFFULL  // key of an empty heap entry (FLT_MAX, slot ~0)
+// Heap keys: (bits(d32) << 32 | slot) + 2^52.  The offset adds one to the
+// exponent field of the key read as an FP64 number, so every key is a
+// normal, finite, positive double whose order is the integer order: the
+// largest of several keys is one v_max_f64 per pair.
+#define MX_KOFF (1ULL << 52)
+#define MX_EMPTY (0x7F7FFFFFFFFFFFFFULL + MX_KOFF)  // empty heap entry (FLT_MAX, slot ~0)
 
 __device__ __forceinline__ uint64_t mkkey(float d, int32_t s) {
-    return ((uint64_t)__float_as_uint(d) << 32) | (uint32_t)s;
+    return (((uint64_t)__float_as_uint(d) << 32) | (uint32_t)s) + MX_KOFF;
 }
-__device__ __forceinline__ float keyd(uint64_t k) { return __uint_as_float((uint32_t)(k >> 32)); }
+__device__ __forceinline__ float keyd(uint64_t k) { return __uint_as_float((uint32_t)((k - MX_KOFF) >> 32)); }
 __device__ __forceinline__ int32_t keys(uint64_t k) { return (int32_t)(uint32_t)k; }
+__device__ __forceinline__ double kasd(uint64_t k) { return __longlong_as_double((long long)k); }
+__device__ __forceinline__ uint64_t dask(double v) { return (uint64_t)__double_as_longlong(v); }
+
+// Largest of four keys and its index 0..3.
+__device__ __forceinline__ void kmax4(uint64_t x0, uint64_t x1, uint64_t x2, uint64_t x3, uint64_t &bc, int &bi) {
+    bc = dask(fmax(fmax(kasd(x0), kasd(x1)), fmax(kasd(x2), kasd(x3))));
+    bi = bc == x1 ? 1 : (bc == x2 ? 2 : (bc == x3 ? 3 : 0));
+}
 
 #define HK(p) hk[(p) * NBX_WAVE + lane]
 
@@ -67,12 +80,9 @@ __device__ __forceinline__ uint64_t kheap_replace_root(uint64_t *hk, int lane, i
         const int c = 4 * p + 1;
         go = go && c < hc;
         const int cc = go ? c : 1;
-        const uint64_t x0 = HK(cc), x1 = HK(cc + 1), x2 = HK(cc + 2), x3 = HK(cc + 3);
-        const bool s01 = x1 > x0, s23 = x3 > x2;
-        const uint64_t m01 = s01 ? x1 : x0, m23 = s23 ? x3 : x2;
-        const bool s = m23 > m01;
-        const uint64_t bc = s ? m23 : m01;
-        const int bi = s ? (s23 ? 3 : 2) : (s01 ? 1 : 0);
+        uint64_t bc;
+        int bi;
+        kmax4(HK(cc), HK(cc + 1), HK(cc + 2), HK(cc + 3), bc, bi);
         go = go && bc > v;
         HK(p) = go ? bc : v;
         if (lv == 0) top = go ? bc : v;
@@ -98,11 +108,9 @@ __device__ __forceinline__ void kheap_sift_bounded(uint64_t *hk, int lane, int m
             const uint64_t t = HK(cc + u);
             x[u] = cc + u < m ? t : 0ULL;
         }
-        const bool s01 = x[1] > x[0], s23 = x[3] > x[2];
-        const uint64_t m01 = s01 ? x[1] : x[0], m23 = s23 ? x[3] : x[2];
-        const bool s = m23 > m01;
-        const uint64_t bc = s ? m23 : m01;
-        const int bi = s ? (s23 ? 3 : 2) : (s01 ? 1 : 0);
+        uint64_t bc;
+        int bi;
+        kmax4(x[0], x[1], x[2], x[3], bc, bi);
         go = go && bc > v;
         HK(p) = go ? bc : v;
         p = go ? cc + bi : p;
