@@ -13,112 +13,186 @@
 //   dL/dx_i += a_ij (sigma/2) W'(q) h^-4 (x_i-x_j)/r   (and minus on x_j),
 //   dL/dh_i  = sum_j a_ij (-(sigma/2) h^-4 (q W'(q) + 3 W(q))),
 //   dL/dx_i += dL/dh_i (x_i-x_K)/(2 r_iK)               (and minus on x_K).
-// One thread per particle i; contributions to other particles use hardware
-// FP64 atomics.
+// One thread per particle i; contributions to other particles use FP64
+// atomics (LDS ones for particles of the same block).
 #include "nbx_kernels.h"
 
+// Cubic spline without branches: with t = max(2 - q, 0), s = max(1 - q, 0),
+// W = t^3/4 - s^3 (= 1 - 1.5q^2 + 0.75q^3 on [0,1], (2-q)^3/4 on [1,2], 0
+// beyond) and W' = -0.75 t^2 + 3 s^2.
 __device__ __forceinline__ double w_cubic(double q) {
-    if (q <= 1.0) return 1.0 - 1.5 * q * q + 0.75 * q * q * q;
-    if (q <= 2.0) { double t = 2.0 - q; return 0.25 * t * t * t; }
-    return 0.0;
+    const double t = fmax(2.0 - q, 0.0), s = fmax(1.0 - q, 0.0);
+    return 0.25 * t * t * t - s * s * s;
 }
 __device__ __forceinline__ double dw_cubic(double q) {
-    if (q <= 1.0) return -3.0 * q + 2.25 * q * q;
-    if (q <= 2.0) { double t = 2.0 - q; return -0.75 * t * t; }
-    return 0.0;
+    const double t = fmax(2.0 - q, 0.0), s = fmax(1.0 - q, 0.0);
+    return -0.75 * t * t + 3.0 * s * s;
 }
 
 #define SIGMA3 0.31830988618379067154  // 1/pi
 
-__global__ __launch_bounds__(256) void k_sph_fwd(const double *__restrict__ pos, const double *__restrict__ mass,
-                                                  const int32_t *__restrict__ nbr, int64_t n, int k, double *rho,
-                                                  double *hout) {
-    int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    const double *X = pos, *Y = pos + n, *Z = pos + 2 * n;
-    double xi = X[i], yi = Y[i], zi = Z[i], mi = mass[i];
-    const int32_t *row = nbr + i * k;
-    int64_t K = row[k - 1];
-    double dx = xi - X[K], dy = yi - Y[K], dz = zi - Z[K];
-    double h = 0.5 * sqrt(dx * dx + dy * dy + dz * dz);
-    if (hout) hout[i] = h;
-    double ih = 1.0 / h;
-    double norm = 0.5 * SIGMA3 * ih * ih * ih;
-    double acc = 0.0;
-    for (int j = 0; j < k; ++j) {
-        int64_t q = row[j];
-        double ex = xi - X[q], ey = yi - Y[q], ez = zi - Z[q];
-        double r = sqrt(ex * ex + ey * ey + ez * ez);
-        double w = norm * w_cubic(r * ih);
-        acc += w * mass[q];
-        unsafeAtomicAdd(rho + q, w * mi);
-    }
-    unsafeAtomicAdd(rho + i, acc);
-}
+// Contributions to particles of the same 1024-particle block (most of them
+// when the arrays are in tree order) are summed in LDS and flushed with one
+// atomic per particle and component; the rest go straight to HBM.
+#define ADJ_B 1024
+#define ADJ_G 8   // neighbours whose gathers are issued together (forward)
+#define ADJ_GB 4  // (adjoint: more live values per neighbour)
 
-__global__ __launch_bounds__(256) void k_sph_adj(const double *__restrict__ pos, const double *__restrict__ mass,
-                                                  const int32_t *__restrict__ nbr, int64_t n, int k,
-                                                  const double *__restrict__ g, double *dpos, double *dmass) {
-    int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    const double *X = pos, *Y = pos + n, *Z = pos + 2 * n;
-    double *DX = dpos, *DY = dpos + n, *DZ = dpos + 2 * n;
-    double xi = X[i], yi = Y[i], zi = Z[i], mi = mass[i], gi = g[i];
-    const int32_t *row = nbr + i * k;
-    int64_t K = row[k - 1];
-    double kx = xi - X[K], ky = yi - Y[K], kz = zi - Z[K];
-    double rk = sqrt(kx * kx + ky * ky + kz * kz);
-    double h = 0.5 * rk;
-    double ih = 1.0 / h;
-    double ih3 = ih * ih * ih, ih4 = ih3 * ih;
-    double gx = 0.0, gy = 0.0, gz = 0.0, gm = 0.0, Hs = 0.0;
-    for (int j = 0; j < k; ++j) {
-        int64_t q = row[j];
-        double ex = xi - X[q], ey = yi - Y[q], ez = zi - Z[q];
-        double r = sqrt(ex * ex + ey * ey + ez * ez);
-        double qq = r * ih;
-        double W = w_cubic(qq), dW = dw_cubic(qq);
-        double w = 0.5 * SIGMA3 * W * ih3;
-        double gq = g[q];
-        double a = gi * mass[q] + gq * mi;
-        unsafeAtomicAdd(dmass + q, w * gi);
-        gm += w * gq;
-        Hs += a * (-0.5 * SIGMA3 * (qq * dW + 3.0 * W) * ih4);
-        if (r > 0.0) {
-            double f = a * 0.5 * SIGMA3 * dW * ih4 / r;
-            gx += f * ex; gy += f * ey; gz += f * ez;
-            unsafeAtomicAdd(DX + q, -f * ex);
-            unsafeAtomicAdd(DY + q, -f * ey);
-            unsafeAtomicAdd(DZ + q, -f * ez);
+__global__ __launch_bounds__(ADJ_B) void k_sph_fwd(const double *__restrict__ pos, const double *__restrict__ mass,
+                                                   const int32_t *__restrict__ nbr, int64_t n, int k, double *rho,
+                                                   double *hout) {
+    __shared__ double lrho[ADJ_B];
+    const int t = threadIdx.x;
+    const int64_t i0 = (int64_t)blockIdx.x * ADJ_B;
+    const int64_t i = i0 + t;
+    lrho[t] = 0.0;
+    __syncthreads();
+    double acc = 0.0;
+    if (i < n) {
+        const double *X = pos, *Y = pos + n, *Z = pos + 2 * n;
+        const double xi = X[i], yi = Y[i], zi = Z[i], mi = mass[i];
+        const int32_t *row = nbr + i * k;
+        const int64_t K = row[k - 1];
+        const double dx = xi - X[K], dy = yi - Y[K], dz = zi - Z[K];
+        const double h = 0.5 * sqrt(dx * dx + dy * dy + dz * dz);
+        if (hout) hout[i] = h;
+        const double ih = 1.0 / h;
+        const double norm = 0.5 * SIGMA3 * ih * ih * ih;
+        for (int j0 = 0; j0 < k; j0 += ADJ_G) {
+            int64_t qv[ADJ_G];
+            double rv[ADJ_G], mv[ADJ_G];
+#pragma unroll
+            for (int u = 0; u < ADJ_G; ++u) {
+                qv[u] = j0 + u < k ? row[j0 + u] : i;
+                const double ex = xi - X[qv[u]], ey = yi - Y[qv[u]], ez = zi - Z[qv[u]];
+                rv[u] = sqrt(ex * ex + ey * ey + ez * ez);
+                mv[u] = mass[qv[u]];
+            }
+#pragma unroll
+            for (int u = 0; u < ADJ_G; ++u) {
+                if (j0 + u >= k) break;
+                const double w = norm * w_cubic(rv[u] * ih);
+                acc += w * mv[u];
+                const uint64_t r = (uint64_t)(qv[u] - i0);
+                if (r < ADJ_B) atomicAdd(&lrho[r], w * mi);
+                else unsafeAtomicAdd(rho + qv[u], w * mi);
+            }
         }
     }
-    if (rk > 0.0) {
-        double f = Hs * 0.5 / rk;
-        gx += f * kx; gy += f * ky; gz += f * kz;
-        unsafeAtomicAdd(DX + K, -f * kx);
-        unsafeAtomicAdd(DY + K, -f * ky);
-        unsafeAtomicAdd(DZ + K, -f * kz);
+    __syncthreads();
+    if (i < n) unsafeAtomicAdd(rho + i, acc + lrho[t]);
+}
+
+__global__ __launch_bounds__(ADJ_B) void k_sph_adj(const double *__restrict__ pos, const double *__restrict__ mass,
+                                                   const int32_t *__restrict__ nbr, int64_t n, int k,
+                                                   const double *__restrict__ g, double *dpos, double *dmass) {
+    __shared__ double lx[ADJ_B], ly[ADJ_B], lz[ADJ_B], lm[ADJ_B];
+    const int t = threadIdx.x;
+    const int64_t i0 = (int64_t)blockIdx.x * ADJ_B;
+    const int64_t i = i0 + t;
+    lx[t] = 0.0;
+    ly[t] = 0.0;
+    lz[t] = 0.0;
+    lm[t] = 0.0;
+    __syncthreads();
+    double *DX = dpos, *DY = dpos + n, *DZ = dpos + 2 * n;
+    double gx = 0.0, gy = 0.0, gz = 0.0, gm = 0.0;
+    if (i < n) {
+        const double *X = pos, *Y = pos + n, *Z = pos + 2 * n;
+        const double xi = X[i], yi = Y[i], zi = Z[i], mi = mass[i], gi = g[i];
+        const int32_t *row = nbr + i * k;
+        const int64_t K = row[k - 1];
+        const double kx = xi - X[K], ky = yi - Y[K], kz = zi - Z[K];
+        const double rk = sqrt(kx * kx + ky * ky + kz * kz);
+        const double h = 0.5 * rk;
+        const double ih = 1.0 / h;
+        const double ih3 = ih * ih * ih, ih4 = ih3 * ih;
+        double Hs = 0.0;
+        for (int j0 = 0; j0 < k; j0 += ADJ_GB) {
+            int64_t qv[ADJ_GB];
+            double exv[ADJ_GB], eyv[ADJ_GB], ezv[ADJ_GB], mv[ADJ_GB], gv[ADJ_GB];
+#pragma unroll
+            for (int u = 0; u < ADJ_GB; ++u) {
+                qv[u] = j0 + u < k ? row[j0 + u] : i;
+                exv[u] = xi - X[qv[u]];
+                eyv[u] = yi - Y[qv[u]];
+                ezv[u] = zi - Z[qv[u]];
+                mv[u] = mass[qv[u]];
+                gv[u] = g[qv[u]];
+            }
+#pragma unroll
+            for (int u = 0; u < ADJ_GB; ++u) {
+                if (j0 + u >= k) break;
+                const int64_t q = qv[u];
+                const double ex = exv[u], ey = eyv[u], ez = ezv[u];
+                const double r = sqrt(ex * ex + ey * ey + ez * ez);
+                const double qq = r * ih;
+                const double W = w_cubic(qq), dW = dw_cubic(qq);
+                const double w = 0.5 * SIGMA3 * W * ih3;
+                const double a = gi * mv[u] + gv[u] * mi;
+                const uint64_t lr = (uint64_t)(q - i0);
+                const bool loc = lr < ADJ_B;
+                if (loc) atomicAdd(&lm[lr], w * gi);
+                else unsafeAtomicAdd(dmass + q, w * gi);
+                gm += w * gv[u];
+                Hs += a * (-0.5 * SIGMA3 * (qq * dW + 3.0 * W) * ih4);
+                if (r > 0.0) {
+                    const double f = a * 0.5 * SIGMA3 * dW * ih4 / r;
+                    gx += f * ex;
+                    gy += f * ey;
+                    gz += f * ez;
+                    if (loc) {
+                        atomicAdd(&lx[lr], -f * ex);
+                        atomicAdd(&ly[lr], -f * ey);
+                        atomicAdd(&lz[lr], -f * ez);
+                    } else {
+                        unsafeAtomicAdd(DX + q, -f * ex);
+                        unsafeAtomicAdd(DY + q, -f * ey);
+                        unsafeAtomicAdd(DZ + q, -f * ez);
+                    }
+                }
+            }
+        }
+        if (rk > 0.0) {
+            const double f = Hs * 0.5 / rk;
+            gx += f * kx;
+            gy += f * ky;
+            gz += f * kz;
+            const uint64_t lr = (uint64_t)(K - i0);
+            if (lr < ADJ_B) {
+                atomicAdd(&lx[lr], -f * kx);
+                atomicAdd(&ly[lr], -f * ky);
+                atomicAdd(&lz[lr], -f * kz);
+            } else {
+                unsafeAtomicAdd(DX + K, -f * kx);
+                unsafeAtomicAdd(DY + K, -f * ky);
+                unsafeAtomicAdd(DZ + K, -f * kz);
+            }
+        }
     }
-    unsafeAtomicAdd(DX + i, gx);
-    unsafeAtomicAdd(DY + i, gy);
-    unsafeAtomicAdd(DZ + i, gz);
-    unsafeAtomicAdd(dmass + i, gm);
+    __syncthreads();
+    if (i < n) {
+        unsafeAtomicAdd(DX + i, gx + lx[t]);
+        unsafeAtomicAdd(DY + i, gy + ly[t]);
+        unsafeAtomicAdd(DZ + i, gz + lz[t]);
+        unsafeAtomicAdd(dmass + i, gm + lm[t]);
+    }
 }
 
 hipError_t nbx_launch_sph_fwd(const double *pos, const double *mass, const int32_t *nbr, int64_t n, int k, double *rho,
                               double *h, hipStream_t st) {
-    int blocks = (int)((n + 255) / 256);
+    int blocks = (int)((n + ADJ_B - 1) / ADJ_B);
     if (!blocks) return hipSuccess;
     ProfScope ps("k_sph_fwd", st);
-    hipLaunchKernelGGL(k_sph_fwd, dim3(blocks), dim3(256), 0, st, pos, mass, nbr, n, k, rho, h);
+    hipLaunchKernelGGL(k_sph_fwd, dim3(blocks), dim3(ADJ_B), 0, st, pos, mass, nbr, n, k, rho, h);
     return hipGetLastError();
 }
 
 hipError_t nbx_launch_sph_adj(const double *pos, const double *mass, const int32_t *nbr, int64_t n, int k,
                               const double *g, double *dpos, double *dmass, hipStream_t st) {
-    int blocks = (int)((n + 255) / 256);
+    int blocks = (int)((n + ADJ_B - 1) / ADJ_B);
     if (!blocks) return hipSuccess;
     ProfScope ps("k_sph_adj", st);
-    hipLaunchKernelGGL(k_sph_adj, dim3(blocks), dim3(256), 0, st, pos, mass, nbr, n, k, g, dpos, dmass);
+    hipLaunchKernelGGL(k_sph_adj, dim3(blocks), dim3(ADJ_B), 0, st, pos, mass, nbr, n, k, g, dpos, dmass);
     return hipGetLastError();
 }
