@@ -13,9 +13,8 @@
 // read from the ends of its lists, the lower half of the chosen list flags
 // the particles that go left, and every list is stably partitioned by that
 // flag (a global exclusive scan gives each element its rank on its side).
-// Lower levels run inside one workgroup per KD_LOCAL-slot range: FP32 copies
-// of the coordinates in LDS, per-range extents by LDS atomics, and bitonic
-// sorts of (coordinate, element) keys within each node range.
+// Lower levels run inside one workgroup per KD_LOCAL-slot range with the
+// same scheme on FP32 copies of the coordinates and 16-bit index lists in LDS.
 //
 // Split coordinates are compared as FP32 (order-preserving 32-bit keys): a
 // split only decides which particles share a node, and every box is later
@@ -28,14 +27,10 @@
 #include "nbx_kernels.h"
 
 #define KD_LOCAL 2048  // slots per workgroup in the local stage (64 leaves)
-#define KD_EBITS 12    // element index bits in a local sort key
 
 __device__ __forceinline__ uint32_t fkey(float f) {
     const uint32_t b = __float_as_uint(f);
     return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
-}
-__device__ __forceinline__ float fdekey(uint32_t k) {
-    return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
 }
 
 __global__ __launch_bounds__(256) void k_kd_key(const double *__restrict__ c, int64_t n, uint32_t *__restrict__ key,
@@ -104,86 +99,110 @@ __global__ __launch_bounds__(256) void k_kd_scatter(const int32_t *__restrict__ 
     S2[dst] = S[i];
 }
 
+// loc[id] = position of particle id in its block of the first list, the
+// local element index the local stage works with.
+__global__ __launch_bounds__(256) void k_kd_loc(const int32_t *__restrict__ S0, int64_t n, int lblk,
+                                                int32_t *__restrict__ loc) {
+    int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    loc[S0[i]] = (int32_t)(i & (((int64_t)1 << lblk) - 1));
+}
+
 // Local stage: the node ranges of one KD_LOCAL-slot block (or the whole set
-// when it is smaller), split level by level down to single leaves.
+// when it is smaller), split level by level down to single leaves with the
+// same per-dimension sorted lists as the global levels, here as 16-bit local
+// element indices in LDS; the stable partitions use block-wide scans.
 template <int ND>
-__global__ __launch_bounds__(256) void k_kd_local(KdLists L, int64_t n, int lblk, int32_t *__restrict__ order) {
+__global__ __launch_bounds__(256) void k_kd_local(KdLists L, const int32_t *__restrict__ loc, int64_t n, int lblk,
+                                                  int32_t *__restrict__ order) {
     __shared__ float cf[ND][KD_LOCAL];
-    __shared__ int32_t ids[KD_LOCAL];
-    __shared__ uint64_t key[KD_LOCAL];
-    __shared__ uint32_t mn[KD_LOCAL / (2 * NBX_LEAF)][ND], mx[KD_LOCAL / (2 * NBX_LEAF)][ND];
+    __shared__ uint16_t lst[2][ND][KD_LOCAL];
+    __shared__ uint16_t pre[KD_LOCAL];
+    __shared__ uint8_t fl[KD_LOCAL];
     __shared__ uint8_t dsel[KD_LOCAL / (2 * NBX_LEAF)];
-    const int tid = threadIdx.x;
+    __shared__ int32_t wsum[4];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int ss = 1 << lblk;  // block span in slots (power of two, <= KD_LOCAL)
     const int64_t a = (int64_t)blockIdx.x << lblk;
     if (a >= n) return;
     const int cnt = (int)min((int64_t)ss, n - a);
-    for (int j = tid; j < ss; j += 256) {
-        if (j < cnt) {
-            const int32_t id = L.S[0][a + j];
-            ids[j] = id;
+    for (int j = tid; j < cnt; j += 256) {
+        const int32_t id = L.S[0][a + j];
 #pragma unroll
-            for (int d = 0; d < ND; ++d) cf[d][j] = (float)L.c[d][id];
-            key[j] = (uint64_t)j;
-        } else {
-            key[j] = ~0ULL;
-        }
+        for (int d = 0; d < ND; ++d) cf[d][j] = (float)L.c[d][id];
+        lst[0][0][j] = (uint16_t)j;
+#pragma unroll
+        for (int d = 1; d < ND; ++d) lst[0][d][j] = (uint16_t)loc[L.S[d][a + j]];
     }
     __syncthreads();
-    const uint64_t EM = (1ULL << KD_EBITS) - 1;
+    const int per = ss >= 256 ? ss / 256 : 1;  // <= KD_LOCAL / 256 = 8
+    const int base = tid * per;
+    int cur = 0;
     for (int s = ss; s >= 2 * NBX_LEAF; s >>= 1) {
         const int nsub = ss / s;
-        for (int t = tid; t < nsub * ND; t += 256) {
-            mn[t / ND][t % ND] = 0xffffffffu;
-            mx[t / ND][t % ND] = 0u;
-        }
-        __syncthreads();
-        for (int j = tid; j < cnt; j += 256) {
-            const int e = (int)(key[j] & EM);
-            const int sub = j / s;
-#pragma unroll
-            for (int d = 0; d < ND; ++d) {
-                const uint32_t k = fkey(cf[d][e]);
-                atomicMin(&mn[sub][d], k);
-                atomicMax(&mx[sub][d], k);
-            }
-        }
-        __syncthreads();
+        // split dimension of each node range: largest extent (list ends)
         for (int t = tid; t < nsub; t += 256) {
+            const int u = t * s;
             int best = 0;
-            float bext = -1.0f;
-            for (int d = 0; d < ND; ++d) {
-                const float e = mx[t][d] >= mn[t][d] ? fdekey(mx[t][d]) - fdekey(mn[t][d]) : 0.0f;
-                if (e > bext) { bext = e; best = d; }
+            if (u < cnt) {
+                const int e = min(u + s, cnt) - 1;
+                float bext = -1.0f;
+                for (int d = 0; d < ND; ++d) {
+                    const float x = cf[d][lst[cur][d][e]] - cf[d][lst[cur][d][u]];
+                    if (x > bext) { bext = x; best = d; }
+                }
             }
             dsel[t] = (uint8_t)best;
         }
         __syncthreads();
         for (int j = tid; j < cnt; j += 256) {
-            const int e = (int)(key[j] & EM);
-            const int d = dsel[j / s];
-            key[j] = ((uint64_t)fkey(cf[d][e]) << KD_EBITS) | (uint64_t)e;
+            const int sub = j / s;
+            const int d = dsel[sub];
+            fl[lst[cur][d][j]] = j < sub * s + s / 2 ? 1 : 0;
         }
         __syncthreads();
-        // bitonic sort of every aligned run of s keys, all ascending
-        for (int k = 2; k <= s; k <<= 1) {
-            for (int j = k >> 1; j > 0; j >>= 1) {
-                for (int i = tid; i < ss; i += 256) {
-                    const int ixj = i ^ j;
-                    if (ixj > i) {
-                        const bool up = (k == s) || ((i & k) == 0);
-                        const uint64_t x = key[i], y = key[ixj];
-                        if ((x > y) == up) {
-                            key[i] = y;
-                            key[ixj] = x;
-                        }
-                    }
-                }
-                __syncthreads();
+        for (int d = 0; d < ND; ++d) {
+            int fv[KD_LOCAL / 256];
+            int tsum = 0;
+#pragma unroll
+            for (int q = 0; q < KD_LOCAL / 256; ++q) {
+                const int j = base + q;
+                fv[q] = (q < per && j < cnt) ? fl[lst[cur][d][j]] : 0;
+                tsum += fv[q];
             }
+            int incl = tsum;
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const int y = __shfl_up(incl, off);
+                if (lane >= off) incl += y;
+            }
+            if (lane == 63) wsum[w] = incl;
+            __syncthreads();
+            int run = incl - tsum;
+            for (int v = 0; v < w; ++v) run += wsum[v];
+#pragma unroll
+            for (int q = 0; q < KD_LOCAL / 256; ++q) {
+                const int j = base + q;
+                if (q < per && j < ss) pre[j] = (uint16_t)run;
+                run += fv[q];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < KD_LOCAL / 256; ++q) {
+                const int j = base + q;
+                if (q < per && j < cnt) {
+                    const int u = (j / s) * s;
+                    const int m = min(u + s / 2, cnt);
+                    const int nl = (int)pre[j] - (int)pre[u];
+                    const int dst = fv[q] ? u + nl : m + (j - u - nl);
+                    lst[cur ^ 1][d][dst] = lst[cur][d][j];
+                }
+            }
+            __syncthreads();
         }
+        cur ^= 1;
     }
-    for (int j = tid; j < cnt; j += 256) order[a + j] = ids[(int)(key[j] & EM)];
+    for (int j = tid; j < cnt; j += 256) order[a + j] = L.S[0][a + lst[cur][0][j]];
 }
 
 static inline int nblk256(int64_t n) { return (int)((n + 255) / 256); }
@@ -201,7 +220,7 @@ hipError_t nbx_kd_order(const double *const *c, int nd, int64_t n, int32_t nleaf
     memset(&Lh, 0, sizeof(Lh));
     hipError_t e = hipSuccess;
     uint32_t *k0 = nullptr, *k1 = nullptr;
-    int32_t *iota = nullptr, *f = nullptr, *pos = nullptr;
+    int32_t *iota = nullptr, *f = nullptr, *pos = nullptr, *loc = nullptr;
     uint8_t *flag = nullptr, *dsel = nullptr;
     void *tmp = nullptr;
     size_t tb = 0, tb2 = 0;
@@ -223,7 +242,7 @@ hipError_t nbx_kd_order(const double *const *c, int nd, int64_t n, int32_t nleaf
                                  rocprim::plus<int32_t>(), st, false));
     KDCK(hipMallocAsync(&tmp, tb > tb2 ? tb : tb2, st));
     // one index list per dimension, sorted by that coordinate (FP32 keys)
-    for (int d = 0; d < (ltot > lblk ? nd : 1); ++d) {
+    for (int d = 0; d < nd; ++d) {
         hipLaunchKernelGGL(k_kd_key, dim3(nblk256(n)), dim3(256), 0, st, c[d], n, k0, iota);
         KDCK(rocprim::radix_sort_pairs(tmp, tb, k0, k1, iota, Lh.S[d], (unsigned int)n, 0, 32, st, false));
     }
@@ -246,11 +265,13 @@ hipError_t nbx_kd_order(const double *const *c, int nd, int64_t n, int32_t nleaf
         }
     }
     {
+        KDCK(hipMallocAsync((void **)&loc, sizeof(int32_t) * n, st));
+        hipLaunchKernelGGL(k_kd_loc, dim3(nblk256(n)), dim3(256), 0, st, Lh.S[0], n, lblk, loc);
         const int64_t nb = (n + ((int64_t)1 << lblk) - 1) >> lblk;
         if (nd == 3)
-            hipLaunchKernelGGL(k_kd_local<3>, dim3((unsigned)nb), dim3(256), 0, st, Lh, n, lblk, order);
+            hipLaunchKernelGGL(k_kd_local<3>, dim3((unsigned)nb), dim3(256), 0, st, Lh, loc, n, lblk, order);
         else
-            hipLaunchKernelGGL(k_kd_local<6>, dim3((unsigned)nb), dim3(256), 0, st, Lh, n, lblk, order);
+            hipLaunchKernelGGL(k_kd_local<6>, dim3((unsigned)nb), dim3(256), 0, st, Lh, loc, n, lblk, order);
     }
     e = hipGetLastError();
 done:
@@ -265,6 +286,7 @@ done:
     if (pos) hipFreeAsync(pos, st);
     if (flag) hipFreeAsync(flag, st);
     if (dsel) hipFreeAsync(dsel, st);
+    if (loc) hipFreeAsync(loc, st);
     if (tmp) hipFreeAsync(tmp, st);
 #undef KDCK
     return e;
