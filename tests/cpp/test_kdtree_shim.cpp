@@ -38,7 +38,9 @@ int main(int argc, char **argv) {
     int *labels = nullptr;
     int ngroup = 0;
     {
+        fprintf(stderr, "stage: build n=%d periodic=%d\n", n, periodic);
         KDTree tree(parts.data(), n, 16, KDTree::TPHYS, KDTree::KSPH, 1000, 0, 0, 0, periodic ? period : nullptr);
+        fprintf(stderr, "stage: batched FindNearest k=%d\n", k);
         // batched kNN, mapped to IDs through the reordered particle array
         std::vector<int> nnflat((size_t)n * k);
         std::vector<double> d2flat((size_t)n * k);
@@ -50,6 +52,7 @@ int main(int argc, char **argv) {
             int id = parts[i].GetID();
             for (int j = 0; j < k; ++j) ids[(size_t)id * k + j] = nnflat[(size_t)i * k + j] >= 0 ? parts[nnflat[(size_t)i * k + j]].GetID() : -1;
         }
+        fprintf(stderr, "stage: per-particle FindNearest (OpenMP)\n");
         // per-particle calls from an OpenMP loop agree with the batch rows
         int bad = 0;
 #pragma omp parallel for schedule(guided) reduction(+ : bad)
@@ -60,8 +63,11 @@ int main(int argc, char **argv) {
             for (int j = 0; j < k; ++j) bad += (one[j] != nnflat[(size_t)i * k + j]) || (od[j] != d2flat[(size_t)i * k + j]);
         }
         flags[0] = bad;
+        fprintf(stderr, "stage: CalcDensity\n");
         tree.CalcDensity(k);
+        fprintf(stderr, "stage: FOF b=%g\n", b);
         labels = tree.FOF(b, ngroup, 8);
+        fprintf(stderr, "stage: destroy\n");
         // pid and type ride along with the reordering
         int moved = 0;
         for (int i = 0; i < n; ++i) moved += (parts[i].GetPID() != 1000 + parts[i].GetID()) || parts[i].GetType() != 1;
@@ -75,6 +81,7 @@ int main(int argc, char **argv) {
     }
     flags[2] = order;
     flags[3] = ngroup;
+    fprintf(stderr, "stage: write\n");
     FILE *o = fopen(argv[2], "wb");
     fwrite(&n, 4, 1, o);
     fwrite(&k, 4, 1, o);
