@@ -288,27 +288,38 @@ def run_phase6d(args):
     vs = torch.from_numpy(np.ascontiguousarray((vel / sv).T)).to(dev)
     rho = torch.empty(n, dtype=torch.float64, device=dev)
     hh = torch.empty(n, dtype=torch.float64, device=dev)
-    builds = []
-    for _ in range(2):
-        torch.cuda.synchronize()
-        with Timer(torch) as tb:
-            t6 = DeviceTree(xs, vel=vs, bucket_size=16, treetype=nb.TPHS, kerntype=nb.KSPH, aniso=-1)
-        builds.append(tb.ms)
-    log(f"6-D tree build {builds[-1]:.1f} ms; warmup {args.warmup}")
-    for _ in range(args.warmup):
-        t6.density(k, rho, hh)
-    nb.prof_reset()
-    nb.prof_enable(True)
-    times = []
-    for _ in range(args.steps):
-        with Timer(torch) as tm:
+
+    def timed(form, label):
+        builds = []
+        for _ in range(2):
+            torch.cuda.synchronize()
+            with Timer(torch) as tb:
+                t6 = DeviceTree(xs, vel=vs, bucket_size=16, treetype=nb.TPHS, kerntype=nb.KSPH, aniso=-1, form=form)
+            builds.append(tb.ms)
+        log(f"{label}: 6-D tree build {builds[-1]:.1f} ms; warmup {args.warmup}")
+        for _ in range(args.warmup):
             t6.density(k, rho, hh)
-        times.append(tm.ms)
-    nb.prof_enable(False)
-    kms, kl = nb.prof_get("k_knn_fp64")
-    ms = _median(times)
-    log(f"6-D k={k} kNN + density: {ms:.2f} ms (k_knn_fp64 {kms / max(kl, 1):.2f} ms/launch)")
-    del t6
+        nb.prof_reset()
+        nb.prof_enable(True)
+        times = []
+        for _ in range(args.steps):
+            with Timer(torch) as tm:
+                t6.density(k, rho, hh)
+            times.append(tm.ms)
+        nb.prof_enable(False)
+        kname = "k_knn_mixed" if form == nb.FORM_MIXED else "k_knn_fp64"
+        kms, kl = nb.prof_get(kname)
+        ms = _median(times)
+        fb = t6.last_fallbacks()
+        log(f"{label}: 6-D k={k} kNN + density {ms:.2f} ms ({kname} {kms / max(kl, 1):.2f} ms/launch, "
+            f"fallbacks {fb})")
+        del t6
+        return ms, kms / max(kl, 1), builds[-1], kname, fb
+
+    plain_ms, _, _, _, _ = timed(nb.FORM_PLAIN, "plain-fp64")
+    rho_plain = rho.clone()
+    ms, kern_ms, build_ms, kname, fb = timed(nb.FORM_MIXED, "mixed")
+    rel = float(torch.max(torch.abs(rho / rho_plain - 1.0)))
     # 3-D FOF at b = 0.2 mean spacing on the same particles
     pd = torch.from_numpy(np.ascontiguousarray(pos.T)).to(dev)
     t3 = DeviceTree(pd, bucket_size=16, kerntype=nb.KSPH, period=L)
@@ -322,16 +333,18 @@ def run_phase6d(args):
         fts.append(tf.ms)
     fms = _median(fts)
     log(f"3-D FOF b={b:.5f}: {fms:.1f} ms, {ng} groups")
-    kern_ms = kms / max(kl, 1)
     bpq = 48 + 8 + 4 * k + 16
     ach = bpq * n / (kern_ms / 1e3) / 1e9
-    _line(f"Mqueries/s 6-D kNN (k={k}) + 6-D SPH density, N={n}", n / (ms / 1e3) / 1e6, "Mqueries/s", args, ms,
-          "f64", f"6-D phase space (x/sigma_x, v/sigma_v), clustered NFW N={n}, k={k} CalcDensity on a TPHS tree; "
-                 f"3-D FOF b=0.2 mean spacing", {
-              "build_ms": round(builds[-1], 2), "fof_ms": round(fms, 2), "fof_mparticles_s": round(n / fms / 1e3, 3),
+    _line(f"Mqueries/s 6-D kNN (k={k}) + 6-D SPH density, N={n}, precision form", n / (ms / 1e3) / 1e6,
+          "Mqueries/s", args, ms, "f32+f64",
+          f"6-D phase space (x/sigma_x, v/sigma_v), clustered NFW N={n}, k={k} CalcDensity on a TPHS tree; "
+          f"3-D FOF b=0.2 mean spacing", {
+              "plain_fp64_mqps": round(n / (plain_ms / 1e3) / 1e6, 3), "mixed_vs_plain": round(plain_ms / ms, 3),
+              "mixed_density_max_rel_vs_fp64": rel, "mixed_fp64_fallback_queries": fb,
+              "build_ms": round(build_ms, 2), "fof_ms": round(fms, 2), "fof_mparticles_s": round(n / fms / 1e3, 3),
               "fof_groups": ng,
               "roofline": {"bound": "hbm", "achieved": round(ach, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                           "frac": round(ach / PEAK_HBM_GBS, 5), "traffic": None, "kernel": "k_knn_fp64",
+                           "frac": round(ach / PEAK_HBM_GBS, 5), "traffic": None, "kernel": kname,
                            "kernel_ms": round(kern_ms, 3), "bytes_per_query": bpq},
               "cpu_baseline": None})
 

@@ -471,7 +471,7 @@ extern "C" int nbx_knn_dev(nbx_tree *t, int32_t k, int32_t variant, int32_t layo
     A.nn = nn_dev;
     A.d2 = d2_dev;
     hipStream_t st = (hipStream_t)stream;
-    if (t->p.form == NBX_FORM_MIXED && t->nd == 3 && A.qsrc == NBX_Q_SLOTS && A.kh == k && d2_dev == nullptr) {
+    if (t->p.form == NBX_FORM_MIXED && A.qsrc == NBX_Q_SLOTS && A.kh == k && d2_dev == nullptr) {
         HIPCK(run_mixed(t, A, k, st));
         return NBX_OK;
     }
@@ -554,7 +554,7 @@ static int density_impl(nbx_tree *t, int32_t k, int32_t variant, int32_t nsearch
     } else
         return fail(NBX_ERR_ARG, "unknown density variant");
     if (A.kh > heap_cap(t)) return fail(NBX_ERR_UNSUPPORTED, "k too large (k <= 64 supported)");
-    const bool mixed = t->p.form == NBX_FORM_MIXED && t->nd == 3 && variant != NBX_DENS_VEL;
+    const bool mixed = t->p.form == NBX_FORM_MIXED && variant != NBX_DENS_VEL;
     const int64_t n = t->n;
     double *rho = nullptr, *h = nullptr;
     int32_t *err = nullptr;

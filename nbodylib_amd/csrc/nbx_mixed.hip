@@ -2,8 +2,9 @@
 // re-check band, exact neighbour sets).
 //
 // Same call as CalcDensity (KDCalcSmoothQuantities.cxx:203-305) and the
-// target-based FindNearest (KDFindNearest.cxx:247-318) on a non-periodic
-// search, with the same wave/leaf structure as the FP64 kernel (nbx_knn.hip).
+// target-based FindNearest / FindNearestPhase (KDFindNearest.cxx:247-318,
+// :347-361) on a non-periodic search, in 3-D or on the 6-D phase-space
+// tree, with the same wave/leaf structure as the FP64 kernel (nbx_knn.hip).
 //
 // Search: particle coordinates are stored as FP32 offsets from their leaf's
 // FP64 centre; a query is re-expressed per leaf as fl32(q - centre) so every
@@ -12,9 +13,12 @@
 // 8-byte entries compared with one 64-bit integer compare.
 //
 // Error bound: with u = 2^-24 and E the half-diagonal of the leaf a
-// candidate came from, |dist32 - dist64| <= 12 u dist32 + 5 u E (rounding
-// of the two offsets, of the difference, of the FMA sum and of sqrt, with a
-// factor 2 of margin).  Each lane tracks E_max over the leaves it evaluated
+// candidate came from, |dist32 - dist64| <= 12 u dist32 + 5 u E.  Per
+// component the two offset roundings and the difference add at most
+// u(|q - o| + |x - o| + |t|) <= u(dist + 2E + dist); the FMA chain over ND
+// <= 6 squares adds ND u relative to the sum, sqrt halves it and adds u:
+// at most (2 + ND/2 + 1) u dist + 2 u E = 6 u dist + 2 u E in 6-D, so the
+// constants keep a factor 2 of margin.  Each lane tracks E_max over the leaves it evaluated
 // as reachable or accepted a candidate from (candidates of other leaves are
 // outside by the FP64 node bound).
 //
@@ -121,39 +125,50 @@ __device__ __forceinline__ void kheap_sift_bounded(uint64_t *hk, int lane, int m
 // Heap capacity for k+1 = kh kept keys: kh rounded up to 1 (mod 4).
 __host__ __device__ __forceinline__ int kheap_cap(int kh) { return ((kh + 2) & ~3) + 1; }
 
+template <int ND>
 struct MixCtx {
-    double q[3];
+    double q[ND];
     bool act;
     int32_t self;
     uint64_t root;  // current largest key kept
     double emax;    // largest leaf half-diagonal that matters for this lane
 };
 
-__device__ __forceinline__ void mix_leaf(const DevTree &T, const double *ext, int leaf, MixCtx &c, bool want,
+template <int ND>
+__device__ __forceinline__ void mix_leaf(const DevTree &T, const double *ext, int leaf, MixCtx<ND> &c, bool want,
                                          uint64_t *hk, float *cx, int lane, int hc) {
     const int base = leaf * NBX_LEAF;
     // the LDS copy of the leaf for phase 2: loaded now, stored after phase 1
     // so the vector-load latency overlaps the candidate tests
-    float st[3];
+    float st[ND];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) st[d] = T.xf[d][base + (lane & (NBX_LEAF - 1))];
+    for (int d = 0; d < ND; ++d) st[d] = T.xf[d][base + (lane & (NBX_LEAF - 1))];
     // query relative to this leaf's centre, rounded once to FP32
-    const double *o = T.org + (size_t)leaf * 3;
-    float qx = (float)(c.q[0] - ((cdptr)o)[0]);
-    float qy = (float)(c.q[1] - ((cdptr)o)[1]);
-    float qz = (float)(c.q[2] - ((cdptr)o)[2]);
+    const double *o = T.org + (size_t)leaf * ND;
+    float qv[ND];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) qv[d] = (float)(c.q[d] - ((cdptr)o)[d]);
     const float th = keyd(c.root);
     uint32_t mask = 0;
+    // candidates in 32- or 16-byte scalar-load chunks per dimension
+    constexpr int CH = ND == 6 ? 4 : 8;
+    typedef float fxc __attribute__((ext_vector_type(CH)));
+    typedef const __attribute__((address_space(4))) fxc *cfcptr;
 #pragma unroll
-    for (int jj = 0; jj < NBX_LEAF; jj += 8) {
-        fx8 px = *(cf8ptr)(T.xf[0] + base + jj);
-        fx8 py = *(cf8ptr)(T.xf[1] + base + jj);
-        fx8 pz = *(cf8ptr)(T.xf[2] + base + jj);
+    for (int jj = 0; jj < NBX_LEAF; jj += CH) {
+        fxc pv[ND];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            float tx = qx - px[u], ty = qy - py[u], tz = qz - pz[u];
-            float d = __builtin_fmaf(tz, tz, __builtin_fmaf(ty, ty, tx * tx));
-            mask |= (d <= th) ? (1u << (jj + u)) : 0u;
+        for (int d = 0; d < ND; ++d) pv[d] = *(cfcptr)(T.xf[d] + base + jj);
+#pragma unroll
+        for (int u = 0; u < CH; ++u) {
+            float t = qv[0] - pv[0][u];
+            float dd = t * t;
+#pragma unroll
+            for (int d = 1; d < ND; ++d) {
+                t = qv[d] - pv[d][u];
+                dd = __builtin_fmaf(t, t, dd);
+            }
+            mask |= (dd <= th) ? (1u << (jj + u)) : 0u;
         }
     }
     if (c.self >= base && c.self < base + NBX_LEAF) mask &= ~(1u << (c.self - base));
@@ -161,22 +176,28 @@ __device__ __forceinline__ void mix_leaf(const DevTree &T, const double *ext, in
     if (c.act && (want || mask)) c.emax = fmax(c.emax, ((cdptr)ext)[leaf]);
     if (lane < NBX_LEAF) {
 #pragma unroll
-        for (int d = 0; d < 3; ++d) cx[d * NBX_LEAF + lane] = st[d];
+        for (int d = 0; d < ND; ++d) cx[d * NBX_LEAF + lane] = st[d];
     }
     __syncthreads();
     while (ballot(mask != 0)) {
         const bool has = mask != 0;
         const int j = __builtin_ctz(mask | 0x80000000u);
         mask &= mask - 1;
-        float tx = qx - cx[j], ty = qy - cx[NBX_LEAF + j], tz = qz - cx[2 * NBX_LEAF + j];
-        float d = __builtin_fmaf(tz, tz, __builtin_fmaf(ty, ty, tx * tx));
-        uint64_t key = mkkey(d, base + j);
+        float t = qv[0] - cx[j];
+        float dd = t * t;
+#pragma unroll
+        for (int d = 1; d < ND; ++d) {
+            t = qv[d] - cx[d * NBX_LEAF + j];
+            dd = __builtin_fmaf(t, t, dd);
+        }
+        uint64_t key = mkkey(dd, base + j);
         c.root = kheap_replace_root(hk, lane, hc, has && key < c.root, key, c.root);
     }
     __syncthreads();
 }
 
-__device__ __forceinline__ void mix_walk(const DevTree &T, const double *ext, MixCtx &c, const float *qc, int anc,
+template <int ND>
+__device__ __forceinline__ void mix_walk(const DevTree &T, const double *ext, MixCtx<ND> &c, const float *qc, int anc,
                                          uint64_t *hk, float *cx, int lane, int hc) {
     const int D = T.depth;
     int node = 1, dep = 0;
@@ -184,18 +205,18 @@ __device__ __forceinline__ void mix_walk(const DevTree &T, const double *ext, Mi
     bool down = true;
     for (;;) {
         if (down) {
-            cdptr bx = (cdptr)node_box(T, node, 3);
-            bool want = c.act && box_lb<3>(c.q, bx) <= (double)keyd(c.root);
+            cdptr bx = (cdptr)node_box(T, node, ND);
+            bool want = c.act && box_lb<ND>(c.q, bx) <= (double)keyd(c.root);
             if (ballot(want)) {
                 if (dep == D) {
-                    mix_leaf(T, ext, node - T.nleafp, c, want, hk, cx, lane, hc);
+                    mix_leaf<ND>(T, ext, node - T.nleafp, c, want, hk, cx, lane, hc);
                     down = false;
                 } else {
                     int c0 = 2 * node;
                     int first = c0;
                     if (anc >= 0 && (anc >> (D - dep)) == node) {
                         first = anc >> (D - dep - 1);
-                    } else if (first_is_right<3>(T, node, qc)) {
+                    } else if (first_is_right<ND>(T, node, qc)) {
                         first = c0 + 1;
                     }
                     dep += 1;
@@ -219,10 +240,11 @@ __device__ __forceinline__ void mix_walk(const DevTree &T, const double *ext, Mi
     }
 }
 
+template <int ND>
 __device__ __forceinline__ double d64(const DevTree &T, const double *q, int32_t s) {
     double d2 = 0.0;
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
+    for (int d = 0; d < ND; ++d) {
         double t = q[d] - T.x[d][s];
         d2 += t * t;
     }
@@ -233,30 +255,35 @@ __device__ __forceinline__ double d64(const DevTree &T, const double *q, int32_t
 // (d2, ID) is below (Dk, IDk) -- self and d2 == 0 excluded, as in the target
 // search -- within the ball of radius^2 Dk, for the lanes with act set.
 // Wave-uniform left-first walk; leaves through 64-byte scalar loads.
+template <int ND>
 __device__ __forceinline__ int verify_walk(const DevTree &T, const double *q, bool act, double Dk, int32_t IDk,
                                            int32_t self) {
+    constexpr int CH = ND == 6 ? 4 : 8;
+    typedef double dxc __attribute__((ext_vector_type(CH)));
+    typedef const __attribute__((address_space(4))) dxc *cdcptr;
     int cnt = 0;
     int node = 1;
     bool down = true;
     for (;;) {
         if (down) {
-            cdptr bx = (cdptr)node_box(T, node, 3);
-            bool want = act && box_lb<3>(q, bx) <= Dk;
+            cdptr bx = (cdptr)node_box(T, node, ND);
+            bool want = act && box_lb<ND>(q, bx) <= Dk;
             if (ballot(want)) {
                 if (node >= T.nleafp) {
                     const int base = (node - T.nleafp) * NBX_LEAF;
 #pragma unroll
-                    for (int jj = 0; jj < NBX_LEAF; jj += 8) {
-                        dx8 cv0 = *(cd8ptr)(T.x[0] + base + jj);
-                        dx8 cv1 = *(cd8ptr)(T.x[1] + base + jj);
-                        dx8 cv2 = *(cd8ptr)(T.x[2] + base + jj);
+                    for (int jj = 0; jj < NBX_LEAF; jj += CH) {
+                        dxc cv[ND];
 #pragma unroll
-                        for (int u = 0; u < 8; ++u) {
-                            double t0 = q[0] - cv0[u], t1 = q[1] - cv1[u], t2 = q[2] - cv2[u];
+                        for (int d = 0; d < ND; ++d) cv[d] = *(cdcptr)(T.x[d] + base + jj);
+#pragma unroll
+                        for (int u = 0; u < CH; ++u) {
                             double d2 = 0.0;
-                            d2 += t0 * t0;
-                            d2 += t1 * t1;
-                            d2 += t2 * t2;
+#pragma unroll
+                            for (int d = 0; d < ND; ++d) {
+                                const double t = q[d] - cv[d][u];
+                                d2 += t * t;
+                            }
                             const int32_t cs = base + jj + u;
                             bool below = d2 < Dk;
                             if (d2 == Dk && act && cs != self) below = T.perm[cs] < IDk;
@@ -284,11 +311,11 @@ __device__ __forceinline__ int verify_walk(const DevTree &T, const double *q, bo
 }
 
 // k = kh - 1 neighbours; A.dens: NBX_DO_NONE / NBX_DO_SYM / NBX_DO_GATHER.
-template <int KMAX>
+template <int ND, int KMAX>
 __global__ __launch_bounds__(64) void k_knn_mixed(DevTree T, KnnArgs A, const double *ext, int32_t *fb_list,
                                                    int32_t *fb_count) {
     __shared__ uint64_t hk[KMAX * NBX_WAVE];
-    __shared__ float cx[3 * NBX_LEAF];
+    __shared__ float cx[ND * NBX_LEAF];
     __shared__ double racc[NBX_WAVE];  // symmetric scatter into the wave's own queries
     const int lane = threadIdx.x;
     const int g = xcd_wave(blockIdx.x, gridDim.x);
@@ -297,19 +324,19 @@ __global__ __launch_bounds__(64) void k_knn_mixed(DevTree T, KnnArgs A, const do
     const int qi = g * NBX_WAVE + lane;
     const bool valid = qi < T.n;
     const int kres = T.kernres;
-    MixCtx c;
+    MixCtx<ND> c;
 #pragma unroll
-    for (int d = 0; d < 3; ++d) c.q[d] = valid ? T.x[d][qi] : 0.0;
+    for (int d = 0; d < ND; ++d) c.q[d] = valid ? T.x[d][qi] : 0.0;
     c.act = valid;
     c.self = qi;
     c.root = MX_EMPTY;
     c.emax = 0.0;
     const int hc = kheap_cap(kh);
     for (int p = 0; p < hc; ++p) HK(p) = MX_EMPTY;
-    float qc[3];
+    float qc[ND];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) qc[d] = (float)readfirst_d(c.q[d]);
-    mix_walk(T, ext, c, qc, T.nleafp + 2 * g, hk, cx, lane, hc);
+    for (int d = 0; d < ND; ++d) qc[d] = (float)readfirst_d(c.q[d]);
+    mix_walk<ND>(T, ext, c, qc, T.nleafp + 2 * g, hk, cx, lane, hc);
 
     // ---------------------------------------------------------------- band
     bool ok = valid;
@@ -351,7 +378,7 @@ __global__ __launch_bounds__(64) void k_knn_mixed(DevTree T, KnnArgs A, const do
             bd[b] = 0.0;
             if (b < nb) {
                 bs[b] = keys(HK(i0 + b));
-                bd[b] = d64(T, c.q, bs[b]);
+                bd[b] = d64<ND>(T, c.q, bs[b]);
                 if (bd[b] == 0.0) { nzero++; drop = b; }
             }
         }
@@ -388,7 +415,7 @@ __global__ __launch_bounds__(64) void k_knn_mixed(DevTree T, KnnArgs A, const do
         int32_t imax = -1;
         for (int p = 0; p < kh; ++p) {
             int32_t s = keys(HK(p));
-            double d = d64(T, c.q, s);
+            double d = d64<ND>(T, c.q, s);
             if (d == 0.0) {
                 nzero++;
                 zero_at = p;
@@ -405,14 +432,14 @@ __global__ __launch_bounds__(64) void k_knn_mixed(DevTree T, KnnArgs A, const do
             IDk = -1;
             for (int p = 0; p < k; ++p) {
                 int32_t s = keys(HK(p));
-                double d = d64(T, c.q, s);
+                double d = d64<ND>(T, c.q, s);
                 int32_t id = T.perm[s];
                 if (d > Dk || (d == Dk && id > IDk)) { Dk = d; IDk = id; }
             }
         }
     }
     if (ballot(verify)) {
-        int cnt = verify_walk(T, c.q, verify, Dk, IDk, qi);
+        int cnt = verify_walk<ND>(T, c.q, verify, Dk, IDk, qi);
         if (verify && cnt == k - 1) ok = true;
     }
     // fallback list for the FP64 kernel
@@ -437,7 +464,7 @@ __global__ __launch_bounds__(64) void k_knn_mixed(DevTree T, KnnArgs A, const do
         // weights from FP64 distances of the k selected neighbours (the FP32
         // offsets carry a storage error proportional to the leaf size)
         const double hi = 0.5 * sqrt(Dk);
-        const double norm = 1.0 / pow(hi, 3.0);
+        const double norm = 1.0 / pow(hi, (double)ND);
         const double mi = T.mass[qi];
         double acc = 0.0;
         // neighbours in groups of 8 so their coordinate/mass gathers are in
@@ -448,7 +475,7 @@ __global__ __launch_bounds__(64) void k_knn_mixed(DevTree T, KnnArgs A, const do
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
                 sv[u] = p0 + u < k ? keys(HK(p0 + u)) : qi;
-                d2v[u] = d64(T, c.q, sv[u]);
+                d2v[u] = d64<ND>(T, c.q, sv[u]);
                 mv[u] = T.mass[sv[u]];
             }
 #pragma unroll
@@ -494,13 +521,18 @@ hipError_t nbx_launch_knn_mixed(const DevTree &T, const KnnArgs &A, const double
                                 int32_t *fallback_count, hipStream_t st) {
     int blocks = (T.n + NBX_WAVE - 1) / NBX_WAVE;
     if (blocks == 0) return hipSuccess;
-    if (T.nd != 3 || A.kh > 65 || A.kh < 2 || !T.kernf || !T.org || !ext) return hipErrorInvalidValue;
+    if ((T.nd != 3 && T.nd != 6) || A.kh > 65 || A.kh < 2 || !T.kernf || !T.org || !ext) return hipErrorInvalidValue;
     ProfScope ps("k_knn_mixed", st);
-    if (A.kh <= 33)
-        hipLaunchKernelGGL((k_knn_mixed<33>), dim3(blocks), dim3(NBX_WAVE), 0, st, T, A, ext, fallback_list,
-                           fallback_count);
-    else
-        hipLaunchKernelGGL((k_knn_mixed<65>), dim3(blocks), dim3(NBX_WAVE), 0, st, T, A, ext, fallback_list,
-                           fallback_count);
+#define MIXL(ND, KM)                                                                                                  \
+    hipLaunchKernelGGL((k_knn_mixed<ND, KM>), dim3(blocks), dim3(NBX_WAVE), 0, st, T, A, ext, fallback_list, \
+                       fallback_count)
+    if (T.nd == 3) {
+        if (A.kh <= 33) MIXL(3, 33);
+        else MIXL(3, 65);
+    } else {
+        if (A.kh <= 33) MIXL(6, 33);
+        else MIXL(6, 65);
+    }
+#undef MIXL
     return hipGetLastError();
 }

@@ -79,3 +79,42 @@ def test_mixed_duplicates_fall_back(gpu):
     ro = oracle.Tree(pos, b=16, kerntype=oracle.KSPH).calc_density(k)
     assert np.max(np.abs(rho / ro - 1.0)) < MIXED_TOL
     assert t.last_fallbacks() > 0
+
+
+def _phase_set(n, seed):
+    pos, vel, _ = datagen.clustered(n, 100.0, nhalo=max(4, n // 2000), seed=seed, with_vel=True)
+    return pos / pos.std(0), vel / vel.std(0)
+
+
+@pytest.mark.parametrize("k", [16, 64])
+def test_mixed_phase_knn_sets(gpu, k):
+    """6-D phase-space tree, precision form: FindNearestPhase sets equal the
+    FP64 brute force."""
+    import nbodylib_amd as nb
+
+    n = 6000
+    say(f"mixed phase knn n={n} k={k}")
+    x, v = _phase_set(n, 34)
+    t = nb.KDTree(x, vel=v, treetype=nb.TPHS, aniso=-1, kerntype=nb.KSPH, form=nb.FORM_MIXED)
+    nn = np.zeros((n, k), dtype=np.int32)
+    nb._check(nb.lib().nbx_knn(t._h, k, nb.KNN_FINDNEARESTPHASE, nb.LAYOUT_BY_ID, nb._p(nn), None))
+    bi, _ = oracle.knn_brute(x, k, mode=0, vel=v, nd=6)
+    assert np.array_equal(np.sort(nn, 1), np.sort(bi, 1))
+    say(f"  fallbacks={t.last_fallbacks()}")
+
+
+def test_mixed_phase_density(gpu):
+    """6-D CalcDensity on a TPHS tree in the precision form against the FP64
+    oracle (6-D norm 3024/(762 pi^3), metric factor 1)."""
+    import nbodylib_amd as nb
+
+    n, k = 20000, 64
+    say(f"mixed phase density n={n} k={k}")
+    x, v = _phase_set(n, 35)
+    m = datagen.lognormal_masses(n, 35)
+    t = nb.KDTree(x, mass=m, vel=v, treetype=nb.TPHS, aniso=-1, kerntype=nb.KSPH, form=nb.FORM_MIXED)
+    rho = t.CalcDensity(k)
+    ro = oracle.Tree(x, mass=m, vel=v, b=16, treetype=oracle.TPHS, kerntype=oracle.KSPH).calc_density(k)
+    rel = np.max(np.abs(rho / ro - 1.0))
+    say(f"  max rel {rel:.3e} fallbacks={t.last_fallbacks()}")
+    assert rel < MIXED_TOL
