@@ -11,8 +11,9 @@
 // Top levels (node ranges longer than KD_LOCAL slots) run over the whole set
 // with one sorted index list per dimension: the split dimension of a node is
 // read from the ends of its lists, the lower half of the chosen list flags
-// the particles that go left, and every list is stably partitioned by that
-// flag (a global exclusive scan gives each element its rank on its side).
+// the particles that go left (one bit each), and every list is stably
+// partitioned by that flag (a global exclusive scan gives each element its
+// rank on its side).
 // Lower levels run inside one workgroup per KD_LOCAL-slot range with the
 // same scheme on FP32 copies of the coordinates and 16-bit index lists in LDS.
 //
@@ -65,38 +66,49 @@ __global__ __launch_bounds__(256) void k_kd_dims(KdLists L, int nd, int64_t n, i
     dsel[s] = (uint8_t)best;
 }
 
-// flag[id] = 1 for the particles of the lower half of each node along its
-// split dimension.
+// Bit id of fw set for the particles of the lower half of each node along
+// its split dimension (fw cleared first).  One bit per particle keeps the
+// flag array at N/8 bytes, so the gathers below mostly hit in L2.
+__global__ __launch_bounds__(256) void k_kd_clear(uint32_t *__restrict__ fw, int64_t nw) {
+    int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < nw) fw[i] = 0u;
+}
 __global__ __launch_bounds__(256) void k_kd_flag(KdLists L, int64_t n, int lseg, const uint8_t *__restrict__ dsel,
-                                                 uint8_t *__restrict__ flag) {
+                                                 uint32_t *__restrict__ fw) {
     int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     const int64_t s = i >> lseg;
     const int64_t a = s << lseg;
     const int64_t m = min(a + ((int64_t)1 << (lseg - 1)), n);
-    const int d = dsel[s];
-    flag[L.S[d][i]] = i < m ? 1 : 0;
+    if (i < m) {
+        const int32_t id = L.S[dsel[s]][i];
+        atomicOr(fw + (id >> 5), 1u << (id & 31));
+    }
 }
 
-__global__ __launch_bounds__(256) void k_kd_getf(const int32_t *__restrict__ S, int64_t n,
-                                                 const uint8_t *__restrict__ flag, int32_t *__restrict__ f) {
-    int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    f[i] = flag[S[i]];
-}
+__device__ __forceinline__ int32_t kd_bit(const uint32_t *fw, int32_t id) { return (int32_t)((fw[id >> 5] >> (id & 31)) & 1u); }
+
+// flag of the i-th element of a list, as the scan input
+struct KdFlagOf {
+    const int32_t *S;
+    const uint32_t *fw;
+    __device__ __forceinline__ int32_t operator()(int64_t i) const { return kd_bit(fw, S[i]); }
+};
 
 // Stable partition of one list inside every node: left elements keep their
-// order in [a, m), right ones in [m, b).
+// order in [a, m), right ones in [m, b); pos is the exclusive scan of the
+// flags in list order.
 __global__ __launch_bounds__(256) void k_kd_scatter(const int32_t *__restrict__ S, int32_t *__restrict__ S2, int64_t n,
-                                                    int lseg, const int32_t *__restrict__ f,
+                                                    int lseg, const uint32_t *__restrict__ fw,
                                                     const int32_t *__restrict__ pos) {
     int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
+    const int32_t id = S[i];
     const int64_t a = (i >> lseg) << lseg;
     const int64_t m = min(a + ((int64_t)1 << (lseg - 1)), n);
     const int64_t nl = (int64_t)pos[i] - pos[a];
-    const int64_t dst = f[i] ? a + nl : m + (i - a - nl);
-    S2[dst] = S[i];
+    const int64_t dst = kd_bit(fw, id) ? a + nl : m + (i - a - nl);
+    S2[dst] = id;
 }
 
 // loc[id] = position of particle id in its block of the first list, the
@@ -220,8 +232,11 @@ hipError_t nbx_kd_order(const double *const *c, int nd, int64_t n, int32_t nleaf
     memset(&Lh, 0, sizeof(Lh));
     hipError_t e = hipSuccess;
     uint32_t *k0 = nullptr, *k1 = nullptr;
-    int32_t *iota = nullptr, *f = nullptr, *pos = nullptr, *loc = nullptr;
-    uint8_t *flag = nullptr, *dsel = nullptr;
+    int32_t *iota = nullptr, *pos = nullptr, *loc = nullptr;
+    uint32_t *fw = nullptr;
+    uint8_t *dsel = nullptr;
+    const int64_t nw = (n + 31) / 32;
+    typedef rocprim::transform_iterator<rocprim::counting_iterator<int64_t>, KdFlagOf, int32_t> FlagIt;
     void *tmp = nullptr;
     size_t tb = 0, tb2 = 0;
 #define KDCK(x)                          \
@@ -238,8 +253,8 @@ hipError_t nbx_kd_order(const double *const *c, int nd, int64_t n, int32_t nleaf
     KDCK(hipMallocAsync((void **)&k1, sizeof(uint32_t) * n, st));
     KDCK(hipMallocAsync((void **)&iota, sizeof(int32_t) * n, st));
     KDCK(rocprim::radix_sort_pairs((void *)nullptr, tb, k0, k1, iota, Lh.S[0], (unsigned int)n, 0, 32, st, false));
-    KDCK(rocprim::exclusive_scan((void *)nullptr, tb2, (int32_t *)nullptr, (int32_t *)nullptr, 0, (size_t)n,
-                                 rocprim::plus<int32_t>(), st, false));
+    KDCK(rocprim::exclusive_scan((void *)nullptr, tb2, FlagIt(rocprim::counting_iterator<int64_t>(0), KdFlagOf{}),
+                                 (int32_t *)nullptr, 0, (size_t)n, rocprim::plus<int32_t>(), st, false));
     KDCK(hipMallocAsync(&tmp, tb > tb2 ? tb : tb2, st));
     // one index list per dimension, sorted by that coordinate (FP32 keys)
     for (int d = 0; d < nd; ++d) {
@@ -247,18 +262,18 @@ hipError_t nbx_kd_order(const double *const *c, int nd, int64_t n, int32_t nleaf
         KDCK(rocprim::radix_sort_pairs(tmp, tb, k0, k1, iota, Lh.S[d], (unsigned int)n, 0, 32, st, false));
     }
     if (ltot > lblk) {
-        KDCK(hipMallocAsync((void **)&flag, (size_t)n, st));
-        KDCK(hipMallocAsync((void **)&f, sizeof(int32_t) * n, st));
+        KDCK(hipMallocAsync((void **)&fw, sizeof(uint32_t) * nw, st));
         KDCK(hipMallocAsync((void **)&pos, sizeof(int32_t) * n, st));
         KDCK(hipMallocAsync((void **)&dsel, (size_t)1 << (ltot - lblk), st));
         for (int lseg = ltot; lseg > lblk; --lseg) {
             const int64_t nseg = (int64_t)1 << (ltot - lseg);
             hipLaunchKernelGGL(k_kd_dims, dim3(nblk256(nseg)), dim3(256), 0, st, Lh, nd, n, lseg, nseg, dsel);
-            hipLaunchKernelGGL(k_kd_flag, dim3(nblk256(n)), dim3(256), 0, st, Lh, n, lseg, dsel, flag);
+            hipLaunchKernelGGL(k_kd_clear, dim3(nblk256(nw)), dim3(256), 0, st, fw, nw);
+            hipLaunchKernelGGL(k_kd_flag, dim3(nblk256(n)), dim3(256), 0, st, Lh, n, lseg, dsel, fw);
             for (int d = 0; d < nd; ++d) {
-                hipLaunchKernelGGL(k_kd_getf, dim3(nblk256(n)), dim3(256), 0, st, Lh.S[d], n, flag, f);
-                KDCK(rocprim::exclusive_scan(tmp, tb2, f, pos, 0, (size_t)n, rocprim::plus<int32_t>(), st, false));
-                hipLaunchKernelGGL(k_kd_scatter, dim3(nblk256(n)), dim3(256), 0, st, Lh.S[d], Lh.S2[d], n, lseg, f,
+                FlagIt fi(rocprim::counting_iterator<int64_t>(0), KdFlagOf{Lh.S[d], fw});
+                KDCK(rocprim::exclusive_scan(tmp, tb2, fi, pos, 0, (size_t)n, rocprim::plus<int32_t>(), st, false));
+                hipLaunchKernelGGL(k_kd_scatter, dim3(nblk256(n)), dim3(256), 0, st, Lh.S[d], Lh.S2[d], n, lseg, fw,
                                    pos);
             }
             for (int d = 0; d < nd; ++d) std::swap(Lh.S[d], Lh.S2[d]);
@@ -282,9 +297,8 @@ done:
     if (k0) hipFreeAsync(k0, st);
     if (k1) hipFreeAsync(k1, st);
     if (iota) hipFreeAsync(iota, st);
-    if (f) hipFreeAsync(f, st);
     if (pos) hipFreeAsync(pos, st);
-    if (flag) hipFreeAsync(flag, st);
+    if (fw) hipFreeAsync(fw, st);
     if (dsel) hipFreeAsync(dsel, st);
     if (loc) hipFreeAsync(loc, st);
     if (tmp) hipFreeAsync(tmp, st);
