@@ -237,30 +237,38 @@ __device__ __forceinline__ void knn_walk(const DevTree &T, LeafCtx<ND> &c, const
     const int D = T.depth;
     int node = 1, dep = 0;
     uint64_t firstright = 0;
+    bool down = true;
     for (;;) {
-        cdptr bx = (cdptr)node_box(T, node, ND);
-        const bool want = c.act && box_lb<ND>(c.q, bx) <= c.kth;
-        if (ballot(want)) {
-            if (dep == D) {
-                knn_leaf<ND>(T, node - T.nleafp, c, hd, hid, cx, cid, lane, hc);
+        if (down) {
+            cdptr bx = (cdptr)node_box(T, node, ND);
+            bool want = c.act && box_lb<ND>(c.q, bx) <= c.kth;
+            if (ballot(want)) {
+                if (dep == D) {
+                    knn_leaf<ND>(T, node - T.nleafp, c, hd, hid, cx, cid, lane, hc);
+                    down = false;
+                } else {
+                    int c0 = 2 * node;
+                    int first = c0;
+                    if (anc >= 0 && (anc >> (D - dep)) == node) first = anc >> (D - dep - 1);
+                    else if (first_is_right<ND>(T, node, qc)) first = c0 + 1;
+                    dep += 1;
+                    if (first & 1) firstright |= (1ULL << dep);
+                    else firstright &= ~(1ULL << dep);
+                    node = first;
+                    continue;
+                }
             } else {
-                int first;
-                if (anc >= 0 && (anc >> (D - dep)) == node) first = anc >> (D - dep - 1);
-                else first = 2 * node + (first_is_right<ND>(T, node, qc) ? 1 : 0);
-                dep += 1;
-                firstright = (firstright & ~(1ULL << dep)) | ((uint64_t)(first & 1) << dep);
-                node = first;
-                continue;
+                down = false;
             }
         }
-        // next node: climb while this is the second child visited at its
-        // depth, then step to its sibling (always present in the padded tree)
-        while (node != 1 && (uint64_t)(node & 1) != ((firstright >> dep) & 1ULL)) {
+        if (node == 1) break;
+        if ((uint64_t)(node & 1) == ((firstright >> dep) & 1ULL)) {
+            node ^= 1;  // the sibling always exists in the padded tree
+            down = true;
+        } else {
             node >>= 1;
             dep -= 1;
         }
-        if (node == 1) break;
-        node ^= 1;
     }
 }
 

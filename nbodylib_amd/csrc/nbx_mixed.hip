@@ -196,38 +196,79 @@ __device__ __forceinline__ void mix_leaf(const DevTree &T, const double *ext, in
     __syncthreads();
 }
 
+// Stackless depth-first walk: the ancestor of the wave's own leaves first,
+// else the child nearer to the wave's first query; firstright[dep] records
+// which child was entered first at each depth.  Two loop forms visit the
+// nodes in the same order; the compiler schedules them differently, and the
+// measured faster one is used for each dimensionality (3-D: climb to the
+// next unvisited sibling in an inner loop; 6-D: one node step per trip).
 template <int ND>
 __device__ __forceinline__ void mix_walk(const DevTree &T, const double *ext, MixCtx<ND> &c, const float *qc, int anc,
                                          uint64_t *hk, float *cx, int lane, int hc) {
     const int D = T.depth;
     int node = 1, dep = 0;
     uint64_t firstright = 0;
-    for (;;) {
-        cdptr bx = (cdptr)node_box(T, node, ND);
-        const bool want = c.act && box_lb<ND>(c.q, bx) <= (double)keyd(c.root);
-        if (ballot(want)) {
-            if (dep == D) {
-                mix_leaf<ND>(T, ext, node - T.nleafp, c, want, hk, cx, lane, hc);
+    if constexpr (ND == 3) {
+        for (;;) {
+            cdptr bx = (cdptr)node_box(T, node, ND);
+            const bool want = c.act && box_lb<ND>(c.q, bx) <= (double)keyd(c.root);
+            if (ballot(want)) {
+                if (dep == D) {
+                    mix_leaf<ND>(T, ext, node - T.nleafp, c, want, hk, cx, lane, hc);
+                } else {
+                    int first;
+                    if ((anc >> (D - dep)) == node) first = anc >> (D - dep - 1);
+                    else first = 2 * node + (first_is_right<ND>(T, node, qc) ? 1 : 0);
+                    dep += 1;
+                    firstright = (firstright & ~(1ULL << dep)) | ((uint64_t)(first & 1) << dep);
+                    node = first;
+                    continue;
+                }
+            }
+            while (node != 1 && (uint64_t)(node & 1) != ((firstright >> dep) & 1ULL)) {
+                node >>= 1;
+                dep -= 1;
+            }
+            if (node == 1) break;
+            node ^= 1;
+        }
+    } else {
+        bool down = true;
+        for (;;) {
+            if (down) {
+                cdptr bx = (cdptr)node_box(T, node, ND);
+                const bool want = c.act && box_lb<ND>(c.q, bx) <= (double)keyd(c.root);
+                if (ballot(want)) {
+                    if (dep == D) {
+                        mix_leaf<ND>(T, ext, node - T.nleafp, c, want, hk, cx, lane, hc);
+                        down = false;
+                    } else {
+                        const int c0 = 2 * node;
+                        int first = c0;
+                        if (anc >= 0 && (anc >> (D - dep)) == node) {
+                            first = anc >> (D - dep - 1);
+                        } else if (first_is_right<ND>(T, node, qc)) {
+                            first = c0 + 1;
+                        }
+                        dep += 1;
+                        if (first & 1) firstright |= (1ULL << dep);
+                        else firstright &= ~(1ULL << dep);
+                        node = first;
+                        continue;
+                    }
+                } else {
+                    down = false;
+                }
+            }
+            if (node == 1) break;
+            if ((uint64_t)(node & 1) == ((firstright >> dep) & 1ULL)) {
+                node ^= 1;
+                down = true;
             } else {
-                // descend: the ancestor of the wave's own leaves first, else
-                // the child nearer to the wave's first query
-                int first;
-                if ((anc >> (D - dep)) == node) first = anc >> (D - dep - 1);
-                else first = 2 * node + (first_is_right<ND>(T, node, qc) ? 1 : 0);
-                dep += 1;
-                firstright = (firstright & ~(1ULL << dep)) | ((uint64_t)(first & 1) << dep);
-                node = first;
-                continue;
+                node >>= 1;
+                dep -= 1;
             }
         }
-        // next node: climb while this is the second child visited at its
-        // depth, then step to its sibling
-        while (node != 1 && (uint64_t)(node & 1) != ((firstright >> dep) & 1ULL)) {
-            node >>= 1;
-            dep -= 1;
-        }
-        if (node == 1) break;
-        node ^= 1;
     }
 }
 
