@@ -68,38 +68,84 @@ __device__ __forceinline__ void kmax4(uint64_t x0, uint64_t x1, uint64_t x2, uin
 
 #define HK(p) hk[(p) * NBX_WAVE + lane]
 
-// Replace the root of a full 4-ary max-heap of hc entries, hc = 1 (mod 4)
-// so every parent has four children, and sift the new key down -- for the
-// lanes with ins set; the others rewrite the root with its own value.  Three
-// unrolled levels cover hc <= 85, and nothing branches, so the wave never
-// splits on how deep a key sinks.  Returns the new root.
-__device__ __forceinline__ uint64_t kheap_replace_root(uint64_t *hk, int lane, int hc, bool ins, uint64_t key,
-                                                       uint64_t root) {
-    const uint64_t v = ins ? key : root;
-    uint64_t top = v;
-    int p = 0;
-    bool go = ins;
+// The kh kept keys live in G groups of S LDS slots (slot p = g * S + s,
+// G * S = KMAX >= kh; slots p >= kh hold 0, which never outranks a key),
+// with every group's largest key and its slot in registers.  Replacing the
+// largest kept key is one LDS store and one S-wide reload of its group --
+// one LDS round trip, where a 4-ary heap of 33 needs three dependent ones.
+template <int KMAX>
+struct GShape {
+    static constexpr int G = KMAX <= 33 ? 3 : 5;
+    static constexpr int S = KMAX / G;
+    static_assert(G * S == KMAX, "group shape must tile KMAX");
+};
+
+template <int G>
+struct GTop {
+    uint64_t gm[G];  // largest key of each group
+    int gs[G];       // its slot within the group
+    int rg;          // group holding the largest kept key
+};
+
+template <int G, int S>
+__device__ __forceinline__ void gtop_init(uint64_t *hk, int lane, int kh, GTop<G> &H) {
+    for (int p = 0; p < G * S; ++p) HK(p) = p < kh ? MX_EMPTY : 0ULL;
 #pragma unroll
-    for (int lv = 0; lv < 3; ++lv) {
-        const int c = 4 * p + 1;
-        go = go && c < hc;
-        const int cc = go ? c : 1;
-        uint64_t bc;
-        int bi;
-        kmax4(HK(cc), HK(cc + 1), HK(cc + 2), HK(cc + 3), bc, bi);
-        go = go && bc > v;
-        HK(p) = go ? bc : v;
-        if (lv == 0) top = go ? bc : v;
-        p = go ? cc + bi : p;
+    for (int g = 0; g < G; ++g) {
+        H.gm[g] = g * S < kh ? MX_EMPTY : 0ULL;
+        H.gs[g] = 0;
     }
-    HK(p) = v;
+    H.rg = 0;
+}
+
+// Largest of S keys (a max tree) and the first slot holding it.
+template <int S>
+__device__ __forceinline__ void gmax(const uint64_t (&x)[S], uint64_t &bc, int &bs) {
+    double v[S];
+#pragma unroll
+    for (int u = 0; u < S; ++u) v[u] = kasd(x[u]);
+#pragma unroll
+    for (int w = 1; w < S; w <<= 1)
+#pragma unroll
+        for (int u = 0; u + w < S; u += 2 * w) v[u] = fmax(v[u], v[u + w]);
+    bc = dask(v[0]);
+    bs = S - 1;
+#pragma unroll
+    for (int u = S - 2; u >= 0; --u) bs = x[u] == bc ? u : bs;
+}
+
+// Replace the largest kept key (root) by `key` in the lanes with ins set;
+// the others store the root back into its own slot, so nothing branches.
+// Returns the new largest.
+template <int G, int S>
+__device__ __forceinline__ uint64_t gtop_replace(uint64_t *hk, int lane, GTop<G> &H, bool ins, uint64_t key,
+                                                 uint64_t root) {
+    const int g0 = H.rg;
+    int s0 = H.gs[0];
+#pragma unroll
+    for (int g = 1; g < G; ++g) s0 = g0 == g ? H.gs[g] : s0;
+    const int b = g0 * S;
+    HK(b + s0) = ins ? key : root;
+    uint64_t x[S];
+#pragma unroll
+    for (int u = 0; u < S; ++u) x[u] = HK(b + u);
+    uint64_t bc;
+    int bs;
+    gmax<S>(x, bc, bs);
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        H.gm[g] = g0 == g ? bc : H.gm[g];
+        H.gs[g] = g0 == g ? bs : H.gs[g];
+    }
+    uint64_t top;
+    gmax<G>(H.gm, top, H.rg);
     return top;
 }
 
-// Sift v down from the root of a heap of m entries (any m <= hc); children
-// at positions >= m read as 0, which never outranks a key.
-__device__ __forceinline__ void kheap_sift_bounded(uint64_t *hk, int lane, int m, uint64_t v) {
-    int p = 0;
+// Sift v down from slot p of a 4-ary max-heap of m <= KMAX entries; KMAX is
+// 1 mod 4, so the four children of any parent below m lie inside the KMAX
+// slots; children at positions >= m read as 0, which never outranks a key.
+__device__ __forceinline__ void kheap_sift_at(uint64_t *hk, int lane, int m, int p, uint64_t v) {
     bool go = true;
 #pragma unroll
     for (int lv = 0; lv < 3; ++lv) {
@@ -122,8 +168,6 @@ __device__ __forceinline__ void kheap_sift_bounded(uint64_t *hk, int lane, int m
     HK(p) = v;
 }
 
-// Heap capacity for k+1 = kh kept keys: kh rounded up to 1 (mod 4).
-__host__ __device__ __forceinline__ int kheap_cap(int kh) { return ((kh + 2) & ~3) + 1; }
 
 template <int ND>
 struct MixCtx {
@@ -134,9 +178,9 @@ struct MixCtx {
     double emax;    // largest leaf half-diagonal that matters for this lane
 };
 
-template <int ND>
+template <int ND, int G, int S>
 __device__ __forceinline__ void mix_leaf(const DevTree &T, const double *ext, int leaf, MixCtx<ND> &c, bool want,
-                                         uint64_t *hk, float *cx, int lane, int hc) {
+                                         uint64_t *hk, float *cx, int lane, GTop<G> &H) {
     const int base = leaf * NBX_LEAF;
     // the LDS copy of the leaf for phase 2: loaded now, stored after phase 1
     // so the vector-load latency overlaps the candidate tests
@@ -179,7 +223,13 @@ __device__ __forceinline__ void mix_leaf(const DevTree &T, const double *ext, in
         for (int d = 0; d < ND; ++d) cx[d * NBX_LEAF + lane] = st[d];
     }
     __syncthreads();
-    while (ballot(mask != 0)) {
+    // as many steps as the lane with the most candidates needs (a counted
+    // loop keeps the kept-key state in place across steps)
+    int nit = __popc(mask);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) nit = max(nit, __shfl_xor(nit, off));
+    nit = __builtin_amdgcn_readfirstlane(nit);
+    for (int it = 0; it < nit; ++it) {
         const bool has = mask != 0;
         const int j = __builtin_ctz(mask | 0x80000000u);
         mask &= mask - 1;
@@ -190,8 +240,8 @@ __device__ __forceinline__ void mix_leaf(const DevTree &T, const double *ext, in
             t = qv[d] - cx[d * NBX_LEAF + j];
             dd = __builtin_fmaf(t, t, dd);
         }
-        uint64_t key = mkkey(dd, base + j);
-        c.root = kheap_replace_root(hk, lane, hc, has && key < c.root, key, c.root);
+        const uint64_t key = mkkey(dd, base + j);
+        c.root = gtop_replace<G, S>(hk, lane, H, has && key < c.root, key, c.root);
     }
     __syncthreads();
 }
@@ -202,9 +252,9 @@ __device__ __forceinline__ void mix_leaf(const DevTree &T, const double *ext, in
 // nodes in the same order; the compiler schedules them differently, and the
 // measured faster one is used for each dimensionality (3-D: climb to the
 // next unvisited sibling in an inner loop; 6-D: one node step per trip).
-template <int ND>
+template <int ND, int G, int S>
 __device__ __forceinline__ void mix_walk(const DevTree &T, const double *ext, MixCtx<ND> &c, const float *qc, int anc,
-                                         uint64_t *hk, float *cx, int lane, int hc) {
+                                         uint64_t *hk, float *cx, int lane, GTop<G> &H) {
     const int D = T.depth;
     int node = 1, dep = 0;
     uint64_t firstright = 0;
@@ -214,7 +264,7 @@ __device__ __forceinline__ void mix_walk(const DevTree &T, const double *ext, Mi
             const bool want = c.act && box_lb<ND>(c.q, bx) <= (double)keyd(c.root);
             if (ballot(want)) {
                 if (dep == D) {
-                    mix_leaf<ND>(T, ext, node - T.nleafp, c, want, hk, cx, lane, hc);
+                    mix_leaf<ND, G, S>(T, ext, node - T.nleafp, c, want, hk, cx, lane, H);
                 } else {
                     int first;
                     if ((anc >> (D - dep)) == node) first = anc >> (D - dep - 1);
@@ -240,7 +290,7 @@ __device__ __forceinline__ void mix_walk(const DevTree &T, const double *ext, Mi
                 const bool want = c.act && box_lb<ND>(c.q, bx) <= (double)keyd(c.root);
                 if (ballot(want)) {
                     if (dep == D) {
-                        mix_leaf<ND>(T, ext, node - T.nleafp, c, want, hk, cx, lane, hc);
+                        mix_leaf<ND, G, S>(T, ext, node - T.nleafp, c, want, hk, cx, lane, H);
                         down = false;
                     } else {
                         const int c0 = 2 * node;
@@ -363,20 +413,23 @@ __global__ __launch_bounds__(64) void k_knn_mixed(DevTree T, KnnArgs A, const do
     c.self = qi;
     c.root = MX_EMPTY;
     c.emax = 0.0;
-    const int hc = kheap_cap(kh);
-    for (int p = 0; p < hc; ++p) HK(p) = MX_EMPTY;
+    constexpr int G = GShape<KMAX>::G, S = GShape<KMAX>::S;
+    GTop<G> H;
+    gtop_init<G, S>(hk, lane, kh, H);
     float qc[ND];
 #pragma unroll
     for (int d = 0; d < ND; ++d) qc[d] = (float)readfirst_d(c.q[d]);
-    mix_walk<ND>(T, ext, c, qc, T.nleafp + 2 * g, hk, cx, lane, hc);
+    mix_walk<ND, G, S>(T, ext, c, qc, T.nleafp + 2 * g, hk, cx, lane, H);
 
     // ---------------------------------------------------------------- band
     bool ok = valid;
     if (valid) {
-        for (int m = hc - 1; m > 0; --m) {  // heap sort: ascending keys
+        // heap sort of the kh kept keys: ascending keys in slots 0 .. kh-1
+        for (int p = (kh - 2) >> 2; p >= 0; --p) kheap_sift_at(hk, lane, kh, p, HK(p));
+        for (int m = kh - 1; m > 0; --m) {
             uint64_t top = HK(0), last = HK(m);
             HK(m) = top;
-            kheap_sift_bounded(hk, lane, m, last);
+            kheap_sift_at(hk, lane, m, 0, last);
         }
         if (HK(kh - 1) == MX_EMPTY) ok = false;  // fewer than k+1 candidates
     }
