@@ -9,8 +9,12 @@
 // Search: particle coordinates are stored as FP32 offsets from their leaf's
 // FP64 centre; a query is re-expressed per leaf as fl32(q - centre) so every
 // difference is formed between two small FP32 numbers.  Each lane keeps the
-// k+1 smallest packed keys (bits(d32) << 32 | slot) in a 4-ary LDS heap --
-// 8-byte entries compared with one 64-bit integer compare.
+// k+1 smallest FP32 d2: their bits in LDS (4 bytes per entry, so LDS does
+// not limit how many waves a CU holds) and their particle slots in a
+// per-workgroup scratch array in device memory.  The kernel is persistent:
+// each workgroup takes waves of queries from its XCD's share of the tree
+// order through an atomic counter, so the scratch is sized by the resident
+// workgroups, not by n.
 //
 // Error bound: with u = 2^-24 and E the half-diagonal of the leaf a
 // candidate came from, |dist32 - dist64| <= 12 u dist32 + 5 u E.  Per
@@ -45,34 +49,16 @@ typedef float fx8 __attribute__((ext_vector_type(8)));
 typedef const __attribute__((address_space(4))) fx8 *cf8ptr;
 
 #define MX_U 5.9604644775390625e-8  // 2^-24
-// Heap keys: (bits(d32) << 32 | slot) + 2^52.  The offset adds one to the
-// exponent field of the key read as an FP64 number, so every key is a
-// normal, finite, positive double whose order is the integer order: the
-// largest of several keys is one v_max_f64 per pair.
-#define MX_KOFF (1ULL << 52)
-#define MX_EMPTY (0x7F7FFFFFFFFFFFFFULL + MX_KOFF)  // empty heap entry (FLT_MAX, slot ~0)
+#define MX_EMPTY 0x7F7FFFFFu  // empty kept entry: the bits of FLT_MAX
 
-__device__ __forceinline__ uint64_t mkkey(float d, int32_t s) {
-    return (((uint64_t)__float_as_uint(d) << 32) | (uint32_t)s) + MX_KOFF;
-}
-__device__ __forceinline__ float keyd(uint64_t k) { return __uint_as_float((uint32_t)((k - MX_KOFF) >> 32)); }
-__device__ __forceinline__ int32_t keys(uint64_t k) { return (int32_t)(uint32_t)k; }
-__device__ __forceinline__ double kasd(uint64_t k) { return __longlong_as_double((long long)k); }
-__device__ __forceinline__ uint64_t dask(double v) { return (uint64_t)__double_as_longlong(v); }
+#define HK(p) hk[(p) * NBX_WAVE + lane]   // kept d2 bits (LDS)
+#define SLA(p) sl[(p) * NBX_WAVE + lane]  // their slots (scratch, insertion positions)
 
-// Largest of four keys and its index 0..3.
-__device__ __forceinline__ void kmax4(uint64_t x0, uint64_t x1, uint64_t x2, uint64_t x3, uint64_t &bc, int &bi) {
-    bc = dask(fmax(fmax(kasd(x0), kasd(x1)), fmax(kasd(x2), kasd(x3))));
-    bi = bc == x1 ? 1 : (bc == x2 ? 2 : (bc == x3 ? 3 : 0));
-}
-
-#define HK(p) hk[(p) * NBX_WAVE + lane]
-
-// The kh kept keys live in G groups of S LDS slots (slot p = g * S + s,
-// G * S = KMAX >= kh; slots p >= kh hold 0, which never outranks a key),
-// with every group's largest key and its slot in registers.  Replacing the
-// largest kept key is one LDS store and one S-wide reload of its group --
-// one LDS round trip, where a 4-ary heap of 33 needs three dependent ones.
+// The kh = k+1 kept candidates live in G groups of S positions (position
+// p = g * S + s, G * S = KMAX >= kh; positions p >= kh hold 0, which never
+// outranks a candidate), with every group's largest d2 and its position in
+// registers.  Replacing the largest kept candidate is one LDS store, one
+// scratch store and one S-wide LDS reload of its group: one round trip.
 template <int KMAX>
 struct GShape {
     static constexpr int G = KMAX <= 33 ? 3 : 5;
@@ -82,105 +68,79 @@ struct GShape {
 
 template <int G>
 struct GTop {
-    uint64_t gm[G];  // largest key of each group
-    int gs[G];       // its slot within the group
-    int rg;          // group holding the largest kept key
+    float gm[G];  // largest kept d2 of each group
+    int gs[G];    // its position within the group
+    int rg;       // group holding the largest kept d2
 };
 
 template <int G, int S>
-__device__ __forceinline__ void gtop_init(uint64_t *hk, int lane, int kh, GTop<G> &H) {
-    for (int p = 0; p < G * S; ++p) HK(p) = p < kh ? MX_EMPTY : 0ULL;
+__device__ __forceinline__ void gtop_init(uint32_t *hk, int lane, int kh, GTop<G> &H) {
+    for (int p = 0; p < G * S; ++p) HK(p) = p < kh ? MX_EMPTY : 0u;
 #pragma unroll
     for (int g = 0; g < G; ++g) {
-        H.gm[g] = g * S < kh ? MX_EMPTY : 0ULL;
+        H.gm[g] = g * S < kh ? __uint_as_float(MX_EMPTY) : 0.0f;
         H.gs[g] = 0;
     }
     H.rg = 0;
 }
 
-// Largest of S keys (a max tree) and the first slot holding it.
+// Largest of S values (a max tree) and the first position holding it.
 template <int S>
-__device__ __forceinline__ void gmax(const uint64_t (&x)[S], uint64_t &bc, int &bs) {
-    double v[S];
+__device__ __forceinline__ void gmaxf(const float (&x)[S], float &bc, int &bs) {
+    float v[S];
 #pragma unroll
-    for (int u = 0; u < S; ++u) v[u] = kasd(x[u]);
+    for (int u = 0; u < S; ++u) v[u] = x[u];
 #pragma unroll
     for (int w = 1; w < S; w <<= 1)
 #pragma unroll
-        for (int u = 0; u + w < S; u += 2 * w) v[u] = fmax(v[u], v[u + w]);
-    bc = dask(v[0]);
+        for (int u = 0; u + w < S; u += 2 * w) v[u] = fmaxf(v[u], v[u + w]);
+    bc = v[0];
     bs = S - 1;
 #pragma unroll
     for (int u = S - 2; u >= 0; --u) bs = x[u] == bc ? u : bs;
 }
 
-// Replace the largest kept key (root) by `key` in the lanes with ins set;
-// the others store the root back into its own slot, so nothing branches.
-// Returns the new largest.
+// Replace the largest kept candidate (d2 = root) by (d, slot) in the lanes
+// with ins set; the others store the root back into its own position, so
+// nothing but the slot store branches.  Returns the new largest d2.
 template <int G, int S>
-__device__ __forceinline__ uint64_t gtop_replace(uint64_t *hk, int lane, GTop<G> &H, bool ins, uint64_t key,
-                                                 uint64_t root) {
+__device__ __forceinline__ float gtop_replace(uint32_t *hk, uint32_t *sl, int lane, GTop<G> &H, bool ins, float d,
+                                              int32_t slot, float root) {
     const int g0 = H.rg;
     int s0 = H.gs[0];
 #pragma unroll
     for (int g = 1; g < G; ++g) s0 = g0 == g ? H.gs[g] : s0;
     const int b = g0 * S;
-    HK(b + s0) = ins ? key : root;
-    uint64_t x[S];
+    HK(b + s0) = __float_as_uint(ins ? d : root);
+    if (ins) SLA(b + s0) = (uint32_t)slot;
+    float x[S];
 #pragma unroll
-    for (int u = 0; u < S; ++u) x[u] = HK(b + u);
-    uint64_t bc;
+    for (int u = 0; u < S; ++u) x[u] = __uint_as_float(HK(b + u));
+    float bc;
     int bs;
-    gmax<S>(x, bc, bs);
+    gmaxf<S>(x, bc, bs);
 #pragma unroll
     for (int g = 0; g < G; ++g) {
         H.gm[g] = g0 == g ? bc : H.gm[g];
         H.gs[g] = g0 == g ? bs : H.gs[g];
     }
-    uint64_t top;
-    gmax<G>(H.gm, top, H.rg);
+    float top;
+    gmaxf<G>(H.gm, top, H.rg);
     return top;
 }
-
-// Sift v down from slot p of a 4-ary max-heap of m <= KMAX entries; KMAX is
-// 1 mod 4, so the four children of any parent below m lie inside the KMAX
-// slots; children at positions >= m read as 0, which never outranks a key.
-__device__ __forceinline__ void kheap_sift_at(uint64_t *hk, int lane, int m, int p, uint64_t v) {
-    bool go = true;
-#pragma unroll
-    for (int lv = 0; lv < 3; ++lv) {
-        const int c = 4 * p + 1;
-        go = go && c < m;
-        const int cc = go ? c : 1;
-        uint64_t x[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const uint64_t t = HK(cc + u);
-            x[u] = cc + u < m ? t : 0ULL;
-        }
-        uint64_t bc;
-        int bi;
-        kmax4(x[0], x[1], x[2], x[3], bc, bi);
-        go = go && bc > v;
-        HK(p) = go ? bc : v;
-        p = go ? cc + bi : p;
-    }
-    HK(p) = v;
-}
-
 
 template <int ND>
 struct MixCtx {
     double q[ND];
     bool act;
     int32_t self;
-    uint64_t root;  // current largest key kept
+    float root;     // largest kept d2 (FP32)
     double emax;    // largest leaf half-diagonal that matters for this lane
 };
 
 template <int ND, int G, int S>
 __device__ __forceinline__ void mix_leaf(const DevTree &T, const double *ext, int leaf, MixCtx<ND> &c, bool want,
-                                         uint64_t *hk, float *cx, int lane, GTop<G> &H) {
+                                         uint32_t *hk, uint32_t *sl, float *cx, int lane, GTop<G> &H) {
     const int base = leaf * NBX_LEAF;
     // the LDS copy of the leaf for phase 2: loaded now, stored after phase 1
     // so the vector-load latency overlaps the candidate tests
@@ -192,7 +152,7 @@ __device__ __forceinline__ void mix_leaf(const DevTree &T, const double *ext, in
     float qv[ND];
 #pragma unroll
     for (int d = 0; d < ND; ++d) qv[d] = (float)(c.q[d] - ((cdptr)o)[d]);
-    const float th = keyd(c.root);
+    const float th = c.root;
     uint32_t mask = 0;
     // candidates in 32- or 16-byte scalar-load chunks per dimension
     constexpr int CH = ND == 6 ? 4 : 8;
@@ -240,8 +200,7 @@ __device__ __forceinline__ void mix_leaf(const DevTree &T, const double *ext, in
             t = qv[d] - cx[d * NBX_LEAF + j];
             dd = __builtin_fmaf(t, t, dd);
         }
-        const uint64_t key = mkkey(dd, base + j);
-        c.root = gtop_replace<G, S>(hk, lane, H, has && key < c.root, key, c.root);
+        c.root = gtop_replace<G, S>(hk, sl, lane, H, has && dd < c.root, dd, base + j, c.root);
     }
     __syncthreads();
 }
@@ -254,17 +213,17 @@ __device__ __forceinline__ void mix_leaf(const DevTree &T, const double *ext, in
 // next unvisited sibling in an inner loop; 6-D: one node step per trip).
 template <int ND, int G, int S>
 __device__ __forceinline__ void mix_walk(const DevTree &T, const double *ext, MixCtx<ND> &c, const float *qc, int anc,
-                                         uint64_t *hk, float *cx, int lane, GTop<G> &H) {
+                                         uint32_t *hk, uint32_t *sl, float *cx, int lane, GTop<G> &H) {
     const int D = T.depth;
     int node = 1, dep = 0;
     uint64_t firstright = 0;
     if constexpr (ND == 3) {
         for (;;) {
             cdptr bx = (cdptr)node_box(T, node, ND);
-            const bool want = c.act && box_lb<ND>(c.q, bx) <= (double)keyd(c.root);
+            const bool want = c.act && box_lb<ND>(c.q, bx) <= (double)c.root;
             if (ballot(want)) {
                 if (dep == D) {
-                    mix_leaf<ND, G, S>(T, ext, node - T.nleafp, c, want, hk, cx, lane, H);
+                    mix_leaf<ND, G, S>(T, ext, node - T.nleafp, c, want, hk, sl, cx, lane, H);
                 } else {
                     int first;
                     if ((anc >> (D - dep)) == node) first = anc >> (D - dep - 1);
@@ -287,10 +246,10 @@ __device__ __forceinline__ void mix_walk(const DevTree &T, const double *ext, Mi
         for (;;) {
             if (down) {
                 cdptr bx = (cdptr)node_box(T, node, ND);
-                const bool want = c.act && box_lb<ND>(c.q, bx) <= (double)keyd(c.root);
+                const bool want = c.act && box_lb<ND>(c.q, bx) <= (double)c.root;
                 if (ballot(want)) {
                     if (dep == D) {
-                        mix_leaf<ND, G, S>(T, ext, node - T.nleafp, c, want, hk, cx, lane, H);
+                        mix_leaf<ND, G, S>(T, ext, node - T.nleafp, c, want, hk, sl, cx, lane, H);
                         down = false;
                     } else {
                         const int c0 = 2 * node;
@@ -392,16 +351,19 @@ __device__ __forceinline__ int verify_walk(const DevTree &T, const double *q, bo
     return cnt;
 }
 
+#define SLB(p) slb[(p) * NBX_WAVE + lane]  // slots in ascending d2 order (scratch)
+#define SKC(p) skc[(p) * NBX_WAVE + lane]  // the matching d2 bits (scratch)
+
+// One wave of 64 consecutive queries (tree order), wave index g.
 // k = kh - 1 neighbours; A.dens: NBX_DO_NONE / NBX_DO_SYM / NBX_DO_GATHER.
 template <int ND, int KMAX>
-__global__ __launch_bounds__(64) void k_knn_mixed(DevTree T, KnnArgs A, const double *ext, int32_t *fb_list,
-                                                   int32_t *fb_count) {
-    __shared__ uint64_t hk[KMAX * NBX_WAVE];
-    __shared__ float cx[ND * NBX_LEAF];
-    __shared__ double racc[NBX_WAVE];  // symmetric scatter into the wave's own queries
-    const int lane = threadIdx.x;
-    const int g = xcd_wave(blockIdx.x, gridDim.x);
-    const int kh = A.kh;  // k + 1
+__device__ __forceinline__ void mix_task(const DevTree &T, const KnnArgs &A, const double *ext, int32_t *fb_list,
+                                         int32_t *fb_count, uint32_t *hk, float *cx, double *racc, uint32_t *sl,
+                                         uint32_t *slb, uint32_t *skc, int g, int lane) {
+    // k + 1, opaque to the compiler so nothing derived from it is hoisted
+    // out of the workgroup's loop into registers
+    int kh = A.kh;
+    asm volatile("" : "+s"(kh));
     const int k = kh - 1;
     const int qi = g * NBX_WAVE + lane;
     const bool valid = qi < T.n;
@@ -411,27 +373,35 @@ __global__ __launch_bounds__(64) void k_knn_mixed(DevTree T, KnnArgs A, const do
     for (int d = 0; d < ND; ++d) c.q[d] = valid ? T.x[d][qi] : 0.0;
     c.act = valid;
     c.self = qi;
-    c.root = MX_EMPTY;
+    c.root = __uint_as_float(MX_EMPTY);
     c.emax = 0.0;
     constexpr int G = GShape<KMAX>::G, S = GShape<KMAX>::S;
     GTop<G> H;
     gtop_init<G, S>(hk, lane, kh, H);
+    for (int p = 0; p < kh; ++p) SLA(p) = 0u;  // every slot read later is a valid index
     float qc[ND];
 #pragma unroll
     for (int d = 0; d < ND; ++d) qc[d] = (float)readfirst_d(c.q[d]);
-    mix_walk<ND, G, S>(T, ext, c, qc, T.nleafp + 2 * g, hk, cx, lane, H);
+    mix_walk<ND, G, S>(T, ext, c, qc, T.nleafp + 2 * g, hk, sl, cx, lane, H);
 
     // ---------------------------------------------------------------- band
     bool ok = valid;
     if (valid) {
-        // heap sort of the kh kept keys: ascending keys in slots 0 .. kh-1
-        for (int p = (kh - 2) >> 2; p >= 0; --p) kheap_sift_at(hk, lane, kh, p, HK(p));
-        for (int m = kh - 1; m > 0; --m) {
-            uint64_t top = HK(0), last = HK(m);
-            HK(m) = top;
-            kheap_sift_at(hk, lane, m, 0, last);
+        // rank sort of the kh kept candidates by (d2 bits, position): d2
+        // bits to SKC, slots from their insertion positions to SLB
+        uint64_t kq[KMAX];
+#pragma unroll
+        for (int p = 0; p < KMAX; ++p) kq[p] = p < kh ? (((uint64_t)HK(p) << 7) | (uint64_t)p) : ~0ULL;
+        for (int i = 0; i < kh; ++i) {
+            const uint32_t hi_ = HK(i);
+            const uint64_t ki = ((uint64_t)hi_ << 7) | (uint64_t)i;
+            int r = 0;
+#pragma unroll
+            for (int j = 0; j < KMAX; ++j) r += kq[j] < ki ? 1 : 0;
+            SKC(r) = hi_;
+            SLB(r) = SLA(i);
         }
-        if (HK(kh - 1) == MX_EMPTY) ok = false;  // fewer than k+1 candidates
+        if (SKC(kh - 1) == MX_EMPTY) ok = false;  // fewer than k+1 candidates
     }
     const double eu = 12.0 * MX_U, ea = 5.0 * MX_U * c.emax;
     int i0 = k - 1;
@@ -440,15 +410,15 @@ __global__ __launch_bounds__(64) void k_knn_mixed(DevTree T, KnnArgs A, const do
     double Dk = 0.0;
     if (ok) {
         // band: entries whose interval overlaps the k-th entry's
-        double dk = sqrt((double)keyd(HK(k - 1)));
+        double dk = sqrt((double)__uint_as_float(SKC(k - 1)));
         double lo_k = dk - (eu * dk + ea);
         while (i0 > 0) {
-            double di = sqrt((double)keyd(HK(i0 - 1)));
+            double di = sqrt((double)__uint_as_float(SKC(i0 - 1)));
             if (di + (eu * di + ea) < lo_k) break;
             --i0;
         }
         // a certain entry that could be an exact duplicate (d2 == 0) must be checked too
-        double d0 = sqrt((double)keyd(HK(0)));
+        double d0 = sqrt((double)__uint_as_float(SKC(0)));
         if (d0 - (eu * d0 + ea) <= 0.0 && i0 > 0) ok = false;
         if (kh - i0 > 4) ok = false;
     }
@@ -462,44 +432,53 @@ __global__ __launch_bounds__(64) void k_knn_mixed(DevTree T, KnnArgs A, const do
             bs[b] = -1;
             bd[b] = 0.0;
             if (b < nb) {
-                bs[b] = keys(HK(i0 + b));
+                bs[b] = (int32_t)SLB(i0 + b);
                 bd[b] = d64<ND>(T, c.q, bs[b]);
                 if (bd[b] == 0.0) { nzero++; drop = b; }
             }
         }
         if (nzero > 1) ok = false;
         if (drop < 0) {
+            // the largest (d2, ID) of the band (static indices only)
             drop = 0;
+            double dd = bd[0];
+            int32_t pd = T.perm[bs[0]];
 #pragma unroll
-            for (int b = 1; b < 4; ++b)
-                if (b < nb && (bd[b] > bd[drop] || (bd[b] == bd[drop] && T.perm[bs[b]] > T.perm[bs[drop]]))) drop = b;
+            for (int b = 1; b < 4; ++b) {
+                const int32_t pb = b < nb ? T.perm[bs[b]] : 0;
+                if (b < nb && (bd[b] > dd || (bd[b] == dd && pb > pd))) {
+                    drop = b;
+                    dd = bd[b];
+                    pd = pb;
+                }
+            }
         }
         Dk = 0.0;
 #pragma unroll
         for (int b = 0; b < 4; ++b)
             if (b < nb && b != drop && bd[b] > Dk) Dk = bd[b];
         // certificate against everything outside the heap
-        double dl = sqrt((double)keyd(HK(kh - 1)));
+        double dl = sqrt((double)__uint_as_float(SKC(kh - 1)));
         double lout = dl - (eu * dl + ea);
         if (!(lout > 0.0 && Dk < lout * lout)) ok = false;
         if (ok) {
             // the kept band entries fill positions i0 .. k-1 (FP32 order)
 #pragma unroll
             for (int b = 0; b < 4; ++b)
-                if (b < nb && b != drop) HK(i0 + b - (b > drop ? 1 : 0)) = (uint64_t)(uint32_t)bs[b];
+                if (b < nb && b != drop) SLB(i0 + b - (b > drop ? 1 : 0)) = (uint32_t)bs[b];
         }
     }
     // exact verification for lanes the error bound could not decide: keep
     // the k smallest (d2, ID) of the k+1 candidates in FP64, then count in
     // FP64 the particles below the k-th of them
-    bool verify = valid && !ok && HK(kh - 1) != MX_EMPTY;
+    bool verify = valid && !ok && SKC(kh - 1) != MX_EMPTY;
     int32_t IDk = 0;
     if (verify) {
         int zero_at = -1, nzero = 0, maxp = -1;
         double dmax = -1.0;
         int32_t imax = -1;
         for (int p = 0; p < kh; ++p) {
-            int32_t s = keys(HK(p));
+            int32_t s = (int32_t)SLB(p);
             double d = d64<ND>(T, c.q, s);
             if (d == 0.0) {
                 nzero++;
@@ -512,11 +491,11 @@ __global__ __launch_bounds__(64) void k_knn_mixed(DevTree T, KnnArgs A, const do
         const int drop = nzero == 1 ? zero_at : maxp;
         if (nzero > 1) verify = false;
         if (verify) {
-            for (int p = drop; p < kh - 1; ++p) HK(p) = HK(p + 1);
+            for (int p = drop; p < kh - 1; ++p) SLB(p) = SLB(p + 1);
             Dk = 0.0;
             IDk = -1;
             for (int p = 0; p < k; ++p) {
-                int32_t s = keys(HK(p));
+                int32_t s = (int32_t)SLB(p);
                 double d = d64<ND>(T, c.q, s);
                 int32_t id = T.perm[s];
                 if (d > Dk || (d == Dk && id > IDk)) { Dk = d; IDk = id; }
@@ -559,7 +538,7 @@ __global__ __launch_bounds__(64) void k_knn_mixed(DevTree T, KnnArgs A, const do
             double d2v[8], mv[8];
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
-                sv[u] = p0 + u < k ? keys(HK(p0 + u)) : qi;
+                sv[u] = p0 + u < k ? (int32_t)SLB(p0 + u) : qi;
                 d2v[u] = d64<ND>(T, c.q, sv[u]);
                 mv[u] = T.mass[sv[u]];
             }
@@ -596,28 +575,95 @@ __global__ __launch_bounds__(64) void k_knn_mixed(DevTree T, KnnArgs A, const do
         int qr = g * NBX_WAVE + r;
         if (qr >= T.n) break;
         if (!((okm >> r) & 1ULL)) continue;
-        int32_t s = keys(hk[t * NBX_WAVE + r]);
+        int32_t s = (int32_t)slb[t * NBX_WAVE + r];
         int64_t row = A.layout == 1 ? qr : T.perm[qr];
         A.nn[row * k + t] = A.layout == 1 ? s : T.perm[s];
     }
 }
 
+// Persistent precision-form kernel, one wave per workgroup.  Workgroup b
+// stays on XCD (b mod 8) (workgroups are dealt to XCDs round robin), takes
+// waves of queries from that XCD's contiguous share of the tree order (the
+// xcd_wave split) through the XCD's counter, and owns one scratch block of
+// 3 * KMAX * 64 words (slots by insertion position, slots and d2 bits in
+// ascending order) for its whole life, so the scratch is sized by the
+// resident workgroups and only ever touched from one XCD.  Every workgroup
+// exits once its counter passes the share's end.
+#define MX_CTR_STRIDE 16  // XCD counters 64 bytes apart
+template <int ND, int KMAX>
+__global__ __launch_bounds__(64) void k_knn_mixed(DevTree T, KnnArgs A, const double *ext, int32_t *fb_list,
+                                                   int32_t *fb_count, uint32_t *scratch, int32_t *ctr) {
+    __shared__ uint32_t hk[KMAX * NBX_WAVE];
+    __shared__ float cx[ND * NBX_LEAF];
+    __shared__ double racc[NBX_WAVE];  // symmetric scatter into the wave's own queries
+    const int lane = threadIdx.x;
+    uint32_t *sl = scratch + (size_t)blockIdx.x * (3 * KMAX * NBX_WAVE);
+    uint32_t *slb = sl + KMAX * NBX_WAVE, *skc = slb + KMAX * NBX_WAVE;
+    const int nw = (T.n + NBX_WAVE - 1) / NBX_WAVE;
+    const int x = blockIdx.x & 7, q8 = nw >> 3, r8 = nw & 7;
+    const int wbeg = x * q8 + (x < r8 ? x : r8), wcnt = q8 + (x < r8 ? 1 : 0);
+    for (;;) {
+        int t = 0;
+        if (lane == 0) t = atomicAdd(ctr + x * MX_CTR_STRIDE, 1);
+        t = __builtin_amdgcn_readfirstlane(__shfl(t, 0));
+        if (t >= wcnt) break;
+        mix_task<ND, KMAX>(T, A, ext, fb_list, fb_count, hk, cx, racc, sl, slb, skc, wbeg + t, lane);
+    }
+}
+
+__global__ void k_mx_zero(int32_t *p, int cnt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < cnt) p[i] = 0;
+}
+
+// Resident workgroups of one instance over the device, a multiple of 8
+// (every XCD gets workgroups) and at least 8.
+template <int ND, int KMAX>
+static int mx_grid() {
+    static int grid = 0;
+    if (grid == 0) {
+        int dev = 0, ncu = 0, per = 0;
+        if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+        if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0) ncu = 256;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_knn_mixed<ND, KMAX>, NBX_WAVE, 0) != hipSuccess ||
+            per <= 0)
+            per = 8;
+        const int gsz = (per * ncu) & ~7;
+        grid = gsz < 8 ? 8 : gsz;
+    }
+    return grid;
+}
+
 hipError_t nbx_launch_knn_mixed(const DevTree &T, const KnnArgs &A, const double *ext, int32_t *fallback_list,
                                 int32_t *fallback_count, hipStream_t st) {
-    int blocks = (T.n + NBX_WAVE - 1) / NBX_WAVE;
-    if (blocks == 0) return hipSuccess;
+    const int nw = (T.n + NBX_WAVE - 1) / NBX_WAVE;
+    if (nw == 0) return hipSuccess;
     if ((T.nd != 3 && T.nd != 6) || A.kh > 65 || A.kh < 2 || !T.kernf || !T.org || !ext) return hipErrorInvalidValue;
-    ProfScope ps("k_knn_mixed", st);
-#define MIXL(ND, KM)                                                                                                  \
-    hipLaunchKernelGGL((k_knn_mixed<ND, KM>), dim3(blocks), dim3(NBX_WAVE), 0, st, T, A, ext, fallback_list, \
-                       fallback_count)
-    if (T.nd == 3) {
-        if (A.kh <= 33) MIXL(3, 33);
-        else MIXL(3, 65);
-    } else {
-        if (A.kh <= 33) MIXL(6, 33);
-        else MIXL(6, 65);
-    }
+    const int km = A.kh <= 33 ? 33 : 65;
+    const int grid = T.nd == 3 ? (km == 33 ? mx_grid<3, 33>() : mx_grid<3, 65>())
+                               : (km == 33 ? mx_grid<6, 33>() : mx_grid<6, 65>());
+    const size_t nscr = (size_t)grid * 3 * km * NBX_WAVE;
+    void *mem = nullptr;
+    hipError_t e = hipMallocAsync(&mem, nscr * sizeof(uint32_t) + 8 * MX_CTR_STRIDE * sizeof(int32_t), st);
+    if (e != hipSuccess) return e;
+    uint32_t *scr = (uint32_t *)mem;
+    int32_t *ctr = (int32_t *)(scr + nscr);
+    hipLaunchKernelGGL(k_mx_zero, dim3(1), dim3(8 * MX_CTR_STRIDE), 0, st, ctr, 8 * MX_CTR_STRIDE);
+    {
+        ProfScope ps("k_knn_mixed", st);
+#define MIXL(ND, KM)                                                                                            \
+    hipLaunchKernelGGL((k_knn_mixed<ND, KM>), dim3(grid), dim3(NBX_WAVE), 0, st, T, A, ext, fallback_list, \
+                       fallback_count, scr, ctr)
+        if (T.nd == 3) {
+            if (km == 33) MIXL(3, 33);
+            else MIXL(3, 65);
+        } else {
+            if (km == 33) MIXL(6, 33);
+            else MIXL(6, 65);
+        }
 #undef MIXL
-    return hipGetLastError();
+    }
+    e = hipGetLastError();
+    hipFreeAsync(mem, st);
+    return e;
 }
