@@ -49,6 +49,7 @@ typedef float fx8 __attribute__((ext_vector_type(8)));
 typedef const __attribute__((address_space(4))) fx8 *cf8ptr;
 
 #define MX_U 5.9604644775390625e-8  // 2^-24
+#define MX_DB 4               // neighbours gathered together in the density pass
 #define MX_EMPTY 0x7F7FFFFFu  // empty kept entry: the bits of FLT_MAX
 
 #define HK(p) hk[(p) * NBX_WAVE + lane]   // kept d2 bits (LDS)
@@ -387,19 +388,35 @@ __device__ __forceinline__ void mix_task(const DevTree &T, const KnnArgs &A, con
     // ---------------------------------------------------------------- band
     bool ok = valid;
     if (valid) {
-        // rank sort of the kh kept candidates by (d2 bits, position): d2
-        // bits to SKC, slots from their insertion positions to SLB
-        uint64_t kq[KMAX];
+        // rank sort of the kh kept candidates by (d2, position): d2 bits to
+        // SKC, slots from their insertion positions to SLB
+        if constexpr (KMAX <= 33) {
+            uint64_t kq[KMAX];
 #pragma unroll
-        for (int p = 0; p < KMAX; ++p) kq[p] = p < kh ? (((uint64_t)HK(p) << 7) | (uint64_t)p) : ~0ULL;
-        for (int i = 0; i < kh; ++i) {
-            const uint32_t hi_ = HK(i);
-            const uint64_t ki = ((uint64_t)hi_ << 7) | (uint64_t)i;
-            int r = 0;
+            for (int p = 0; p < KMAX; ++p) kq[p] = p < kh ? (((uint64_t)HK(p) << 7) | (uint64_t)p) : ~0ULL;
+            for (int i = 0; i < kh; ++i) {
+                const uint32_t hb = HK(i);
+                const uint64_t ki = ((uint64_t)hb << 7) | (uint64_t)i;
+                int r = 0;
 #pragma unroll
-            for (int j = 0; j < KMAX; ++j) r += kq[j] < ki ? 1 : 0;
-            SKC(r) = hi_;
-            SLB(r) = SLA(i);
+                for (int j = 0; j < KMAX; ++j) r += kq[j] < ki ? 1 : 0;
+                SKC(r) = hb;
+                SLB(r) = SLA(i);
+            }
+        } else {
+            // half the registers: FP32 d2 with the position tie-break
+            float kf[KMAX];
+#pragma unroll
+            for (int p = 0; p < KMAX; ++p) kf[p] = p < kh ? __uint_as_float(HK(p)) : __builtin_inff();
+            for (int i = 0; i < kh; ++i) {
+                const uint32_t hb = HK(i);
+                const float ki = __uint_as_float(hb);
+                int r = 0;
+#pragma unroll
+                for (int j = 0; j < KMAX; ++j) r += (kf[j] < ki || (kf[j] == ki && j < i)) ? 1 : 0;
+                SKC(r) = hb;
+                SLB(r) = SLA(i);
+            }
         }
         if (SKC(kh - 1) == MX_EMPTY) ok = false;  // fewer than k+1 candidates
     }
@@ -531,19 +548,19 @@ __device__ __forceinline__ void mix_task(const DevTree &T, const KnnArgs &A, con
         const double norm = 1.0 / pow(hi, (double)ND);
         const double mi = T.mass[qi];
         double acc = 0.0;
-        // neighbours in groups of 8 so their coordinate/mass gathers are in
+        // neighbours in groups of MX_DB so their coordinate/mass gathers are in
         // flight together
-        for (int p0 = 0; p0 < k; p0 += 8) {
-            int32_t sv[8];
-            double d2v[8], mv[8];
+        for (int p0 = 0; p0 < k; p0 += MX_DB) {
+            int32_t sv[MX_DB];
+            double d2v[MX_DB], mv[MX_DB];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
+            for (int u = 0; u < MX_DB; ++u) {
                 sv[u] = p0 + u < k ? (int32_t)SLB(p0 + u) : qi;
                 d2v[u] = d64<ND>(T, c.q, sv[u]);
                 mv[u] = T.mass[sv[u]];
             }
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
+            for (int u = 0; u < MX_DB; ++u) {
                 if (p0 + u >= k) break;
                 const int32_t s = sv[u];
                 double W = wsm_at(sqrt(d2v[u]), hi, T.kern, kres);
