@@ -22,7 +22,8 @@
 // u(|q - o| + |x - o| + |t|) <= u(dist + 2E + dist); the FMA chain over ND
 // <= 6 squares adds ND u relative to the sum, sqrt halves it and adds u:
 // at most (2 + ND/2 + 1) u dist + 2 u E = 6 u dist + 2 u E in 6-D, so the
-// constants keep a factor 2 of margin.  Each lane tracks E_max over the leaves it evaluated
+// constants keep a factor 2 of margin.  A kept key adds its tag error (2^TB
+// u dist, see the kept keys below) to the 12 u dist term.  Each lane tracks E_max over the leaves it evaluated
 // as reachable or accepted a candidate from (candidates of other leaves are
 // outside by the FP64 node bound).
 //
@@ -52,82 +53,93 @@ typedef const __attribute__((address_space(4))) fx8 *cf8ptr;
 #define MX_DB 4               // neighbours gathered together in the density pass
 #define MX_EMPTY 0x7F7FFFFFu  // empty kept entry: the bits of FLT_MAX
 
-#define HK(p) hk[(p) * NBX_WAVE + lane]   // kept d2 bits (LDS)
-#define SLA(p) sl[(p) * NBX_WAVE + lane]  // their slots (scratch, insertion positions)
+#define HK(p) hk[(p) * NBX_WAVE + lane]   // kept keys (LDS)
+#define SLA(p) sl[(p) * NBX_WAVE + lane]  // their slots (scratch, by position)
 
-// The kh = k+1 kept candidates live in G groups of S positions (position
-// p = g * S + s, G * S = KMAX >= kh; positions p >= kh hold 0, which never
-// outranks a candidate), with every group's largest d2 and its position in
-// registers.  Replacing the largest kept candidate is one LDS store, one
-// scratch store and one S-wide LDS reload of its group: one round trip.
+// Kept keys.  The kh = k+1 kept candidates of a lane sit at positions
+// p < kh of KMAX = G * S (G groups of S positions; positions p >= kh hold
+// 0, which never outranks a key).  A kept key is the FP32 d2's bits with the
+// low TB bits replaced by a tag: its position p when KMAX <= 33 (keys are
+// then unique), else its position within its group (the group holding the
+// largest key is found among the group maxima).  Keys compare as unsigned
+// integers in d2 order up to the TB replaced bits, and the largest key names
+// its own position.  A key overstates or understates d2 by less than 2^TB
+// ulps -- a relative 2^TB * 2^-23 in d2, 2^TB * u in the distance -- which
+// the error bound below carries.  Each group's largest key stays in a
+// register; replacing the largest kept key is one LDS store, one scratch
+// store and one S-wide LDS reload of its group.
 template <int KMAX>
 struct GShape {
     static constexpr int G = KMAX <= 33 ? 3 : 5;
     static constexpr int S = KMAX / G;
-    static_assert(G * S == KMAX, "group shape must tile KMAX");
+    static constexpr bool GT = KMAX <= 33;    // tag = position p (else position in group)
+    static constexpr int TB = GT ? 6 : 4;      // tag bits
+    static constexpr uint32_t TM = (1u << TB) - 1u;
+    static_assert(G * S == KMAX && (GT ? KMAX : S) <= (1 << TB), "group shape must tile KMAX");
 };
 
-template <int G>
-struct GTop {
-    float gm[G];  // largest kept d2 of each group
-    int gs[G];    // its position within the group
-    int rg;       // group holding the largest kept d2
-};
+__device__ __forceinline__ uint32_t umax3(uint32_t a, uint32_t b, uint32_t c) { return max(max(a, b), c); }
 
-template <int G, int S>
-__device__ __forceinline__ void gtop_init(uint32_t *hk, int lane, int kh, GTop<G> &H) {
-    for (int p = 0; p < G * S; ++p) HK(p) = p < kh ? MX_EMPTY : 0u;
-#pragma unroll
-    for (int g = 0; g < G; ++g) {
-        H.gm[g] = g * S < kh ? __uint_as_float(MX_EMPTY) : 0.0f;
-        H.gs[g] = 0;
-    }
-    H.rg = 0;
-}
-
-// Largest of S values (a max tree) and the first position holding it.
+// Largest of S keys (a max tree).
 template <int S>
-__device__ __forceinline__ void gmaxf(const float (&x)[S], float &bc, int &bs) {
-    float v[S];
+__device__ __forceinline__ uint32_t gmaxu(const uint32_t (&x)[S]) {
+    uint32_t v[S];
 #pragma unroll
     for (int u = 0; u < S; ++u) v[u] = x[u];
 #pragma unroll
-    for (int w = 1; w < S; w <<= 1)
+    for (int w = 1; w < S; w *= 3)
 #pragma unroll
-        for (int u = 0; u + w < S; u += 2 * w) v[u] = fmaxf(v[u], v[u + w]);
-    bc = v[0];
-    bs = S - 1;
-#pragma unroll
-    for (int u = S - 2; u >= 0; --u) bs = x[u] == bc ? u : bs;
+        for (int u = 0; u + w < S; u += 3 * w) v[u] = u + 2 * w < S ? umax3(v[u], v[u + w], v[u + 2 * w]) : max(v[u], v[u + w]);
+    return v[0];
 }
 
-// Replace the largest kept candidate (d2 = root) by (d, slot) in the lanes
-// with ins set; the others store the root back into its own position, so
-// nothing but the slot store branches.  Returns the new largest d2.
-template <int G, int S>
-__device__ __forceinline__ float gtop_replace(uint32_t *hk, uint32_t *sl, int lane, GTop<G> &H, bool ins, float d,
-                                              int32_t slot, float root) {
-    const int g0 = H.rg;
-    int s0 = H.gs[0];
-#pragma unroll
-    for (int g = 1; g < G; ++g) s0 = g0 == g ? H.gs[g] : s0;
-    const int b = g0 * S;
-    HK(b + s0) = __float_as_uint(ins ? d : root);
-    if (ins) SLA(b + s0) = (uint32_t)slot;
-    float x[S];
-#pragma unroll
-    for (int u = 0; u < S; ++u) x[u] = __uint_as_float(HK(b + u));
-    float bc;
-    int bs;
-    gmaxf<S>(x, bc, bs);
+template <int KMAX>
+__device__ __forceinline__ uint32_t gtop_init(uint32_t *hk, int lane, int kh, uint32_t (&gm)[GShape<KMAX>::G]) {
+    constexpr int G = GShape<KMAX>::G, S = GShape<KMAX>::S;
+    constexpr uint32_t TM = GShape<KMAX>::TM;
 #pragma unroll
     for (int g = 0; g < G; ++g) {
-        H.gm[g] = g0 == g ? bc : H.gm[g];
-        H.gs[g] = g0 == g ? bs : H.gs[g];
+        uint32_t m = 0u;
+        for (int u = 0; u < S; ++u) {
+            const int p = g * S + u;
+            const uint32_t v = p < kh ? ((MX_EMPTY & ~TM) | (uint32_t)(GShape<KMAX>::GT ? p : u)) : 0u;
+            HK(p) = v;
+            m = max(m, v);
+        }
+        gm[g] = m;
     }
-    float top;
-    gmaxf<G>(H.gm, top, H.rg);
-    return top;
+    return gmaxu<G>(gm);
+}
+
+// Replace the largest kept key (root) by key in the lanes with ins set (key
+// carries the root's position); the others store the root back, so nothing
+// but the slot store branches.  Returns the new largest key.
+template <int KMAX>
+__device__ __forceinline__ uint32_t gtop_replace(uint32_t *hk, uint32_t *sl, int lane,
+                                                 uint32_t (&gm)[GShape<KMAX>::G], bool ins, uint32_t key,
+                                                 int32_t slot, uint32_t root) {
+    constexpr int G = GShape<KMAX>::G, S = GShape<KMAX>::S;
+    constexpr uint32_t TM = GShape<KMAX>::TM;
+    int g0 = 0, p;
+    if constexpr (GShape<KMAX>::GT) {
+        p = (int)(root & TM);
+#pragma unroll
+        for (int g = 1; g < G; ++g) g0 += p >= g * S ? 1 : 0;
+    } else {
+#pragma unroll
+        for (int g = G - 1; g > 0; --g) g0 = gm[g] == root ? g : g0;
+        p = g0 * S + (int)(root & TM);
+    }
+    const int b = g0 * S;
+    HK(p) = ins ? key : root;
+    if (ins) SLA(p) = (uint32_t)slot;
+    uint32_t x[S];
+#pragma unroll
+    for (int u = 0; u < S; ++u) x[u] = HK(b + u);
+    const uint32_t bc = gmaxu<S>(x);
+#pragma unroll
+    for (int g = 0; g < G; ++g) gm[g] = g0 == g ? bc : gm[g];
+    return gmaxu<G>(gm);
 }
 
 template <int ND>
@@ -135,13 +147,14 @@ struct MixCtx {
     double q[ND];
     bool act;
     int32_t self;
-    float root;     // largest kept d2 (FP32)
+    uint32_t root;  // largest kept key
     double emax;    // largest leaf half-diagonal that matters for this lane
 };
 
-template <int ND, int G, int S>
+template <int ND, int KMAX>
 __device__ __forceinline__ void mix_leaf(const DevTree &T, const double *ext, int leaf, MixCtx<ND> &c, bool want,
-                                         uint32_t *hk, uint32_t *sl, float *cx, int lane, GTop<G> &H) {
+                                         uint32_t *hk, uint32_t *sl, float *cx, int lane,
+                                         uint32_t (&gm)[GShape<KMAX>::G]) {
     const int base = leaf * NBX_LEAF;
     // the LDS copy of the leaf for phase 2: loaded now, stored after phase 1
     // so the vector-load latency overlaps the candidate tests
@@ -153,7 +166,7 @@ __device__ __forceinline__ void mix_leaf(const DevTree &T, const double *ext, in
     float qv[ND];
 #pragma unroll
     for (int d = 0; d < ND; ++d) qv[d] = (float)(c.q[d] - ((cdptr)o)[d]);
-    const float th = c.root;
+    const float th = __uint_as_float(c.root);
     uint32_t mask = 0;
     // candidates in 32- or 16-byte scalar-load chunks per dimension
     constexpr int CH = ND == 6 ? 4 : 8;
@@ -201,7 +214,9 @@ __device__ __forceinline__ void mix_leaf(const DevTree &T, const double *ext, in
             t = qv[d] - cx[d * NBX_LEAF + j];
             dd = __builtin_fmaf(t, t, dd);
         }
-        c.root = gtop_replace<G, S>(hk, sl, lane, H, has && dd < c.root, dd, base + j, c.root);
+        constexpr uint32_t TM = GShape<KMAX>::TM;
+        const uint32_t key = (__float_as_uint(dd) & ~TM) | (c.root & TM);
+        c.root = gtop_replace<KMAX>(hk, sl, lane, gm, has && key < c.root, key, base + j, c.root);
     }
     __syncthreads();
 }
@@ -212,19 +227,20 @@ __device__ __forceinline__ void mix_leaf(const DevTree &T, const double *ext, in
 // nodes in the same order; the compiler schedules them differently, and the
 // measured faster one is used for each dimensionality (3-D: climb to the
 // next unvisited sibling in an inner loop; 6-D: one node step per trip).
-template <int ND, int G, int S>
+template <int ND, int KMAX>
 __device__ __forceinline__ void mix_walk(const DevTree &T, const double *ext, MixCtx<ND> &c, const float *qc, int anc,
-                                         uint32_t *hk, uint32_t *sl, float *cx, int lane, GTop<G> &H) {
+                                         uint32_t *hk, uint32_t *sl, float *cx, int lane,
+                                         uint32_t (&gm)[GShape<KMAX>::G]) {
     const int D = T.depth;
     int node = 1, dep = 0;
     uint64_t firstright = 0;
     if constexpr (ND == 3) {
         for (;;) {
             cdptr bx = (cdptr)node_box(T, node, ND);
-            const bool want = c.act && box_lb<ND>(c.q, bx) <= (double)c.root;
+            const bool want = c.act && box_lb<ND>(c.q, bx) <= (double)__uint_as_float(c.root);
             if (ballot(want)) {
                 if (dep == D) {
-                    mix_leaf<ND, G, S>(T, ext, node - T.nleafp, c, want, hk, sl, cx, lane, H);
+                    mix_leaf<ND, KMAX>(T, ext, node - T.nleafp, c, want, hk, sl, cx, lane, gm);
                 } else {
                     int first;
                     if ((anc >> (D - dep)) == node) first = anc >> (D - dep - 1);
@@ -247,10 +263,10 @@ __device__ __forceinline__ void mix_walk(const DevTree &T, const double *ext, Mi
         for (;;) {
             if (down) {
                 cdptr bx = (cdptr)node_box(T, node, ND);
-                const bool want = c.act && box_lb<ND>(c.q, bx) <= (double)c.root;
+                const bool want = c.act && box_lb<ND>(c.q, bx) <= (double)__uint_as_float(c.root);
                 if (ballot(want)) {
                     if (dep == D) {
-                        mix_leaf<ND, G, S>(T, ext, node - T.nleafp, c, want, hk, sl, cx, lane, H);
+                        mix_leaf<ND, KMAX>(T, ext, node - T.nleafp, c, want, hk, sl, cx, lane, gm);
                         down = false;
                     } else {
                         const int c0 = 2 * node;
@@ -374,53 +390,39 @@ __device__ __forceinline__ void mix_task(const DevTree &T, const KnnArgs &A, con
     for (int d = 0; d < ND; ++d) c.q[d] = valid ? T.x[d][qi] : 0.0;
     c.act = valid;
     c.self = qi;
-    c.root = __uint_as_float(MX_EMPTY);
+    c.root = 0u;
     c.emax = 0.0;
-    constexpr int G = GShape<KMAX>::G, S = GShape<KMAX>::S;
-    GTop<G> H;
-    gtop_init<G, S>(hk, lane, kh, H);
+    uint32_t gm[GShape<KMAX>::G];
+    c.root = gtop_init<KMAX>(hk, lane, kh, gm);
     for (int p = 0; p < kh; ++p) SLA(p) = 0u;  // every slot read later is a valid index
     float qc[ND];
 #pragma unroll
     for (int d = 0; d < ND; ++d) qc[d] = (float)readfirst_d(c.q[d]);
-    mix_walk<ND, G, S>(T, ext, c, qc, T.nleafp + 2 * g, hk, sl, cx, lane, H);
+    mix_walk<ND, KMAX>(T, ext, c, qc, T.nleafp + 2 * g, hk, sl, cx, lane, gm);
 
     // ---------------------------------------------------------------- band
     bool ok = valid;
     if (valid) {
-        // rank sort of the kh kept candidates by (d2, position): d2 bits to
-        // SKC, slots from their insertion positions to SLB
-        if constexpr (KMAX <= 33) {
-            uint64_t kq[KMAX];
+        // rank sort of the kh kept keys (ties, possible only between
+        // groups, by position): keys to SKC, slots from their positions to SLB
+        uint32_t kq[KMAX];
 #pragma unroll
-            for (int p = 0; p < KMAX; ++p) kq[p] = p < kh ? (((uint64_t)HK(p) << 7) | (uint64_t)p) : ~0ULL;
-            for (int i = 0; i < kh; ++i) {
-                const uint32_t hb = HK(i);
-                const uint64_t ki = ((uint64_t)hb << 7) | (uint64_t)i;
-                int r = 0;
+        for (int p = 0; p < KMAX; ++p) kq[p] = p < kh ? HK(p) : 0xffffffffu;
+        for (int i = 0; i < kh; ++i) {
+            const uint32_t ki = HK(i);
+            int r = 0;
 #pragma unroll
-                for (int j = 0; j < KMAX; ++j) r += kq[j] < ki ? 1 : 0;
-                SKC(r) = hb;
-                SLB(r) = SLA(i);
+            for (int j = 0; j < KMAX; ++j) {
+                if constexpr (GShape<KMAX>::GT) r += kq[j] < ki ? 1 : 0;
+                else r += (kq[j] < ki || (kq[j] == ki && j < i)) ? 1 : 0;
             }
-        } else {
-            // half the registers: FP32 d2 with the position tie-break
-            float kf[KMAX];
-#pragma unroll
-            for (int p = 0; p < KMAX; ++p) kf[p] = p < kh ? __uint_as_float(HK(p)) : __builtin_inff();
-            for (int i = 0; i < kh; ++i) {
-                const uint32_t hb = HK(i);
-                const float ki = __uint_as_float(hb);
-                int r = 0;
-#pragma unroll
-                for (int j = 0; j < KMAX; ++j) r += (kf[j] < ki || (kf[j] == ki && j < i)) ? 1 : 0;
-                SKC(r) = hb;
-                SLB(r) = SLA(i);
-            }
+            SKC(r) = ki;
+            SLB(r) = SLA(i);
         }
-        if (SKC(kh - 1) == MX_EMPTY) ok = false;  // fewer than k+1 candidates
+        if ((SKC(kh - 1) & ~GShape<KMAX>::TM) == (MX_EMPTY & ~GShape<KMAX>::TM)) ok = false;  // < k+1 candidates
     }
-    const double eu = 12.0 * MX_U, ea = 5.0 * MX_U * c.emax;
+    // FP32 search error plus the key's position bits
+    const double eu = (12.0 + (double)(1 << GShape<KMAX>::TB)) * MX_U, ea = 5.0 * MX_U * c.emax;
     int i0 = k - 1;
     int32_t bs[4];
     double bd[4];
@@ -488,7 +490,7 @@ __device__ __forceinline__ void mix_task(const DevTree &T, const KnnArgs &A, con
     // exact verification for lanes the error bound could not decide: keep
     // the k smallest (d2, ID) of the k+1 candidates in FP64, then count in
     // FP64 the particles below the k-th of them
-    bool verify = valid && !ok && SKC(kh - 1) != MX_EMPTY;
+    bool verify = valid && !ok && (SKC(kh - 1) & ~GShape<KMAX>::TM) != (MX_EMPTY & ~GShape<KMAX>::TM);
     int32_t IDk = 0;
     if (verify) {
         int zero_at = -1, nzero = 0, maxp = -1;
