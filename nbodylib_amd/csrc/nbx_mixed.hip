@@ -54,7 +54,9 @@ typedef const __attribute__((address_space(4))) fx8 *cf8ptr;
 #define MX_EMPTY 0x7F7FFFFFu  // empty kept entry: the bits of FLT_MAX
 
 #define HK(p) hk[(p) * NBX_WAVE + lane]   // kept keys (LDS)
-#define SLA(p) sl[(p) * NBX_WAVE + lane]  // their slots (scratch, by position)
+#define SLA(p) sl[(p) * NBX_WAVE + lane]  // their slots (scratch, by position; KMAX > 33)
+#define SLH(p) ((uint16_t *)sl)[(p) * NBX_WAVE + lane]  // their handles (scratch, by position; KMAX <= 33)
+#define MX_V 2048  // leaf visits a wave can record: handle = visit << 5 | index in leaf
 
 // Kept keys.  The kh = k+1 kept candidates of a lane sit at positions
 // p < kh of KMAX = G * S (G groups of S positions; positions p >= kh hold
@@ -113,7 +115,8 @@ __device__ __forceinline__ uint32_t gtop_init(uint32_t *hk, int lane, int kh, ui
 
 // Replace the largest kept key (root) by key in the lanes with ins set (key
 // carries the root's position); the others store the root back, so nothing
-// but the slot store branches.  Returns the new largest key.
+// but the slot store branches.  slot is the candidate's handle when
+// KMAX <= 33, else its slot.  Returns the new largest key.
 template <int KMAX>
 __device__ __forceinline__ uint32_t gtop_replace(uint32_t *hk, uint32_t *sl, int lane,
                                                  uint32_t (&gm)[GShape<KMAX>::G], bool ins, uint32_t key,
@@ -132,7 +135,11 @@ __device__ __forceinline__ uint32_t gtop_replace(uint32_t *hk, uint32_t *sl, int
     }
     const int b = g0 * S;
     HK(p) = ins ? key : root;
-    if (ins) SLA(p) = (uint32_t)slot;
+    if constexpr (GShape<KMAX>::GT) {
+        if (ins) SLH(p) = (uint16_t)slot;
+    } else {
+        if (ins) SLA(p) = (uint32_t)slot;
+    }
     uint32_t x[S];
 #pragma unroll
     for (int u = 0; u < S; ++u) x[u] = HK(b + u);
@@ -149,6 +156,7 @@ struct MixCtx {
     int32_t self;
     uint32_t root;  // largest kept key
     double emax;    // largest leaf half-diagonal that matters for this lane
+    int nvis;       // leaves visited (wave-uniform), the next handle's visit number
 };
 
 template <int ND, int KMAX>
@@ -156,6 +164,15 @@ __device__ __forceinline__ void mix_leaf(const DevTree &T, const double *ext, in
                                          uint32_t *hk, uint32_t *sl, float *cx, int lane,
                                          uint32_t (&gm)[GShape<KMAX>::G]) {
     const int base = leaf * NBX_LEAF;
+    // what a kept candidate stores: its slot, or (KMAX <= 33) a 16-bit handle
+    // into the wave's visit table (vt, right after the handles in scratch)
+    int rbase = base;
+    if constexpr (GShape<KMAX>::GT) {
+        uint32_t *vt = sl + KMAX * (NBX_WAVE / 2);
+        if (lane == 0) vt[c.nvis] = (uint32_t)leaf;
+        rbase = c.nvis << 5;
+        c.nvis += 1;
+    }
     // the LDS copy of the leaf for phase 2: loaded now, stored after phase 1
     // so the vector-load latency overlaps the candidate tests
     float st[ND];
@@ -216,7 +233,7 @@ __device__ __forceinline__ void mix_leaf(const DevTree &T, const double *ext, in
         }
         constexpr uint32_t TM = GShape<KMAX>::TM;
         const uint32_t key = (__float_as_uint(dd) & ~TM) | (c.root & TM);
-        c.root = gtop_replace<KMAX>(hk, sl, lane, gm, has && key < c.root, key, base + j, c.root);
+        c.root = gtop_replace<KMAX>(hk, sl, lane, gm, has && key < c.root, key, rbase + j, c.root);
     }
     __syncthreads();
 }
@@ -240,6 +257,7 @@ __device__ __forceinline__ void mix_walk(const DevTree &T, const double *ext, Mi
             const bool want = c.act && box_lb<ND>(c.q, bx) <= (double)__uint_as_float(c.root);
             if (ballot(want)) {
                 if (dep == D) {
+                    if (GShape<KMAX>::GT && c.nvis >= MX_V) break;  // visit table full: lanes fall back
                     mix_leaf<ND, KMAX>(T, ext, node - T.nleafp, c, want, hk, sl, cx, lane, gm);
                 } else {
                     int first;
@@ -266,6 +284,7 @@ __device__ __forceinline__ void mix_walk(const DevTree &T, const double *ext, Mi
                 const bool want = c.act && box_lb<ND>(c.q, bx) <= (double)__uint_as_float(c.root);
                 if (ballot(want)) {
                     if (dep == D) {
+                        if (GShape<KMAX>::GT && c.nvis >= MX_V) break;  // visit table full: lanes fall back
                         mix_leaf<ND, KMAX>(T, ext, node - T.nleafp, c, want, hk, sl, cx, lane, gm);
                         down = false;
                     } else {
@@ -368,8 +387,12 @@ __device__ __forceinline__ int verify_walk(const DevTree &T, const double *q, bo
     return cnt;
 }
 
-#define SLB(p) slb[(p) * NBX_WAVE + lane]  // slots in ascending d2 order (scratch)
-#define SKC(p) skc[(p) * NBX_WAVE + lane]  // the matching d2 bits (scratch)
+// After the sort: kept keys in ascending order (SKC) and their slots (SLOT).
+// When KMAX <= 33 the keys are sorted in place in LDS and a key's tag is
+// its slot's scratch position; else keys and slots are sorted into two more
+// scratch regions.
+#define SKC(p) kb[(p) * NBX_WAVE + lane]
+#define SLB(p) slb[(p) * NBX_WAVE + lane]
 
 // One wave of 64 consecutive queries (tree order), wave index g.
 // k = kh - 1 neighbours; A.dens: NBX_DO_NONE / NBX_DO_SYM / NBX_DO_GATHER.
@@ -377,6 +400,23 @@ template <int ND, int KMAX>
 __device__ __forceinline__ void mix_task(const DevTree &T, const KnnArgs &A, const double *ext, int32_t *fb_list,
                                          int32_t *fb_count, uint32_t *hk, float *cx, double *racc, uint32_t *sl,
                                          uint32_t *slb, uint32_t *skc, int g, int lane) {
+    constexpr bool GT = GShape<KMAX>::GT;
+    constexpr uint32_t TM = GShape<KMAX>::TM;
+    uint32_t *const kb = GT ? hk : skc;
+    // slot of sorted entry p; move sorted entry src to dst
+    uint32_t *const vt = sl + KMAX * (NBX_WAVE / 2);  // visit table (KMAX <= 33)
+    auto slot_at = [&](int p) -> int32_t {
+        if constexpr (GT) {
+            const uint32_t h = SLH(SKC(p) & TM);
+            return (int32_t)(vt[h >> 5] * NBX_LEAF + (h & 31u));
+        } else {
+            return (int32_t)SLB(p);
+        }
+    };
+    auto move_entry = [&](int dst, int src) {
+        if constexpr (GT) SKC(dst) = SKC(src);
+        else SLB(dst) = SLB(src);
+    };
     // k + 1, opaque to the compiler so nothing derived from it is hoisted
     // out of the workgroup's loop into registers
     int kh = A.kh;
@@ -394,30 +434,50 @@ __device__ __forceinline__ void mix_task(const DevTree &T, const KnnArgs &A, con
     c.emax = 0.0;
     uint32_t gm[GShape<KMAX>::G];
     c.root = gtop_init<KMAX>(hk, lane, kh, gm);
-    for (int p = 0; p < kh; ++p) SLA(p) = 0u;  // every slot read later is a valid index
+    // every slot read later is a valid index (handle 0 names the first leaf
+    // visited, which every valid lane visits)
+    for (int p = 0; p < kh; ++p) {
+        if constexpr (GT) SLH(p) = 0;
+        else SLA(p) = 0u;
+    }
+    c.nvis = 0;
     float qc[ND];
 #pragma unroll
     for (int d = 0; d < ND; ++d) qc[d] = (float)readfirst_d(c.q[d]);
     mix_walk<ND, KMAX>(T, ext, c, qc, T.nleafp + 2 * g, hk, sl, cx, lane, gm);
 
     // ---------------------------------------------------------------- band
-    bool ok = valid;
+    const bool ovf = GT && c.nvis >= MX_V;  // walk stopped early: no certificate
+    bool ok = valid && !ovf;
     if (valid) {
-        // rank sort of the kh kept keys (ties, possible only between
-        // groups, by position): keys to SKC, slots from their positions to SLB
-        uint32_t kq[KMAX];
+        // rank sort of the kh kept keys: keys to SKC, slots to SLB
+        if constexpr (GT) {
+            // unique keys, all in registers first: sorted in place in LDS
+            uint32_t kq[KMAX];
 #pragma unroll
-        for (int p = 0; p < KMAX; ++p) kq[p] = p < kh ? HK(p) : 0xffffffffu;
-        for (int i = 0; i < kh; ++i) {
-            const uint32_t ki = HK(i);
-            int r = 0;
+            for (int p = 0; p < KMAX; ++p) kq[p] = p < kh ? HK(p) : 0xffffffffu;
 #pragma unroll
-            for (int j = 0; j < KMAX; ++j) {
-                if constexpr (GShape<KMAX>::GT) r += kq[j] < ki ? 1 : 0;
-                else r += (kq[j] < ki || (kq[j] == ki && j < i)) ? 1 : 0;
+            for (int i = 0; i < KMAX; ++i) {
+                if (i < kh) {
+                    int r = 0;
+#pragma unroll
+                    for (int j = 0; j < KMAX; ++j) r += kq[j] < kq[i] ? 1 : 0;
+                    HK(r) = kq[i];
+                }
             }
-            SKC(r) = ki;
-            SLB(r) = SLA(i);
+        } else {
+            // ties (possible only between groups) by position
+            uint32_t kq[KMAX];
+#pragma unroll
+            for (int p = 0; p < KMAX; ++p) kq[p] = p < kh ? HK(p) : 0xffffffffu;
+            for (int i = 0; i < kh; ++i) {
+                const uint32_t ki = HK(i);
+                int r = 0;
+#pragma unroll
+                for (int j = 0; j < KMAX; ++j) r += (kq[j] < ki || (kq[j] == ki && j < i)) ? 1 : 0;
+                SKC(r) = ki;
+                SLB(r) = SLA(i);
+            }
         }
         if ((SKC(kh - 1) & ~GShape<KMAX>::TM) == (MX_EMPTY & ~GShape<KMAX>::TM)) ok = false;  // < k+1 candidates
     }
@@ -425,6 +485,7 @@ __device__ __forceinline__ void mix_task(const DevTree &T, const KnnArgs &A, con
     const double eu = (12.0 + (double)(1 << GShape<KMAX>::TB)) * MX_U, ea = 5.0 * MX_U * c.emax;
     int i0 = k - 1;
     int32_t bs[4];
+    uint32_t bk[4];
     double bd[4];
     double Dk = 0.0;
     if (ok) {
@@ -451,7 +512,8 @@ __device__ __forceinline__ void mix_task(const DevTree &T, const KnnArgs &A, con
             bs[b] = -1;
             bd[b] = 0.0;
             if (b < nb) {
-                bs[b] = (int32_t)SLB(i0 + b);
+                bk[b] = SKC(i0 + b);
+                bs[b] = slot_at(i0 + b);
                 bd[b] = d64<ND>(T, c.q, bs[b]);
                 if (bd[b] == 0.0) { nzero++; drop = b; }
             }
@@ -484,20 +546,24 @@ __device__ __forceinline__ void mix_task(const DevTree &T, const KnnArgs &A, con
             // the kept band entries fill positions i0 .. k-1 (FP32 order)
 #pragma unroll
             for (int b = 0; b < 4; ++b)
-                if (b < nb && b != drop) SLB(i0 + b - (b > drop ? 1 : 0)) = (uint32_t)bs[b];
+                if (b < nb && b != drop) {
+                    const int dst = i0 + b - (b > drop ? 1 : 0);
+                    if constexpr (GT) SKC(dst) = bk[b];
+                    else SLB(dst) = (uint32_t)bs[b];
+                }
         }
     }
     // exact verification for lanes the error bound could not decide: keep
     // the k smallest (d2, ID) of the k+1 candidates in FP64, then count in
     // FP64 the particles below the k-th of them
-    bool verify = valid && !ok && (SKC(kh - 1) & ~GShape<KMAX>::TM) != (MX_EMPTY & ~GShape<KMAX>::TM);
+    bool verify = valid && !ok && !ovf && (SKC(kh - 1) & ~GShape<KMAX>::TM) != (MX_EMPTY & ~GShape<KMAX>::TM);
     int32_t IDk = 0;
     if (verify) {
         int zero_at = -1, nzero = 0, maxp = -1;
         double dmax = -1.0;
         int32_t imax = -1;
         for (int p = 0; p < kh; ++p) {
-            int32_t s = (int32_t)SLB(p);
+            int32_t s = slot_at(p);
             double d = d64<ND>(T, c.q, s);
             if (d == 0.0) {
                 nzero++;
@@ -510,11 +576,11 @@ __device__ __forceinline__ void mix_task(const DevTree &T, const KnnArgs &A, con
         const int drop = nzero == 1 ? zero_at : maxp;
         if (nzero > 1) verify = false;
         if (verify) {
-            for (int p = drop; p < kh - 1; ++p) SLB(p) = SLB(p + 1);
+            for (int p = drop; p < kh - 1; ++p) move_entry(p, p + 1);
             Dk = 0.0;
             IDk = -1;
             for (int p = 0; p < k; ++p) {
-                int32_t s = (int32_t)SLB(p);
+                int32_t s = slot_at(p);
                 double d = d64<ND>(T, c.q, s);
                 int32_t id = T.perm[s];
                 if (d > Dk || (d == Dk && id > IDk)) { Dk = d; IDk = id; }
@@ -557,7 +623,7 @@ __device__ __forceinline__ void mix_task(const DevTree &T, const KnnArgs &A, con
             double d2v[MX_DB], mv[MX_DB];
 #pragma unroll
             for (int u = 0; u < MX_DB; ++u) {
-                sv[u] = p0 + u < k ? (int32_t)SLB(p0 + u) : qi;
+                sv[u] = p0 + u < k ? slot_at(p0 + u) : qi;
                 d2v[u] = d64<ND>(T, c.q, sv[u]);
                 mv[u] = T.mass[sv[u]];
             }
@@ -594,7 +660,12 @@ __device__ __forceinline__ void mix_task(const DevTree &T, const KnnArgs &A, con
         int qr = g * NBX_WAVE + r;
         if (qr >= T.n) break;
         if (!((okm >> r) & 1ULL)) continue;
-        int32_t s = (int32_t)slb[t * NBX_WAVE + r];
+        int32_t s;
+        if constexpr (GT) {
+            const uint32_t h = ((const uint16_t *)sl)[(hk[t * NBX_WAVE + r] & TM) * NBX_WAVE + r];
+            s = (int32_t)(vt[h >> 5] * NBX_LEAF + (h & 31u));
+        }
+        else s = (int32_t)slb[t * NBX_WAVE + r];
         int64_t row = A.layout == 1 ? qr : T.perm[qr];
         A.nn[row * k + t] = A.layout == 1 ? s : T.perm[s];
     }
@@ -604,11 +675,14 @@ __device__ __forceinline__ void mix_task(const DevTree &T, const KnnArgs &A, con
 // stays on XCD (b mod 8) (workgroups are dealt to XCDs round robin), takes
 // waves of queries from that XCD's contiguous share of the tree order (the
 // xcd_wave split) through the XCD's counter, and owns one scratch block of
-// 3 * KMAX * 64 words (slots by insertion position, slots and d2 bits in
-// ascending order) for its whole life, so the scratch is sized by the
+// MX_BLKW(KMAX) words (16-bit handles by position and the visit table; for
+// KMAX > 33 slots by position and slots and keys in ascending order) for its
+// whole life, so the scratch is sized by the
 // resident workgroups and only ever touched from one XCD.  Every workgroup
 // exits once its counter passes the share's end.
 #define MX_CTR_STRIDE 16  // XCD counters 64 bytes apart
+// scratch words per workgroup: 16-bit handles + visit table, or 3 regions of slots/keys
+#define MX_BLKW(KM) ((KM) <= 33 ? (KM) * (NBX_WAVE / 2) + MX_V : 3 * (KM) * NBX_WAVE)
 template <int ND, int KMAX>
 __global__ __launch_bounds__(64) void k_knn_mixed(DevTree T, KnnArgs A, const double *ext, int32_t *fb_list,
                                                    int32_t *fb_count, uint32_t *scratch, int32_t *ctr) {
@@ -616,7 +690,7 @@ __global__ __launch_bounds__(64) void k_knn_mixed(DevTree T, KnnArgs A, const do
     __shared__ float cx[ND * NBX_LEAF];
     __shared__ double racc[NBX_WAVE];  // symmetric scatter into the wave's own queries
     const int lane = threadIdx.x;
-    uint32_t *sl = scratch + (size_t)blockIdx.x * (3 * KMAX * NBX_WAVE);
+    uint32_t *sl = scratch + (size_t)blockIdx.x * MX_BLKW(KMAX);
     uint32_t *slb = sl + KMAX * NBX_WAVE, *skc = slb + KMAX * NBX_WAVE;
     const int nw = (T.n + NBX_WAVE - 1) / NBX_WAVE;
     const int x = blockIdx.x & 7, q8 = nw >> 3, r8 = nw & 7;
@@ -661,7 +735,7 @@ hipError_t nbx_launch_knn_mixed(const DevTree &T, const KnnArgs &A, const double
     const int km = A.kh <= 33 ? 33 : 65;
     const int grid = T.nd == 3 ? (km == 33 ? mx_grid<3, 33>() : mx_grid<3, 65>())
                                : (km == 33 ? mx_grid<6, 33>() : mx_grid<6, 65>());
-    const size_t nscr = (size_t)grid * 3 * km * NBX_WAVE;
+    const size_t nscr = (size_t)grid * MX_BLKW(km);
     void *mem = nullptr;
     hipError_t e = hipMallocAsync(&mem, nscr * sizeof(uint32_t) + 8 * MX_CTR_STRIDE * sizeof(int32_t), st);
     if (e != hipSuccess) return e;
