@@ -173,8 +173,8 @@ __device__ __forceinline__ void mix_leaf(const DevTree &T, const double *ext, in
         rbase = c.nvis << 5;
         c.nvis += 1;
     }
-    // the LDS copy of the leaf for phase 2: loaded now, stored after phase 1
-    // so the vector-load latency overlaps the candidate tests
+    // candidate (lane mod 32) of this leaf: the tested point in phase 1 and
+    // this lane's part of the LDS copy used in phase 2
     float st[ND];
 #pragma unroll
     for (int d = 0; d < ND; ++d) st[d] = T.xf[d][base + (lane & (NBX_LEAF - 1))];
@@ -184,27 +184,71 @@ __device__ __forceinline__ void mix_leaf(const DevTree &T, const double *ext, in
 #pragma unroll
     for (int d = 0; d < ND; ++d) qv[d] = (float)(c.q[d] - ((cdptr)o)[d]);
     const float th = __uint_as_float(c.root);
-    uint32_t mask = 0;
-    // candidates in 32- or 16-byte scalar-load chunks per dimension
-    constexpr int CH = ND == 6 ? 4 : 8;
-    typedef float fxc __attribute__((ext_vector_type(CH)));
-    typedef const __attribute__((address_space(4))) fxc *cfcptr;
+    uint32_t mask = 0u;
+    if constexpr (ND == 3) {
+        // Phase 1: every lane tests the 32 candidates, read in 32-byte scalar
+        // loads per dimension (cheaper than the transposed form below at 3
+        // flops per dimension)
+        typedef float fxc __attribute__((ext_vector_type(8)));
+        typedef const __attribute__((address_space(4))) fxc *cfcptr;
 #pragma unroll
-    for (int jj = 0; jj < NBX_LEAF; jj += CH) {
-        fxc pv[ND];
+        for (int jj = 0; jj < NBX_LEAF; jj += 8) {
+            fxc pv[ND];
 #pragma unroll
-        for (int d = 0; d < ND; ++d) pv[d] = *(cfcptr)(T.xf[d] + base + jj);
+            for (int d = 0; d < ND; ++d) pv[d] = *(cfcptr)(T.xf[d] + base + jj);
 #pragma unroll
-        for (int u = 0; u < CH; ++u) {
-            float t = qv[0] - pv[0][u];
+            for (int u = 0; u < 8; ++u) {
+                float t = qv[0] - pv[0][u];
+                float dd = t * t;
+#pragma unroll
+                for (int d = 1; d < ND; ++d) {
+                    t = qv[d] - pv[d][u];
+                    dd = __builtin_fmaf(t, t, dd);
+                }
+                mask |= (dd <= th) ? (1u << (jj + u)) : 0u;
+            }
+        }
+    } else {
+        // Phase 1, transposed: the lanes that want this leaf put their query
+        // offsets and thresholds in LDS (qb, tb, which alias cx), and each
+        // step tests two of them against the 32 candidates, one candidate per
+        // lane; the ballot's halves are the two queries' accept masks,
+        // selected into the owning lanes' mask register.  Lanes that do not
+        // want the leaf have no candidate in it: their FP64 bound already
+        // exceeds their largest kept key.
+        const uint64_t wm = ballot(want);
+        const int nwq = __popcll(wm);
+        float *qb = cx;
+        float *tb = cx + ND * NBX_WAVE;
+        if (want) {
+            const int wr = __builtin_amdgcn_mbcnt_hi((uint32_t)(wm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)wm, 0u));
+#pragma unroll
+            for (int d = 0; d < ND; ++d) qb[d * NBX_WAVE + wr] = qv[d];
+            tb[wr] = th;
+        }
+        __syncthreads();
+        uint64_t wrest = wm;  // wanting lanes not yet given a mask, in rank order
+        for (int it = 0; it < nwq; it += 2) {
+            const int qs = it + (lane >> 5);
+            const int qc = qs < nwq ? qs : it;
+            float t = qb[qc] - st[0];
             float dd = t * t;
 #pragma unroll
             for (int d = 1; d < ND; ++d) {
-                t = qv[d] - pv[d][u];
+                t = qb[d * NBX_WAVE + qc] - st[d];
                 dd = __builtin_fmaf(t, t, dd);
             }
-            mask |= (dd <= th) ? (1u << (jj + u)) : 0u;
+            const uint64_t pm = ballot(qs < nwq && dd <= tb[qc]);
+            const int l0 = (int)__builtin_ctzll(wrest);
+            wrest &= wrest - 1;
+            mask = lane == l0 ? (uint32_t)pm : mask;
+            if (it + 1 < nwq) {
+                const int l1 = (int)__builtin_ctzll(wrest);
+                wrest &= wrest - 1;
+                mask = lane == l1 ? (uint32_t)(pm >> 32) : mask;
+            }
         }
+        __syncthreads();
     }
     if (c.self >= base && c.self < base + NBX_LEAF) mask &= ~(1u << (c.self - base));
     if (!c.act) mask = 0;
@@ -687,8 +731,14 @@ template <int ND, int KMAX>
 __global__ __launch_bounds__(64) void k_knn_mixed(DevTree T, KnnArgs A, const double *ext, int32_t *fb_list,
                                                    int32_t *fb_count, uint32_t *scratch, int32_t *ctr) {
     __shared__ uint32_t hk[KMAX * NBX_WAVE];
-    __shared__ float cx[ND * NBX_LEAF];
-    __shared__ double racc[NBX_WAVE];  // symmetric scatter into the wave's own queries
+    // one buffer for three uses at different times: the leaf copy (phase 2),
+    // the compacted queries and masks (phase 1) and the symmetric scatter into
+    // the wave's own queries (density)
+    constexpr int UB1 = ND == 3 ? 0 : ND * NBX_WAVE + NBX_WAVE;  // transposed phase 1 (6-D only)
+    constexpr int UBN = UB1 > ND * NBX_LEAF + 2 * NBX_WAVE ? UB1 : ND * NBX_LEAF + 2 * NBX_WAVE;
+    __shared__ __attribute__((aligned(8))) float ub[UBN];
+    float *cx = ub;
+    double *racc = (double *)(ub + ND * NBX_LEAF);
     const int lane = threadIdx.x;
     uint32_t *sl = scratch + (size_t)blockIdx.x * MX_BLKW(KMAX);
     uint32_t *slb = sl + KMAX * NBX_WAVE, *skc = slb + KMAX * NBX_WAVE;
