@@ -48,39 +48,58 @@ __global__ __launch_bounds__(256) void k_fof_nodeinit(DevTree T, double b2, int3
     if (br[0] <= br[3]) uf_union(P, node_first(T, node), node_first(T, right));
 }
 
-// link test from the (possibly image-shifted) position q to candidate slot cs
+// Links from the (possibly image-shifted) position q to the particles of one
+// leaf.  Candidates arrive as 64-byte scalar-load chunks (padding slots are
+// 1e300 away and never link).  A leaf that is internally connected needs one
+// union with any particle it links to; otherwise every link is a union (in
+// the first pass only toward larger slots: the pair is met from both sides).
+// rq is a hint for the root of qi (any former root of its set), kept across
+// unions so each find starts near the top.
 template <int PASS0>
 __device__ __forceinline__ void fof_leaf(const DevTree &T, int leaf, const double *q, bool act, int32_t qi,
-                                         double b2, int32_t *P, const uint8_t *conn) {
+                                         double b2, int32_t *P, const uint8_t *conn, int32_t &rq) {
     const int base = leaf * NBX_LEAF;
-    int cnt = T.n - base;
-    cnt = cnt < NBX_LEAF ? cnt : NBX_LEAF;
+    const bool lconn = conn[T.nleafp + leaf];
     cdptr bx = (cdptr)node_box(T, T.nleafp + leaf, 3);
-    bool encl = act && conn[T.nleafp + leaf] && box_ub<3>(q, bx) < b2;
-    if (encl) uf_union(P, qi, base);
-    bool pair = act && !encl;
+    const bool encl = act && lconn && box_ub<3>(q, bx) < b2;
+    if (encl) rq = uf_union_r(P, rq, base);
+    const bool pair = act && !encl;
     if (!ballot(pair)) return;
     uint32_t mask = 0;
-    for (int j = 0; j < cnt; ++j) {
-        double d2 = 0.0;
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            double t = q[d] - ((cdptr)T.x[d])[base + j];
+    for (int jj = 0; jj < NBX_LEAF; jj += 8) {
+        const dx8 cx = *(cd8ptr)(T.x[0] + base + jj);
+        const dx8 cy = *(cd8ptr)(T.x[1] + base + jj);
+        const dx8 cz = *(cd8ptr)(T.x[2] + base + jj);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            double d2 = 0.0;
+            double t = q[0] - cx[u];
             d2 += t * t;
+            t = q[1] - cy[u];
+            d2 += t * t;
+            t = q[2] - cz[u];
+            d2 += t * t;
+            const int32_t cs = base + jj + u;
+            const bool l = d2 < b2 && (PASS0 ? (lconn ? cs != qi : cs > qi) : cs != qi);
+            mask |= l ? (1u << (jj + u)) : 0u;
         }
-        bool l = pair && d2 < b2 && (PASS0 ? (base + j > qi) : (base + j != qi));
-        mask |= l ? (1u << j) : 0u;
+    }
+    if (!pair) mask = 0;
+    if (lconn) {
+        if (mask) rq = uf_union_r(P, rq, base + __builtin_ctz(mask));
+        return;
     }
     while (mask) {
-        int j = __builtin_ctz(mask);
+        const int j = __builtin_ctz(mask);
         mask &= mask - 1;
-        uf_union(P, qi, base + j);
+        rq = uf_union_r(P, rq, base + j);
     }
 }
 
 template <int PASS0>
 __device__ __forceinline__ void fof_walk(const DevTree &T, const double *q, bool act, int32_t qi, int g, double b2,
-                                         int32_t *P, uint8_t *conn) {
+                                         int32_t *P, uint8_t *conn, int32_t &rq) {
     const int D = T.depth;
     // the wave's own 64 particles form the parent of leaves 2g, 2g+1 (or the
     // single leaf when the tree has one)
@@ -97,7 +116,7 @@ __device__ __forceinline__ void fof_walk(const DevTree &T, const double *q, bool
                     // every wave particle links to every particle of the node:
                     // wave and node form one component
                     const int32_t nfirst = node_first(T, node);
-                    if (act) uf_union(P, qi, nfirst);
+                    if (act) rq = uf_union_r(P, rq, nfirst);
                     if (!conn[node]) {
                         const int32_t nlast = node_end(T, node);
                         for (int32_t s = nfirst + 1 + threadIdx.x; s < nlast; s += NBX_WAVE) uf_union(P, s, nfirst);
@@ -105,7 +124,7 @@ __device__ __forceinline__ void fof_walk(const DevTree &T, const double *q, bool
                     }
                     down = false;
                 } else if (node >= T.nleafp) {
-                    fof_leaf<PASS0>(T, node - T.nleafp, q, act, qi, b2, P, conn);
+                    fof_leaf<PASS0>(T, node - T.nleafp, q, act, qi, b2, P, conn, rq);
                     down = false;
                 } else {
                     node = 2 * node;
@@ -133,7 +152,8 @@ __global__ __launch_bounds__(64) void k_fof(DevTree T, FofArgs A, uint8_t *conn)
     double q0[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) q0[d] = valid ? T.x[d][qi] : 0.0;
-    fof_walk<1>(T, q0, valid, qi, g, A.b2, A.parent, conn);
+    int32_t rq = valid ? qi : 0;
+    fof_walk<1>(T, q0, valid, qi, g, A.b2, A.parent, conn, rq);
     if (!A.periodic) return;
     // FOFSearchBallPeriodic: image toward the far wall when b^2 > s^2
     const double *per = A.period;
@@ -156,17 +176,36 @@ __global__ __launch_bounds__(64) void k_fof(DevTree T, FofArgs A, uint8_t *conn)
         }
         bool act = valid && A.b2 > sval * sval;
         if (!ballot(act)) continue;
-        fof_walk<0>(T, xp, act, qi, g, A.b2, A.parent, conn);
+        fof_walk<0>(T, xp, act, qi, g, A.b2, A.parent, conn, rq);
     }
 }
 
+// Group sizes and smallest member IDs.  Neighbouring slots mostly share a
+// root, so the lanes whose root equals the first active lane's are summed
+// in the wave and counted with one atomic pair; the rest count alone.
 __global__ __launch_bounds__(256) void k_fof_compress(int32_t *P, int64_t n, const int32_t *perm, int32_t *cnt,
                                                         int32_t *minid) {
     int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    int32_t r = uf_find(P, (int32_t)i);
-    atomicAdd(cnt + r, 1);
-    atomicMin(minid + r, perm[i]);
+    const bool in = i < n;
+    const int32_t r = in ? uf_find(P, (int32_t)i) : -1;
+    const int32_t id = in ? perm[i] : 0x7fffffff;
+    const uint64_t act = ballot(in);
+    if (!act) return;
+    const int lead = __builtin_ctzll(act);
+    const int32_t r0 = __shfl(r, lead);
+    const bool same = in && r == r0;
+    const uint64_t m = ballot(same);
+    int32_t mn = same ? id : 0x7fffffff;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) mn = min(mn, __shfl_xor(mn, off));
+    if ((int)(threadIdx.x & 63) == lead) {
+        atomicAdd(cnt + r0, (int32_t)__popcll(m));
+        atomicMin(minid + r0, mn);
+    }
+    if (in && !same) {
+        atomicAdd(cnt + r, 1);
+        atomicMin(minid + r, id);
+    }
 }
 __global__ __launch_bounds__(256) void k_fof_store_root(int32_t *P, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;

@@ -191,3 +191,18 @@ __device__ __forceinline__ void uf_union(int32_t *P, int32_t a, int32_t b) {
             return;
     }
 }
+
+// Union of the sets of a and b; returns the root the joined set had when
+// the link was made (a valid starting point for later finds from a).
+__device__ __forceinline__ int32_t uf_union_r(int32_t *P, int32_t a, int32_t b) {
+    for (;;) {
+        a = uf_find(P, a);
+        b = uf_find(P, b);
+        if (a == b) return a;
+        if (a < b) { int32_t t = a; a = b; b = t; }
+        int32_t expected = a;
+        if (__hip_atomic_compare_exchange_strong(P + a, &expected, b, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT))
+            return b;
+    }
+}
