@@ -238,13 +238,23 @@ static int build_tree(nbx_tree *t, const double *pos, const double *vel, const d
     HIPCK(hipMalloc((void **)&t->mass, sizeof(double) * npad));
     HIPCK(hipMalloc((void **)&t->perm, sizeof(int32_t) * npad));
     HIPCK(hipMalloc((void **)&t->inv, sizeof(int32_t) * n));
+    HIPCK(hipMalloc((void **)&t->box, sizeof(double) * 2 * nd * (size_t)t->nnodes));
+    HIPCK(hipMalloc((void **)&t->plane, sizeof(float) * (nd == 6 ? 8 : 4) * (size_t)t->nleafp));
+    if (t->p.form == NBX_FORM_MIXED) {
+        HIPCK(hipMalloc((void **)&t->org, sizeof(double) * nd * (size_t)t->nleaf));
+        HIPCK(hipMalloc((void **)&t->ext, sizeof(double) * (size_t)t->nleaf));
+        for (int d = 0; d < nd; ++d) HIPCK(hipMalloc((void **)&t->xf[d], sizeof(float) * npad));
+    }
     {
         ProfScope ps("k_build_gather", st);
         GatherArgs g;
         memset(&g, 0, sizeof(g));
         g.n = n;
         g.npad = npad;
-        g.nd = ncoord;
+        g.ncoord = ncoord;
+        g.nd = nd;
+        g.nleaf = t->nleaf;
+        g.nleafp = t->nleafp;
         g.order = order;
         for (int d = 0; d < ncoord; ++d) { g.in[d] = c[d]; g.out[d] = t->xs[d]; }
         g.mass_in = mass;
@@ -252,43 +262,21 @@ static int build_tree(nbx_tree *t, const double *pos, const double *vel, const d
         g.mass_out = t->mass;
         g.perm = t->perm;
         g.inv = t->inv;
-        hipLaunchKernelGGL(k_gather, dim3(nblk(npad)), dim3(256), 0, st, g);
+        g.box = t->box;
+        g.org = t->org;
+        g.ext = t->ext;
+        for (int d = 0; d < nd; ++d) g.xf[d] = t->xf[d];
+        hipLaunchKernelGGL(k_gather_leaf, dim3(nblk(npad)), dim3(256), 0, st, g);
     }
     HIPCK(hipFreeAsync(order, st));
-
-    HIPCK(hipMalloc((void **)&t->box, sizeof(double) * 2 * nd * (size_t)t->nnodes));
-    HIPCK(hipMalloc((void **)&t->plane, sizeof(float) * (nd == 6 ? 8 : 4) * (size_t)t->nleafp));
-    if (t->p.form == NBX_FORM_MIXED) {
-        HIPCK(hipMalloc((void **)&t->org, sizeof(double) * nd * (size_t)t->nleaf));
-        HIPCK(hipMalloc((void **)&t->ext, sizeof(double) * (size_t)t->nleaf));
-    }
     {
         ProfScope ps("k_build_boxes", st);
-        LeafBoxArgs a;
-        memset(&a, 0, sizeof(a));
-        a.n = n;
-        a.nleaf = t->nleaf;
-        a.nleafp = t->nleafp;
-        a.nd = nd;
-        for (int d = 0; d < nd; ++d) a.c[d] = t->xs[d];
-        a.box = t->box;
-        a.org = t->org;
-        a.ext = t->ext;
-        hipLaunchKernelGGL(k_leafbox, dim3(nblk(t->nleafp)), dim3(256), 0, st, a);
+        if (t->nleafp > t->nleaf)
+            hipLaunchKernelGGL(k_leafbox_pad, dim3(nblk(t->nleafp - t->nleaf)), dim3(256), 0, st, t->box, t->nleaf,
+                               t->nleafp, nd);
         for (int dep = t->depth - 1; dep >= 0; --dep)
             hipLaunchKernelGGL(k_levelbox, dim3(nblk((int64_t)1 << dep)), dim3(256), 0, st, t->box, t->plane,
                                1 << dep, 1 << dep, nd);
-    }
-    if (t->p.form == NBX_FORM_MIXED) {
-        for (int d = 0; d < nd; ++d) HIPCK(hipMalloc((void **)&t->xf[d], sizeof(float) * npad));
-        MixedArgs m;
-        memset(&m, 0, sizeof(m));
-        m.n = n;
-        m.npad = npad;
-        m.nd = nd;
-        for (int d = 0; d < nd; ++d) { m.c[d] = t->xs[d]; m.xf[d] = t->xf[d]; }
-        m.org = t->org;
-        hipLaunchKernelGGL(k_mixed_offsets, dim3(nblk(npad)), dim3(256), 0, st, m);
     }
     HIPCK(hipGetLastError());
     // kernel table

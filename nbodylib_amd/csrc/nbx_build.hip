@@ -2,9 +2,10 @@
 //
 // Replaces KDTree::BuildNodes (KDTree.cxx:970-1055): instead of a recursive
 // median split that swaps 88-byte particles in place, the split order is
-// computed on index lists (nbx_kdbuild.hip), particles are gathered into SoA
-// arrays in that order, cut into fixed buckets of NBX_LEAF slots and boxed
-// bottom-up.  All passes here are streaming and coalesced (HBM-bound).
+// computed by physical partitions (nbx_kdbuild.hip), particles are gathered
+// into SoA arrays in that order, cut into fixed buckets of NBX_LEAF slots and
+// boxed bottom-up.  All passes here are streaming and coalesced (HBM-bound)
+// apart from the gather's read of the input by ID.
 #include <cstring>
 #include <rocprim/rocprim.hpp>
 
@@ -37,56 +38,73 @@ __global__ __launch_bounds__(256) void k_bbox(const double *__restrict__ c, int6
 }
 
 // ---------------------------------------------------------------- gather
-__global__ __launch_bounds__(256) void k_gather(GatherArgs a) {
-    int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (s >= a.npad) return;
-    if (s >= a.n) {  // padding up to a whole leaf
-        for (int d = 0; d < a.nd; ++d) a.out[d][s] = NBX_PAD_COORD;
+// One thread per (padded) slot, gathering its particle into tree order.  A
+// half-wave is exactly one leaf of NBX_LEAF slots, so the leaf's tight
+// FP64 box is a 32-lane min/max reduction over the values just gathered
+// (padding slots count as +inf / -inf), and for the precision form the
+// leaf origin (box centre), half-diagonal and FP32 offsets follow from it
+// without reading the coordinates again.
+__global__ __launch_bounds__(256) void k_gather_leaf(GatherArgs a) {
+    const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const bool in = s < a.npad;
+    const bool real = s < a.n;
+    const int lane = threadIdx.x & 63;
+    double v[6];
+    if (real) {
+        const int32_t id = a.order[s];
+        for (int d = 0; d < a.ncoord; ++d) v[d] = a.in[d][id];
+        a.mass_out[s] = a.mass_in ? a.mass_in[id] : a.mass_default;
+        a.perm[s] = id;
+        if (a.inv) a.inv[id] = (int32_t)s;
+    } else if (in) {  // padding up to a whole leaf
+        for (int d = 0; d < a.ncoord; ++d) v[d] = NBX_PAD_COORD;
         a.mass_out[s] = 0.0;
         a.perm[s] = -1;
-        return;
     }
-    int32_t id = a.order[s];
-    for (int d = 0; d < a.nd; ++d) a.out[d][s] = a.in[d][id];
-    a.mass_out[s] = a.mass_in ? a.mass_in[id] : a.mass_default;
-    a.perm[s] = id;
-    if (a.inv) a.inv[id] = (int32_t)s;
-}
-
-// ---------------------------------------------------------------- boxes
-// One thread per (padded) leaf: tight min/max over its NBX_LEAF slots;
-// padding leaves get the empty box (+inf, -inf).
-__global__ __launch_bounds__(256) void k_leafbox(LeafBoxArgs a) {
-    int32_t L = blockIdx.x * 256 + threadIdx.x;
-    if (L >= a.nleafp) return;
-    double *b = a.box + (size_t)(a.nleafp + L) * 2 * a.nd;
-    if (L >= a.nleaf) {
-        for (int d = 0; d < a.nd; ++d) { b[d] = 1.0 / 0.0; b[a.nd + d] = -1.0 / 0.0; }
-        return;
-    }
-    int64_t s0 = (int64_t)L * NBX_LEAF, s1 = s0 + NBX_LEAF;
-    if (s1 > a.n) s1 = a.n;
+    if (in)
+        for (int d = 0; d < a.ncoord; ++d) a.out[d][s] = v[d];
+    const int64_t L = s / NBX_LEAF;
+    double mn[6], mx[6];
     for (int d = 0; d < a.nd; ++d) {
-        const double *c = a.c[d];
-        double mn = c[s0], mx = mn;
-        for (int64_t s = s0 + 1; s < s1; ++s) {
-            double v = c[s];
-            mn = v < mn ? v : mn;
-            mx = v > mx ? v : mx;
+        mn[d] = real ? v[d] : 1.0 / 0.0;
+        mx[d] = real ? v[d] : -1.0 / 0.0;
+#pragma unroll
+        for (int off = 16; off > 0; off >>= 1) {
+            mn[d] = fmin(mn[d], __shfl_xor(mn[d], off));
+            mx[d] = fmax(mx[d], __shfl_xor(mx[d], off));
         }
-        b[d] = mn;
-        b[a.nd + d] = mx;
-        if (a.org) a.org[(size_t)L * a.nd + d] = 0.5 * (mn + mx);
     }
-    if (a.ext) {
-        // half-diagonal from the centre, rounded up
+    if (!in || L >= a.nleaf) return;
+    if ((lane & 31) == 0) {
+        double *b = a.box + (size_t)(a.nleafp + L) * 2 * a.nd;
         double e2 = 0.0;
         for (int d = 0; d < a.nd; ++d) {
-            double o = 0.5 * (b[d] + b[a.nd + d]);
-            double w = fmax(o - b[d], b[a.nd + d] - o);
+            b[d] = mn[d];
+            b[a.nd + d] = mx[d];
+            if (a.org) a.org[(size_t)L * a.nd + d] = 0.5 * (mn[d] + mx[d]);
+            // half-diagonal from the centre, rounded up
+            const double o = 0.5 * (mn[d] + mx[d]);
+            const double w = fmax(o - mn[d], mx[d] - o);
             e2 += w * w;
         }
-        a.ext[L] = sqrt(e2) * (1.0 + 1e-12);
+        if (a.ext) a.ext[L] = sqrt(e2) * (1.0 + 1e-12);
+    }
+    if (a.org) {
+        for (int d = 0; d < a.nd; ++d) {
+            const double o = 0.5 * (mn[d] + mx[d]);  // the same in every lane of the leaf
+            a.xf[d][s] = real ? (float)(v[d] - o) : NBX_PAD_COORD_F;
+        }
+    }
+}
+
+// Padding leaves [nleaf, nleafp) hold the empty box (+inf, -inf).
+__global__ __launch_bounds__(256) void k_leafbox_pad(double *box, int32_t nleaf, int32_t nleafp, int nd) {
+    const int32_t L = nleaf + (int32_t)(blockIdx.x * 256 + threadIdx.x);
+    if (L >= nleafp) return;
+    double *b = box + (size_t)(nleafp + L) * 2 * nd;
+    for (int d = 0; d < nd; ++d) {
+        b[d] = 1.0 / 0.0;
+        b[nd + d] = -1.0 / 0.0;
     }
 }
 
@@ -118,15 +136,6 @@ __global__ __launch_bounds__(256) void k_levelbox(double *__restrict__ box, floa
     }
     pl[nd] = empty ? 1.0f : (float)off;
     for (int d = nd + 1; d < plw; ++d) pl[d] = 0.0f;
-}
-
-// Mixed form: FP32 offsets of each particle from its leaf's FP64 origin.
-__global__ __launch_bounds__(256) void k_mixed_offsets(MixedArgs a) {
-    int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (s >= a.npad) return;
-    int64_t L = s / NBX_LEAF;
-    for (int d = 0; d < a.nd; ++d)
-        a.xf[d][s] = s < a.n ? (float)(a.c[d][s] - a.org[L * a.nd + d]) : NBX_PAD_COORD_F;
 }
 
 // ---------------------------------------------------------------- sort

@@ -163,34 +163,70 @@ __global__ __launch_bounds__(KD_T) void k_kd_hist(KdState S, int64_t n, int lseg
     const uint32_t *K = S.key[dsel[s]];
     const uint32_t p = pre[s];
     const uint32_t hm = sh >= 24 ? 0u : (0xffffffffu << (sh + 8));  // higher digits already fixed
+    const int lane = t & 63;
 #pragma unroll
     for (int q = 0; q < KD_PER; ++q) {
         const int64_t i = i0 + q * KD_T + t;
-        if (i < n) {
-            const uint32_t k = K[i];
-            if ((k & hm) == (p & hm)) atomicAdd(&h[(k >> sh) & 255u], 1u);
+        uint32_t k = 0;
+        if (i < n) k = K[i];
+        const bool in = i < n && (k & hm) == (p & hm);
+        const uint32_t bin = (k >> sh) & 255u;
+        // the high digits of a node's keys are mostly one bin: a wave whose
+        // counted keys all share it adds them with one LDS atomic
+        const uint64_t m = ballot(in);
+        if (!m) continue;
+        const uint32_t b0 = (uint32_t)__shfl((int)bin, __builtin_ctzll(m));
+        if (!ballot(in && bin != b0)) {
+            if (lane == 0) atomicAdd(&h[b0], (uint32_t)__popcll(m));
+        } else if (in) {
+            atomicAdd(&h[bin], 1u);
         }
     }
     __syncthreads();
     if (h[t]) atomicAdd(hist + s * 256 + t, h[t]);
 }
 
-// Radix select step: the digit bin holding the node's remaining rank.
+// Radix select step: the digit bin holding the node's remaining rank.  One
+// wave per node, four bins per lane and a wave prefix sum; the node's bins
+// are cleared for the next pass.
 __global__ __launch_bounds__(KD_T) void k_kd_select(int64_t n, int lseg, int64_t nseg, int sh,
-                                                    const uint32_t *__restrict__ hist, uint32_t *__restrict__ pre,
+                                                    uint32_t *__restrict__ hist, uint32_t *__restrict__ pre,
                                                     int32_t *__restrict__ need) {
-    int64_t s = (int64_t)blockIdx.x * KD_T + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int64_t s = (int64_t)blockIdx.x * (KD_T / 64) + (threadIdx.x >> 6);
     if (s >= nseg || (s << lseg) >= n) return;
-    int32_t r = need[s];
-    const uint32_t *h = hist + s * 256;
-    int b = 0;
-    for (; b < 255; ++b) {
-        const int32_t c = (int32_t)h[b];
-        if (r <= c) break;
-        r -= c;
+    uint32_t *h = hist + s * 256;
+    const uint4 v = *(const uint4 *)(h + 4 * lane);
+    const uint32_t c4 = v.x + v.y + v.z + v.w;
+    uint32_t inc = c4;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)inc, off);
+        if (lane >= off) inc += y;
     }
-    pre[s] |= (uint32_t)b << sh;
-    need[s] = r;
+    const uint32_t exc = inc - c4;
+    const uint32_t r = (uint32_t)need[s];
+    const uint64_t bm = ballot(exc < r && r <= inc);
+    const int L = bm ? __builtin_ctzll(bm) : 63;
+    if (lane == L) {
+        uint32_t rr = r - exc;
+        int b = 4 * lane;
+        if (rr > v.x) {
+            rr -= v.x;
+            ++b;
+            if (rr > v.y) {
+                rr -= v.y;
+                ++b;
+                if (rr > v.z) {
+                    rr -= v.z;
+                    ++b;
+                }
+            }
+        }
+        pre[s] |= (uint32_t)b << sh;
+        need[s] = (int32_t)rr;
+    }
+    *(uint4 *)(h + 4 * lane) = make_uint4(0u, 0u, 0u, 0u);
 }
 
 // Per-block (below, equal) counts of the split keys against the node's
@@ -544,6 +580,7 @@ hipError_t nbx_kd_order(const double *const *c, int nd, int64_t n, int32_t nleaf
         KDCK(hipMallocAsync((void **)&need, sizeof(int32_t) * nsegmax, st));
         KDCK(hipMallocAsync((void **)&dsel, (size_t)nsegmax, st));
         KDCK(hipMallocAsync((void **)&hist, sizeof(uint32_t) * 256 * nsegmax, st));
+        hipLaunchKernelGGL(k_kd_zero, dim3(nblk(256 * nsegmax)), dim3(KD_T), 0, st, hist, (int64_t)256 * nsegmax);
         KDCK(hipMallocAsync((void **)&blk, sizeof(uint64_t) * nb, st));
         KDCK(hipMallocAsync((void **)&bpre, sizeof(uint64_t) * nb, st));
         // root extents; deeper levels get theirs from the previous move
@@ -561,10 +598,10 @@ hipError_t nbx_kd_order(const double *const *c, int nd, int64_t n, int32_t nleaf
                 hipLaunchKernelGGL(k_kd_ext_init, dim3(nblk(2 * nd * nseg)), dim3(KD_T), 0, st, ext,
                                    (int64_t)2 * nd * nseg);
             for (int sh = 24; sh >= 0; sh -= 8) {
-                hipLaunchKernelGGL(k_kd_zero, dim3(nblk(256 * nseg)), dim3(KD_T), 0, st, hist, (int64_t)256 * nseg);
+                // the select step leaves every bin it read at zero
                 hipLaunchKernelGGL(k_kd_hist, dim3(nb), dim3(KD_T), 0, st, Sh, n, lseg, sh, dsel, pre, hist);
-                hipLaunchKernelGGL(k_kd_select, dim3(nblk(nseg)), dim3(KD_T), 0, st, n, lseg, nseg, sh, hist, pre,
-                                   need);
+                hipLaunchKernelGGL(k_kd_select, dim3(nblk(nseg, KD_T / 64)), dim3(KD_T), 0, st, n, lseg, nseg, sh,
+                                   hist, pre, need);
             }
             hipLaunchKernelGGL(k_kd_count, dim3(nb), dim3(KD_T), 0, st, Sh, n, lseg, dsel, pre, blk);
             hipLaunchKernelGGL(k_kd_bscan, dim3(1), dim3(KD_ST), 0, st, blk, (int64_t)nb, bpre);

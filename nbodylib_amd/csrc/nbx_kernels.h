@@ -6,9 +6,13 @@
 
 #include "nbx_internal.h"
 
+// Gather into tree order fused with the leaf boxes and, for the precision
+// form, the leaf origins, half-diagonals and FP32 offsets.
 struct GatherArgs {
     int64_t n, npad;
-    int nd;
+    int ncoord;  // coordinates gathered (3, or 6 with velocities)
+    int nd;      // tree dimensions boxed
+    int32_t nleaf, nleafp;
     const int32_t *order;
     const double *in[6];
     double *out[6];
@@ -16,29 +20,16 @@ struct GatherArgs {
     double mass_default;
     double *mass_out;
     int32_t *perm, *inv;
-};
-struct LeafBoxArgs {
-    int64_t n;
-    int32_t nleaf, nleafp;
-    int nd;
-    const double *c[6];
     double *box;
-    double *org;  // nullable: leaf centre (mixed-form origin)
-    double *ext;  // nullable: leaf half-diagonal (mixed-form error bound)
-};
-struct MixedArgs {
-    int64_t n, npad;
-    int nd;
-    const double *c[6];
-    const double *org;
-    float *xf[6];
+    double *org;  // nullable: leaf centre (precision-form origin)
+    double *ext;  // nullable: leaf half-diagonal (precision-form error bound)
+    float *xf[6];  // precision form: FP32 offsets from the leaf origin
 };
 
 __global__ void k_bbox(const double *c, int64_t n, double *part);
-__global__ void k_gather(GatherArgs a);
-__global__ void k_leafbox(LeafBoxArgs a);
+__global__ void k_gather_leaf(GatherArgs a);
+__global__ void k_leafbox_pad(double *box, int32_t nleaf, int32_t nleafp, int nd);
 __global__ void k_levelbox(double *box, float *plane, int32_t first, int32_t count, int nd);
-__global__ void k_mixed_offsets(MixedArgs a);
 
 // order (slot -> ID) of the implicit kd tree: midpoint splits along the
 // widest dimension of every node (nbx_kdbuild.hip)
