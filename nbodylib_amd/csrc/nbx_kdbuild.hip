@@ -320,6 +320,7 @@ __global__ __launch_bounds__(KD_T) void k_kd_move(KdState S, int64_t n, int lseg
                                                   const uint64_t *__restrict__ bpre, uint32_t *__restrict__ ext) {
     __shared__ uint32_t ws[KD_T / 64];
     __shared__ uint32_t red[KD_T / 64][4 * ND];
+    __shared__ uint32_t stg[(1 + ND) * KD_LOCAL];  // the block's elements in destination order
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int64_t i0 = (int64_t)blockIdx.x * KD_LOCAL;
     const int64_t s = i0 >> lseg;
@@ -354,9 +355,19 @@ __global__ __launch_bounds__(KD_T) void k_kd_move(KdState S, int64_t n, int lseg
     }
     if (lane == 63) ws[w] = incl;
     __syncthreads();
-    uint32_t run = incl - tsum;
-    for (int v = 0; v < w; ++v) run += ws[v];
+    uint32_t run = incl - tsum, tot = 0;
+    for (int v = 0; v < KD_T / 64; ++v) {
+        run += v < w ? ws[v] : 0u;
+        tot += ws[v];
+    }
     const int64_t lb0 = (int64_t)(base >> 32) + (run >> 16), eb0 = (int64_t)(base & 0xffffffffu) + (run & 0xffffu);
+    // left elements of the node before this block and through its end: the
+    // block's left elements go to [a + lrb, a + lre), its right ones follow
+    // the node's earlier right elements from m
+    const int64_t bb = (int64_t)(base >> 32), be = (int64_t)(base & 0xffffffffu);
+    const int64_t lrb = bb + (be < ne ? be : ne);
+    const int64_t lre = bb + (tot >> 16) + (be + (tot & 0xffffu) < ne ? be + (tot & 0xffffu) : ne);
+    const int nl = (int)(lre - lrb);
     uint32_t mn[2][ND], mx[2][ND];
 #pragma unroll
     for (int d = 0; d < ND; ++d) mn[0][d] = mn[1][d] = 0xffffffffu, mx[0][d] = mx[1][d] = 0u;
@@ -370,11 +381,12 @@ __global__ __launch_bounds__(KD_T) void k_kd_move(KdState S, int64_t n, int lseg
             const int64_t lb = lb0 + (fl[q] >> 16), eb = eb0 + (fl[q] & 0xffffu);
             const bool left = k < p || (k == p && eb < ne);
             const int64_t lr = lb + (eb < ne ? eb : ne);
-            const int64_t dst = left ? a + lr : m + (i + q - a - lr);
-            S.id2[dst] = (int32_t)id[q];
+            const int j = (int)(i + q - i0), jl = (int)(lr - lrb);
+            const int sp = left ? jl : nl + (j - jl);
+            stg[sp] = id[q];
 #pragma unroll
             for (int d = 0; d < ND; ++d) {
-                S.key2[d][dst] = kk[d][q];
+                stg[(1 + d) * KD_LOCAL + sp] = kk[d][q];
                 const uint32_t lo = min(mn[0][d], kk[d][q]), hi = max(mx[0][d], kk[d][q]);
                 const uint32_t lo1 = min(mn[1][d], kk[d][q]), hi1 = max(mx[1][d], kk[d][q]);
                 mn[0][d] = left ? lo : mn[0][d];
@@ -383,6 +395,16 @@ __global__ __launch_bounds__(KD_T) void k_kd_move(KdState S, int64_t n, int lseg
                 mx[1][d] = left ? mx[1][d] : hi1;
             }
         }
+    }
+    __syncthreads();
+    // coalesced write-out of the two runs
+    const int cnt = (int)min((int64_t)KD_LOCAL, n - i0);
+    const int64_t rb = m + (i0 - a - lrb);
+    for (int j = t; j < cnt; j += KD_T) {
+        const int64_t dst = j < nl ? a + lrb + j : rb + (j - nl);
+        S.id2[dst] = (int32_t)stg[j];
+#pragma unroll
+        for (int d = 0; d < ND; ++d) S.key2[d][dst] = stg[(1 + d) * KD_LOCAL + j];
     }
     if (ext == nullptr) return;
 #pragma unroll
