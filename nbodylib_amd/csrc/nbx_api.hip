@@ -605,7 +605,6 @@ extern "C" int nbx_density(nbx_tree *t, int32_t k, int32_t variant, int32_t nsea
 extern "C" int nbx_fof_dev(nbx_tree *t, double fdist, int32_t minnum, int32_t *labels_by_id_dev, int64_t *ngroups,
                            void *stream) {
     if (!t || !labels_by_id_dev || !ngroups) return fail(NBX_ERR_ARG, "NULL argument");
-    if (t->nd != 3) return fail(NBX_ERR_UNSUPPORTED, "FOF on TPHS trees is not supported (use a TPHYS tree)");
     if (!(fdist >= 0)) return fail(NBX_ERR_ARG, "linking length must be >= 0");
     hipStream_t st = (hipStream_t)stream;
     const int64_t n = t->n;

@@ -16,16 +16,72 @@
 //    is < b, all cross pairs link, so one union joins the node to the wave.
 // All bounds are monotone in the roundings (nbx_internal.h), so a shortcut
 // is taken only when every computed particle distance is below b^2.
+//
+// On a 6-D (TPHS) tree the link distance is the reference's
+// DistanceSqd(positions) + DistanceSqd(velocities) (KDLeafNode.cxx:570-571):
+// two sums of three squares, added.  Every particle distance and every box
+// bound here is summed with that same association, so the bounds stay
+// monotone.  Periodic images shift the positions only (KDSplitNode.cxx:
+// 1609-1641).
 #include "nbx_kernels.h"
 
+// Sum of ND per-dimension squares in the reference's association: one
+// running sum for 3-D, position and velocity sums added for 6-D.
+template <int ND>
+__device__ __forceinline__ double fof_sum(const double (&t2)[ND]) {
+    double s = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) s += t2[d];
+    if constexpr (ND == 6) {
+        double v = 0.0;
+#pragma unroll
+        for (int d = 3; d < 6; ++d) v += t2[d];
+        s += v;
+    }
+    return s;
+}
+template <int ND>
+__device__ __forceinline__ double fof_lb(const double *q, cdptr b) {
+    double t2[ND];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        const double t = fmax(fmax(b[d] - q[d], q[d] - b[ND + d]), 0.0);
+        t2[d] = t * t;
+    }
+    return fof_sum<ND>(t2);
+}
+template <int ND>
+__device__ __forceinline__ double fof_ub(const double *q, cdptr b) {
+    double t2[ND];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        const double a = q[d] - b[d], c = b[ND + d] - q[d];
+        const double a2 = a * a, c2 = c * c;
+        t2[d] = (a2 > c2) ? a2 : c2;
+    }
+    return fof_sum<ND>(t2);
+}
+template <int ND>
+__device__ __forceinline__ double fof_bbub(cdptr a, cdptr b) {
+    double t2[ND];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        const double u = b[ND + d] - a[d], v = a[ND + d] - b[d];
+        const double m = u > v ? u : v;
+        t2[d] = m * m;
+    }
+    return fof_sum<ND>(t2);
+}
+
+template <int ND>
 __global__ __launch_bounds__(256) void k_fof_leafinit(DevTree T, double b2, int32_t *P, uint8_t *conn) {
     int32_t L = blockIdx.x * 256 + threadIdx.x;
     if (L >= T.nleaf) return;
     const int node = T.nleafp + L;
-    cdptr bx = (cdptr)node_box(T, node, 3);
+    cdptr bx = (cdptr)node_box(T, node, ND);
     int32_t s0 = L * NBX_LEAF;
     int32_t s1 = s0 + NBX_LEAF < T.n ? s0 + NBX_LEAF : T.n;
-    bool c = boxbox_ub<3>(bx, bx) < b2;
+    bool c = fof_bbub<ND>(bx, bx) < b2;
     conn[node] = c ? 1 : 0;
     for (int32_t s = s0; s < s1; ++s) P[s] = c ? s0 : s;
 }
@@ -34,18 +90,19 @@ __global__ __launch_bounds__(256) void k_fof_leafinit(DevTree T, double b2, int3
 // diagonals are smaller) and every cross pair links, so one union connects
 // it.  Nodes [first, first+count) of one depth.  An empty box (padding) has
 // an infinite diagonal and is never connected.
+template <int ND>
 __global__ __launch_bounds__(256) void k_fof_nodeinit(DevTree T, double b2, int32_t *P, uint8_t *conn, int32_t first,
                                                        int32_t count) {
     int32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i >= count) return;
     const int node = first + i;
-    cdptr bx = (cdptr)node_box(T, node, 3);
-    bool c = boxbox_ub<3>(bx, bx) < b2;
+    cdptr bx = (cdptr)node_box(T, node, ND);
+    bool c = fof_bbub<ND>(bx, bx) < b2;
     conn[node] = c ? 1 : 0;
     if (!c) return;
     const int right = 2 * node + 1;
-    cdptr br = (cdptr)node_box(T, right, 3);
-    if (br[0] <= br[3]) uf_union(P, node_first(T, node), node_first(T, right));
+    cdptr br = (cdptr)node_box(T, right, ND);
+    if (br[0] <= br[ND]) uf_union(P, node_first(T, node), node_first(T, right));
 }
 
 // Links from the (possibly image-shifted) position q to the particles of one
@@ -55,31 +112,35 @@ __global__ __launch_bounds__(256) void k_fof_nodeinit(DevTree T, double b2, int3
 // the first pass only toward larger slots: the pair is met from both sides).
 // rq is a hint for the root of qi (any former root of its set), kept across
 // unions so each find starts near the top.
-template <int PASS0>
+template <int PASS0, int ND>
 __device__ __forceinline__ void fof_leaf(const DevTree &T, int leaf, const double *q, bool act, int32_t qi,
                                          double b2, int32_t *P, const uint8_t *conn, int32_t &rq) {
     const int base = leaf * NBX_LEAF;
     const bool lconn = conn[T.nleafp + leaf];
-    cdptr bx = (cdptr)node_box(T, T.nleafp + leaf, 3);
-    const bool encl = act && lconn && box_ub<3>(q, bx) < b2;
+    cdptr bx = (cdptr)node_box(T, T.nleafp + leaf, ND);
+    const bool encl = act && lconn && fof_ub<ND>(q, bx) < b2;
     if (encl) rq = uf_union_r(P, rq, base);
     const bool pair = act && !encl;
     if (!ballot(pair)) return;
     uint32_t mask = 0;
+    // 64-byte (3-D) or 32-byte (6-D) scalar-load chunks per dimension
+    constexpr int CH = ND == 6 ? 4 : 8;
+    typedef double dxc __attribute__((ext_vector_type(CH)));
+    typedef const __attribute__((address_space(4))) dxc *cdcptr;
 #pragma unroll
-    for (int jj = 0; jj < NBX_LEAF; jj += 8) {
-        const dx8 cx = *(cd8ptr)(T.x[0] + base + jj);
-        const dx8 cy = *(cd8ptr)(T.x[1] + base + jj);
-        const dx8 cz = *(cd8ptr)(T.x[2] + base + jj);
+    for (int jj = 0; jj < NBX_LEAF; jj += CH) {
+        dxc cv[ND];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            double d2 = 0.0;
-            double t = q[0] - cx[u];
-            d2 += t * t;
-            t = q[1] - cy[u];
-            d2 += t * t;
-            t = q[2] - cz[u];
-            d2 += t * t;
+        for (int d = 0; d < ND; ++d) cv[d] = *(cdcptr)(T.x[d] + base + jj);
+#pragma unroll
+        for (int u = 0; u < CH; ++u) {
+            double t2[ND];
+#pragma unroll
+            for (int d = 0; d < ND; ++d) {
+                const double t = q[d] - cv[d][u];
+                t2[d] = t * t;
+            }
+            const double d2 = fof_sum<ND>(t2);
             const int32_t cs = base + jj + u;
             const bool l = d2 < b2 && (PASS0 ? (lconn ? cs != qi : cs > qi) : cs != qi);
             mask |= l ? (1u << (jj + u)) : 0u;
@@ -97,22 +158,22 @@ __device__ __forceinline__ void fof_leaf(const DevTree &T, int leaf, const doubl
     }
 }
 
-template <int PASS0>
+template <int PASS0, int ND>
 __device__ __forceinline__ void fof_walk(const DevTree &T, const double *q, bool act, int32_t qi, int g, double b2,
                                          int32_t *P, uint8_t *conn, int32_t &rq) {
     const int D = T.depth;
     // the wave's own 64 particles form the parent of leaves 2g, 2g+1 (or the
     // single leaf when the tree has one)
     const int gnode = D >= 1 ? (T.nleafp + 2 * g) >> 1 : 1;
-    cdptr gb = (cdptr)node_box(T, gnode, 3);
+    cdptr gb = (cdptr)node_box(T, gnode, ND);
     int node = 1;
     bool down = true;
     for (;;) {
         if (down) {
-            cdptr bx = (cdptr)node_box(T, node, 3);
-            bool want = act && box_lb<3>(q, bx) < b2;
+            cdptr bx = (cdptr)node_box(T, node, ND);
+            bool want = act && fof_lb<ND>(q, bx) < b2;
             if (ballot(want)) {
-                if (PASS0 && boxbox_ub<3>(gb, bx) < b2) {
+                if (PASS0 && fof_bbub<ND>(gb, bx) < b2) {
                     // every wave particle links to every particle of the node:
                     // wave and node form one component
                     const int32_t nfirst = node_first(T, node);
@@ -124,7 +185,7 @@ __device__ __forceinline__ void fof_walk(const DevTree &T, const double *q, bool
                     }
                     down = false;
                 } else if (node >= T.nleafp) {
-                    fof_leaf<PASS0>(T, node - T.nleafp, q, act, qi, b2, P, conn, rq);
+                    fof_leaf<PASS0, ND>(T, node - T.nleafp, q, act, qi, b2, P, conn, rq);
                     down = false;
                 } else {
                     node = 2 * node;
@@ -144,21 +205,24 @@ __device__ __forceinline__ void fof_walk(const DevTree &T, const double *q, bool
     }
 }
 
+template <int ND>
 __global__ __launch_bounds__(64) void k_fof(DevTree T, FofArgs A, uint8_t *conn) {
     const int lane = threadIdx.x;
     const int g = blockIdx.x;
     const int32_t qi = g * NBX_WAVE + lane;
     const bool valid = qi < T.n;
-    double q0[3];
+    double q0[ND];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) q0[d] = valid ? T.x[d][qi] : 0.0;
+    for (int d = 0; d < ND; ++d) q0[d] = valid ? T.x[d][qi] : 0.0;
     int32_t rq = valid ? qi : 0;
-    fof_walk<1>(T, q0, valid, qi, g, A.b2, A.parent, conn, rq);
+    fof_walk<1, ND>(T, q0, valid, qi, g, A.b2, A.parent, conn, rq);
     if (!A.periodic) return;
     // FOFSearchBallPeriodic: image toward the far wall when b^2 > s^2
     const double *per = A.period;
     for (int pass = 1; pass < 8; ++pass) {
-        double xp[3] = {q0[0], q0[1], q0[2]};
+        double xp[ND];
+#pragma unroll
+        for (int d = 0; d < ND; ++d) xp[d] = q0[d];
         double sval = 0.0;
         if (pass <= 3) {
             int k = pass - 1;
@@ -176,7 +240,7 @@ __global__ __launch_bounds__(64) void k_fof(DevTree T, FofArgs A, uint8_t *conn)
         }
         bool act = valid && A.b2 > sval * sval;
         if (!ballot(act)) continue;
-        fof_walk<0>(T, xp, act, qi, g, A.b2, A.parent, conn, rq);
+        fof_walk<0, ND>(T, xp, act, qi, g, A.b2, A.parent, conn, rq);
     }
 }
 
@@ -241,14 +305,23 @@ hipError_t nbx_launch_fof(const DevTree &T, const FofArgs &A, hipStream_t st) {
     uint8_t *conn = (uint8_t *)(A.parent + T.n);
     {
         ProfScope ps("k_fof_init", st);
-        hipLaunchKernelGGL(k_fof_leafinit, dim3(nblk(T.nleaf)), dim3(256), 0, st, T, A.b2, A.parent, conn);
-        for (int dep = T.depth - 1; dep >= 0; --dep)
-            hipLaunchKernelGGL(k_fof_nodeinit, dim3(nblk((int64_t)1 << dep)), dim3(256), 0, st, T, A.b2, A.parent,
-                               conn, 1 << dep, 1 << dep);
+        if (T.nd == 3)
+            hipLaunchKernelGGL(k_fof_leafinit<3>, dim3(nblk(T.nleaf)), dim3(256), 0, st, T, A.b2, A.parent, conn);
+        else
+            hipLaunchKernelGGL(k_fof_leafinit<6>, dim3(nblk(T.nleaf)), dim3(256), 0, st, T, A.b2, A.parent, conn);
+        for (int dep = T.depth - 1; dep >= 0; --dep) {
+            if (T.nd == 3)
+                hipLaunchKernelGGL(k_fof_nodeinit<3>, dim3(nblk((int64_t)1 << dep)), dim3(256), 0, st, T, A.b2,
+                                   A.parent, conn, 1 << dep, 1 << dep);
+            else
+                hipLaunchKernelGGL(k_fof_nodeinit<6>, dim3(nblk((int64_t)1 << dep)), dim3(256), 0, st, T, A.b2,
+                                   A.parent, conn, 1 << dep, 1 << dep);
+        }
     }
     int groups = (T.n + NBX_WAVE - 1) / NBX_WAVE;
     ProfScope ps("k_fof_link", st);
-    hipLaunchKernelGGL(k_fof, dim3(groups), dim3(NBX_WAVE), 0, st, T, A, conn);
+    if (T.nd == 3) hipLaunchKernelGGL(k_fof<3>, dim3(groups), dim3(NBX_WAVE), 0, st, T, A, conn);
+    else hipLaunchKernelGGL(k_fof<6>, dim3(groups), dim3(NBX_WAVE), 0, st, T, A, conn);
     return hipGetLastError();
 }
 

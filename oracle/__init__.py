@@ -68,6 +68,8 @@ def lib():
         L.orc_fof_canonical.restype = _i64
         L.orc_fof_brute.argtypes = [_dp, _i64, _cd, _ci, ctypes.c_void_p, _ip]
         L.orc_fof_brute.restype = _i64
+        L.orc_fof_brute_nd.argtypes = [_dp, _i64, _ci, _cd, _ci, ctypes.c_void_p, _ip]
+        L.orc_fof_brute_nd.restype = _i64
         L.orc_density_analytic.argtypes = [_dp, _dp, _i64, _ip, _ci, _dp, ctypes.c_void_p]
         L.orc_density_adjoint.argtypes = [_dp, _dp, _i64, _ip, _ci, _dp, _dp, _dp]
         L.orc_knn_density_sample.argtypes = [_vp, _ci, np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"),
@@ -223,12 +225,14 @@ def fof_canonical(labels, minnum=1):
     return out, g
 
 
-def fof_brute(pos, b, minnum=8, period=None):
-    ph = _phase(pos)
+def fof_brute(pos, b, minnum=8, period=None, vel=None):
+    """All-pairs FOF, canonical labels.  With vel, links in 6-D phase space
+    with the reference's d2 = |dx|^2 + |dv|^2 (periodic images shift x only)."""
+    ph = _phase(pos, vel)
     n = ph.shape[0]
     lab = np.zeros(n, dtype=np.int32)
     per = None if period is None else np.ascontiguousarray(np.broadcast_to(period, (3,)), dtype=np.float64)
-    g = lib().orc_fof_brute(ph, n, b, minnum, _ptr(per), lab)
+    g = lib().orc_fof_brute_nd(ph, n, 6 if vel is not None else 3, b, minnum, _ptr(per), lab)
     return lab, g
 
 

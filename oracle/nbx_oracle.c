@@ -1061,37 +1061,41 @@ int64_t orc_fof_canonical(int64_t n, const int32_t *labels, int minnum, int32_t 
 /* link test with the reference arithmetic: primary distance, or an image of
  * the first particle shifted toward the far wall on the axes in `mask`
  * (DistFunc.h:326-355 forms the image point first, then the distance). */
-static int link_dir(const double *a, const double *b, double b2, const double *p, int periodic) {
+/* d2 between phase points a and b in the reference FOF's association
+ * (KDLeafNode.cxx:570-571): DistanceSqd of the positions, plus DistanceSqd of
+ * the velocities on a 6-D tree. */
+static inline double fof_d2(const double *a, const double *b, int nd) {
     double d2 = (a[0] - b[0]) * (a[0] - b[0]) + (a[1] - b[1]) * (a[1] - b[1]) + (a[2] - b[2]) * (a[2] - b[2]);
-    if (d2 < b2) return 1;
+    if (nd == 6)
+        d2 += (a[3] - b[3]) * (a[3] - b[3]) + (a[4] - b[4]) * (a[4] - b[4]) + (a[5] - b[5]) * (a[5] - b[5]);
+    return d2;
+}
+
+/* Does a link to b, directly or through one of the images the reference
+ * searches (KDSplitNode.cxx:1602-1646: image iff b^2 > s^2; only the position
+ * is shifted)? */
+static int link_dir(const double *a, const double *b, double b2, const double *p, int periodic, int nd) {
+    if (fof_d2(a, b, nd) < b2) return 1;
     if (!periodic) return 0;
-    double xp[3], s;
+    double xp[6], s;
+    for (int j = 3; j < 6; j++) xp[j] = a[j];
     for (int k = 0; k < 3; k++) {
         s = refl1(a, xp, p, k);
-        if (b2 > s * s) {
-            d2 = (xp[0] - b[0]) * (xp[0] - b[0]) + (xp[1] - b[1]) * (xp[1] - b[1]) + (xp[2] - b[2]) * (xp[2] - b[2]);
-            if (d2 < b2) return 1;
-        }
+        if (b2 > s * s && fof_d2(xp, b, nd) < b2) return 1;
     }
     const int pairs[3][2] = {{0, 1}, {0, 2}, {1, 2}};
     for (int c = 0; c < 3; c++) {
         s = refl2(a, xp, p, pairs[c][0], pairs[c][1]);
-        if (b2 > s * s) {
-            d2 = (xp[0] - b[0]) * (xp[0] - b[0]) + (xp[1] - b[1]) * (xp[1] - b[1]) + (xp[2] - b[2]) * (xp[2] - b[2]);
-            if (d2 < b2) return 1;
-        }
+        if (b2 > s * s && fof_d2(xp, b, nd) < b2) return 1;
     }
     s = reflN(a, xp, p, 3);
-    if (b2 > s * s) {
-        d2 = (xp[0] - b[0]) * (xp[0] - b[0]) + (xp[1] - b[1]) * (xp[1] - b[1]) + (xp[2] - b[2]) * (xp[2] - b[2]);
-        if (d2 < b2) return 1;
-    }
+    if (b2 > s * s && fof_d2(xp, b, nd) < b2) return 1;
     return 0;
 }
 
-/* All-pairs union-find FOF (3-D positions; phase rows n x 6), links are the
- * union of both directions.  Returns G; labels canonical. */
-int64_t orc_fof_brute(const double *phase, int64_t n, double b, int minnum, const double *period, int32_t *labels) {
+/* All-pairs FOF on nd = 3 or 6 phase coordinates (n x 6 rows), canonical labels. */
+int64_t orc_fof_brute_nd(const double *phase, int64_t n, int nd, double b, int minnum, const double *period,
+                         int32_t *labels) {
     int64_t *par = (int64_t *)malloc(sizeof(int64_t) * (size_t)n);
     for (int64_t i = 0; i < n; i++) par[i] = i;
     double b2 = b * b;
@@ -1099,7 +1103,7 @@ int64_t orc_fof_brute(const double *phase, int64_t n, double b, int minnum, cons
     for (int64_t i = 0; i < n; i++)
         for (int64_t j = i + 1; j < n; j++) {
             const double *a = phase + 6 * i, *c = phase + 6 * j;
-            if (link_dir(a, c, b2, period, periodic) || (periodic && link_dir(c, a, b2, period, periodic)))
+            if (link_dir(a, c, b2, period, periodic, nd) || (periodic && link_dir(c, a, b2, period, periodic, nd)))
                 uf_union(par, i, j);
         }
     int32_t *raw = (int32_t *)malloc(sizeof(int32_t) * (size_t)n);
@@ -1107,6 +1111,10 @@ int64_t orc_fof_brute(const double *phase, int64_t n, double b, int minnum, cons
     int64_t G = orc_fof_canonical(n, raw, minnum, labels);
     free(par); free(raw);
     return G;
+}
+
+int64_t orc_fof_brute(const double *phase, int64_t n, double b, int minnum, const double *period, int32_t *labels) {
+    return orc_fof_brute_nd(phase, n, 3, b, minnum, period, labels);
 }
 
 /* ===================================================== differentiation form */

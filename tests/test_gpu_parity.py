@@ -95,6 +95,41 @@ def test_fof_vs_brute(gpu, periodic):
     assert np.array_equal(lab, ol)
 
 
+@pytest.mark.parametrize("periodic", [False, True])
+def test_fof6d_vs_brute(gpu, periodic):
+    # FOF on a phase-space (TPHS) tree: 6-D links, d2 = |dx|^2 + |dv|^2
+    nb = _nb()
+    n = 6000
+    pos = datagen.uniform(n, 1.0, 31)
+    vel = np.random.default_rng(32).random((n, 3)) * 0.3
+    b = 0.85 * (0.3 ** 3 / n) ** (1 / 6)
+    say(f"fof6d n={n} b={b} periodic={periodic}")
+    per = 1.0 if periodic else None
+    t = nb.KDTree(pos, vel=vel, treetype=nb.TPHS, aniso=-1, kerntype=nb.KSPH, period=per)
+    lab, ng = t.FOF(b, 8)
+    ol, og = oracle.fof_brute(pos, b, 8, period=per, vel=vel)
+    assert og > 5
+    assert ng == og
+    assert np.array_equal(lab, ol)
+
+
+def test_fof6d_clustered_vs_tree_oracle(gpu):
+    nb = _nb()
+    n = 32768
+    pos, _ = datagen.clustered(n, 1.0, nhalo=40, seed=33)
+    vel = np.random.default_rng(34).normal(size=(n, 3)) * 0.02
+    b = 0.25 * datagen.mean_spacing(n, 1.0)
+    say(f"fof6d clustered n={n} b={b}")
+    t = nb.KDTree(pos, vel=vel, treetype=nb.TPHS, aniso=-1, kerntype=nb.KSPH)
+    lab, ng = t.FOF(b, 8)
+    ot = oracle.Tree(pos, vel=vel, b=16, treetype=oracle.TPHS, kerntype=oracle.KSPH)
+    raw, og = ot.fof(b, 8)
+    can, _ = oracle.fof_canonical(raw, 8)
+    assert og > 0
+    assert ng == og
+    assert np.array_equal(lab, can)
+
+
 def test_fof_clustered_vs_tree_oracle(gpu):
     nb = _nb()
     n = 65536

@@ -230,6 +230,32 @@ def test_fof_tree_vs_brute(periodic):
     assert np.array_equal(can, bl)
 
 
+@pytest.mark.parametrize("periodic", [False, True])
+def test_fof6d_tree_vs_brute(periodic):
+    # FOF on a phase-space (TPHS) tree links in 6-D with d2 = |dx|^2 + |dv|^2
+    # (KDLeafNode.cxx:570-571); the restated BFS must equal all-pairs linking
+    n = 3000
+    pos = datagen.uniform(n, 1.0, 21)
+    vel = np.random.default_rng(22).random((n, 3)) * 0.3
+    b = 0.85 * (0.3 ** 3 / n) ** (1 / 6)
+    per = 1.0 if periodic else None
+    raw, g = oracle.Tree(pos, vel=vel, b=16, treetype=oracle.TPHS, kerntype=oracle.KSPH, period=per).fof(b, 8)
+    can, g2 = oracle.fof_canonical(raw, 8)
+    bl, bg = oracle.fof_brute(pos, b, 8, period=per, vel=vel)
+    assert g == g2 == bg and bg > 5
+    assert np.array_equal(can, bl)
+
+
+def test_fof6d_zero_velocity_is_3d():
+    # with all velocities equal the 6-D distance is the 3-D one
+    n = 3000
+    pos = datagen.uniform(n, 1.0, 23)
+    b = 0.6 * datagen.mean_spacing(n, 1.0)
+    l6, g6 = oracle.fof_brute(pos, b, 8, vel=np.zeros((n, 3)))
+    l3, g3 = oracle.fof_brute(pos, b, 8)
+    assert g6 == g3 > 0 and np.array_equal(l6, l3)
+
+
 def test_fof_clustered_vs_brute():
     pos, _ = datagen.clustered(5000, 100.0, nhalo=15, seed=15)
     b = 0.2 * datagen.mean_spacing(5000, 100.0)
