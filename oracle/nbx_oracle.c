@@ -24,7 +24,8 @@
  *
  *  (2) Brute-force restatements that share no code with (1): all-pairs kNN with
  *      the same acceptance rules, the sequential image rule applied to all
- *      points, all-pairs union-find FOF, long-double density from neighbour
+ *      points, all-pairs union-find FOF (3-D, and 6-D phase space with the
+ *      reference's |dx|^2 + |dv|^2, KDLeafNode.cxx:570-571), long-double density from neighbour
  *      lists, the FOF canonicaliser, the analytic (C1) cubic-spline density used
  *      by the differentiation form and its hand-derived adjoint.
  *
@@ -32,6 +33,7 @@
  * cases; kernel normalisation against closed forms (1/pi in 3-D,
  * 3024/(762 pi^3) in 6-D) and against the normalisation integral; mean
  * density of a uniform box against sum(m)/V; FOF on constructed chains;
+ * 6-D FOF with zero velocities against 3-D FOF;
  * adjoint against central finite differences.
  *
  * Arithmetic follows the reference's x86-64 build: IEEE double, no FMA
