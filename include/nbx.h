@@ -121,6 +121,16 @@ int nbx_knn_density_dev(nbx_tree *t, int32_t k, int32_t *nn_by_slot_dev, double 
  * Replaces SearchBallPosTagged(Double_t*x, fdist2, Int_t*tagged)
  * KDFindNearest.cxx:618-688 (period-aware).  out: capacity cap; *nout = total found. */
 int nbx_ball_points(nbx_tree *t, const double *x, double r2, int32_t *out, int64_t cap, int64_t *nout);
+/* Ball search that marks: group_by_id[id] = imark and d2_by_id[id] = d2 for
+ * every particle with d2 < r2 around x, except the particle with ID
+ * exclude_id (pass -1 to exclude none); other entries are left as they are.
+ * Replaces SearchBallPos(Double_t *x, fdist2, imark, nn, dist2) and, with
+ * x = the target's position and exclude_id = its ID, SearchBallPos(Int_t tt,
+ * ...) KDFindNearest.cxx:567-583 (leaf rules KDLeafNode.cxx:247-300).  On a
+ * periodic tree r must be below half the period (then a particle is within r
+ * of at most one image and its d2 is that image's). */
+int nbx_ball_mark(nbx_tree *t, const double *x, int64_t exclude_id, double r2, int32_t imark, int32_t *group_by_id,
+                  double *d2_by_id);
 
 /* Friends-of-friends.  Replaces KDTree::FOF(fdist, numgroup, minnum, order)
  * KDFOF.cxx:29-153 (periodic when the tree is).  labels_by_id: canonical labels

@@ -6,6 +6,7 @@
  *   Particle      src/NBody/Particle.h:264-293 (88-byte layout, default build)
  *   KDTree ctor   src/KDTree/KDTree.h:229-274
  *   FindNearest*  KDTree.h:300-331      SearchBallPosTagged  KDTree.h:363-365
+ *   SearchBallPos KDTree.h:342, :348
  *   CalcDensity*  KDTree.h:411-416      FOF                  KDTree.h:480-482
  *   OverWriteInputOrder / SetResetOrder KDTree.h:515-518
  *
@@ -121,6 +122,11 @@ class KDTree {
     void FindNearestPos(Double_t *x, Int_t *nn, Double_t *dist2, Int_t Nsearch = 64);
 
     // tree indices of all particles with d2 < fdist2 around a position / particle
+    // nn[id] = imark, dist2[id] = d2 for every particle with d2 < fdist2
+    // (KDTree.h:342, :348; KDFindNearest.cxx:567-583; the target itself is not
+    // marked)
+    void SearchBallPos(Int_t tt, Double_t fdist2, Int_t imark, Int_t *nn, Double_t *dist2);
+    void SearchBallPos(Double_t *x, Double_t fdist2, Int_t imark, Int_t *nn, Double_t *dist2);
     Int_t SearchBallPosTagged(Double_t *x, Double_t fdist2, Int_t *tagged);
     Int_t SearchBallPosTagged(Int_t tt, Double_t fdist2, Int_t *tagged);
     std::vector<Int_t> SearchBallPosTagged(Double_t *x, Double_t fdist2);

@@ -30,7 +30,7 @@ NBX_ERR_ARG, NBX_ERR_HIP, NBX_ERR_UNSUPPORTED, NBX_ERR_NOMEM, NBX_ERR_SEARCH = -
 EXPORTS = [
     "nbx_params_default", "nbx_tree_create", "nbx_tree_create_dev", "nbx_tree_destroy", "nbx_tree_order",
     "nbx_tree_size", "nbx_tree_kernel", "nbx_knn", "nbx_knn_dev", "nbx_knn_points", "nbx_density",
-    "nbx_density_dev", "nbx_knn_density_dev", "nbx_ball_points", "nbx_fof", "nbx_fof_dev",
+    "nbx_density_dev", "nbx_knn_density_dev", "nbx_ball_points", "nbx_ball_mark", "nbx_fof", "nbx_fof_dev",
     "nbx_sph_density_fwd_dev", "nbx_sph_density_adjoint_dev", "nbx_prof_enable", "nbx_prof_reset", "nbx_prof_get",
     "nbx_tree_last_fallbacks", "nbx_last_error",
 ]
@@ -81,6 +81,7 @@ def lib():
     L.nbx_density_dev.argtypes = [vp, i32, i32, i32, vp, vp, vp]
     L.nbx_knn_density_dev.argtypes = [vp, i32, vp, vp, vp, vp]
     L.nbx_ball_points.argtypes = [vp, vp, dbl, vp, i64, P(i64)]
+    L.nbx_ball_mark.argtypes = [vp, vp, i64, dbl, i32, vp, vp]
     L.nbx_fof.argtypes = [vp, dbl, i32, vp, P(i64)]
     L.nbx_fof_dev.argtypes = [vp, dbl, i32, vp, P(i64), vp]
     L.nbx_sph_density_fwd_dev.argtypes = [vp, vp, vp, i64, i32, vp, vp, vp]
@@ -247,6 +248,17 @@ class KDTree:
         nout = ctypes.c_int64()
         _check(lib().nbx_ball_points(self._h, _p(x), float(fdist2), _p(out), cap, ctypes.byref(nout)))
         return out[: min(nout.value, cap)].copy()
+
+    def SearchBallPos(self, x, fdist2, imark, nn, dist2, exclude_id=-1):
+        """SearchBallPos(x, fdist2, imark, nn, dist2) (KDFindNearest.cxx:575-582): nn[id] = imark and
+        dist2[id] = d2 for every particle with d2 < fdist2 (ID-indexed arrays of n, updated in place);
+        exclude_id skips one particle (the target of the SearchBallPos(tt, ...) form)."""
+        x = np.ascontiguousarray(np.asarray(x, dtype=np.float64).reshape(3))
+        if nn.dtype != np.int32 or dist2.dtype != np.float64 or nn.shape != (self.n,) or dist2.shape != (self.n,):
+            raise ValueError("nn must be int32[n] and dist2 float64[n]")
+        if not (nn.flags.c_contiguous and dist2.flags.c_contiguous):
+            raise ValueError("nn and dist2 must be contiguous")
+        _check(lib().nbx_ball_mark(self._h, _p(x), int(exclude_id), float(fdist2), int(imark), _p(nn), _p(dist2)))
 
     def FOF(self, fdist, minnum=8):
         """KDTree::FOF (KDFOF.cxx:29-153): canonical labels by ID and the group count."""

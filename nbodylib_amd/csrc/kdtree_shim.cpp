@@ -179,6 +179,15 @@ void KDTree::FindNearestPos(Double_t *x, Int_t *nn, Double_t *dist2, Int_t Nsear
 }
 
 // ------------------------------------------------------------------ ball search
+void KDTree::SearchBallPos(Double_t *x, Double_t fdist2, Int_t imark, Int_t *nn, Double_t *dist2) {
+    std::lock_guard<std::mutex> lk(mu);
+    check(nbx_ball_mark(tree, x, -1, fdist2, imark, nn, dist2), "SearchBallPos");
+}
+void KDTree::SearchBallPos(Int_t tt, Double_t fdist2, Int_t imark, Int_t *nn, Double_t *dist2) {
+    Double_t x[3] = {bucket[tt].GetPosition(0), bucket[tt].GetPosition(1), bucket[tt].GetPosition(2)};
+    std::lock_guard<std::mutex> lk(mu);
+    check(nbx_ball_mark(tree, x, bucket[tt].GetID(), fdist2, imark, nn, dist2), "SearchBallPos");
+}
 std::vector<Int_t> KDTree::SearchBallPosTagged(Double_t *x, Double_t fdist2) {
     std::lock_guard<std::mutex> lk(mu);
     int64_t cap = numparts, nout = 0;

@@ -16,8 +16,8 @@
 
 hipError_t nbx_fof_canonical(const DevTree &T, int32_t *P, int32_t minnum, int32_t *labels_by_id, int64_t *ngroups,
                              hipStream_t st);
-__global__ void k_ball(DevTree T, double qx, double qy, double qz, double r2, int32_t *out, int64_t cap,
-                       unsigned long long *count);
+__global__ void k_ball(DevTree T, double qx, double qy, double qz, double r2, int32_t *out, double *dout,
+                       int64_t cap, unsigned long long *count);
 
 // ------------------------------------------------------------------ errors
 static thread_local std::string g_err;
@@ -640,13 +640,18 @@ extern "C" int nbx_fof(nbx_tree *t, double fdist, int32_t minnum, int32_t *label
 }
 
 // ------------------------------------------------------------------ ball search
-extern "C" int nbx_ball_points(nbx_tree *t, const double *x, double r2, int32_t *out, int64_t cap, int64_t *nout) {
+// IDs (and d2, when d2out is given) of the particles with d2 < r2 around x,
+// through the images of SearchBallPosPeriodicTagged (KDSplitNode.cxx:1388-1414).
+static int ball_impl(nbx_tree *t, const double *x, double r2, int32_t *out, double *d2out, int64_t cap,
+                     int64_t *nout) {
     if (!t || !x || !nout) return fail(NBX_ERR_ARG, "NULL argument");
     if (t->nd != 3) return fail(NBX_ERR_UNSUPPORTED, "ball search needs a TPHYS tree");
     int32_t *dout = nullptr;
+    double *dd2 = nullptr;
     unsigned long long *dc = nullptr;
     int64_t dcap = cap > 0 ? cap : 1;
     HIPCK(hipMalloc((void **)&dout, sizeof(int32_t) * dcap));
+    if (d2out) HIPCK(hipMalloc((void **)&dd2, sizeof(double) * dcap));
     HIPCK(hipMalloc((void **)&dc, sizeof(unsigned long long)));
     HIPCK(hipMemset(dc, 0, sizeof(unsigned long long)));
     DevTree T = t->view();
@@ -675,15 +680,52 @@ extern "C" int nbx_ball_points(nbx_tree *t, const double *x, double r2, int32_t 
         }
     }
     for (int i = 0; i < np; ++i)
-        hipLaunchKernelGGL(k_ball, dim3(1), dim3(NBX_WAVE), 0, 0, T, pts[i][0], pts[i][1], pts[i][2], r2, dout, cap, dc);
+        hipLaunchKernelGGL(k_ball, dim3(1), dim3(NBX_WAVE), 0, 0, T, pts[i][0], pts[i][1], pts[i][2], r2, dout, dd2, cap,
+                           dc);
     unsigned long long hc = 0;
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpy(&hc, dc, sizeof(hc), hipMemcpyDeviceToHost);
     if (e == hipSuccess && cap > 0) e = hipMemcpy(out, dout, sizeof(int32_t) * std::min<int64_t>(cap, (int64_t)hc), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && cap > 0 && d2out)
+        e = hipMemcpy(d2out, dd2, sizeof(double) * std::min<int64_t>(cap, (int64_t)hc), hipMemcpyDeviceToHost);
     hipFree(dout);
+    if (dd2) hipFree(dd2);
     hipFree(dc);
     if (e != hipSuccess) return hipfail(e, "ball");
     *nout = (int64_t)hc;
+    return NBX_OK;
+}
+
+extern "C" int nbx_ball_points(nbx_tree *t, const double *x, double r2, int32_t *out, int64_t cap, int64_t *nout) {
+    return ball_impl(t, x, r2, out, nullptr, cap, nout);
+}
+
+extern "C" int nbx_ball_mark(nbx_tree *t, const double *x, int64_t exclude_id, double r2, int32_t imark,
+                             int32_t *group_by_id, double *d2_by_id) {
+    if (!t || !x || !group_by_id || !d2_by_id) return fail(NBX_ERR_ARG, "NULL argument");
+    if (t->p.periodic) {
+        for (int d = 0; d < 3; ++d)
+            if (!(4.0 * r2 < t->p.period[d] * t->p.period[d]))
+                return fail(NBX_ERR_UNSUPPORTED, "periodic ball search needs a radius below half the period");
+    }
+    // with r below half the period a particle lies within r of at most one
+    // image of x, so its d2 is the one the reference's image pass writes
+    int64_t cap = std::min<int64_t>(t->n, 1 << 20), nout = 0;
+    std::vector<int32_t> ids;
+    std::vector<double> d2;
+    for (;;) {
+        ids.resize(cap);
+        d2.resize(cap);
+        int rc = ball_impl(t, x, r2, ids.data(), d2.data(), cap, &nout);
+        if (rc != NBX_OK) return rc;
+        if (nout <= cap) break;
+        cap = nout;
+    }
+    for (int64_t i = 0; i < nout; ++i) {
+        if (ids[i] == exclude_id) continue;
+        group_by_id[ids[i]] = imark;
+        d2_by_id[ids[i]] = d2[i];
+    }
     return NBX_OK;
 }
 

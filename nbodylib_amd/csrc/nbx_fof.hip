@@ -403,11 +403,13 @@ __global__ __launch_bounds__(256) void k_iota(int32_t *p, int64_t n) {
 }
 
 // ------------------------------------------------------------------ ball search
-// SearchBallPosTagged(x) (KDFindNearest.cxx:618-688, KDLeafNode.cxx:336-356):
-// every particle with d2 < r2 around one position; one wave walks the tree
-// and each lane tests one candidate of a leaf.
+// SearchBallPosTagged(x) (KDFindNearest.cxx:618-688, KDLeafNode.cxx:336-356)
+// and SearchBallPos(x) (KDFindNearest.cxx:575-582, KDLeafNode.cxx:279-300):
+// every particle with d2 < r2 around one position, its ID and (dout, when
+// given) its d2 from that position; one wave walks the tree and each lane
+// tests one candidate of a leaf.
 __global__ __launch_bounds__(64) void k_ball(DevTree T, double qx, double qy, double qz, double r2, int32_t *out,
-                                             int64_t cap, unsigned long long *count) {
+                                             double *dout, int64_t cap, unsigned long long *count) {
     const double q[3] = {qx, qy, qz};
     int node = 1;
     bool down = true;
@@ -425,7 +427,10 @@ __global__ __launch_bounds__(64) void k_ball(DevTree T, double qx, double qy, do
                         }
                         if (d2 < r2) {
                             unsigned long long at = atomicAdd(count, 1ULL);
-                            if ((int64_t)at < cap) out[at] = T.perm[cs];
+                            if ((int64_t)at < cap) {
+                                out[at] = T.perm[cs];
+                                if (dout) dout[at] = d2;
+                            }
                         }
                     }
                     down = false;

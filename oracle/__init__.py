@@ -70,6 +70,8 @@ def lib():
         L.orc_fof_brute.restype = _i64
         L.orc_fof_brute_nd.argtypes = [_dp, _i64, _ci, _cd, _ci, ctypes.c_void_p, _ip]
         L.orc_fof_brute_nd.restype = _i64
+        L.orc_ball_mark.argtypes = [_dp, _i64, _dp, _i64, _cd, ctypes.c_void_p, _ci, _ip, _dp]
+        L.orc_ball_mark.restype = None
         L.orc_density_analytic.argtypes = [_dp, _dp, _i64, _ip, _ci, _dp, ctypes.c_void_p]
         L.orc_density_adjoint.argtypes = [_dp, _dp, _i64, _ip, _ci, _dp, _dp, _dp]
         L.orc_knn_density_sample.argtypes = [_vp, _ci, np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"),
@@ -234,6 +236,14 @@ def fof_brute(pos, b, minnum=8, period=None, vel=None):
     per = None if period is None else np.ascontiguousarray(np.broadcast_to(period, (3,)), dtype=np.float64)
     g = lib().orc_fof_brute_nd(ph, n, 6 if vel is not None else 3, b, minnum, _ptr(per), lab)
     return lab, g
+
+
+def ball_mark(pos, x, r2, imark, group, d2, exclude=-1, period=None):
+    """SearchBallPos restated over all particles: group[id] = imark, d2[id] = d2 (in place, by ID)."""
+    ph = _phase(pos)
+    x = np.ascontiguousarray(np.asarray(x, dtype=np.float64).reshape(3))
+    per = None if period is None else np.ascontiguousarray(np.broadcast_to(period, (3,)), dtype=np.float64)
+    lib().orc_ball_mark(ph, ph.shape[0], x, int(exclude), float(r2), _ptr(per), int(imark), group, d2)
 
 
 def density_analytic(pos, mass, nbr):

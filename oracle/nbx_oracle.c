@@ -1095,6 +1095,42 @@ static int link_dir(const double *a, const double *b, double b2, const double *p
     return 0;
 }
 
+/* SearchBallPos(x | tt, fdist2, imark, nn, dist2) restated over all particles
+ * (KDFindNearest.cxx:567-583, leaf rules KDLeafNode.cxx:247-300): every
+ * particle with d2 < r2 gets group[id] = imark and d2out[id] = d2, where d2 =
+ * sum over x, y, z of (x - p)^2.  Periodic (SearchBallPosPeriodic,
+ * KDSplitNode.cxx:1341-1459): the images with r2 > s^2 are searched after the
+ * primary pass, in the reference's order, and a later pass overwrites.  The
+ * particle with ID exclude is skipped (the target of the tt form). */
+void orc_ball_mark(const double *phase, int64_t n, const double *x, int64_t exclude, double r2,
+                   const double *period, int32_t imark, int32_t *group, double *d2out) {
+    double pts[8][3];
+    int np = 0;
+    pts[np][0] = x[0]; pts[np][1] = x[1]; pts[np][2] = x[2]; np++;
+    if (period) {
+        double xp[3], s;
+        for (int k = 0; k < 3; k++) {
+            s = refl1(x, xp, period, k);
+            if (r2 > s * s) { memcpy(pts[np], xp, sizeof(xp)); np++; }
+        }
+        const int pairs[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+        for (int c = 0; c < 3; c++) {
+            s = refl2(x, xp, period, pairs[c][0], pairs[c][1]);
+            if (r2 > s * s) { memcpy(pts[np], xp, sizeof(xp)); np++; }
+        }
+        s = reflN(x, xp, period, 3);
+        if (r2 > s * s) { memcpy(pts[np], xp, sizeof(xp)); np++; }
+    }
+    for (int q = 0; q < np; q++)
+        for (int64_t i = 0; i < n; i++) {
+            if (i == exclude) continue;
+            const double *pi = phase + 6 * i;
+            double d2 = 0.0;
+            for (int d = 0; d < 3; d++) d2 += (pts[q][d] - pi[d]) * (pts[q][d] - pi[d]);
+            if (d2 < r2) { group[i] = imark; d2out[i] = d2; }
+        }
+}
+
 /* All-pairs FOF on nd = 3 or 6 phase coordinates (n x 6 rows), canonical labels. */
 int64_t orc_fof_brute_nd(const double *phase, int64_t n, int nd, double b, int minnum, const double *period,
                          int32_t *labels) {

@@ -161,6 +161,27 @@ def test_clustered_knn_density(gpu):
     assert np.array_equal(d2, od)
 
 
+@pytest.mark.parametrize("periodic", [False, True])
+def test_ball_mark(gpu, periodic):
+    # SearchBallPos(x | tt, fdist2, imark, nn, dist2): marks and d2 by ID
+    nb = _nb()
+    n = 20000
+    pos = datagen.uniform(n, 1.0, 43)
+    per = 1.0 if periodic else None
+    say(f"ball mark n={n} periodic={periodic}")
+    t = nb.KDTree(pos, kerntype=nb.KSPH, period=per)
+    for x, excl in (([0.5, 0.4, 0.6], -1), ([0.02, 0.98, 0.03], -1), (pos[123], 123)):
+        g = np.zeros(n, dtype=np.int32)
+        d2 = np.full(n, -1.0)
+        t.SearchBallPos(np.asarray(x), 0.004, 5, g, d2, exclude_id=excl)
+        og = np.zeros(n, dtype=np.int32)
+        od2 = np.full(n, -1.0)
+        oracle.ball_mark(pos, x, 0.004, 5, og, od2, exclude=excl, period=per)
+        assert (og == 5).sum() > 2
+        assert np.array_equal(g, og)
+        assert np.array_equal(d2, od2)
+
+
 def test_ball_search(gpu):
     nb = _nb()
     n = 5000

@@ -256,6 +256,43 @@ def test_fof6d_zero_velocity_is_3d():
     assert g6 == g3 > 0 and np.array_equal(l6, l3)
 
 
+@pytest.mark.parametrize("periodic", [False, True])
+def test_ball_mark_vs_numpy(periodic):
+    # SearchBallPos restatement against an independent computation of the
+    # same rule: d2 = sum (x - p)^2 in x, y, z order, strict < r2; periodic
+    # queries near a corner also reach the particles across the walls
+    n = 4000
+    pos = datagen.uniform(n, 1.0, 41)
+    r2 = 0.012
+    per = 1.0 if periodic else None
+    for x, excl in (([0.5, 0.4, 0.6], -1), ([0.03, 0.97, 0.05], -1), (pos[17], 17)):
+        x = np.asarray(x, dtype=np.float64)
+        g = np.zeros(n, dtype=np.int32)
+        d2 = np.full(n, -1.0)
+        oracle.ball_mark(pos, x, r2, 7, g, d2, exclude=excl, period=per)
+        shifts = [np.zeros(3)]
+        if periodic:
+            for ax in ([0], [1], [2], [0, 1], [0, 2], [1, 2], [0, 1, 2]):
+                sh = np.zeros(3)
+                for a in ax:
+                    sh[a] = 1.0 if x[a] < 0.5 else -1.0
+                shifts.append(sh)
+        best = np.full(n, np.inf)
+        for sh in shifts:
+            xp = x + sh
+            dd = (xp[0] - pos[:, 0]) ** 2
+            dd = dd + (xp[1] - pos[:, 1]) ** 2
+            dd = dd + (xp[2] - pos[:, 2]) ** 2
+            best = np.where(dd < r2, dd, best)
+        hit = best < r2
+        if excl >= 0:
+            hit[excl] = False
+        assert hit.sum() > 2
+        assert np.array_equal(g == 7, hit)
+        assert np.array_equal(d2[hit], best[hit])
+        assert np.all(d2[~hit] == -1.0)
+
+
 def test_fof_clustered_vs_brute():
     pos, _ = datagen.clustered(5000, 100.0, nhalo=15, seed=15)
     b = 0.2 * datagen.mean_spacing(5000, 100.0)
