@@ -100,6 +100,13 @@ int nbx_knn_dev(nbx_tree *t, int32_t k, int32_t variant, int32_t layout, int32_t
 /* kNN of arbitrary positions (q: 3*m SoA).  Replaces FindNearestPos(Double_t*x,...)
  * KDFindNearest.cxx:462-564 (period-aware like the reference).  Entries are IDs. */
 int nbx_knn_points(nbx_tree *t, const double *q, int64_t m, int32_t k, int32_t *nn, double *d2);
+/* Gather SPH density at arbitrary positions (q: 3*m SoA), TPHYS trees.
+ * Replaces CalcDensityPosition(Double_t *x, Nsmooth, v) KDCalcSmoothQuantities.cxx:1092-1149
+ * for m positions: non-periodic search with no exclusion (a particle at x is
+ * its own neighbour), h = r_k/2, rho = sum_j W(r_j/h) h^-3 m_j.  rho[m];
+ * h[m] nullable.  NBX_ERR_SEARCH when the tree holds fewer than k particles
+ * (the reference exits). */
+int nbx_density_points(nbx_tree *t, const double *q, int64_t m, int32_t k, double *rho, double *h);
 
 /* SPH densities, results keyed by ID.
  * SYMMETRIC replaces KDTree::CalcDensity KDCalcSmoothQuantities.cxx:203-305;

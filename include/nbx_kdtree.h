@@ -134,6 +134,8 @@ class KDTree {
     void CalcDensity(Int_t Nsmooth = 64);
     void CalcVelDensity(Int_t Nsmooth = 64, Int_t Nsearch = 64);
     Double_t CalcDensityParticle(Int_t target, Int_t Nsmooth = 64);
+    // KDTree.h:432 (TPHYS trees; v is unused there); KDCalcSmoothQuantities.cxx:1092-1149
+    Double_t CalcDensityPosition(Double_t *x, Int_t Nsmooth = 64, Double_t *v = NULL);
 
     Int_t *FOF(Double_t fdist, Int_t &numgroup, Int_t minnum = 8, int order = 0, Int_tree_t *pHead = nullptr,
                Int_tree_t *pNext = nullptr, Int_tree_t *pTail = nullptr, Int_tree_t *pLen = nullptr);

@@ -29,7 +29,7 @@ NBX_ERR_ARG, NBX_ERR_HIP, NBX_ERR_UNSUPPORTED, NBX_ERR_NOMEM, NBX_ERR_SEARCH = -
 
 EXPORTS = [
     "nbx_params_default", "nbx_tree_create", "nbx_tree_create_dev", "nbx_tree_destroy", "nbx_tree_order",
-    "nbx_tree_size", "nbx_tree_kernel", "nbx_knn", "nbx_knn_dev", "nbx_knn_points", "nbx_density",
+    "nbx_tree_size", "nbx_tree_kernel", "nbx_knn", "nbx_knn_dev", "nbx_knn_points", "nbx_density_points", "nbx_density",
     "nbx_density_dev", "nbx_knn_density_dev", "nbx_ball_points", "nbx_ball_mark", "nbx_fof", "nbx_fof_dev",
     "nbx_sph_density_fwd_dev", "nbx_sph_density_adjoint_dev", "nbx_prof_enable", "nbx_prof_reset", "nbx_prof_get",
     "nbx_tree_last_fallbacks", "nbx_last_error",
@@ -77,6 +77,7 @@ def lib():
     L.nbx_knn.argtypes = [vp, i32, i32, i32, vp, vp]
     L.nbx_knn_dev.argtypes = [vp, i32, i32, i32, vp, vp, vp]
     L.nbx_knn_points.argtypes = [vp, vp, i64, i32, vp, vp]
+    L.nbx_density_points.argtypes = [vp, vp, i64, i32, vp, vp]
     L.nbx_density.argtypes = [vp, i32, i32, i32, vp, vp]
     L.nbx_density_dev.argtypes = [vp, i32, i32, i32, vp, vp, vp]
     L.nbx_knn_density_dev.argtypes = [vp, i32, vp, vp, vp, vp]
@@ -232,6 +233,21 @@ class KDTree:
         h = np.zeros(self.n, dtype=np.float64)
         _check(lib().nbx_density(self._h, Nsmooth, DENS_GATHER, 0, _p(rho), _p(h)))
         return rho, h
+
+    def CalcDensityPosition(self, x, Nsmooth=64, return_h=False):
+        """CalcDensityPosition(x, Nsmooth) (KDCalcSmoothQuantities.cxx:1092-1149) at each row of x (m, 3);
+        a single position (3,) gives a float."""
+        x = np.asarray(x, dtype=np.float64)
+        one = x.ndim == 1
+        x = x.reshape(-1, 3)
+        m = x.shape[0]
+        q = np.ascontiguousarray(x.T)
+        rho = np.zeros(m, dtype=np.float64)
+        h = np.zeros(m, dtype=np.float64)
+        _check(lib().nbx_density_points(self._h, _p(q), m, Nsmooth, _p(rho), _p(h)))
+        if one:
+            return (float(rho[0]), float(h[0])) if return_h else float(rho[0])
+        return (rho, h) if return_h else rho
 
     def CalcVelDensity(self, Nsmooth=64, Nsearch=64):
         """KDTree::CalcVelDensity (KDCalcSmoothQuantities.cxx:309-389); result by ID."""

@@ -234,6 +234,12 @@ Double_t KDTree::CalcDensityParticle(Int_t target, Int_t Nsmooth) {
     }
     return dcache[bucket[target].GetID()];
 }
+Double_t KDTree::CalcDensityPosition(Double_t *x, Int_t Nsmooth, Double_t *) {
+    std::lock_guard<std::mutex> lk(mu);
+    double den = 0.0;
+    check(nbx_density_points(tree, x, 1, Nsmooth, &den, nullptr), "CalcDensity failed");
+    return den;
+}
 
 // ------------------------------------------------------------------ FOF
 Int_t *KDTree::FOF(Double_t fdist, Int_t &numgroup, Int_t minnum, int, Int_tree_t *, Int_tree_t *, Int_tree_t *,

@@ -486,36 +486,69 @@ extern "C" int nbx_knn(nbx_tree *t, int32_t k, int32_t variant, int32_t layout, 
     return rc;
 }
 
-extern "C" int nbx_knn_points(nbx_tree *t, const double *q, int64_t m, int32_t k, int32_t *nn, double *d2) {
-    if (!t || !q || !nn) return fail(NBX_ERR_ARG, "NULL argument");
-    if (t->nd != 3) return fail(NBX_ERR_UNSUPPORTED, "position queries need a TPHYS tree");
-    if (k < 1 || k > 64) return fail(NBX_ERR_UNSUPPORTED, "k out of range (1..64)");
+// Position queries through the plain kernel: kNN (nn/d2, nullable) and/or the
+// gather density with h = r_k/2 (rho/h, nullable).  periodic: the image passes
+// of the period-aware search.
+static int points_impl(nbx_tree *t, const double *q, int64_t m, int32_t k, bool periodic, int32_t *nn, double *d2,
+                       double *rho, double *h, const char *what) {
     if (m <= 0) return NBX_OK;
-    double *dq = nullptr, *dd2 = nullptr;
-    int32_t *dnn = nullptr;
+    if (m > INT32_MAX) return fail(NBX_ERR_ARG, "too many positions (m < 2^31)");
+    double *dq = nullptr, *dd2 = nullptr, *drho = nullptr, *dh = nullptr;
+    int32_t *dnn = nullptr, *err = nullptr;
     HIPCK(hipMalloc((void **)&dq, sizeof(double) * 3 * m));
     HIPCK(hipMemcpy(dq, q, sizeof(double) * 3 * m, hipMemcpyHostToDevice));
-    HIPCK(hipMalloc((void **)&dnn, sizeof(int32_t) * m * k));
+    if (nn) HIPCK(hipMalloc((void **)&dnn, sizeof(int32_t) * m * k));
     if (d2) HIPCK(hipMalloc((void **)&dd2, sizeof(double) * m * k));
+    if (rho) HIPCK(hipMalloc((void **)&drho, sizeof(double) * m));
+    if (h) HIPCK(hipMalloc((void **)&dh, sizeof(double) * m));
+    HIPCK(hipMalloc((void **)&err, sizeof(int32_t)));
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, nullptr, err, 0, (int64_t)1);
     KnnArgs A;
     memset(&A, 0, sizeof(A));
     A.qsrc = NBX_Q_POINTS;
     A.kh = k;
     A.kout = k;
     A.m = (int32_t)m;
-    A.periodic = t->p.periodic && t->n > t->p.bucket_size;
+    A.periodic = periodic && t->p.periodic && t->n > t->p.bucket_size;
     for (int d = 0; d < 3; ++d) { A.period[d] = t->p.period[d]; A.q[d] = dq + d * m; }
     A.nn = dnn;
     A.d2 = dd2;
+    A.dens = rho ? NBX_DO_GATHER : NBX_DO_NONE;
+    A.rho = drho;
+    A.hout = dh;
+    A.err = err;
+    int32_t herr = 0;
     hipError_t e = nbx_launch_knn(t->view(), A, (int)m, nullptr);
     if (e == hipSuccess) e = hipDeviceSynchronize();
-    if (e == hipSuccess) e = hipMemcpy(nn, dnn, sizeof(int32_t) * m * k, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(&herr, err, sizeof(int32_t), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && nn) e = hipMemcpy(nn, dnn, sizeof(int32_t) * m * k, hipMemcpyDeviceToHost);
     if (e == hipSuccess && d2) e = hipMemcpy(d2, dd2, sizeof(double) * m * k, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && rho) e = hipMemcpy(rho, drho, sizeof(double) * m, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && h) e = hipMemcpy(h, dh, sizeof(double) * m, hipMemcpyDeviceToHost);
     hipFree(dq);
-    hipFree(dnn);
+    if (dnn) hipFree(dnn);
     if (dd2) hipFree(dd2);
-    if (e != hipSuccess) return hipfail(e, "nbx_knn_points");
+    if (drho) hipFree(drho);
+    if (dh) hipFree(dh);
+    hipFree(err);
+    if (e != hipSuccess) return hipfail(e, what);
+    if (herr) return fail(NBX_ERR_SEARCH, "CalcDensityPosition failed: fewer than Nsmooth particles");
     return NBX_OK;
+}
+
+extern "C" int nbx_knn_points(nbx_tree *t, const double *q, int64_t m, int32_t k, int32_t *nn, double *d2) {
+    if (!t || !q || !nn) return fail(NBX_ERR_ARG, "NULL argument");
+    if (t->nd != 3) return fail(NBX_ERR_UNSUPPORTED, "position queries need a TPHYS tree");
+    if (k < 1 || k > 64) return fail(NBX_ERR_UNSUPPORTED, "k out of range (1..64)");
+    return points_impl(t, q, m, k, true, nn, d2, nullptr, nullptr, "nbx_knn_points");
+}
+
+extern "C" int nbx_density_points(nbx_tree *t, const double *q, int64_t m, int32_t k, double *rho, double *h) {
+    if (!t || !q || !rho) return fail(NBX_ERR_ARG, "NULL argument");
+    if (t->nd != 3) return fail(NBX_ERR_UNSUPPORTED, "CalcDensityPosition here needs a TPHYS tree");
+    if (k < 1 || k > 64) return fail(NBX_ERR_UNSUPPORTED, "Nsmooth out of range (1..64)");
+    if (k > t->n) return fail(NBX_ERR_SEARCH, "CalcDensityPosition failed: fewer than Nsmooth particles");
+    return points_impl(t, q, m, k, false, nullptr, nullptr, rho, h, "nbx_density_points");
 }
 
 // ------------------------------------------------------------------ density

@@ -67,8 +67,8 @@ struct KnnArgs {
     const double *q[3];  // external points (SoA)
     int32_t *nn;        // nullable
     double *d2;         // nullable
-    double *rho;        // by slot (density)
-    double *hout;       // by slot (nullable)
+    double *rho;        // by slot (density); by query row for position queries
+    double *hout;       // as rho (nullable)
     int32_t *err;       // set to 1 if a row has fewer than kh neighbours
     const int32_t *qlist;  // nullable: query slots (FP64 re-run of mixed-form fallbacks)
 };

@@ -365,7 +365,9 @@ __global__ __launch_bounds__(64) void k_knn(DevTree T, KnnArgs A) {
     double racc_own = 0.0;
     if (own) racc[lane] = 0.0;
     __syncthreads();
-    if (QSRC == NBX_Q_SLOTS && A.dens != NBX_DO_NONE && valid) {
+    // position queries take the gather form only (mass and slot of the
+    // query are undefined there)
+    if (A.dens != NBX_DO_NONE && valid && (QSRC == NBX_Q_SLOTS || A.dens == NBX_DO_GATHER)) {
         bool bad = false;
         if (A.dens == NBX_DO_VEL) {
             // CalcVelDensity: |dv| of the Nsearch physical neighbours, the
@@ -398,7 +400,7 @@ __global__ __launch_bounds__(64) void k_knn(DevTree T, KnnArgs A) {
         } else {
             double hi = 0.5 * sqrt(kth);
             double norm = 1.0 / pow(hi, (double)ND);
-            double mi = T.mass[qi];
+            const double mi = QSRC == NBX_Q_SLOTS ? T.mass[qi] : 0.0;
             double acc = 0.0;
             // entries in groups of 8 so their slot/mass gathers are in flight
             // together; summed in heap-position order
@@ -412,7 +414,7 @@ __global__ __launch_bounds__(64) void k_knn(DevTree T, KnnArgs A) {
                     dv[u] = in ? HD(p0 + u) : 0.0;
                 }
 #pragma unroll
-                for (int u = 0; u < 8; ++u) sv[u] = (idv[u] >= 0 && idv[u] != NBX_NOID) ? T.inv[idv[u]] : qi;
+                for (int u = 0; u < 8; ++u) sv[u] = (idv[u] >= 0 && idv[u] != NBX_NOID) ? T.inv[idv[u]] : 0;
 #pragma unroll
                 for (int u = 0; u < 8; ++u) mv[u] = T.mass[sv[u]];
 #pragma unroll

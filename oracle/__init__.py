@@ -59,6 +59,8 @@ def lib():
         L.orc_calc_density_gather.restype = _ci
         L.orc_calc_veldensity.argtypes = [_vp, _ci, _ci, _dp, _ci]
         L.orc_calc_veldensity.restype = _ci
+        L.orc_calc_density_position.argtypes = [_vp, _dp, _i64, _ci, _dp, ctypes.c_void_p, _ci]
+        L.orc_calc_density_position.restype = _ci
         L.orc_fof_tree.argtypes = [_vp, _cd, _ci, _ci, _ip]
         L.orc_fof_tree.restype = _i64
         L.orc_knn_brute.argtypes = [_dp, _i64, _ci, _ci, _ci, ctypes.c_void_p, _ip, ctypes.c_void_p, _ci]
@@ -170,6 +172,17 @@ class Tree:
         rc = lib().orc_calc_density_gather(self.h, k, rho, _ptr(h), nthreads)
         if rc != 0:
             raise RuntimeError("CalcDensityParticle failed")
+        return rho, h
+
+    def calc_density_position(self, q, k, nthreads=0):
+        """CalcDensityPosition(x, k) at each row of q (m, 3): (rho, h)."""
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        m = q.shape[0]
+        rho = np.zeros(m, dtype=np.float64)
+        h = np.zeros(m, dtype=np.float64)
+        rc = lib().orc_calc_density_position(self.h, q, m, k, rho, _ptr(h), nthreads)
+        if rc != 0:
+            raise RuntimeError("CalcDensityPosition failed")
         return rho, h
 
     def calc_veldensity(self, nsmooth, nsearch, nthreads=0):

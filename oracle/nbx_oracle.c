@@ -637,6 +637,43 @@ int orc_calc_density_gather(void *h, int k, double *rho_by_id, double *h_by_id, 
     return rc;
 }
 
+/* CalcDensityPosition(x, Nsmooth) for m positions (TPHYS tree),
+ * KDCalcSmoothQuantities.cxx:1092-1149: the node search root->FindNearestPos
+ * (non-periodic, no exclusion: a particle at x is its own neighbour, d2 = 0),
+ * h = sqrt(top)/2, then the gather sum of W(r/h) h^-3 m_j in pop order
+ * (farthest first).  Returns 0, or -1 when fewer than Nsmooth were found. */
+int orc_calc_density_position(void *h, const double *q, int64_t m, int k, double *rho, double *hout, int nthreads) {
+    otree *t = (otree *)h;
+    int rc = 0;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel reduction(min : rc)
+    {
+        pq_t pq;
+        pq_init(&pq, k);
+#pragma omp for schedule(guided)
+        for (int64_t i = 0; i < m; i++) {
+            double off[6] = {0, 0, 0, 0, 0, 0}, den = 0.;
+            pq.n = 0;
+            for (int j = 0; j < k; j++) pq_push(&pq, -1, ORC_MAXVALUE);
+            nn_pos(t, t->root, 0.0, &pq, off, q + 3 * i, 0, 3);
+            double hi = 0.5 * sqrt(pq_top(&pq));
+            double norm = 1.0 / pow(hi, 3.0);
+            for (int j = 0; j < k; j++) {
+                if (pq_topq(&pq) == -1) { rc = -1; break; }
+                double rij = sqrt(pq_top(&pq));
+                den += wsm_at(rij, hi, t->kernres, t->kern) * norm * t->p[pq_topq(&pq)].m;
+                pq_pop(&pq);
+            }
+            rho[i] = den;
+            if (hout) hout[i] = hi;
+        }
+        pq_free(&pq);
+    }
+    return rc;
+}
+
 /* CalcVelDensity, KDCalcSmoothQuantities.cxx:309-389 (TPHYS tree):
  * Nsearch physical neighbours, the Nsmooth nearest of them in |dv|,
  * h_v = max|dv|/2, sum of W(|dv|/h_v) h_v^-3 (no mass, no 1/2). */

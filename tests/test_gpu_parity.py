@@ -80,6 +80,38 @@ def test_density_gather(gpu):
     assert np.array_equal(h, ho)
 
 
+@pytest.mark.parametrize("form", ["plain", "mixed"])
+def test_density_position(gpu, form):
+    """CalcDensityPosition at positions inside and outside the box of a
+    periodic clustered tree (the search ignores the period), at particle
+    positions (the particle is its own neighbour) and for k = 1, 2, 64."""
+    nb = _nb()
+    n = 6000
+    say(f"density position form={form} n={n}")
+    pos, _ = datagen.clustered(n, 1.0, nhalo=6, seed=33)
+    m = datagen.lognormal_masses(n, 33)
+    rng = np.random.default_rng(33)
+    x = np.concatenate([rng.uniform(-0.2, 1.2, (500, 3)), pos[:300], pos[300:400] + 1e-4])
+    fm = nb.FORM_PLAIN if form == "plain" else nb.FORM_MIXED
+    t = nb.KDTree(pos, mass=m, kerntype=nb.KSPH, period=1.0, form=fm)
+    ot = oracle.Tree(pos, mass=m, b=16, kerntype=oracle.KSPH, period=1.0)
+    for k in (32, 1, 2, 64):
+        # k = 1 at a particle's own position gives h = 0 (0/0 in the reference)
+        xs = x if k > 1 else np.concatenate([x[:500], x[800:]])
+        say(f"  k={k} m={len(xs)}")
+        rho, h = t.CalcDensityPosition(xs, k, return_h=True)
+        ro, ho = ot.calc_density_position(xs, k)
+        assert np.array_equal(h, ho)
+        ok = ro != 0  # k = 1: the one neighbour sits at q = 2, where W = 0
+        assert np.all(rho[~ok] == 0)
+        assert not ok.any() or np.max(np.abs(rho[ok] / ro[ok] - 1.0)) < DENS_TOL
+    one = t.CalcDensityPosition(x[7], 32)
+    assert abs(one / ot.calc_density_position(x[7:8], 32)[0][0] - 1.0) < DENS_TOL
+    small = nb.KDTree(pos[:20], mass=m[:20], kerntype=nb.KSPH)
+    with pytest.raises(Exception):
+        small.CalcDensityPosition(x[:2], 32)
+
+
 @pytest.mark.parametrize("periodic", [False, True])
 def test_fof_vs_brute(gpu, periodic):
     nb = _nb()

@@ -189,6 +189,67 @@ def test_gather_density_vs_lists():
     assert abs(rg[inner].mean() - 1.0) < 0.03
 
 
+def _density_position_analytic(pos, mass, x, k):
+    """All-pairs restatement with the closed-form cubic spline (1/pi)W(q)."""
+    d2 = ((x[:, None, :] - pos[None, :, :]) ** 2).sum(-1)
+    idx = np.argsort(d2, axis=1, kind="stable")[:, :k]
+    dk = np.take_along_axis(d2, idx, 1)
+    h = 0.5 * np.sqrt(dk[:, -1])
+    q = np.sqrt(dk) / h[:, None]
+    w = np.where(q <= 1, 1 - 0.75 * (2 - q) * q * q, np.where(q < 2, 0.25 * (2 - q) ** 3, 0.0)) / math.pi
+    return (w * mass[idx]).sum(1) / h ** 3, h, mass[idx].sum(1) / h ** 3
+
+
+def _table_err_ok(rho, ra, scale):
+    """Linear interpolation on the 1000-entry table errs by at most
+    delta^2/8 max|W''| = (2/999)^2/8 * 3/pi < 5e-7 per unit kernel weight."""
+    return np.all(np.abs(rho - ra) <= 1e-6 * scale)
+
+
+def test_density_position_vs_brute():
+    """CalcDensityPosition against an all-pairs restatement with the analytic
+    kernel (agreement to the 1000-entry table's interpolation error), on
+    positions inside and outside the particle box of a periodic tree (the
+    search ignores the period), with h exact."""
+    pos, _ = datagen.clustered(3000, 1.0, nhalo=6, seed=31)
+    mass = datagen.lognormal_masses(3000, 31)
+    rng = np.random.default_rng(31)
+    x = np.concatenate([rng.uniform(-0.2, 1.2, (300, 3)), pos[:50] + 1e-3])
+    t = oracle.Tree(pos, mass=mass, b=16, period=1.0)
+    rho, h = t.calc_density_position(x, 32)
+    ra, ha, sc = _density_position_analytic(pos, mass, x, 32)
+    assert np.array_equal(h, ha)
+    assert _table_err_ok(rho, ra, sc)
+    assert np.median(np.abs(rho / ra - 1)) < 1e-5
+    for k in (1, 2, 64):
+        r2, h2 = t.calc_density_position(x[:40], k)
+        ra2, ha2, sc2 = _density_position_analytic(pos, mass, x[:40], k)
+        assert np.array_equal(h2, ha2)
+        assert _table_err_ok(r2, ra2, sc2)
+    with pytest.raises(RuntimeError):
+        oracle.Tree(pos[:20], b=16).calc_density_position(x[:2], 32)
+
+
+def test_density_position_at_particles_is_gather_plus_self():
+    """At a particle's own position the position search finds the particle
+    itself at d2 = 0 and then its k nearest others, so
+    rho_pos(x_i, k+1) = rho_gather(i, k) + W(0) h^-3 m_i with the same h
+    (CalcDensityParticle excludes the target).  A dropped self term or an
+    off-by-one in the neighbour count fails it."""
+    pos = datagen.uniform(4000, 1.0, 32)
+    mass = datagen.lognormal_masses(4000, 32)
+    t = oracle.Tree(pos, mass=mass, b=16)
+    rg, hg = t.calc_density_gather(32)
+    rp, hp = t.calc_density_position(pos, 33)
+    assert np.array_equal(hp, hg)
+    self_term = (1.0 / math.pi) / hg ** 3 * mass
+    assert np.max(np.abs(rp / (rg + self_term) - 1)) < 1e-12
+    big = datagen.uniform(32768, 1.0, 20261202)
+    inner = datagen.uniform(2000, 1.0, 7) * 0.7 + 0.15
+    rb, _ = oracle.Tree(big, b=16).calc_density_position(inner, 32)
+    assert abs(rb.mean() - 1.0) < 0.03
+
+
 def _veldens_numpy(pos, vel, nsmooth, nsearch):
     ids, _ = oracle.knn_brute(pos, nsearch, mode=0)
     kn, tab = oracle.kernel(3, oracle.KSPH, 1000)
