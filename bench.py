@@ -37,6 +37,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
+# VALU issue peak: 256 CUs x 4 SIMD16 x 2.4 GHz, one wave64 VALU instruction
+# per SIMD every 4 cycles = 614.4 G wave-instructions/s (x 64 lanes x 2 FLOP
+# per FMA x 2 packed = the 157.3 TFLOP/s FP32-vector peak)
+PEAK_VALU_GINST = 614.4
 
 
 def log(msg):
@@ -175,14 +179,22 @@ def main():
     achieved = bytes_per_q * n / (kern_ms / 1e3) / 1e9
     kname = "k_knn_mixed" if headline["form"] == "mixed" else "k_knn_fp64"
     traffic = None
+    valu = None
     if os.path.exists(args.traffic):
         tj = json.load(open(args.traffic))
         ent = tj.get(kname)
         if ent and ent.get("n") == n and ent.get("k") == k:
             traffic = ent["bytes_per_launch"]
+            # VALU wave-instructions per launch (rocprofv3 SQ_INSTS_VALU on the
+            # same seeded workload) over this run's launch time
+            vi = ent.get("valu_insts")
+            if vi:
+                vach = vi / (kern_ms / 1e3) / 1e9
+                valu = {"insts_per_launch": vi, "achieved": round(vach, 1), "peak": PEAK_VALU_GINST,
+                        "unit": "G VALU wave-instr/s", "frac": round(vach / PEAK_VALU_GINST, 4)}
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                 "frac": round(achieved / PEAK_HBM_GBS, 5), "traffic": traffic,
-                "kernel": kname, "kernel_ms": round(kern_ms, 3), "bytes_per_query": bytes_per_q}
+                "kernel": kname, "kernel_ms": round(kern_ms, 3), "bytes_per_query": bytes_per_q, "valu": valu}
 
     cpu = None
     if not args.no_cpu:
