@@ -1,0 +1,139 @@
+"""Edge cases on the GPU, in both forms where the call has two: trees of one
+or a few leaves (the root is a leaf below 33 particles), fewer particles than
+k, exact ties on a lattice, a box far from the origin for the precision
+form's leaf-relative FP32 offsets, and FOF with no links or one group.
+
+Each case prints its name and sizes before the GPU call."""
+import numpy as np
+import pytest
+
+import oracle
+from nbodylib_amd import datagen
+
+pytestmark = pytest.mark.gpu
+
+
+def _nb():
+    import nbodylib_amd as nb
+
+    return nb
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def _form(nb, form):
+    return nb.FORM_PLAIN if form == "plain" else nb.FORM_MIXED
+
+
+@pytest.mark.parametrize("form", ["plain", "mixed"])
+@pytest.mark.parametrize("n", [1, 2, 17, 32, 33, 95])
+def test_small_trees_knn(gpu, n, form):
+    """FindNearest(8) on trees of 1-3 leaves; rows with fewer than 8 others
+    end in -1 with d2 = 1e32, like the reference's filler entries."""
+    nb = _nb()
+    k = 8
+    say(f"small tree knn n={n} k={k} form={form}")
+    pos = datagen.uniform(n, 1.0, 50 + n)
+    t = nb.KDTree(pos, kerntype=nb.KSPH, form=_form(nb, form))
+    nn, d2 = t.FindNearest(k)
+    bi, bd = oracle.knn_brute(pos, k, mode=0)
+    assert np.array_equal(nn, bi)
+    assert np.array_equal(d2, bd)
+
+
+@pytest.mark.parametrize("form", ["plain", "mixed"])
+def test_small_tree_density_and_too_few(gpu, form):
+    """k = n - 1 works (every other particle is a neighbour); k >= n is the
+    reference's "CalcDensity failed" exit, an error here."""
+    nb = _nb()
+    n = 20
+    say(f"small tree density n={n} form={form}")
+    pos = datagen.uniform(n, 1.0, 61)
+    m = datagen.lognormal_masses(n, 61)
+    t = nb.KDTree(pos, mass=m, kerntype=nb.KSPH, form=_form(nb, form))
+    rho = t.CalcDensity(n - 1)
+    ro = oracle.Tree(pos, mass=m, b=16, kerntype=oracle.KSPH).calc_density(n - 1)
+    assert np.max(np.abs(rho / ro - 1.0)) < 1e-12
+    with pytest.raises(nb.NbxError):
+        t.CalcDensity(n)
+
+
+@pytest.mark.parametrize("form", ["plain", "mixed"])
+@pytest.mark.parametrize("k", [6, 7, 18, 19])
+def test_lattice_ties(gpu, form, k):
+    """A cubic lattice: 6 nearest at one distance, 12 at the next.  k = 7 and
+    19 split a shell, where the neighbour set is decided by the tie rule
+    (smaller ID first); distances are exact ties in FP32 too, so the precision
+    form must settle them through its FP64 checks.  The gather density, which
+    does not depend on which tied particle is taken, matches the oracle."""
+    nb = _nb()
+    g = 12
+    say(f"lattice ties g={g} k={k} form={form}")
+    ax = np.arange(g) + 0.5  # half-integers: every distance is exact in FP64 and FP32
+    pos = np.stack(np.meshgrid(ax, ax, ax, indexing="ij"), -1).reshape(-1, 3)
+    t = nb.KDTree(pos, kerntype=nb.KSPH, form=_form(nb, form))
+    nn, d2 = t.FindNearest(k)
+    bi, bd = oracle.knn_brute(pos, k, mode=0)
+    assert np.array_equal(d2, bd)
+    assert np.array_equal(np.sort(nn, 1), np.sort(bi, 1))
+    rho, h = t.CalcDensityParticleAll(k)
+    ro, ho = oracle.Tree(pos, b=16, kerntype=oracle.KSPH).calc_density_gather(k)
+    assert np.array_equal(h, ho)
+    # k = 6: the whole first shell sits at 2h, where W = 0
+    z = ro == 0
+    assert np.array_equal(rho == 0, z)
+    assert z.all() or np.max(np.abs(rho[~z] / ro[~z] - 1.0)) < 1e-12
+
+
+@pytest.mark.parametrize("offset", [1.0e4, 1.0e6])
+def test_precision_form_far_box(gpu, offset):
+    """A unit box placed far from the origin: the precision form works on
+    FP32 offsets from each leaf's FP64 centre, so its neighbour sets stay
+    exact and its densities within 1e-12 of the FP64 form."""
+    nb = _nb()
+    n, k = 20000, 32
+    say(f"far box offset={offset:g} n={n} k={k}")
+    pos = datagen.uniform(n, 1.0, 62) + offset
+    tm = nb.KDTree(pos, kerntype=nb.KSPH, form=nb.FORM_MIXED)
+    tp = nb.KDTree(pos, kerntype=nb.KSPH, form=nb.FORM_PLAIN)
+    nm, dm = tm.FindNearest(k)
+    npl, dp = tp.FindNearest(k)
+    assert np.array_equal(nm, npl)
+    assert np.array_equal(dm, dp)
+    bi, bd = oracle.knn_brute(pos, k, mode=0)
+    assert np.array_equal(np.sort(npl, 1), np.sort(bi, 1))
+    rm, rp = tm.CalcDensity(k), tp.CalcDensity(k)
+    assert np.max(np.abs(rm / rp - 1.0)) < 1e-12
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_fof_extremes(gpu, periodic):
+    """No pair closer than b: every particle is its own group, so with
+    minnum = 2 all labels are 0; b larger than the box diagonal: one group."""
+    nb = _nb()
+    n = 3000
+    say(f"fof extremes n={n} periodic={periodic}")
+    pos = datagen.uniform(n, 1.0, 63)
+    per = 1.0 if periodic else None
+    t = nb.KDTree(pos, kerntype=nb.KSPH, period=per)
+    lab, ng = t.FOF(1e-9, minnum=2)
+    assert ng == 0 and not lab.any()
+    bl, bg = oracle.fof_brute(pos, 1e-9, 2, period=per)
+    assert bg == 0 and np.array_equal(lab, bl)
+    if not periodic:  # minimum image needs b < L/2
+        lab, ng = t.FOF(2.0, minnum=8)
+        assert ng == 1 and np.all(lab == 1)
+    for n_small in (1, 5, 16, 17, 31):
+        say(f"  fof small n={n_small}")
+        ps = datagen.uniform(n_small, 1.0, 64 + n_small)
+        ts = nb.KDTree(ps, kerntype=nb.KSPH, period=per)
+        b = 0.3
+        lab, ng = ts.FOF(b, minnum=2)
+        # up to the bucket size (16) the reference's root is a leaf, whose
+        # periodic search is the plain one (KDLeafNode.cxx:841-844)
+        bl, bg = oracle.fof_brute(ps, b, 2, period=per if n_small > 16 else None)
+        assert ng == bg and np.array_equal(lab, bl)
+        raw, _ = oracle.Tree(ps, b=16, period=per).fof(b, 2)
+        assert np.array_equal(lab, oracle.fof_canonical(raw, 2)[0])

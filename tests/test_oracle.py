@@ -135,6 +135,20 @@ def test_root_leaf_periodic_edge():
     assert np.array_equal(np.sort(ids, 1), np.sort(bi[:, 1:], 1))
 
 
+def test_root_leaf_periodic_fof():
+    """N <= b: the root is a leaf, whose periodic FOF search is the plain one
+    (KDLeafNode.cxx:841-844); one particle more and the images count.  The
+    seeds are ones where the two answers differ."""
+    for n, per_used in ((16, False), (17, True)):
+        pos = datagen.uniform(n, 1.0, 0)
+        raw, _ = oracle.Tree(pos, b=16, period=1.0).fof(0.3, 2)
+        can, _ = oracle.fof_canonical(raw, 2)
+        bp, _ = oracle.fof_brute(pos, 0.3, 2, period=1.0)
+        bn, _ = oracle.fof_brute(pos, 0.3, 2, period=None)
+        assert not np.array_equal(bp, bn)
+        assert np.array_equal(can, bp if per_used else bn)
+
+
 def test_phase_knn_vs_brute():
     pos, vel, _ = datagen.clustered(3000, 100.0, nhalo=10, seed=10, with_vel=True)
     ph_pos = pos / pos.std(0)
