@@ -722,8 +722,11 @@ __device__ __forceinline__ void mix_task(const DevTree &T, const KnnArgs &A, con
 // MX_BLKW(KMAX) words (16-bit handles by position and the visit table; for
 // KMAX > 33 slots by position and slots and keys in ascending order) for its
 // whole life, so the scratch is sized by the
-// resident workgroups and only ever touched from one XCD.  Every workgroup
-// exits once its counter passes the share's end.
+// resident workgroups and only ever touched from one XCD.  Once its own
+// share is drained a workgroup takes waves from the other XCDs' counters in
+// turn (each ticket names one wave, so every wave runs once), which evens out
+// shares of unequal cost; it exits when all eight counters have passed their
+// shares' ends.
 #define MX_CTR_STRIDE 16  // XCD counters 64 bytes apart
 // scratch words per workgroup: 16-bit handles + visit table, or 3 regions of slots/keys
 #define MX_BLKW(KM) ((KM) <= 33 ? (KM) * (NBX_WAVE / 2) + MX_V : 3 * (KM) * NBX_WAVE)
@@ -744,13 +747,16 @@ __global__ __launch_bounds__(64) void k_knn_mixed(DevTree T, KnnArgs A, const do
     uint32_t *slb = sl + KMAX * NBX_WAVE, *skc = slb + KMAX * NBX_WAVE;
     const int nw = (T.n + NBX_WAVE - 1) / NBX_WAVE;
     const int x = blockIdx.x & 7, q8 = nw >> 3, r8 = nw & 7;
-    const int wbeg = x * q8 + (x < r8 ? x : r8), wcnt = q8 + (x < r8 ? 1 : 0);
-    for (;;) {
-        int t = 0;
-        if (lane == 0) t = atomicAdd(ctr + x * MX_CTR_STRIDE, 1);
-        t = __builtin_amdgcn_readfirstlane(__shfl(t, 0));
-        if (t >= wcnt) break;
-        mix_task<ND, KMAX>(T, A, ext, fb_list, fb_count, hk, cx, racc, sl, slb, skc, wbeg + t, lane);
+    for (int s = 0; s < 8; ++s) {
+        const int y = (x + s) & 7;
+        const int ybeg = y * q8 + (y < r8 ? y : r8), ycnt = q8 + (y < r8 ? 1 : 0);
+        for (;;) {
+            int t = 0;
+            if (lane == 0) t = atomicAdd(ctr + y * MX_CTR_STRIDE, 1);
+            t = __builtin_amdgcn_readfirstlane(__shfl(t, 0));
+            if (t >= ycnt) break;
+            mix_task<ND, KMAX>(T, A, ext, fb_list, fb_count, hk, cx, racc, sl, slb, skc, ybeg + t, lane);
+        }
     }
 }
 
