@@ -18,7 +18,8 @@ oracle's OpenMP restatement of the reference, timed on a bounded sample).
 Other measured workloads (--workload), each printing its own JSON line:
   phase6d  N=16.8M clustered particles with velocities, metric (x/sigma_x,
            v/sigma_v): 6-D phase-space tree, k=64 kNN + 6-D SPH density,
-           plus 3-D FOF at b=0.2 mean spacing on the same particles.
+           plus 3-D FOF at b=0.2 mean spacing on the same particles and
+           FOFCriterion with FOF6d on the phase-space tree.
   adjoint  N=1M clustered, lognormal masses: SPH density over a frozen k=32
            list and its hand-written adjoint (d sum log rho / d x, d m), in
            input (ID) order and in tree order; list memory vs recompute time.
@@ -345,6 +346,21 @@ def run_phase6d(args):
         fts.append(tf.ms)
     fms = _median(fts)
     log(f"3-D FOF b={b:.5f}: {fms:.1f} ms, {ng} groups")
+    del t3
+    # FOFCriterion with FOF6d on the scaled phase-space tree: 0.2 mean spacing
+    # in the scaled positions, 0.1 in the unit-variance velocities
+    t6 = DeviceTree(xs, vel=vs, bucket_size=16, treetype=nb.TPHS, kerntype=nb.KSPH, aniso=-1)
+    s6 = (0.2 * datagen.mean_spacing(n, L) / float(np.mean(sx))) ** 2
+    prm6 = [nb.TPHS, s6, 0.01, 0, 0, 0, s6, 0.01]
+    t6.fof_criterion(nb.FOF_CMP_6D, prm6, 8, lab)
+    f6 = []
+    for _ in range(3):
+        with Timer(torch) as tf:
+            ng6 = t6.fof_criterion(nb.FOF_CMP_6D, prm6, 8, lab)
+        f6.append(tf.ms)
+    f6ms = _median(f6)
+    log(f"FOF6d (FOFCriterion) params[6]={s6:.3e} params[7]=0.01: {f6ms:.1f} ms, {ng6} groups")
+    del t6
     bpq = 48 + 8 + 4 * k + 16
     ach = bpq * n / (kern_ms / 1e3) / 1e9
     _line(f"Mqueries/s 6-D kNN (k={k}) + 6-D SPH density, N={n}, precision form", n / (ms / 1e3) / 1e6,
@@ -354,7 +370,7 @@ def run_phase6d(args):
               "plain_fp64_mqps": round(n / (plain_ms / 1e3) / 1e6, 3), "mixed_vs_plain": round(plain_ms / ms, 3),
               "mixed_density_max_rel_vs_fp64": rel, "mixed_fp64_fallback_queries": fb,
               "build_ms": round(build_ms, 2), "fof_ms": round(fms, 2), "fof_mparticles_s": round(n / fms / 1e3, 3),
-              "fof_groups": ng,
+              "fof_groups": ng, "fof6d_criterion_ms": round(f6ms, 2), "fof6d_criterion_groups": ng6,
               "roofline": {"bound": "hbm", "achieved": round(ach, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                            "frac": round(ach / PEAK_HBM_GBS, 5), "traffic": None, "kernel": kname,
                            "kernel_ms": round(kern_ms, 3), "bytes_per_query": bpq},

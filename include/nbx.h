@@ -146,6 +146,21 @@ int nbx_ball_mark(nbx_tree *t, const double *x, int64_t exclude_id, double r2, i
  * its heap tie order). */
 int nbx_fof(nbx_tree *t, double fdist, int32_t minnum, int32_t *labels_by_id, int64_t *ngroups);
 int nbx_fof_dev(nbx_tree *t, double fdist, int32_t minnum, int32_t *labels_by_id_dev, int64_t *ngroups, void *stream);
+/* FOF with a built-in comparator.  Replaces KDTree::FOFCriterion(cmp, params,
+ * numgroups, minnum, order) KDFOF.cxx:157-265 for cmp = FOF3d or FOF6d
+ * (FOFFunc.h:30-55), periodic when the tree is (KDSplitNode.cxx:1649-1680).
+ * params[8] as the reference's: [0] treetype, [1] / [2] search scales for
+ * positions / velocities, [6] / [7] comparator scales.  FOF3d: TPHYS tree,
+ * sum_j dx_j^2 / params[6] < 1; FOF6d: TPHS tree, sum_j (dx_j^2 / params[6] +
+ * dv_j^2 / params[7]) < 1.  The search scales must be >= the comparator's
+ * (else the reference's groups depend on its traversal); NBX_ERR_UNSUPPORTED
+ * otherwise and for any other comparator.  Labels as nbx_fof. */
+#define NBX_FOF_CMP_3D 1
+#define NBX_FOF_CMP_6D 2
+int nbx_fof_criterion(nbx_tree *t, int32_t comparator, const double *params, int32_t minnum, int32_t *labels_by_id,
+                      int64_t *ngroups);
+int nbx_fof_criterion_dev(nbx_tree *t, int32_t comparator, const double *params, int32_t minnum,
+                          int32_t *labels_by_id_dev, int64_t *ngroups, void *stream);
 
 /* Differentiation form (no reference counterpart: the reference allocates
  * derKernel but never fills it, KDTree.cxx:1150).  Symmetric SPH density with

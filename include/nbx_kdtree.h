@@ -8,6 +8,7 @@
  *   FindNearest*  KDTree.h:300-331      SearchBallPosTagged  KDTree.h:363-365
  *   SearchBallPos KDTree.h:342, :348
  *   CalcDensity*  KDTree.h:411-416      FOF                  KDTree.h:480-482
+ *   FOFCriterion  KDTree.h:503 with the comparators of FOFFunc.h:24-55
  *   OverWriteInputOrder / SetResetOrder KDTree.h:515-518
  *
  * Semantics kept from the reference: the constructor overwrites Particle::id
@@ -81,6 +82,40 @@ class Particle {
     void SetPotential(const Double_t &p) { phi = p; }
 };
 
+// FOFFunc.h:24-55: comparator and check-function types and the built-in
+// functions.  A comparator links a and b when it returns nonzero; params[6]
+// and params[7] are the position and velocity scales (squared linking
+// lengths).  KDTree::FOFCriterion runs FOF3d and FOF6d on the GPU; it
+// recognises them by address, so pass these functions themselves.
+typedef int (*FOFcompfunc)(Particle &, Particle &, Double_t *);
+typedef int (*FOFcheckfunc)(Particle &, Double_t *);
+inline int FOF3d(Particle &a, Particle &b, Double_t *params) {
+    Double_t total = 0;
+    for (int j = 0; j < 3; j++) {
+        const Double_t t = a.GetPosition(j) - b.GetPosition(j);
+        total += t * t / params[6];
+    }
+    return total < 1;
+}
+inline int FOFVel(Particle &a, Particle &b, Double_t *params) {
+    Double_t total = 0;
+    for (int j = 0; j < 3; j++) {
+        const Double_t t = a.GetVelocity(j) - b.GetVelocity(j);
+        total += t * t / params[6];
+    }
+    return total < 1;
+}
+inline int FOF6d(Particle &a, Particle &b, Double_t *params) {
+    Double_t total = 0;
+    for (int j = 0; j < 3; j++) {
+        const Double_t t = a.GetPosition(j) - b.GetPosition(j), u = a.GetVelocity(j) - b.GetVelocity(j);
+        total += t * t / params[6];
+        total += u * u / params[7];
+    }
+    return total < 1;
+}
+inline int Pnocheck(Particle &, Double_t *) { return 0; }
+
 class KDTree {
   public:
     const static int TPHYS = 0, TPROJ = 1, TVEL = 2, TPHS = 3, TMETRIC = 4;
@@ -139,6 +174,13 @@ class KDTree {
 
     Int_t *FOF(Double_t fdist, Int_t &numgroup, Int_t minnum = 8, int order = 0, Int_tree_t *pHead = nullptr,
                Int_tree_t *pNext = nullptr, Int_tree_t *pTail = nullptr, Int_tree_t *pLen = nullptr);
+    // KDTree.h:503, KDFOF.cxx:157-265: cmp = FOF3d (TPHYS tree) or FOF6d (TPHS
+    // tree) with params as the reference's; any other comparator, a check
+    // function other than Pnocheck, or the linked-list outputs are reported
+    // as unsupported.  Labels are canonical, as FOF's.
+    Int_t *FOFCriterion(FOFcompfunc cmp, Double_t *params, Int_t &numgroups, Int_t minnum = 8, int order = 0,
+                        int ipcheckflag = 0, FOFcheckfunc check = Pnocheck, Int_tree_t *pHead = nullptr,
+                        Int_tree_t *pNext = nullptr, Int_tree_t *pTail = nullptr, Int_tree_t *pLen = nullptr);
 
     void OverWriteInputOrder();
     void SetResetOrder(bool);

@@ -23,6 +23,7 @@ FORM_PLAIN, FORM_MIXED = 0, 1
 KNN_FINDNEAREST, KNN_FINDNEARESTPOS, KNN_FINDNEARESTPHASE, KNN_NONPERIODIC = 0, 1, 2, 3
 LAYOUT_BY_ID, LAYOUT_BY_SLOT = 0, 1
 DENS_SYMMETRIC, DENS_GATHER, DENS_VEL = 0, 1, 2
+FOF_CMP_3D, FOF_CMP_6D = 1, 2  # the built-in FOFCriterion comparators FOF3d, FOF6d
 
 NBX_OK = 0
 NBX_ERR_ARG, NBX_ERR_HIP, NBX_ERR_UNSUPPORTED, NBX_ERR_NOMEM, NBX_ERR_SEARCH = -1, -2, -3, -4, -5
@@ -30,7 +31,7 @@ NBX_ERR_ARG, NBX_ERR_HIP, NBX_ERR_UNSUPPORTED, NBX_ERR_NOMEM, NBX_ERR_SEARCH = -
 EXPORTS = [
     "nbx_params_default", "nbx_tree_create", "nbx_tree_create_dev", "nbx_tree_destroy", "nbx_tree_order",
     "nbx_tree_size", "nbx_tree_kernel", "nbx_knn", "nbx_knn_dev", "nbx_knn_points", "nbx_density_points", "nbx_density",
-    "nbx_density_dev", "nbx_knn_density_dev", "nbx_ball_points", "nbx_ball_mark", "nbx_fof", "nbx_fof_dev",
+    "nbx_density_dev", "nbx_knn_density_dev", "nbx_ball_points", "nbx_ball_mark", "nbx_fof", "nbx_fof_dev", "nbx_fof_criterion", "nbx_fof_criterion_dev",
     "nbx_sph_density_fwd_dev", "nbx_sph_density_adjoint_dev", "nbx_prof_enable", "nbx_prof_reset", "nbx_prof_get",
     "nbx_tree_last_fallbacks", "nbx_last_error",
 ]
@@ -85,6 +86,8 @@ def lib():
     L.nbx_ball_mark.argtypes = [vp, vp, i64, dbl, i32, vp, vp]
     L.nbx_fof.argtypes = [vp, dbl, i32, vp, P(i64)]
     L.nbx_fof_dev.argtypes = [vp, dbl, i32, vp, P(i64), vp]
+    L.nbx_fof_criterion.argtypes = [vp, i32, vp, i32, vp, P(i64)]
+    L.nbx_fof_criterion_dev.argtypes = [vp, i32, vp, i32, vp, P(i64), vp]
     L.nbx_sph_density_fwd_dev.argtypes = [vp, vp, vp, i64, i32, vp, vp, vp]
     L.nbx_sph_density_adjoint_dev.argtypes = [vp, vp, vp, i64, i32, vp, vp, vp, vp]
     L.nbx_prof_enable.argtypes = [ctypes.c_int]
@@ -281,6 +284,19 @@ class KDTree:
         lab = np.zeros(self.n, dtype=np.int32)
         ng = ctypes.c_int64()
         _check(lib().nbx_fof(self._h, float(fdist), minnum, _p(lab), ctypes.byref(ng)))
+        return lab, ng.value
+
+    def FOFCriterion(self, comparator, params, minnum=8):
+        """KDTree::FOFCriterion(cmp, params, numgroups, minnum) (KDFOF.cxx:157-265) with cmp = FOF3d
+        (FOF_CMP_3D) or FOF6d (FOF_CMP_6D), FOFFunc.h:30-55.  params as the reference's (up to 8 values:
+        [0] treetype, [1]/[2] search scales, [6]/[7] comparator scales).  Canonical labels by ID and the
+        group count."""
+        p = np.zeros(8, dtype=np.float64)
+        v = np.asarray(params, dtype=np.float64).ravel()[:8]
+        p[: v.size] = v
+        lab = np.zeros(self.n, dtype=np.int32)
+        ng = ctypes.c_int64()
+        _check(lib().nbx_fof_criterion(self._h, int(comparator), _p(p), minnum, _p(lab), ctypes.byref(ng)))
         return lab, ng.value
 
 

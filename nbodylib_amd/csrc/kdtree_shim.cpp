@@ -252,4 +252,28 @@ Int_t *KDTree::FOF(Double_t fdist, Int_t &numgroup, Int_t minnum, int, Int_tree_
     return pGroup;
 }
 
+Int_t *KDTree::FOFCriterion(FOFcompfunc cmp, Double_t *params, Int_t &numgroups, Int_t minnum, int,
+                            int ipcheckflag, FOFcheckfunc chk, Int_tree_t *pHead, Int_tree_t *pNext, Int_tree_t *pTail,
+                            Int_tree_t *pLen) {
+    std::lock_guard<std::mutex> lk(mu);
+    const int32_t c = cmp == &FOF3d ? NBX_FOF_CMP_3D : cmp == &FOF6d ? NBX_FOF_CMP_6D : 0;
+    int rc = NBX_OK;
+    std::string why;
+    if (!c) why = "only the built-in comparators FOF3d and FOF6d are provided";
+    else if (ipcheckflag && chk != &Pnocheck) why = "check functions other than Pnocheck are not provided";
+    else if (pHead || pNext || pTail || pLen) why = "the linked-list outputs are not provided";
+    if (!why.empty()) {
+        if (g_throw) throw std::runtime_error("FOFCriterion: " + why);
+        std::fprintf(stderr, "FOFCriterion: %s\n", why.c_str());
+        std::exit(1);
+    }
+    Int_t *pGroup = new Int_t[numparts];
+    int64_t ng = 0;
+    rc = nbx_fof_criterion(tree, c, params, minnum, pGroup, &ng);
+    if (rc < 0) delete[] pGroup;
+    check(rc, "FOFCriterion");
+    numgroups = (Int_t)ng;
+    return pGroup;
+}
+
 }  // namespace NBody

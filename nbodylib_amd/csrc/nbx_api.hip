@@ -635,18 +635,16 @@ extern "C" int nbx_density(nbx_tree *t, int32_t k, int32_t variant, int32_t nsea
 }
 
 // ------------------------------------------------------------------ FOF
-extern "C" int nbx_fof_dev(nbx_tree *t, double fdist, int32_t minnum, int32_t *labels_by_id_dev, int64_t *ngroups,
-                           void *stream) {
-    if (!t || !labels_by_id_dev || !ngroups) return fail(NBX_ERR_ARG, "NULL argument");
-    if (!(fdist >= 0)) return fail(NBX_ERR_ARG, "linking length must be >= 0");
-    hipStream_t st = (hipStream_t)stream;
+static int fof_impl(nbx_tree *t, int metric, const FofLink &link, int32_t minnum, int32_t *labels_by_id_dev,
+                    int64_t *ngroups, hipStream_t st) {
     const int64_t n = t->n;
     int32_t *P = nullptr;
     size_t bytes = sizeof(int32_t) * n + (size_t)t->nnodes + 16;
     HIPCK(hipMallocAsync((void **)&P, bytes, st));
     FofArgs A;
     memset(&A, 0, sizeof(A));
-    A.b2 = fdist * fdist;
+    A.link = link;
+    A.metric = metric;
     A.periodic = t->p.periodic && n > t->p.bucket_size;
     for (int d = 0; d < 3; ++d) A.period[d] = t->p.period[d];
     A.parent = P;
@@ -657,6 +655,57 @@ extern "C" int nbx_fof_dev(nbx_tree *t, double fdist, int32_t minnum, int32_t *l
     hipFreeAsync(P, st);
     if (e != hipSuccess) return hipfail(e, "fof");
     return NBX_OK;
+}
+
+extern "C" int nbx_fof_dev(nbx_tree *t, double fdist, int32_t minnum, int32_t *labels_by_id_dev, int64_t *ngroups,
+                           void *stream) {
+    if (!t || !labels_by_id_dev || !ngroups) return fail(NBX_ERR_ARG, "NULL argument");
+    if (!(fdist >= 0)) return fail(NBX_ERR_ARG, "linking length must be >= 0");
+    FofLink L = {fdist * fdist, 0.0, 0.0};
+    return fof_impl(t, NBX_FOFM_PLAIN, L, minnum, labels_by_id_dev, ngroups, (hipStream_t)stream);
+}
+
+// FOFCriterion's tree search scales offsets by params[1] (positions) and
+// params[2] (velocities) and its comparator by params[6] / params[7].  The
+// groups are the comparator's transitive closure exactly when the search is
+// no narrower than the comparator, which is what the check below requires.
+extern "C" int nbx_fof_criterion_dev(nbx_tree *t, int32_t comparator, const double *params, int32_t minnum,
+                                     int32_t *labels_by_id_dev, int64_t *ngroups, void *stream) {
+    if (!t || !params || !labels_by_id_dev || !ngroups) return fail(NBX_ERR_ARG, "NULL argument");
+    FofLink L = {1.0, params[6], comparator == NBX_FOF_CMP_6D ? params[7] : 0.0};
+    int metric;
+    if (comparator == NBX_FOF_CMP_3D) {
+        if (t->nd != 3 || (int)params[0] != NBX_TPHYS)
+            return fail(NBX_ERR_UNSUPPORTED, "FOF3d runs on a TPHYS tree with params[0] = TPHYS");
+        if (!(params[6] > 0) || !(params[1] >= params[6]))
+            return fail(NBX_ERR_UNSUPPORTED, "FOF3d needs params[6] > 0 and a search scale params[1] >= params[6]");
+        metric = NBX_FOFM_3D;
+    } else if (comparator == NBX_FOF_CMP_6D) {
+        if (t->nd != 6 || (int)params[0] != NBX_TPHS)
+            return fail(NBX_ERR_UNSUPPORTED, "FOF6d runs on a TPHS tree with params[0] = TPHS");
+        if (!(params[6] > 0) || !(params[7] > 0) || !(params[1] >= params[6]) || !(params[2] >= params[7]))
+            return fail(NBX_ERR_UNSUPPORTED,
+                        "FOF6d needs params[6], params[7] > 0 and search scales params[1] >= params[6], "
+                        "params[2] >= params[7]");
+        metric = NBX_FOFM_6D;
+    } else {
+        return fail(NBX_ERR_UNSUPPORTED, "only the built-in comparators FOF3d and FOF6d are provided");
+    }
+    return fof_impl(t, metric, L, minnum, labels_by_id_dev, ngroups, (hipStream_t)stream);
+}
+
+extern "C" int nbx_fof_criterion(nbx_tree *t, int32_t comparator, const double *params, int32_t minnum,
+                                 int32_t *labels_by_id, int64_t *ngroups) {
+    if (!t || !labels_by_id || !ngroups) return fail(NBX_ERR_ARG, "NULL argument");
+    int32_t *d = nullptr;
+    HIPCK(hipMalloc((void **)&d, sizeof(int32_t) * t->n));
+    int rc = nbx_fof_criterion_dev(t, comparator, params, minnum, d, ngroups, nullptr);
+    if (rc == NBX_OK) {
+        hipError_t e = hipMemcpy(labels_by_id, d, sizeof(int32_t) * t->n, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = hipfail(e, "nbx_fof_criterion");
+    }
+    hipFree(d);
+    return rc;
 }
 
 extern "C" int nbx_fof(nbx_tree *t, double fdist, int32_t minnum, int32_t *labels_by_id, int64_t *ngroups) {

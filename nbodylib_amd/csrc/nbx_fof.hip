@@ -23,6 +23,13 @@
 // bound here is summed with that same association, so the bounds stay
 // monotone.  Periodic images shift the positions only (KDSplitNode.cxx:
 // 1609-1641).
+//
+// The same kernels run KDTree::FOFCriterion (KDFOF.cxx:157-265) for the
+// built-in comparators FOF3d and FOF6d (FOFFunc.h:30-55): the pair value is
+// then the comparator's scaled sum, computed in its order, and links when
+// below 1 (fof_val).  Its periodic search (KDSplitNode.cxx:1649-1680) tries
+// all seven images; here they are tried when the root's bound allows a link,
+// which changes no link decision.
 #include "nbx_kernels.h"
 
 // Sum of ND per-dimension squares in the reference's association: one
@@ -40,18 +47,46 @@ __device__ __forceinline__ double fof_sum(const double (&t2)[ND]) {
     }
     return s;
 }
-template <int ND>
-__device__ __forceinline__ double fof_lb(const double *q, cdptr b) {
+
+// Link value of one pair from its per-dimension squares; a pair links when
+// the value is below L.lim.  MT 0: the plain search, d2 against b^2.  MT 1
+// (FOF3d) and 2 (FOF6d): each square divided by its comparator scale
+// (params[6] for positions, params[7] for velocities) and summed in the
+// comparator's order (x0 x1 x2, or x0 v0 x1 v1 x2 v2), against 1.  Every
+// step is monotone, so a box bound run through it stays a bound.
+template <int ND, int MT>
+__device__ __forceinline__ double fof_val(const double (&t2)[ND], const FofLink &L) {
+    if constexpr (MT == 0) {
+        return fof_sum<ND>(t2);
+    } else if constexpr (MT == 1) {
+        double s = 0.0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) s += t2[d] / L.s6;
+        return s;
+    } else {
+        static_assert(ND == 6, "FOF6d links in phase space");
+        double s = 0.0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            s += t2[d] / L.s6;
+            s += t2[3 + d] / L.s7;
+        }
+        return s;
+    }
+}
+
+template <int ND, int MT>
+__device__ __forceinline__ double fof_lb(const double *q, cdptr b, const FofLink &L) {
     double t2[ND];
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
         const double t = fmax(fmax(b[d] - q[d], q[d] - b[ND + d]), 0.0);
         t2[d] = t * t;
     }
-    return fof_sum<ND>(t2);
+    return fof_val<ND, MT>(t2, L);
 }
-template <int ND>
-__device__ __forceinline__ double fof_ub(const double *q, cdptr b) {
+template <int ND, int MT>
+__device__ __forceinline__ double fof_ub(const double *q, cdptr b, const FofLink &L) {
     double t2[ND];
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
@@ -59,10 +94,10 @@ __device__ __forceinline__ double fof_ub(const double *q, cdptr b) {
         const double a2 = a * a, c2 = c * c;
         t2[d] = (a2 > c2) ? a2 : c2;
     }
-    return fof_sum<ND>(t2);
+    return fof_val<ND, MT>(t2, L);
 }
-template <int ND>
-__device__ __forceinline__ double fof_bbub(cdptr a, cdptr b) {
+template <int ND, int MT>
+__device__ __forceinline__ double fof_bbub(cdptr a, cdptr b, const FofLink &L) {
     double t2[ND];
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
@@ -70,18 +105,18 @@ __device__ __forceinline__ double fof_bbub(cdptr a, cdptr b) {
         const double m = u > v ? u : v;
         t2[d] = m * m;
     }
-    return fof_sum<ND>(t2);
+    return fof_val<ND, MT>(t2, L);
 }
 
-template <int ND>
-__global__ __launch_bounds__(256) void k_fof_leafinit(DevTree T, double b2, int32_t *P, uint8_t *conn) {
-    int32_t L = blockIdx.x * 256 + threadIdx.x;
-    if (L >= T.nleaf) return;
-    const int node = T.nleafp + L;
+template <int ND, int MT>
+__global__ __launch_bounds__(256) void k_fof_leafinit(DevTree T, FofLink L, int32_t *P, uint8_t *conn) {
+    const int32_t leaf = blockIdx.x * 256 + threadIdx.x;
+    if (leaf >= T.nleaf) return;
+    const int node = T.nleafp + leaf;
     cdptr bx = (cdptr)node_box(T, node, ND);
-    int32_t s0 = L * NBX_LEAF;
+    int32_t s0 = leaf * NBX_LEAF;
     int32_t s1 = s0 + NBX_LEAF < T.n ? s0 + NBX_LEAF : T.n;
-    bool c = fof_bbub<ND>(bx, bx) < b2;
+    bool c = fof_bbub<ND, MT>(bx, bx, L) < L.lim;
     conn[node] = c ? 1 : 0;
     for (int32_t s = s0; s < s1; ++s) P[s] = c ? s0 : s;
 }
@@ -90,14 +125,14 @@ __global__ __launch_bounds__(256) void k_fof_leafinit(DevTree T, double b2, int3
 // diagonals are smaller) and every cross pair links, so one union connects
 // it.  Nodes [first, first+count) of one depth.  An empty box (padding) has
 // an infinite diagonal and is never connected.
-template <int ND>
-__global__ __launch_bounds__(256) void k_fof_nodeinit(DevTree T, double b2, int32_t *P, uint8_t *conn, int32_t first,
+template <int ND, int MT>
+__global__ __launch_bounds__(256) void k_fof_nodeinit(DevTree T, FofLink L, int32_t *P, uint8_t *conn, int32_t first,
                                                        int32_t count) {
     int32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i >= count) return;
     const int node = first + i;
     cdptr bx = (cdptr)node_box(T, node, ND);
-    bool c = fof_bbub<ND>(bx, bx) < b2;
+    bool c = fof_bbub<ND, MT>(bx, bx, L) < L.lim;
     conn[node] = c ? 1 : 0;
     if (!c) return;
     const int right = 2 * node + 1;
@@ -112,13 +147,13 @@ __global__ __launch_bounds__(256) void k_fof_nodeinit(DevTree T, double b2, int3
 // the first pass only toward larger slots: the pair is met from both sides).
 // rq is a hint for the root of qi (any former root of its set), kept across
 // unions so each find starts near the top.
-template <int PASS0, int ND>
+template <int PASS0, int ND, int MT>
 __device__ __forceinline__ void fof_leaf(const DevTree &T, int leaf, const double *q, bool act, int32_t qi,
-                                         double b2, int32_t *P, const uint8_t *conn, int32_t &rq) {
+                                         const FofLink &L, int32_t *P, const uint8_t *conn, int32_t &rq) {
     const int base = leaf * NBX_LEAF;
     const bool lconn = conn[T.nleafp + leaf];
     cdptr bx = (cdptr)node_box(T, T.nleafp + leaf, ND);
-    const bool encl = act && lconn && fof_ub<ND>(q, bx) < b2;
+    const bool encl = act && lconn && fof_ub<ND, MT>(q, bx, L) < L.lim;
     if (encl) rq = uf_union_r(P, rq, base);
     const bool pair = act && !encl;
     if (!ballot(pair)) return;
@@ -140,9 +175,9 @@ __device__ __forceinline__ void fof_leaf(const DevTree &T, int leaf, const doubl
                 const double t = q[d] - cv[d][u];
                 t2[d] = t * t;
             }
-            const double d2 = fof_sum<ND>(t2);
+            const double d2 = fof_val<ND, MT>(t2, L);
             const int32_t cs = base + jj + u;
-            const bool l = d2 < b2 && (PASS0 ? (lconn ? cs != qi : cs > qi) : cs != qi);
+            const bool l = d2 < L.lim && (PASS0 ? (lconn ? cs != qi : cs > qi) : cs != qi);
             mask |= l ? (1u << (jj + u)) : 0u;
         }
     }
@@ -158,9 +193,9 @@ __device__ __forceinline__ void fof_leaf(const DevTree &T, int leaf, const doubl
     }
 }
 
-template <int PASS0, int ND>
-__device__ __forceinline__ void fof_walk(const DevTree &T, const double *q, bool act, int32_t qi, int g, double b2,
-                                         int32_t *P, uint8_t *conn, int32_t &rq) {
+template <int PASS0, int ND, int MT>
+__device__ __forceinline__ void fof_walk(const DevTree &T, const double *q, bool act, int32_t qi, int g,
+                                         const FofLink &L, int32_t *P, uint8_t *conn, int32_t &rq) {
     const int D = T.depth;
     // the wave's own 64 particles form the parent of leaves 2g, 2g+1 (or the
     // single leaf when the tree has one)
@@ -171,9 +206,9 @@ __device__ __forceinline__ void fof_walk(const DevTree &T, const double *q, bool
     for (;;) {
         if (down) {
             cdptr bx = (cdptr)node_box(T, node, ND);
-            bool want = act && fof_lb<ND>(q, bx) < b2;
+            bool want = act && fof_lb<ND, MT>(q, bx, L) < L.lim;
             if (ballot(want)) {
-                if (PASS0 && fof_bbub<ND>(gb, bx) < b2) {
+                if (PASS0 && fof_bbub<ND, MT>(gb, bx, L) < L.lim) {
                     // every wave particle links to every particle of the node:
                     // wave and node form one component
                     const int32_t nfirst = node_first(T, node);
@@ -185,7 +220,7 @@ __device__ __forceinline__ void fof_walk(const DevTree &T, const double *q, bool
                     }
                     down = false;
                 } else if (node >= T.nleafp) {
-                    fof_leaf<PASS0, ND>(T, node - T.nleafp, q, act, qi, b2, P, conn, rq);
+                    fof_leaf<PASS0, ND, MT>(T, node - T.nleafp, q, act, qi, L, P, conn, rq);
                     down = false;
                 } else {
                     node = 2 * node;
@@ -205,7 +240,7 @@ __device__ __forceinline__ void fof_walk(const DevTree &T, const double *q, bool
     }
 }
 
-template <int ND>
+template <int ND, int MT>
 __global__ __launch_bounds__(64) void k_fof(DevTree T, FofArgs A, uint8_t *conn) {
     const int lane = threadIdx.x;
     const int g = blockIdx.x;
@@ -215,9 +250,11 @@ __global__ __launch_bounds__(64) void k_fof(DevTree T, FofArgs A, uint8_t *conn)
 #pragma unroll
     for (int d = 0; d < ND; ++d) q0[d] = valid ? T.x[d][qi] : 0.0;
     int32_t rq = valid ? qi : 0;
-    fof_walk<1, ND>(T, q0, valid, qi, g, A.b2, A.parent, conn, rq);
+    const FofLink L = A.link;
+    fof_walk<1, ND, MT>(T, q0, valid, qi, g, L, A.parent, conn, rq);
     if (!A.periodic) return;
-    // FOFSearchBallPeriodic: image toward the far wall when b^2 > s^2
+    // FOFSearchBallPeriodic: image toward the far wall when b^2 > s^2;
+    // FOFSearchCriterionPeriodic: every image, left to the walk's bounds
     const double *per = A.period;
     for (int pass = 1; pass < 8; ++pass) {
         double xp[ND];
@@ -238,9 +275,9 @@ __global__ __launch_bounds__(64) void k_fof(DevTree T, FofArgs A, uint8_t *conn)
             }
             sval = sqrt(sval);
         }
-        bool act = valid && A.b2 > sval * sval;
+        bool act = valid && (MT != 0 || L.lim > sval * sval);
         if (!ballot(act)) continue;
-        fof_walk<0, ND>(T, xp, act, qi, g, A.b2, A.parent, conn, rq);
+        fof_walk<0, ND, MT>(T, xp, act, qi, g, L, A.parent, conn, rq);
     }
 }
 
@@ -300,28 +337,28 @@ __global__ __launch_bounds__(256) void k_fof_out(const int32_t *P, const int32_t
 
 static inline int nblk(int64_t n) { return (int)((n + 255) / 256); }
 
+template <int ND, int MT>
+static void launch_fof_t(const DevTree &T, const FofArgs &A, uint8_t *conn, hipStream_t st) {
+    {
+        ProfScope ps("k_fof_init", st);
+        hipLaunchKernelGGL((k_fof_leafinit<ND, MT>), dim3(nblk(T.nleaf)), dim3(256), 0, st, T, A.link, A.parent, conn);
+        for (int dep = T.depth - 1; dep >= 0; --dep)
+            hipLaunchKernelGGL((k_fof_nodeinit<ND, MT>), dim3(nblk((int64_t)1 << dep)), dim3(256), 0, st, T, A.link,
+                               A.parent, conn, 1 << dep, 1 << dep);
+    }
+    const int groups = (T.n + NBX_WAVE - 1) / NBX_WAVE;
+    ProfScope ps("k_fof_link", st);
+    hipLaunchKernelGGL((k_fof<ND, MT>), dim3(groups), dim3(NBX_WAVE), 0, st, T, A, conn);
+}
+
 hipError_t nbx_launch_fof(const DevTree &T, const FofArgs &A, hipStream_t st) {
     // per-node connectivity flags live right after the parent array (allocated by the caller)
     uint8_t *conn = (uint8_t *)(A.parent + T.n);
-    {
-        ProfScope ps("k_fof_init", st);
-        if (T.nd == 3)
-            hipLaunchKernelGGL(k_fof_leafinit<3>, dim3(nblk(T.nleaf)), dim3(256), 0, st, T, A.b2, A.parent, conn);
-        else
-            hipLaunchKernelGGL(k_fof_leafinit<6>, dim3(nblk(T.nleaf)), dim3(256), 0, st, T, A.b2, A.parent, conn);
-        for (int dep = T.depth - 1; dep >= 0; --dep) {
-            if (T.nd == 3)
-                hipLaunchKernelGGL(k_fof_nodeinit<3>, dim3(nblk((int64_t)1 << dep)), dim3(256), 0, st, T, A.b2,
-                                   A.parent, conn, 1 << dep, 1 << dep);
-            else
-                hipLaunchKernelGGL(k_fof_nodeinit<6>, dim3(nblk((int64_t)1 << dep)), dim3(256), 0, st, T, A.b2,
-                                   A.parent, conn, 1 << dep, 1 << dep);
-        }
-    }
-    int groups = (T.n + NBX_WAVE - 1) / NBX_WAVE;
-    ProfScope ps("k_fof_link", st);
-    if (T.nd == 3) hipLaunchKernelGGL(k_fof<3>, dim3(groups), dim3(NBX_WAVE), 0, st, T, A, conn);
-    else hipLaunchKernelGGL(k_fof<6>, dim3(groups), dim3(NBX_WAVE), 0, st, T, A, conn);
+    if (A.metric == NBX_FOFM_PLAIN && T.nd == 3) launch_fof_t<3, NBX_FOFM_PLAIN>(T, A, conn, st);
+    else if (A.metric == NBX_FOFM_PLAIN && T.nd == 6) launch_fof_t<6, NBX_FOFM_PLAIN>(T, A, conn, st);
+    else if (A.metric == NBX_FOFM_3D && T.nd == 3) launch_fof_t<3, NBX_FOFM_3D>(T, A, conn, st);
+    else if (A.metric == NBX_FOFM_6D && T.nd == 6) launch_fof_t<6, NBX_FOFM_6D>(T, A, conn, st);
+    else return hipErrorInvalidValue;
     return hipGetLastError();
 }
 

@@ -78,8 +78,19 @@ hipError_t nbx_launch_knn_mixed(const DevTree &T, const KnnArgs &A, const double
                                 int32_t *fallback_count, hipStream_t st);
 
 // ------------------------------------------------------------------ FOF
+// Link rule of a FOF run: pairs link when their value is below lim.  Plain
+// FOF: d2 against b^2.  FOFCriterion with FOF3d / FOF6d: the comparator's sum
+// of squares divided by s6 (positions) and s7 (velocities), against 1.
+#define NBX_FOFM_PLAIN 0
+#define NBX_FOFM_3D 1
+#define NBX_FOFM_6D 2
+struct FofLink {
+    double lim;
+    double s6, s7;
+};
 struct FofArgs {
-    double b2;
+    FofLink link;
+    int32_t metric;  // NBX_FOFM_*
     int32_t periodic;
     double period[3];
     int32_t *parent;

@@ -92,6 +92,19 @@ class DeviceTree:
         _check(lib().nbx_fof_dev(self._h, float(fdist), minnum, _ptr(labels_out), ctypes.byref(ng), _stream()))
         return ng.value
 
+    def fof_criterion(self, comparator, params, minnum, labels_out):
+        """FOFCriterion with FOF3d / FOF6d (nbx_fof_criterion_dev); params as the reference's (8 values)."""
+        import numpy as np
+
+        _need(labels_out, "labels_out", torch.int32, (self.n,))
+        p = np.zeros(8, dtype=np.float64)
+        v = np.asarray(params, dtype=np.float64).ravel()[:8]
+        p[: v.size] = v
+        ng = ctypes.c_int64()
+        _check(lib().nbx_fof_criterion_dev(self._h, int(comparator), p.ctypes.data_as(ctypes.c_void_p), minnum,
+                                           _ptr(labels_out), ctypes.byref(ng), _stream()))
+        return ng.value
+
 
 def _check_list(nbr, n):
     if nbr.dtype != torch.int32 or not nbr.is_cuda or nbr.dim() != 2 or nbr.shape[0] != n:

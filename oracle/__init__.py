@@ -72,6 +72,8 @@ def lib():
         L.orc_fof_brute.restype = _i64
         L.orc_fof_brute_nd.argtypes = [_dp, _i64, _ci, _cd, _ci, ctypes.c_void_p, _ip]
         L.orc_fof_brute_nd.restype = _i64
+        L.orc_fof_criterion_brute.argtypes = [_dp, _i64, _ci, _dp, _ci, ctypes.c_void_p, _ip]
+        L.orc_fof_criterion_brute.restype = _i64
         L.orc_ball_mark.argtypes = [_dp, _i64, _dp, _i64, _cd, ctypes.c_void_p, _ci, _ip, _dp]
         L.orc_ball_mark.restype = None
         L.orc_density_analytic.argtypes = [_dp, _dp, _i64, _ip, _ci, _dp, ctypes.c_void_p]
@@ -248,6 +250,19 @@ def fof_brute(pos, b, minnum=8, period=None, vel=None):
     lab = np.zeros(n, dtype=np.int32)
     per = None if period is None else np.ascontiguousarray(np.broadcast_to(period, (3,)), dtype=np.float64)
     g = lib().orc_fof_brute_nd(ph, n, 6 if vel is not None else 3, b, minnum, _ptr(per), lab)
+    return lab, g
+
+
+def fof_criterion_brute(pos, cmp, params, minnum=8, period=None, vel=None):
+    """All-pairs FOFCriterion with FOF3d (cmp 1) or FOF6d (cmp 2, needs vel); canonical labels."""
+    ph = _phase(pos, vel)
+    n = ph.shape[0]
+    p = np.zeros(8, dtype=np.float64)
+    v = np.asarray(params, dtype=np.float64).ravel()[:8]
+    p[: v.size] = v
+    lab = np.zeros(n, dtype=np.int32)
+    per = None if period is None else np.ascontiguousarray(np.broadcast_to(period, (3,)), dtype=np.float64)
+    g = lib().orc_fof_criterion_brute(ph, n, cmp, p, minnum, _ptr(per), lab)
     return lab, g
 
 

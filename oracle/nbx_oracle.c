@@ -1188,6 +1188,60 @@ int64_t orc_fof_brute_nd(const double *phase, int64_t n, int nd, double b, int m
     return G;
 }
 
+/* The built-in FOFCriterion comparators (FOFFunc.h:30-55) on phase rows:
+ * FOF3d (cmp 1) sums (x_j - y_j)^2 / p[6] over j; FOF6d (cmp 2) adds
+ * (v_j - w_j)^2 / p[7] after each position term; both link when the total is
+ * below 1. */
+static int cmp_link(const double *a, const double *b, int cmp, const double *p) {
+    double total = 0;
+    for (int j = 0; j < 3; j++) {
+        total += (a[j] - b[j]) * (a[j] - b[j]) / p[6];
+        if (cmp == 2) total += (a[3 + j] - b[3 + j]) * (a[3 + j] - b[3 + j]) / p[7];
+    }
+    return total < 1;
+}
+/* FOFSearchCriterionPeriodic (KDSplitNode.cxx:1649-1680): the target, then its
+ * seven images (positions moved by PeriodicReflection*, velocities kept), all
+ * tried without a distance test. */
+static int cmp_link_dir(const double *a, const double *b, int cmp, const double *p, const double *period) {
+    if (cmp_link(a, b, cmp, p)) return 1;
+    if (!period) return 0;
+    double xp[6];
+    for (int j = 3; j < 6; j++) xp[j] = a[j];
+    for (int k = 0; k < 3; k++) {
+        refl1(a, xp, period, k);
+        if (cmp_link(xp, b, cmp, p)) return 1;
+    }
+    const int pairs[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+    for (int c = 0; c < 3; c++) {
+        refl2(a, xp, period, pairs[c][0], pairs[c][1]);
+        if (cmp_link(xp, b, cmp, p)) return 1;
+    }
+    reflN(a, xp, period, 3);
+    return cmp_link(xp, b, cmp, p);
+}
+
+/* KDTree::FOFCriterion (KDFOF.cxx:157-265) restated as the transitive closure
+ * of the comparator over all pairs, canonical labels.  The reference's tree
+ * search reaches every such pair when its search scales params[1] / params[2]
+ * are no smaller than the comparator's params[6] / params[7]. */
+int64_t orc_fof_criterion_brute(const double *phase, int64_t n, int cmp, const double *params, int minnum,
+                                const double *period, int32_t *labels) {
+    int64_t *par = (int64_t *)malloc(sizeof(int64_t) * (size_t)n);
+    for (int64_t i = 0; i < n; i++) par[i] = i;
+    for (int64_t i = 0; i < n; i++)
+        for (int64_t j = i + 1; j < n; j++) {
+            const double *a = phase + 6 * i, *c = phase + 6 * j;
+            if (cmp_link_dir(a, c, cmp, params, period) || (period && cmp_link_dir(c, a, cmp, params, period)))
+                uf_union(par, i, j);
+        }
+    int32_t *raw = (int32_t *)malloc(sizeof(int32_t) * (size_t)n);
+    for (int64_t i = 0; i < n; i++) raw[i] = (int32_t)(uf_find(par, i) + 1);
+    int64_t G = orc_fof_canonical(n, raw, minnum, labels);
+    free(par); free(raw);
+    return G;
+}
+
 int64_t orc_fof_brute(const double *phase, int64_t n, double b, int minnum, const double *period, int32_t *labels) {
     return orc_fof_brute_nd(phase, n, 3, b, minnum, period, labels);
 }

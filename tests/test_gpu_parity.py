@@ -145,6 +145,68 @@ def test_fof6d_vs_brute(gpu, periodic):
     assert np.array_equal(lab, ol)
 
 
+@pytest.mark.parametrize("periodic", [False, True])
+def test_fof_criterion_3d(gpu, periodic):
+    """FOFCriterion with FOF3d on a TPHYS tree: the comparator's closure,
+    labels bit-identical; a wider search scale changes nothing."""
+    nb = _nb()
+    n = 6000
+    pos = datagen.uniform(n, 1.0, 35)
+    per = 1.0 if periodic else None
+    b2 = (0.6 * datagen.mean_spacing(n, 1.0)) ** 2
+    say(f"fof criterion 3d n={n} periodic={periodic}")
+    t = nb.KDTree(pos, kerntype=nb.KSPH, period=per)
+    prm = [nb.TPHYS, b2, 0, 0, 0, 0, b2]
+    lab, ng = t.FOFCriterion(nb.FOF_CMP_3D, prm, 8)
+    ol, og = oracle.fof_criterion_brute(pos, 1, prm, 8, period=per)
+    assert og > 5 and ng == og and np.array_equal(lab, ol)
+    lab2, _ = t.FOFCriterion(nb.FOF_CMP_3D, [nb.TPHYS, 4 * b2, 0, 0, 0, 0, b2], 8)
+    assert np.array_equal(lab2, lab)
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_fof_criterion_6d(gpu, periodic):
+    """FOFCriterion with FOF6d on a TPHS tree, unequal position and velocity
+    scales: labels bit-identical to the comparator's closure."""
+    nb = _nb()
+    n = 6000
+    pos = datagen.uniform(n, 1.0, 36)
+    vel = np.random.default_rng(37).random((n, 3)) * 0.3
+    per = 1.0 if periodic else None
+    b6 = 0.85 * (0.3 ** 3 / n) ** (1 / 6)
+    sx, sv = 0.7 * b6 * b6, 1.5 * b6 * b6
+    say(f"fof criterion 6d n={n} periodic={periodic}")
+    t = nb.KDTree(pos, vel=vel, treetype=nb.TPHS, aniso=-1, kerntype=nb.KSPH, period=per)
+    prm = [nb.TPHS, sx, sv, 0, 0, 0, sx, sv]
+    lab, ng = t.FOFCriterion(nb.FOF_CMP_6D, prm, 8)
+    ol, og = oracle.fof_criterion_brute(pos, 2, prm, 8, period=per, vel=vel)
+    assert og > 5 and ng == og and np.array_equal(lab, ol)
+
+
+def test_fof_criterion_clustered_and_rejected(gpu):
+    """FOF6d on clustered phase-space data (halo cores link through both
+    terms), then the cases the call reports as unsupported: a comparator on
+    the wrong tree type, a search scale narrower than the comparator's, and
+    an unknown comparator."""
+    nb = _nb()
+    n = 12000
+    pos, vel, _ = datagen.clustered(n, 1.0, nhalo=12, seed=38, with_vel=True)
+    vel = vel / vel.std()
+    sx, sv = (0.3 * datagen.mean_spacing(n, 1.0)) ** 2, 0.05
+    say(f"fof criterion 6d clustered n={n}")
+    t6 = nb.KDTree(pos, vel=vel, treetype=nb.TPHS, aniso=-1, kerntype=nb.KSPH)
+    prm = [nb.TPHS, sx, sv, 0, 0, 0, sx, sv]
+    lab, ng = t6.FOFCriterion(nb.FOF_CMP_6D, prm, 8)
+    ol, og = oracle.fof_criterion_brute(pos, 2, prm, 8, vel=vel)
+    assert og > 5 and ng == og and np.array_equal(lab, ol)
+    t3 = nb.KDTree(pos, kerntype=nb.KSPH)
+    for tree, cmp, q in ((t3, nb.FOF_CMP_6D, prm), (t6, nb.FOF_CMP_3D, [nb.TPHYS, sx, 0, 0, 0, 0, sx]),
+                         (t6, nb.FOF_CMP_6D, [nb.TPHS, 0.5 * sx, sv, 0, 0, 0, sx, sv]),
+                         (t3, 7, [nb.TPHYS, sx, 0, 0, 0, 0, sx])):
+        with pytest.raises(nb.NbxError):
+            tree.FOFCriterion(cmp, q, 8)
+
+
 def test_fof6d_clustered_vs_tree_oracle(gpu):
     nb = _nb()
     n = 32768

@@ -321,6 +321,34 @@ def test_fof6d_tree_vs_brute(periodic):
     assert np.array_equal(can, bl)
 
 
+@pytest.mark.parametrize("periodic", [False, True])
+def test_fof_criterion_vs_plain(periodic):
+    """FOFCriterion's comparators (FOFFunc.h:30-55) against the plain FOF
+    brute force, which is pinned on its own: FOF3d with params[6] = b^2 and
+    FOF6d with params[6] = params[7] = b^2 give the groups of d2 < b^2 in 3-D
+    and 6-D; FOF6d with a vanishing velocity term is FOF3d; and swapping the
+    two scales (a planted mistake) changes the groups."""
+    n = 3000
+    pos = datagen.uniform(n, 1.0, 25)
+    vel = np.random.default_rng(26).random((n, 3)) * 0.3
+    per = 1.0 if periodic else None
+    b = 0.6 * datagen.mean_spacing(n, 1.0)
+    l3, g3 = oracle.fof_criterion_brute(pos, 1, [0, b * b, 0, 0, 0, 0, b * b], 8, period=per)
+    bl, bg = oracle.fof_brute(pos, b, 8, period=per)
+    assert g3 == bg and bg > 5 and np.array_equal(l3, bl)
+    b6 = 0.85 * (0.3 ** 3 / n) ** (1 / 6)
+    l6, g6 = oracle.fof_criterion_brute(pos, 2, [3, b6 * b6, b6 * b6, 0, 0, 0, b6 * b6, b6 * b6], 8, period=per,
+                                        vel=vel)
+    bl6, bg6 = oracle.fof_brute(pos, b6, 8, period=per, vel=vel)
+    assert g6 == bg6 and bg6 > 5 and np.array_equal(l6, bl6)
+    lv, gv = oracle.fof_criterion_brute(pos, 2, [3, b * b, 1e300, 0, 0, 0, b * b, 1e300], 8, period=per, vel=vel)
+    assert gv == g3 and np.array_equal(lv, l3)
+    s6, s7 = b6 * b6 * 0.5, b6 * b6 * 2.0
+    la, _ = oracle.fof_criterion_brute(pos, 2, [3, s6, s7, 0, 0, 0, s6, s7], 8, period=per, vel=vel)
+    lb, _ = oracle.fof_criterion_brute(pos, 2, [3, s7, s6, 0, 0, 0, s7, s6], 8, period=per, vel=vel)
+    assert not np.array_equal(la, lb)
+
+
 def test_fof6d_zero_velocity_is_3d():
     # with all velocities equal the 6-D distance is the 3-D one
     n = 3000
