@@ -661,7 +661,7 @@ extern "C" int nbx_fof_dev(nbx_tree *t, double fdist, int32_t minnum, int32_t *l
                            void *stream) {
     if (!t || !labels_by_id_dev || !ngroups) return fail(NBX_ERR_ARG, "NULL argument");
     if (!(fdist >= 0)) return fail(NBX_ERR_ARG, "linking length must be >= 0");
-    FofLink L = {fdist * fdist, 0.0, 0.0};
+    FofLink L = {fdist * fdist, 0.0, 0.0, 0.0, 0.0};
     return fof_impl(t, NBX_FOFM_PLAIN, L, minnum, labels_by_id_dev, ngroups, (hipStream_t)stream);
 }
 
@@ -672,7 +672,9 @@ extern "C" int nbx_fof_dev(nbx_tree *t, double fdist, int32_t minnum, int32_t *l
 extern "C" int nbx_fof_criterion_dev(nbx_tree *t, int32_t comparator, const double *params, int32_t minnum,
                                      int32_t *labels_by_id_dev, int64_t *ngroups, void *stream) {
     if (!t || !params || !labels_by_id_dev || !ngroups) return fail(NBX_ERR_ARG, "NULL argument");
-    FofLink L = {1.0, params[6], comparator == NBX_FOF_CMP_6D ? params[7] : 0.0};
+    FofLink L = {1.0, params[6], comparator == NBX_FOF_CMP_6D ? params[7] : 1.0, 0.0, 0.0};
+    L.r6 = 1.0 / L.s6;
+    L.r7 = 1.0 / L.s7;
     int metric;
     if (comparator == NBX_FOF_CMP_3D) {
         if (t->nd != 3 || (int)params[0] != NBX_TPHYS)

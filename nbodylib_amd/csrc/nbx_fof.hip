@@ -75,6 +75,47 @@ __device__ __forceinline__ double fof_val(const double (&t2)[ND], const FofLink 
     }
 }
 
+// The comparator sum with each division replaced by a multiplication by the
+// correctly rounded reciprocal.  Term by term that is within 3.01u of the
+// quotient (u = 2^-53), and the two sums, of at most six nonnegative terms
+// added in the same order, are within 13.2u of each other.  So a value below
+// 1 - 2^-46 proves the comparator's value below 1, and one at or above
+// 1 + 2^-46 proves it not; only values in between need the divisions.
+template <int ND, int MT>
+__device__ __forceinline__ double fof_val_fast(const double (&t2)[ND], const FofLink &L) {
+    double s = 0.0;
+    if constexpr (MT == 1) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) s += t2[d] * L.r6;
+    } else {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            s += t2[d] * L.r6;
+            s += t2[3 + d] * L.r7;
+        }
+    }
+    return s;
+}
+#define NBX_FOF_BAND_LO (1.0 - 0x1p-46)
+#define NBX_FOF_BAND_HI (1.0 + 0x1p-46)
+// Box bounds for the comparators use the reciprocal sum too, read through
+// the band: a node is skipped only when its lower bound is surely >= 1
+// (fof_maybe fails), and a wholesale link is taken only when the upper bound
+// is surely < 1 (fof_surely).  Pair decisions stay exact either way.
+template <int ND, int MT>
+__device__ __forceinline__ double fof_bval(const double (&t2)[ND], const FofLink &L) {
+    if constexpr (MT == 0) return fof_sum<ND>(t2);
+    else return fof_val_fast<ND, MT>(t2, L);
+}
+template <int MT>
+__device__ __forceinline__ bool fof_maybe(double v, const FofLink &L) {
+    return MT == 0 ? v < L.lim : v < NBX_FOF_BAND_HI;
+}
+template <int MT>
+__device__ __forceinline__ bool fof_surely(double v, const FofLink &L) {
+    return MT == 0 ? v < L.lim : v < NBX_FOF_BAND_LO;
+}
+
 template <int ND, int MT>
 __device__ __forceinline__ double fof_lb(const double *q, cdptr b, const FofLink &L) {
     double t2[ND];
@@ -83,7 +124,7 @@ __device__ __forceinline__ double fof_lb(const double *q, cdptr b, const FofLink
         const double t = fmax(fmax(b[d] - q[d], q[d] - b[ND + d]), 0.0);
         t2[d] = t * t;
     }
-    return fof_val<ND, MT>(t2, L);
+    return fof_bval<ND, MT>(t2, L);
 }
 template <int ND, int MT>
 __device__ __forceinline__ double fof_ub(const double *q, cdptr b, const FofLink &L) {
@@ -94,7 +135,7 @@ __device__ __forceinline__ double fof_ub(const double *q, cdptr b, const FofLink
         const double a2 = a * a, c2 = c * c;
         t2[d] = (a2 > c2) ? a2 : c2;
     }
-    return fof_val<ND, MT>(t2, L);
+    return fof_bval<ND, MT>(t2, L);
 }
 template <int ND, int MT>
 __device__ __forceinline__ double fof_bbub(cdptr a, cdptr b, const FofLink &L) {
@@ -105,7 +146,7 @@ __device__ __forceinline__ double fof_bbub(cdptr a, cdptr b, const FofLink &L) {
         const double m = u > v ? u : v;
         t2[d] = m * m;
     }
-    return fof_val<ND, MT>(t2, L);
+    return fof_bval<ND, MT>(t2, L);
 }
 
 template <int ND, int MT>
@@ -116,7 +157,7 @@ __global__ __launch_bounds__(256) void k_fof_leafinit(DevTree T, FofLink L, int3
     cdptr bx = (cdptr)node_box(T, node, ND);
     int32_t s0 = leaf * NBX_LEAF;
     int32_t s1 = s0 + NBX_LEAF < T.n ? s0 + NBX_LEAF : T.n;
-    bool c = fof_bbub<ND, MT>(bx, bx, L) < L.lim;
+    bool c = fof_surely<MT>(fof_bbub<ND, MT>(bx, bx, L), L);
     conn[node] = c ? 1 : 0;
     for (int32_t s = s0; s < s1; ++s) P[s] = c ? s0 : s;
 }
@@ -132,7 +173,7 @@ __global__ __launch_bounds__(256) void k_fof_nodeinit(DevTree T, FofLink L, int3
     if (i >= count) return;
     const int node = first + i;
     cdptr bx = (cdptr)node_box(T, node, ND);
-    bool c = fof_bbub<ND, MT>(bx, bx, L) < L.lim;
+    bool c = fof_surely<MT>(fof_bbub<ND, MT>(bx, bx, L), L);
     conn[node] = c ? 1 : 0;
     if (!c) return;
     const int right = 2 * node + 1;
@@ -153,7 +194,7 @@ __device__ __forceinline__ void fof_leaf(const DevTree &T, int leaf, const doubl
     const int base = leaf * NBX_LEAF;
     const bool lconn = conn[T.nleafp + leaf];
     cdptr bx = (cdptr)node_box(T, T.nleafp + leaf, ND);
-    const bool encl = act && lconn && fof_ub<ND, MT>(q, bx, L) < L.lim;
+    const bool encl = act && lconn && fof_surely<MT>(fof_ub<ND, MT>(q, bx, L), L);
     if (encl) rq = uf_union_r(P, rq, base);
     const bool pair = act && !encl;
     if (!ballot(pair)) return;
@@ -175,9 +216,17 @@ __device__ __forceinline__ void fof_leaf(const DevTree &T, int leaf, const doubl
                 const double t = q[d] - cv[d][u];
                 t2[d] = t * t;
             }
-            const double d2 = fof_val<ND, MT>(t2, L);
+            bool near;
+            if constexpr (MT == 0) {
+                near = fof_val<ND, MT>(t2, L) < L.lim;
+            } else {
+                const double v = fof_val_fast<ND, MT>(t2, L);
+                near = v < NBX_FOF_BAND_LO;
+                const bool unsure = v >= NBX_FOF_BAND_LO && v < NBX_FOF_BAND_HI;
+                if (ballot(unsure) && unsure) near = fof_val<ND, MT>(t2, L) < 1.0;
+            }
             const int32_t cs = base + jj + u;
-            const bool l = d2 < L.lim && (PASS0 ? (lconn ? cs != qi : cs > qi) : cs != qi);
+            const bool l = near && (PASS0 ? (lconn ? cs != qi : cs > qi) : cs != qi);
             mask |= l ? (1u << (jj + u)) : 0u;
         }
     }
@@ -206,9 +255,9 @@ __device__ __forceinline__ void fof_walk(const DevTree &T, const double *q, bool
     for (;;) {
         if (down) {
             cdptr bx = (cdptr)node_box(T, node, ND);
-            bool want = act && fof_lb<ND, MT>(q, bx, L) < L.lim;
+            bool want = act && fof_maybe<MT>(fof_lb<ND, MT>(q, bx, L), L);
             if (ballot(want)) {
-                if (PASS0 && fof_bbub<ND, MT>(gb, bx, L) < L.lim) {
+                if (PASS0 && fof_surely<MT>(fof_bbub<ND, MT>(gb, bx, L), L)) {
                     // every wave particle links to every particle of the node:
                     // wave and node form one component
                     const int32_t nfirst = node_first(T, node);

@@ -87,6 +87,7 @@ hipError_t nbx_launch_knn_mixed(const DevTree &T, const KnnArgs &A, const double
 struct FofLink {
     double lim;
     double s6, s7;
+    double r6, r7;  // 1 / s6, 1 / s7 (correctly rounded)
 };
 struct FofArgs {
     FofLink link;

@@ -137,3 +137,31 @@ def test_fof_extremes(gpu, periodic):
         assert ng == bg and np.array_equal(lab, bl)
         raw, _ = oracle.Tree(ps, b=16, period=per).fof(b, 2)
         assert np.array_equal(lab, oracle.fof_canonical(raw, 2)[0])
+
+
+@pytest.mark.parametrize("cmp", ["3d", "6d"])
+def test_fof_criterion_at_threshold(gpu, cmp):
+    """Lattice neighbours at exactly the linking scale: with params[6] = 1 =
+    spacing^2 the comparator's value is exactly 1 and nothing links (strict
+    <); with the next double above 1 it is 1 - 2^-52 and the lattice links
+    into one group.  Both values fall in the band where the kernel decides
+    by the comparator's own divisions."""
+    nb = _nb()
+    g = 10
+    ax = np.arange(g) + 0.5
+    pos = np.stack(np.meshgrid(ax, ax, ax, indexing="ij"), -1).reshape(-1, 3)
+    vel = np.zeros_like(pos) + 0.25
+    say(f"fof criterion at threshold cmp={cmp} n={len(pos)}")
+    if cmp == "3d":
+        t = nb.KDTree(pos, kerntype=nb.KSPH)
+        c, tt = nb.FOF_CMP_3D, nb.TPHYS
+    else:
+        t = nb.KDTree(pos, vel=vel, treetype=nb.TPHS, aniso=-1, kerntype=nb.KSPH)
+        c, tt = nb.FOF_CMP_6D, nb.TPHS
+    up = np.nextafter(1.0, 2.0)
+    for s6, groups in ((1.0, 0), (up, 1)):
+        prm = [tt, 2.0, 2.0, 0, 0, 0, s6, 1.0]
+        lab, ng = t.FOFCriterion(c, prm, 2)
+        ol, og = oracle.fof_criterion_brute(pos, 1 if cmp == "3d" else 2, prm, 2,
+                                            vel=None if cmp == "3d" else vel)
+        assert og == groups and ng == og and np.array_equal(lab, ol)
