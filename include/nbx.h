@@ -119,6 +119,19 @@ int nbx_density_points(nbx_tree *t, const double *q, int64_t m, int32_t k, doubl
 int nbx_density(nbx_tree *t, int32_t k, int32_t variant, int32_t nsearch, double *rho_by_id, double *h_by_id);
 int nbx_density_dev(nbx_tree *t, int32_t k, int32_t variant, int32_t nsearch, double *rho_by_id_dev,
                     double *h_by_id_dev, void *stream);
+/* Smoothed velocity mean and dispersion (TPHYS trees built with velocities).
+ * Replaces CalcSmoothVel(Nsmooth, densityset) KDCalcSmoothQuantities.cxx:480-540
+ * and CalcSmoothVelDisp(smvel, Nsmooth, densityset, meanvelset) :542-614:
+ * CalcDensity's kNN (non-periodic, h = r_k/2) and half weights W_ij/2, added
+ * to both ends of each pair as W_ij/2 * m/rho * v (mean) and
+ * W_ij/2 * m/rho * (v - mean)(v - mean)^T (dispersion, about the receiving
+ * particle's mean).  rho_by_id: the densities to divide by (densityset = 1),
+ * or NULL to run CalcDensity(k) first.  mean_in_by_id: n x 3 means for the
+ * dispersion (CalcSmoothVelDisp's smvel with meanvelset = 1), or NULL to
+ * compute them.  smvel_by_id: n x 3 by ID, or NULL; disp_by_id: n x 9
+ * (row-major 3x3) by ID, or NULL; at least one of the two. */
+int nbx_smooth_vel(nbx_tree *t, int32_t k, const double *rho_by_id, const double *mean_in_by_id, double *smvel_by_id,
+                   double *disp_by_id);
 /* Fused: kNN (NBX_KNN_NONPERIODIC semantics) + symmetric density in one pass,
  * neighbour IDs by slot.  The batch form of FindNearest + CalcDensity. */
 int nbx_knn_density_dev(nbx_tree *t, int32_t k, int32_t *nn_by_slot_dev, double *rho_by_id_dev,

@@ -9,6 +9,7 @@
  *   SearchBallPos KDTree.h:342, :348
  *   CalcDensity*  KDTree.h:411-416      FOF                  KDTree.h:480-482
  *   FOFCriterion  KDTree.h:503 with the comparators of FOFFunc.h:24-55
+ *   CalcSmoothVel / CalcSmoothVelDisp  KDTree.h:414-418
  *   OverWriteInputOrder / SetResetOrder KDTree.h:515-518
  *
  * Semantics kept from the reference: the constructor overwrites Particle::id
@@ -80,6 +81,28 @@ class Particle {
     void SetDensity(const Double_t &r) { rho = r; }
     Double_t GetPotential() const { return phi; }
     void SetPotential(const Double_t &p) { phi = p; }
+};
+
+// The results of CalcSmoothVel and CalcSmoothVelDisp: the forms of
+// NBody::Coordinate (src/Math/Coordinate.h:46-78) and the 3x3 NBody::Matrix
+// (src/Math/Matrix.h:23-41) that callers index, c[i] and m(i, j).
+class Coordinate {
+    Double_t c[3];
+
+  public:
+    Coordinate(const Double_t x = 0, const Double_t y = 0, const Double_t z = 0) { c[0] = x; c[1] = y; c[2] = z; }
+    Double_t &operator[](int i) { return c[i]; }
+    const Double_t &operator[](int i) const { return c[i]; }
+};
+class Matrix {
+    Double_t m[9];
+
+  public:
+    Matrix(Double_t a = 0) {
+        for (int i = 0; i < 9; ++i) m[i] = a;
+    }
+    Double_t &operator()(int i, int j) { return m[3 * i + j]; }
+    const Double_t &operator()(int i, int j) const { return m[3 * i + j]; }
 };
 
 // FOFFunc.h:24-55: comparator and check-function types and the built-in
@@ -167,6 +190,12 @@ class KDTree {
     std::vector<Int_t> SearchBallPosTagged(Double_t *x, Double_t fdist2);
 
     void CalcDensity(Int_t Nsmooth = 64);
+    // KDTree.h:414-418, KDCalcSmoothQuantities.cxx:480-614 (TPHYS trees with
+    // velocities): new[] arrays of numparts by ID.  densityset = 1 divides by
+    // Particle::rho, else CalcDensity(Nsmooth) runs first; meanvelset = 1
+    // takes smvel as the means, else CalcSmoothVel runs first.
+    Coordinate *CalcSmoothVel(Int_t Nsmooth = 64, int densityset = 1);
+    Matrix *CalcSmoothVelDisp(Coordinate *smvel, Int_t Nsmooth = 64, int densityset = 1, int meanvelset = 1);
     void CalcVelDensity(Int_t Nsmooth = 64, Int_t Nsearch = 64);
     Double_t CalcDensityParticle(Int_t target, Int_t Nsmooth = 64);
     // KDTree.h:432 (TPHYS trees; v is unused there); KDCalcSmoothQuantities.cxx:1092-1149
@@ -191,6 +220,8 @@ class KDTree {
     static void SetPrecisionForm(bool mixed);
 
   private:
+    void density_to_bucket(Int_t Nsmooth);
+    std::vector<double> rho_by_id();
     Particle *bucket;
     Int_t numparts;
     Int_t b;

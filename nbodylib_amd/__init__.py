@@ -30,7 +30,7 @@ NBX_ERR_ARG, NBX_ERR_HIP, NBX_ERR_UNSUPPORTED, NBX_ERR_NOMEM, NBX_ERR_SEARCH = -
 
 EXPORTS = [
     "nbx_params_default", "nbx_tree_create", "nbx_tree_create_dev", "nbx_tree_destroy", "nbx_tree_order",
-    "nbx_tree_size", "nbx_tree_kernel", "nbx_knn", "nbx_knn_dev", "nbx_knn_points", "nbx_density_points", "nbx_density",
+    "nbx_tree_size", "nbx_tree_kernel", "nbx_knn", "nbx_knn_dev", "nbx_knn_points", "nbx_density_points", "nbx_smooth_vel", "nbx_density",
     "nbx_density_dev", "nbx_knn_density_dev", "nbx_ball_points", "nbx_ball_mark", "nbx_fof", "nbx_fof_dev", "nbx_fof_criterion", "nbx_fof_criterion_dev",
     "nbx_sph_density_fwd_dev", "nbx_sph_density_adjoint_dev", "nbx_prof_enable", "nbx_prof_reset", "nbx_prof_get",
     "nbx_tree_last_fallbacks", "nbx_last_error",
@@ -79,6 +79,7 @@ def lib():
     L.nbx_knn_dev.argtypes = [vp, i32, i32, i32, vp, vp, vp]
     L.nbx_knn_points.argtypes = [vp, vp, i64, i32, vp, vp]
     L.nbx_density_points.argtypes = [vp, vp, i64, i32, vp, vp]
+    L.nbx_smooth_vel.argtypes = [vp, i32, vp, vp, vp, vp]
     L.nbx_density.argtypes = [vp, i32, i32, i32, vp, vp]
     L.nbx_density_dev.argtypes = [vp, i32, i32, i32, vp, vp, vp]
     L.nbx_knn_density_dev.argtypes = [vp, i32, vp, vp, vp, vp]
@@ -251,6 +252,32 @@ class KDTree:
         if one:
             return (float(rho[0]), float(h[0])) if return_h else float(rho[0])
         return (rho, h) if return_h else rho
+
+    def CalcSmoothVel(self, Nsmooth=64, rho=None):
+        """CalcSmoothVel(Nsmooth, densityset) (KDCalcSmoothQuantities.cxx:480-540): smoothed velocities (n, 3)
+        by ID.  rho: densities by ID to divide by (densityset = 1), or None to compute CalcDensity first."""
+        return self._smooth(Nsmooth, rho, False)
+
+    def CalcSmoothVelDisp(self, Nsmooth=64, rho=None, mean=None):
+        """CalcSmoothVelDisp (:542-614): (mean (n, 3), dispersion (n, 3, 3)) by ID.  mean: the means to
+        take (meanvelset = 1), or None to compute CalcSmoothVel first."""
+        return self._smooth(Nsmooth, rho, True, mean)
+
+    def _smooth(self, k, rho, with_disp, mean_in=None):
+        mean = np.zeros((self.n, 3), dtype=np.float64)
+        disp = np.zeros((self.n, 3, 3), dtype=np.float64) if with_disp else None
+        r = mi = None
+        if rho is not None:
+            r = np.ascontiguousarray(rho, dtype=np.float64)
+            if r.shape != (self.n,):
+                raise ValueError("rho must have one value per particle")
+        if mean_in is not None:
+            mi = np.ascontiguousarray(mean_in, dtype=np.float64)
+            if mi.shape != (self.n, 3):
+                raise ValueError("mean must be (n, 3)")
+            mean[:] = mi
+        _check(lib().nbx_smooth_vel(self._h, k, _p(r), _p(mi), None if mi is not None else _p(mean), _p(disp)))
+        return (mean, disp) if with_disp else mean
 
     def CalcVelDensity(self, Nsmooth=64, Nsearch=64):
         """KDTree::CalcVelDensity (KDCalcSmoothQuantities.cxx:309-389); result by ID."""

@@ -213,11 +213,50 @@ Int_t KDTree::SearchBallPosTagged(Int_t tt, Double_t fdist2, Int_t *tagged) {
 }
 
 // ------------------------------------------------------------------ densities
-void KDTree::CalcDensity(Int_t Nsmooth) {
-    std::lock_guard<std::mutex> lk(mu);
+void KDTree::density_to_bucket(Int_t Nsmooth) {
     std::vector<double> rho(numparts);
     check(nbx_density(tree, Nsmooth, NBX_DENS_SYMMETRIC, 0, rho.data(), nullptr), "CalcDensity failed");
     for (Int_t s = 0; s < numparts; ++s) bucket[s].SetDensity(rho[bucket[s].GetID()]);
+}
+std::vector<double> KDTree::rho_by_id() {
+    std::vector<double> rho(numparts);
+    for (Int_t s = 0; s < numparts; ++s) rho[bucket[s].GetID()] = bucket[s].GetDensity();
+    return rho;
+}
+void KDTree::CalcDensity(Int_t Nsmooth) {
+    std::lock_guard<std::mutex> lk(mu);
+    density_to_bucket(Nsmooth);
+}
+Coordinate *KDTree::CalcSmoothVel(Int_t Nsmooth, int densityset) {
+    std::lock_guard<std::mutex> lk(mu);
+    if (densityset != 1) density_to_bucket(Nsmooth);
+    const std::vector<double> rho = rho_by_id();
+    std::vector<double> mean((size_t)3 * numparts);
+    check(nbx_smooth_vel(tree, Nsmooth, rho.data(), nullptr, mean.data(), nullptr), "CalcSmoothVel failed");
+    Coordinate *out = new Coordinate[numparts];
+    for (Int_t i = 0; i < numparts; ++i) out[i] = Coordinate(mean[3 * i], mean[3 * i + 1], mean[3 * i + 2]);
+    return out;
+}
+Matrix *KDTree::CalcSmoothVelDisp(Coordinate *smvel, Int_t Nsmooth, int densityset, int meanvelset) {
+    const int own = meanvelset != 1;
+    if (own) {
+        delete[] smvel;
+        smvel = CalcSmoothVel(Nsmooth, densityset);
+        densityset = 1;
+    }
+    std::lock_guard<std::mutex> lk(mu);
+    if (densityset != 1) density_to_bucket(Nsmooth);
+    const std::vector<double> rho = rho_by_id();
+    std::vector<double> mean((size_t)3 * numparts), disp((size_t)9 * numparts);
+    for (Int_t i = 0; i < numparts; ++i)
+        for (int d = 0; d < 3; ++d) mean[3 * i + d] = smvel[i][d];
+    check(nbx_smooth_vel(tree, Nsmooth, rho.data(), mean.data(), nullptr, disp.data()), "CalcSmoothVelDisp failed");
+    Matrix *out = new Matrix[numparts];
+    for (Int_t i = 0; i < numparts; ++i)
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) out[i](a, b) = disp[9 * i + 3 * a + b];
+    if (own) delete[] smvel;  // the means computed here (the caller's were deleted, as the reference does)
+    return out;
 }
 void KDTree::CalcVelDensity(Int_t Nsmooth, Int_t Nsearch) {
     std::lock_guard<std::mutex> lk(mu);

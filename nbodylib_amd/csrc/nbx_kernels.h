@@ -100,6 +100,18 @@ hipError_t nbx_launch_fof(const DevTree &T, const FofArgs &A, hipStream_t st);
 hipError_t nbx_launch_fof_finish(const DevTree &T, int32_t *parent, int32_t *count, int32_t *minid, int64_t n,
                                  hipStream_t st);
 
+// ------------------------------------------------------------------ smoothed velocities
+struct SmoothArgs {
+    const int32_t *nn;  // n x k neighbour slots, rows by slot
+    int32_t k;
+    const double *rho;  // densities by slot
+    double *mean;       // smoothed velocity by slot, [3][n]; zeroed by the caller
+    double *disp;       // dispersion by slot, [9][n] (upper triangle used) or null; zeroed by the caller
+};
+hipError_t nbx_launch_smooth(const DevTree &T, const SmoothArgs &A, bool with_mean, hipStream_t st);
+__global__ void k_smooth_out(const double *in, const int32_t *perm, int64_t n, int ncomp, double *out);
+__global__ void k_gather_by_id(const double *by_id, const int32_t *perm, int64_t n, double *by_slot);
+
 // ------------------------------------------------------------------ small kernels
 __global__ void k_scatter_by_id(const double *src, const int32_t *perm, double *dst, int64_t n);
 __global__ void k_fill_i32(int32_t *p, int32_t v, int64_t n);

@@ -1,0 +1,151 @@
+// nbx_smooth.hip -- smoothed velocity mean and dispersion on gfx950.
+//
+// Replaces KDTree::CalcSmoothVel (KDCalcSmoothQuantities.cxx:480-540) and
+// CalcSmoothVelDisp (:542-614) on TPHYS trees.  For every particle i the
+// reference takes the non-periodic target kNN of CalcDensity, h_i = r_k / 2,
+// and for each neighbour j adds half-weight shares to both ends:
+//   mean_i += W_ij / rho_j * m_j * v_j,   mean_j += W_ij / rho_i * m_i * v_i,
+// with W_ij = W(r_ij / h_i) h_i^-3 / 2 from the kernel table (the weights of
+// CalcDensity's symmetric sum).  The dispersion adds the same weights times
+// (v - mean)(v - mean)^T about the receiving particle's finished mean.
+//
+// The neighbour lists come from the fused kNN + density pass (rows by slot,
+// entries slots).  One thread per particle walks its row: its own share sums
+// in registers and goes out with one atomic per component, the other end's
+// share goes out as FP64 atomics at once.  Sums are order-independent up to
+// rounding.
+#include "nbx_kernels.h"
+
+__device__ __forceinline__ double smooth_d2(const DevTree &T, const double *xi, int32_t j) {
+    double s = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const double t = xi[d] - T.x[d][j];
+        s += t * t;
+    }
+    return s;
+}
+
+// h of row i: half the largest neighbour distance (the k-th)
+__device__ __forceinline__ double smooth_h(const DevTree &T, const double *xi, const int32_t *row, int k) {
+    double m = 0.0;
+    for (int t = 0; t < k; ++t) m = fmax(m, smooth_d2(T, xi, row[t]));
+    return 0.5 * sqrt(m);
+}
+
+__global__ __launch_bounds__(256) void k_smooth_vel(DevTree T, SmoothArgs A) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= T.n) return;
+    const int k = A.k;
+    const int64_t n = T.n;
+    const int32_t *row = A.nn + i * k;
+    double xi[3], vi[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        xi[d] = T.x[d][i];
+        vi[d] = T.x[3 + d][i];
+    }
+    const double hi = smooth_h(T, xi, row, k);
+    const double norm = 1.0 / pow(hi, 3.0);
+    const double rhoi = A.rho[i], mi = T.mass[i];
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int t = 0; t < k; ++t) {
+        const int32_t j = row[t];
+        const double rij = sqrt(smooth_d2(T, xi, j));
+        const double w = 0.5 * wsm_at(rij, hi, T.kern, T.kernres) * norm;
+        const double tj = w / A.rho[j] * T.mass[j];
+        const double ti = w / rhoi * mi;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            acc[d] += tj * T.x[3 + d][j];
+            unsafeAtomicAdd(A.mean + d * n + j, ti * vi[d]);
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) unsafeAtomicAdd(A.mean + d * n + i, acc[d]);
+}
+
+// Second pass, over the finished means.  The 3x3 products are symmetric
+// (dv_a * dv_b == dv_b * dv_a exactly), so six are summed and mirrored.
+__global__ __launch_bounds__(256) void k_smooth_disp(DevTree T, SmoothArgs A) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= T.n) return;
+    const int k = A.k;
+    const int64_t n = T.n;
+    const int32_t *row = A.nn + i * k;
+    double xi[3], vi[3], ui[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        xi[d] = T.x[d][i];
+        vi[d] = T.x[3 + d][i];
+        ui[d] = A.mean[d * n + i];
+    }
+    const double hi = smooth_h(T, xi, row, k);
+    const double norm = 1.0 / pow(hi, 3.0);
+    const double rhoi = A.rho[i], mi = T.mass[i];
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int t = 0; t < k; ++t) {
+        const int32_t j = row[t];
+        const double rij = sqrt(smooth_d2(T, xi, j));
+        const double w = 0.5 * wsm_at(rij, hi, T.kern, T.kernres) * norm;
+        const double tj = w / A.rho[j] * T.mass[j];
+        const double ti = w / rhoi * mi;
+        double a[3], b[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            a[d] = T.x[3 + d][j] - ui[d];       // neighbour about my mean
+            b[d] = vi[d] - A.mean[d * n + j];  // me about the neighbour's mean
+        }
+        int c = 0;
+#pragma unroll
+        for (int p = 0; p < 3; ++p)
+#pragma unroll
+            for (int q = p; q < 3; ++q, ++c) {
+                acc[c] += tj * a[p] * a[q];
+                unsafeAtomicAdd(A.disp + (3 * p + q) * n + j, ti * b[p] * b[q]);
+            }
+    }
+    int c = 0;
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+        for (int q = p; q < 3; ++q, ++c) unsafeAtomicAdd(A.disp + (3 * p + q) * n + i, acc[c]);
+}
+
+// Slot-major SoA to ID-major rows: out[id * ncomp + c] = in[c * n + slot];
+// the lower triangle of a 3x3 dispersion (ncomp 9) mirrors the upper.
+__global__ __launch_bounds__(256) void k_smooth_out(const double *in, const int32_t *perm, int64_t n, int ncomp,
+                                                    double *out) {
+    const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (s >= n) return;
+    const int64_t id = perm[s];
+    for (int c = 0; c < ncomp; ++c) {
+        int src = c;
+        if (ncomp == 9 && (c / 3) > (c % 3)) src = 3 * (c % 3) + c / 3;
+        out[id * ncomp + c] = in[(int64_t)src * n + s];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_gather_by_id(const double *by_id, const int32_t *perm, int64_t n,
+                                                      double *by_slot) {
+    const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (s < n) by_slot[s] = by_id[perm[s]];
+}
+
+static inline int nblk(int64_t n) { return (int)((n + 255) / 256); }
+
+// with_mean: compute the means first (else A.mean holds the caller's)
+hipError_t nbx_launch_smooth(const DevTree &T, const SmoothArgs &A, bool with_mean, hipStream_t st) {
+    const int64_t n = T.n;
+    if (n == 0) return hipSuccess;
+    if (T.nd != 3 || !T.x[3] || A.k < 1) return hipErrorInvalidValue;
+    if (with_mean) {
+        ProfScope ps("k_smooth_vel", st);
+        hipLaunchKernelGGL(k_smooth_vel, dim3(nblk(n)), dim3(256), 0, st, T, A);
+    }
+    if (A.disp) {
+        ProfScope ps("k_smooth_disp", st);
+        hipLaunchKernelGGL(k_smooth_disp, dim3(nblk(n)), dim3(256), 0, st, T, A);
+    }
+    return hipGetLastError();
+}

@@ -61,6 +61,8 @@ def lib():
         L.orc_calc_veldensity.restype = _ci
         L.orc_calc_density_position.argtypes = [_vp, _dp, _i64, _ci, _dp, ctypes.c_void_p, _ci]
         L.orc_calc_density_position.restype = _ci
+        L.orc_smooth_vel.argtypes = [_vp, _ci, ctypes.c_void_p, _dp, ctypes.c_void_p]
+        L.orc_smooth_vel.restype = _ci
         L.orc_fof_tree.argtypes = [_vp, _cd, _ci, _ci, _ip]
         L.orc_fof_tree.restype = _i64
         L.orc_knn_brute.argtypes = [_dp, _i64, _ci, _ci, _ci, ctypes.c_void_p, _ip, ctypes.c_void_p, _ci]
@@ -186,6 +188,17 @@ class Tree:
         if rc != 0:
             raise RuntimeError("CalcDensityPosition failed")
         return rho, h
+
+    def smooth_vel(self, k, rho=None, disp=False):
+        """CalcSmoothVel(k) [and CalcSmoothVelDisp]: mean (n, 3) [, disp (n, 3, 3)] by ID; rho by ID or None
+        (CalcDensity(k) first)."""
+        mean = np.zeros((self.n, 3), dtype=np.float64)
+        dd = np.zeros((self.n, 9), dtype=np.float64) if disp else None
+        r = None if rho is None else np.ascontiguousarray(rho, dtype=np.float64)
+        rc = lib().orc_smooth_vel(self.h, k, _ptr(r), mean, _ptr(dd))
+        if rc != 0:
+            raise RuntimeError("CalcSmoothVel failed")
+        return (mean, dd.reshape(self.n, 3, 3)) if disp else mean
 
     def calc_veldensity(self, nsmooth, nsearch, nthreads=0):
         rho = np.zeros(self.n, dtype=np.float64)

@@ -602,6 +602,69 @@ int orc_calc_density(void *h, int k, double *rho_by_id) {
     return rc;
 }
 
+/* CalcSmoothVel (KDCalcSmoothQuantities.cxx:480-540) and, when disp is not
+ * NULL, CalcSmoothVelDisp (:542-614) on a TPHYS tree with velocities,
+ * serial as shipped.  rho_by_id gives the densities divided by (the
+ * reference's densityset = 1); NULL runs CalcDensity(k) first.  mean: n x 3
+ * by ID; disp: n x 9 (row-major 3x3) by ID.  Returns 0, or -1 when a
+ * neighbour is missing (the reference exits). */
+int orc_smooth_vel(void *h, int k, const double *rho_by_id, double *mean, double *disp) {
+    otree *t = (otree *)h;
+    const int64_t n = t->n;
+    if (rho_by_id) {
+        for (int64_t i = 0; i < n; i++) t->p[i].rho = rho_by_id[t->p[i].id];
+    } else {
+        double *tmp = (double *)malloc(sizeof(double) * (size_t)n);
+        int rc = orc_calc_density(h, k, tmp);
+        free(tmp);
+        if (rc) return rc;
+    }
+    pq_t pq;
+    pq_init(&pq, k);
+    int rc = 0;
+    for (int pass = 0; pass < (disp ? 2 : 1) && rc == 0; pass++) {
+        double *out = pass ? disp : mean;
+        const int nc = pass ? 9 : 3;
+        for (int64_t i = 0; i < n * nc; i++) out[i] = 0;
+        for (int64_t i = 0; i < n && rc == 0; i++) {
+            double off[6] = {0, 0, 0, 0, 0, 0};
+            pq.n = 0;
+            for (int j = 0; j < k; j++) pq_push(&pq, -1, ORC_MAXVALUE);
+            nn_target(t, t->root, 0.0, &pq, off, i, 0, 3);
+            double hi = 0.5 * sqrt(pq_top(&pq));
+            double norm = 1.0 / pow(hi, 3.0);
+            for (int j = 0; j < k; j++) {
+                if (pq_topq(&pq) == -1) { rc = -1; break; }
+                double rij = sqrt(pq_top(&pq));
+                double Wij = 0.5 * wsm_at(rij, hi, t->kernres, t->kern) * norm;
+                int32_t x = pq_topq(&pq);
+                const int64_t id1 = t->p[i].id, id2 = t->p[x].id;
+                double temp = Wij / t->p[x].rho * t->p[x].m;
+                if (!pass) {
+                    for (int d = 0; d < 3; d++) mean[id1 * 3 + d] += temp * t->p[x].ph[3 + d];
+                } else {
+                    for (int a = 0; a < 3; a++)
+                        for (int b = 0; b < 3; b++)
+                            disp[id1 * 9 + 3 * a + b] += temp * (t->p[x].ph[3 + a] - mean[id1 * 3 + a]) *
+                                                         (t->p[x].ph[3 + b] - mean[id1 * 3 + b]);
+                }
+                temp = Wij / t->p[i].rho * t->p[i].m;
+                if (!pass) {
+                    for (int d = 0; d < 3; d++) mean[id2 * 3 + d] += temp * t->p[i].ph[3 + d];
+                } else {
+                    for (int a = 0; a < 3; a++)
+                        for (int b = 0; b < 3; b++)
+                            disp[id2 * 9 + 3 * a + b] += temp * (t->p[i].ph[3 + a] - mean[id2 * 3 + a]) *
+                                                         (t->p[i].ph[3 + b] - mean[id2 * 3 + b]);
+                }
+                pq_pop(&pq);
+            }
+        }
+    }
+    pq_free(&pq);
+    return rc;
+}
+
 /* CalcDensityParticle for every particle, KDCalcSmoothQuantities.cxx:768-844
  * (gather form, no 1/2, no scatter).  Also the all-cores CPU restatement. */
 int orc_calc_density_gather(void *h, int k, double *rho_by_id, double *h_by_id, int nthreads) {

@@ -120,3 +120,34 @@ def test_adjoint_gradcheck_and_reference(gpu):
     # the adjoint accumulates with FP64 atomics, so repeated backward passes
     # differ in the last bits (summation order): allow that, not more
     assert torch.autograd.gradcheck(f, (sp, sm), eps=1e-6, atol=1e-5, rtol=1e-4, nondet_tol=1e-12)
+
+
+@pytest.mark.parametrize("form,kind", [("plain", "uniform"), ("mixed", "uniform"), ("plain", "clustered")])
+def test_smooth_vel_vs_oracle(gpu, form, kind):
+    """CalcSmoothVel and CalcSmoothVelDisp against the oracle's restatement
+    (sums in another order: 1e-12 of the largest mean, 1e-11 of the largest
+    dispersion entry); densities given (densityset = 1) scale the means
+    exactly; the caller's means (meanvelset = 1) reproduce the dispersion."""
+    import nbodylib_amd as nb
+
+    n, k = 6000, 32
+    say(f"smooth vel form={form} kind={kind} n={n} k={k}")
+    if kind == "uniform":
+        pos = datagen.uniform(n, 1.0, 43)
+        vel = np.random.default_rng(43).normal(size=(n, 3))
+    else:
+        pos, vel, _ = datagen.clustered(n, 1.0, nhalo=8, seed=44, with_vel=True)
+    m = datagen.lognormal_masses(n, 43)
+    fm = nb.FORM_PLAIN if form == "plain" else nb.FORM_MIXED
+    t = nb.KDTree(pos, mass=m, vel=vel, kerntype=nb.KSPH, form=fm)
+    mean, disp = t.CalcSmoothVelDisp(k)
+    ot = oracle.Tree(pos, mass=m, vel=vel, b=16, kerntype=oracle.KSPH)
+    om, od = ot.smooth_vel(k, disp=True)
+    assert np.max(np.abs(mean - om)) < 1e-12 * np.max(np.abs(om))
+    assert np.max(np.abs(disp - od)) < 1e-11 * np.max(np.abs(od))
+    assert np.array_equal(disp, np.transpose(disp, (0, 2, 1)))
+    rho = t.CalcDensity(k)
+    m2 = t.CalcSmoothVel(k, rho=2.0 * rho)
+    assert np.max(np.abs(2.0 * m2 - om)) < 1e-12 * np.max(np.abs(om))
+    _, d2 = t.CalcSmoothVelDisp(k, mean=om)
+    assert np.max(np.abs(d2 - od)) < 1e-11 * np.max(np.abs(od))

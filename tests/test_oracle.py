@@ -264,6 +264,61 @@ def test_density_position_at_particles_is_gather_plus_self():
     assert abs(rb.mean() - 1.0) < 0.03
 
 
+def _wtab(q):
+    """The kernel table read as Wsm does (KDCalcSmoothQuantities.cxx:12-15), 3-D KSPH, kernres 1000."""
+    _, tab = oracle.kernel(3, oracle.KSPH, 1000)
+    j = (q * 0.5 * 999).astype(int)
+    jj = np.minimum(j, 998)
+    return np.where(j < 999, tab[jj] + (tab[jj + 1] - tab[jj]) * (q - (2 / 999) * j) / (2 / 999), tab[999])
+
+
+def test_smooth_vel_mean_with_unit_fields_is_calc_density():
+    """With v = 1 and rho = 1 given, CalcSmoothVel adds exactly CalcDensity's
+    terms, both halves, in CalcDensity's order: bit-identical to it.  The
+    dispersion then has the closed form c_a c_b (1 - rho_i)^2 rho_i for a
+    constant velocity c.  A dropped scatter half fails the first."""
+    n = 3000
+    pos = datagen.uniform(n, 1.0, 41)
+    m = datagen.lognormal_masses(n, 41)
+    c = np.array([1.0, -2.0, 0.5])
+    t = oracle.Tree(pos, mass=m, vel=np.tile(c, (n, 1)), b=16, kerntype=oracle.KSPH)
+    rho = oracle.Tree(pos, mass=m, b=16, kerntype=oracle.KSPH).calc_density(32)
+    mean, disp = t.smooth_vel(32, rho=np.ones(n), disp=True)
+    for d in range(3):  # powers of two scale every term exactly
+        assert np.array_equal(mean[:, d], c[d] * rho)
+    want = np.outer(c, c)[None] * ((1 - rho) ** 2 * rho)[:, None, None]
+    assert np.max(np.abs(disp - want)) <= 1e-12 * np.max(np.abs(want))
+
+
+def test_smooth_vel_vs_numpy_lists():
+    """CalcSmoothVel and CalcSmoothVelDisp on the oracle tree against a NumPy
+    restatement over brute-force neighbour lists, with the computed densities
+    and random velocities (which end's rho and v each share uses)."""
+    n, k = 2000, 32
+    pos = datagen.uniform(n, 1.0, 42)
+    m = datagen.lognormal_masses(n, 42)
+    vel = np.random.default_rng(42).normal(size=(n, 3))
+    t = oracle.Tree(pos, mass=m, vel=vel, b=16, kerntype=oracle.KSPH)
+    mean, disp = t.smooth_vel(k, disp=True)
+    rho = oracle.Tree(pos, mass=m, b=16, kerntype=oracle.KSPH).calc_density(k)
+    ids, d2 = oracle.knn_brute(pos, k, mode=0)
+    h = 0.5 * np.sqrt(d2[:, -1])
+    w = 0.5 * _wtab(np.sqrt(d2) / h[:, None]) / h[:, None] ** 3
+    own = w / rho[ids] * m[ids]
+    oth = w / rho[:, None] * m[:, None]
+    mn = np.zeros((n, 3))
+    rows = np.repeat(np.arange(n), k)
+    np.add.at(mn, rows, (own[..., None] * vel[ids]).reshape(-1, 3))
+    np.add.at(mn, ids.ravel(), (oth.ravel()[:, None] * np.repeat(vel, k, 0)))
+    assert np.max(np.abs(mean - mn)) < 1e-12 * np.max(np.abs(mn))
+    a = vel[ids] - mn[:, None, :]
+    b = np.repeat(vel, k, 0) - mn[ids.ravel()]
+    dp = np.zeros((n, 3, 3))
+    np.add.at(dp, rows, (own[..., None, None] * a[..., :, None] * a[..., None, :]).reshape(-1, 3, 3))
+    np.add.at(dp, ids.ravel(), oth.ravel()[:, None, None] * b[:, :, None] * b[:, None, :])
+    assert np.max(np.abs(disp - dp)) < 1e-11 * np.max(np.abs(dp))
+
+
 def _veldens_numpy(pos, vel, nsmooth, nsearch):
     ids, _ = oracle.knn_brute(pos, nsearch, mode=0)
     kn, tab = oracle.kernel(3, oracle.KSPH, 1000)
