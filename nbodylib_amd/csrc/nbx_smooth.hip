@@ -11,9 +11,9 @@
 //
 // The neighbour lists come from the fused kNN + density pass (rows by slot,
 // entries slots).  One thread per particle walks its row: its own share sums
-// in registers and goes out with one atomic per component, the other end's
-// share goes out as FP64 atomics at once.  Sums are order-independent up to
-// rounding.
+// in registers, the other end's share goes to LDS when that particle is in
+// the same block and to an FP64 atomic otherwise.  Sums are
+// order-independent up to rounding.
 #include "nbx_kernels.h"
 
 __device__ __forceinline__ double smooth_d2(const DevTree &T, const double *xi, int32_t j) {
@@ -33,83 +33,114 @@ __device__ __forceinline__ double smooth_h(const DevTree &T, const double *xi, c
     return 0.5 * sqrt(m);
 }
 
-__global__ __launch_bounds__(256) void k_smooth_vel(DevTree T, SmoothArgs A) {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= T.n) return;
-    const int k = A.k;
+// Blocks of SM_B consecutive slots: a share for a particle of the same block
+// (most neighbours are) is summed with LDS atomics, and each particle's total
+// goes out with one global atomic per component; shares for particles of
+// other blocks go out as global atomics at once.
+#define SM_B 1024
+
+__global__ __launch_bounds__(SM_B) void k_smooth_vel(DevTree T, SmoothArgs A) {
+    __shared__ double lv[3][SM_B];
+    const int tid = threadIdx.x;
+    const int64_t i0 = (int64_t)blockIdx.x * SM_B, i = i0 + tid;
     const int64_t n = T.n;
-    const int32_t *row = A.nn + i * k;
-    double xi[3], vi[3];
+    const bool valid = i < n;
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        xi[d] = T.x[d][i];
-        vi[d] = T.x[3 + d][i];
-    }
-    const double hi = smooth_h(T, xi, row, k);
-    const double norm = 1.0 / pow(hi, 3.0);
-    const double rhoi = A.rho[i], mi = T.mass[i];
+    for (int d = 0; d < 3; ++d) lv[d][tid] = 0.0;
+    __syncthreads();
     double acc[3] = {0.0, 0.0, 0.0};
-    for (int t = 0; t < k; ++t) {
-        const int32_t j = row[t];
-        const double rij = sqrt(smooth_d2(T, xi, j));
-        const double w = 0.5 * wsm_at(rij, hi, T.kern, T.kernres) * norm;
-        const double tj = w / A.rho[j] * T.mass[j];
-        const double ti = w / rhoi * mi;
+    if (valid) {
+        const int k = A.k;
+        const int32_t *row = A.nn + i * k;
+        double xi[3], vi[3];
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            acc[d] += tj * T.x[3 + d][j];
-            unsafeAtomicAdd(A.mean + d * n + j, ti * vi[d]);
+            xi[d] = T.x[d][i];
+            vi[d] = T.x[3 + d][i];
+        }
+        const double hi = smooth_h(T, xi, row, k);
+        const double norm = 1.0 / pow(hi, 3.0);
+        const double rhoi = A.rho[i], mi = T.mass[i];
+        for (int t = 0; t < k; ++t) {
+            const int32_t j = row[t];
+            const double rij = sqrt(smooth_d2(T, xi, j));
+            const double w = 0.5 * wsm_at(rij, hi, T.kern, T.kernres) * norm;
+            const double tj = w / A.rho[j] * T.mass[j];
+            const double ti = w / rhoi * mi;
+            const uint32_t r = (uint32_t)(j - i0);
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                acc[d] += tj * T.x[3 + d][j];
+                if (r < SM_B) atomicAdd(&lv[d][r], ti * vi[d]);
+                else unsafeAtomicAdd(A.mean + d * n + j, ti * vi[d]);
+            }
         }
     }
+    __syncthreads();
+    if (valid) {
 #pragma unroll
-    for (int d = 0; d < 3; ++d) unsafeAtomicAdd(A.mean + d * n + i, acc[d]);
+        for (int d = 0; d < 3; ++d) unsafeAtomicAdd(A.mean + d * n + i, acc[d] + lv[d][tid]);
+    }
 }
 
 // Second pass, over the finished means.  The 3x3 products are symmetric
 // (dv_a * dv_b == dv_b * dv_a exactly), so six are summed and mirrored.
-__global__ __launch_bounds__(256) void k_smooth_disp(DevTree T, SmoothArgs A) {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= T.n) return;
-    const int k = A.k;
+__global__ __launch_bounds__(SM_B) void k_smooth_disp(DevTree T, SmoothArgs A) {
+    __shared__ double ld[6][SM_B];
+    const int tid = threadIdx.x;
+    const int64_t i0 = (int64_t)blockIdx.x * SM_B, i = i0 + tid;
     const int64_t n = T.n;
-    const int32_t *row = A.nn + i * k;
-    double xi[3], vi[3], ui[3];
+    const bool valid = i < n;
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        xi[d] = T.x[d][i];
-        vi[d] = T.x[3 + d][i];
-        ui[d] = A.mean[d * n + i];
-    }
-    const double hi = smooth_h(T, xi, row, k);
-    const double norm = 1.0 / pow(hi, 3.0);
-    const double rhoi = A.rho[i], mi = T.mass[i];
+    for (int c = 0; c < 6; ++c) ld[c][tid] = 0.0;
+    __syncthreads();
     double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    for (int t = 0; t < k; ++t) {
-        const int32_t j = row[t];
-        const double rij = sqrt(smooth_d2(T, xi, j));
-        const double w = 0.5 * wsm_at(rij, hi, T.kern, T.kernres) * norm;
-        const double tj = w / A.rho[j] * T.mass[j];
-        const double ti = w / rhoi * mi;
-        double a[3], b[3];
+    if (valid) {
+        const int k = A.k;
+        const int32_t *row = A.nn + i * k;
+        double xi[3], vi[3], ui[3];
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            a[d] = T.x[3 + d][j] - ui[d];       // neighbour about my mean
-            b[d] = vi[d] - A.mean[d * n + j];  // me about the neighbour's mean
+            xi[d] = T.x[d][i];
+            vi[d] = T.x[3 + d][i];
+            ui[d] = A.mean[d * n + i];
         }
+        const double hi = smooth_h(T, xi, row, k);
+        const double norm = 1.0 / pow(hi, 3.0);
+        const double rhoi = A.rho[i], mi = T.mass[i];
+        for (int t = 0; t < k; ++t) {
+            const int32_t j = row[t];
+            const double rij = sqrt(smooth_d2(T, xi, j));
+            const double w = 0.5 * wsm_at(rij, hi, T.kern, T.kernres) * norm;
+            const double tj = w / A.rho[j] * T.mass[j];
+            const double ti = w / rhoi * mi;
+            const uint32_t r = (uint32_t)(j - i0);
+            double a[3], b[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                a[d] = T.x[3 + d][j] - ui[d];       // neighbour about my mean
+                b[d] = vi[d] - A.mean[d * n + j];  // me about the neighbour's mean
+            }
+            int c = 0;
+#pragma unroll
+            for (int p = 0; p < 3; ++p)
+#pragma unroll
+                for (int q = p; q < 3; ++q, ++c) {
+                    acc[c] += tj * a[p] * a[q];
+                    const double v = ti * b[p] * b[q];
+                    if (r < SM_B) atomicAdd(&ld[c][r], v);
+                    else unsafeAtomicAdd(A.disp + (3 * p + q) * n + j, v);
+                }
+        }
+    }
+    __syncthreads();
+    if (valid) {
         int c = 0;
 #pragma unroll
         for (int p = 0; p < 3; ++p)
 #pragma unroll
-            for (int q = p; q < 3; ++q, ++c) {
-                acc[c] += tj * a[p] * a[q];
-                unsafeAtomicAdd(A.disp + (3 * p + q) * n + j, ti * b[p] * b[q]);
-            }
+            for (int q = p; q < 3; ++q, ++c) unsafeAtomicAdd(A.disp + (3 * p + q) * n + i, acc[c] + ld[c][tid]);
     }
-    int c = 0;
-#pragma unroll
-    for (int p = 0; p < 3; ++p)
-#pragma unroll
-        for (int q = p; q < 3; ++q, ++c) unsafeAtomicAdd(A.disp + (3 * p + q) * n + i, acc[c]);
 }
 
 // Slot-major SoA to ID-major rows: out[id * ncomp + c] = in[c * n + slot];
@@ -141,11 +172,11 @@ hipError_t nbx_launch_smooth(const DevTree &T, const SmoothArgs &A, bool with_me
     if (T.nd != 3 || !T.x[3] || A.k < 1) return hipErrorInvalidValue;
     if (with_mean) {
         ProfScope ps("k_smooth_vel", st);
-        hipLaunchKernelGGL(k_smooth_vel, dim3(nblk(n)), dim3(256), 0, st, T, A);
+        hipLaunchKernelGGL(k_smooth_vel, dim3((n + SM_B - 1) / SM_B), dim3(SM_B), 0, st, T, A);
     }
     if (A.disp) {
         ProfScope ps("k_smooth_disp", st);
-        hipLaunchKernelGGL(k_smooth_disp, dim3(nblk(n)), dim3(256), 0, st, T, A);
+        hipLaunchKernelGGL(k_smooth_disp, dim3((n + SM_B - 1) / SM_B), dim3(SM_B), 0, st, T, A);
     }
     return hipGetLastError();
 }
