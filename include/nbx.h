@@ -132,6 +132,16 @@ int nbx_density_dev(nbx_tree *t, int32_t k, int32_t variant, int32_t nsearch, do
  * (row-major 3x3) by ID, or NULL; at least one of the two. */
 int nbx_smooth_vel(nbx_tree *t, int32_t k, const double *rho_by_id, const double *mean_in_by_id, double *smvel_by_id,
                    double *disp_by_id);
+/* Per-component velocity skewness and kurtosis over the same pairs.  Replaces
+ * CalcSmoothVelSkew(smvel, smveldisp, Nsmooth, densityset, meanvelset,
+ * veldispset) :617-690 and CalcSmoothVelKurtosis :692-767: each share adds
+ * share * dv^3 / s^1.5 and share * dv^4 / s^2 - 3, with dv about the receiving
+ * particle's mean and s its dispersion diagonal (the 3 is taken off every
+ * share, as the reference does).  mean_in_by_id (n x 3) and disp_in_by_id
+ * (n x 9) replace the computed ones (meanvelset / veldispset = 1), or NULL.
+ * skew_by_id, kurt_by_id: n x 3 by ID, either may be NULL (not both). */
+int nbx_smooth_vel_moments(nbx_tree *t, int32_t k, const double *rho_by_id, const double *mean_in_by_id,
+                           const double *disp_in_by_id, double *skew_by_id, double *kurt_by_id);
 /* Fused: kNN (NBX_KNN_NONPERIODIC semantics) + symmetric density in one pass,
  * neighbour IDs by slot.  The batch form of FindNearest + CalcDensity. */
 int nbx_knn_density_dev(nbx_tree *t, int32_t k, int32_t *nn_by_slot_dev, double *rho_by_id_dev,

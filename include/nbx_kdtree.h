@@ -196,6 +196,12 @@ class KDTree {
     // takes smvel as the means, else CalcSmoothVel runs first.
     Coordinate *CalcSmoothVel(Int_t Nsmooth = 64, int densityset = 1);
     Matrix *CalcSmoothVelDisp(Coordinate *smvel, Int_t Nsmooth = 64, int densityset = 1, int meanvelset = 1);
+    // KDTree.h:420-422, KDCalcSmoothQuantities.cxx:617-767; veldispset = 1
+    // takes smveldisp as the dispersions, else CalcSmoothVelDisp runs first.
+    Coordinate *CalcSmoothVelSkew(Coordinate *smvel, Matrix *smveldisp, Int_t Nsmooth = 64, int densityset = 1,
+                                  int meanvelset = 1, int veldispset = 1);
+    Coordinate *CalcSmoothVelKurtosis(Coordinate *smvel, Matrix *smveldisp, Int_t Nsmooth = 64, int densityset = 1,
+                                      int meanvelset = 1, int veldispset = 1);
     void CalcVelDensity(Int_t Nsmooth = 64, Int_t Nsearch = 64);
     Double_t CalcDensityParticle(Int_t target, Int_t Nsmooth = 64);
     // KDTree.h:432 (TPHYS trees; v is unused there); KDCalcSmoothQuantities.cxx:1092-1149
@@ -221,6 +227,8 @@ class KDTree {
 
   private:
     void density_to_bucket(Int_t Nsmooth);
+    Coordinate *smooth_moment(int which, Coordinate *smvel, Matrix *smveldisp, Int_t Nsmooth, int densityset,
+                              int meanvelset, int veldispset);
     std::vector<double> rho_by_id();
     Particle *bucket;
     Int_t numparts;

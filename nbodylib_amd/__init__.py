@@ -30,7 +30,7 @@ NBX_ERR_ARG, NBX_ERR_HIP, NBX_ERR_UNSUPPORTED, NBX_ERR_NOMEM, NBX_ERR_SEARCH = -
 
 EXPORTS = [
     "nbx_params_default", "nbx_tree_create", "nbx_tree_create_dev", "nbx_tree_destroy", "nbx_tree_order",
-    "nbx_tree_size", "nbx_tree_kernel", "nbx_knn", "nbx_knn_dev", "nbx_knn_points", "nbx_density_points", "nbx_smooth_vel", "nbx_density",
+    "nbx_tree_size", "nbx_tree_kernel", "nbx_knn", "nbx_knn_dev", "nbx_knn_points", "nbx_density_points", "nbx_smooth_vel", "nbx_smooth_vel_moments", "nbx_density",
     "nbx_density_dev", "nbx_knn_density_dev", "nbx_ball_points", "nbx_ball_mark", "nbx_fof", "nbx_fof_dev", "nbx_fof_criterion", "nbx_fof_criterion_dev",
     "nbx_sph_density_fwd_dev", "nbx_sph_density_adjoint_dev", "nbx_prof_enable", "nbx_prof_reset", "nbx_prof_get",
     "nbx_tree_last_fallbacks", "nbx_last_error",
@@ -80,6 +80,7 @@ def lib():
     L.nbx_knn_points.argtypes = [vp, vp, i64, i32, vp, vp]
     L.nbx_density_points.argtypes = [vp, vp, i64, i32, vp, vp]
     L.nbx_smooth_vel.argtypes = [vp, i32, vp, vp, vp, vp]
+    L.nbx_smooth_vel_moments.argtypes = [vp, i32, vp, vp, vp, vp, vp]
     L.nbx_density.argtypes = [vp, i32, i32, i32, vp, vp]
     L.nbx_density_dev.argtypes = [vp, i32, i32, i32, vp, vp, vp]
     L.nbx_knn_density_dev.argtypes = [vp, i32, vp, vp, vp, vp]
@@ -262,6 +263,26 @@ class KDTree:
         """CalcSmoothVelDisp (:542-614): (mean (n, 3), dispersion (n, 3, 3)) by ID.  mean: the means to
         take (meanvelset = 1), or None to compute CalcSmoothVel first."""
         return self._smooth(Nsmooth, rho, True, mean)
+
+    def CalcSmoothVelMoments(self, Nsmooth=64, rho=None, mean=None, disp=None):
+        """CalcSmoothVelSkew (:617-690) and CalcSmoothVelKurtosis (:692-767): (skew, kurt), each (n, 3) by ID.
+        mean (n, 3) / disp (n, 3, 3): the means and dispersions to take (meanvelset / veldispset = 1), or None
+        to compute them."""
+        skew = np.zeros((self.n, 3), dtype=np.float64)
+        kurt = np.zeros((self.n, 3), dtype=np.float64)
+        r = None if rho is None else np.ascontiguousarray(rho, dtype=np.float64)
+        mi = None if mean is None else np.ascontiguousarray(mean, dtype=np.float64).reshape(self.n, 3)
+        di = None if disp is None else np.ascontiguousarray(disp, dtype=np.float64).reshape(self.n, 9)
+        _check(lib().nbx_smooth_vel_moments(self._h, Nsmooth, _p(r), _p(mi), _p(di), _p(skew), _p(kurt)))
+        return skew, kurt
+
+    def CalcSmoothVelSkew(self, Nsmooth=64, rho=None, mean=None, disp=None):
+        """CalcSmoothVelSkew (KDCalcSmoothQuantities.cxx:617-690): (n, 3) by ID."""
+        return self.CalcSmoothVelMoments(Nsmooth, rho, mean, disp)[0]
+
+    def CalcSmoothVelKurtosis(self, Nsmooth=64, rho=None, mean=None, disp=None):
+        """CalcSmoothVelKurtosis (KDCalcSmoothQuantities.cxx:692-767): (n, 3) by ID."""
+        return self.CalcSmoothVelMoments(Nsmooth, rho, mean, disp)[1]
 
     def _smooth(self, k, rho, with_disp, mean_in=None):
         mean = np.zeros((self.n, 3), dtype=np.float64)

@@ -280,6 +280,47 @@ Double_t KDTree::CalcDensityPosition(Double_t *x, Int_t Nsmooth, Double_t *) {
     return den;
 }
 
+// which: 0 skewness, 1 kurtosis.  The reference's order of preparation:
+// densities (densityset), then means (meanvelset), then dispersions
+// (veldispset), replacing the caller's arrays it recomputes.
+Coordinate *KDTree::smooth_moment(int which, Coordinate *smvel, Matrix *smveldisp, Int_t Nsmooth, int densityset,
+                                  int meanvelset, int veldispset) {
+    if (densityset != 1) CalcDensity(Nsmooth);
+    const bool own_mean = meanvelset != 1, own_disp = veldispset != 1;
+    if (own_mean) {
+        delete[] smvel;
+        smvel = CalcSmoothVel(Nsmooth, 1);
+    }
+    if (own_disp) {
+        delete[] smveldisp;
+        smveldisp = CalcSmoothVelDisp(smvel, Nsmooth, 1, 1);
+    }
+    std::lock_guard<std::mutex> lk(mu);
+    const std::vector<double> rho = rho_by_id();
+    std::vector<double> mean((size_t)3 * numparts), disp((size_t)9 * numparts), out((size_t)3 * numparts);
+    for (Int_t i = 0; i < numparts; ++i)
+        for (int a = 0; a < 3; ++a) {
+            mean[3 * i + a] = smvel[i][a];
+            for (int b = 0; b < 3; ++b) disp[9 * i + 3 * a + b] = smveldisp[i](a, b);
+        }
+    check(nbx_smooth_vel_moments(tree, Nsmooth, rho.data(), mean.data(), disp.data(), which ? nullptr : out.data(),
+                                 which ? out.data() : nullptr),
+          which ? "CalcSmoothVelKurtosis failed" : "CalcSmoothVelSkew failed");
+    if (own_mean) delete[] smvel;
+    if (own_disp) delete[] smveldisp;
+    Coordinate *res = new Coordinate[numparts];
+    for (Int_t i = 0; i < numparts; ++i) res[i] = Coordinate(out[3 * i], out[3 * i + 1], out[3 * i + 2]);
+    return res;
+}
+Coordinate *KDTree::CalcSmoothVelSkew(Coordinate *smvel, Matrix *smveldisp, Int_t Nsmooth, int densityset,
+                                      int meanvelset, int veldispset) {
+    return smooth_moment(0, smvel, smveldisp, Nsmooth, densityset, meanvelset, veldispset);
+}
+Coordinate *KDTree::CalcSmoothVelKurtosis(Coordinate *smvel, Matrix *smveldisp, Int_t Nsmooth, int densityset,
+                                          int meanvelset, int veldispset) {
+    return smooth_moment(1, smvel, smveldisp, Nsmooth, densityset, meanvelset, veldispset);
+}
+
 // ------------------------------------------------------------------ FOF
 Int_t *KDTree::FOF(Double_t fdist, Int_t &numgroup, Int_t minnum, int, Int_tree_t *, Int_tree_t *, Int_tree_t *,
                    Int_tree_t *) {

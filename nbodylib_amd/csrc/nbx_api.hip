@@ -617,68 +617,92 @@ extern "C" int nbx_knn_density_dev(nbx_tree *t, int32_t k, int32_t *nn_by_slot_d
     return density_impl(t, k, NBX_DENS_SYMMETRIC, 0, rho_by_id_dev, h_by_id_dev, nn_by_slot_dev, (hipStream_t)stream);
 }
 
-// Smoothed velocity mean (and dispersion): the fused kNN + CalcDensity pass
-// gives neighbour rows by slot and rho, then the smoothing kernels run over
-// those rows.  A caller's rho (the reference's densityset = 1) replaces the
-// computed one.
-extern "C" int nbx_smooth_vel(nbx_tree *t, int32_t k, const double *rho_by_id, const double *mean_in_by_id,
-                              double *smvel_by_id, double *disp_by_id) {
-    if (!t || (!smvel_by_id && !disp_by_id)) return fail(NBX_ERR_ARG, "NULL argument");
+// Smoothed velocity mean, dispersion and moments: the fused kNN + CalcDensity
+// pass gives neighbour rows by slot and rho, then the smoothing kernels run
+// over those rows.  A caller's rho (the reference's densityset = 1), means
+// (meanvelset = 1) or dispersions (veldispset = 1) replace computed ones.
+// Host arrays by ID: rows of ncomp values -> [ncomp][n] by slot on the device.
+static hipError_t upload_rows(const nbx_tree *t, const double *rows, int ncomp, double *tmp, double *by_slot) {
+    const int64_t n = t->n;
+    std::vector<double> soa((size_t)ncomp * n);
+    for (int64_t i = 0; i < n; ++i)
+        for (int c = 0; c < ncomp; ++c) soa[(size_t)c * n + i] = rows[ncomp * i + c];
+    hipError_t e = hipMemcpy(tmp, soa.data(), sizeof(double) * ncomp * n, hipMemcpyHostToDevice);
+    for (int c = 0; c < ncomp && e == hipSuccess; ++c)
+        hipLaunchKernelGGL(k_gather_by_id, dim3(nblk(n)), dim3(256), 0, nullptr, tmp + c * n, t->perm, n,
+                           by_slot + c * n);
+    return e;
+}
+
+static int smooth_impl(nbx_tree *t, int32_t k, const double *rho_by_id, const double *mean_in, const double *disp_in,
+                       double *mean_out, double *disp_out, double *skew_out, double *kurt_out) {
     if (t->nd != 3 || !t->has_vel)
         return fail(NBX_ERR_UNSUPPORTED, "CalcSmoothVel needs a TPHYS tree built with velocities");
     if (k < 1 || k > 64) return fail(NBX_ERR_UNSUPPORTED, "Nsmooth out of range (1..64)");
     const int64_t n = t->n;
-    const bool want_disp = disp_by_id != nullptr;
+    const bool moments = skew_out || kurt_out;
+    const bool need_disp = disp_out || moments;
     hipStream_t st = nullptr;
     int32_t *nn = nullptr;
-    double *rid = nullptr, *rslot = nullptr, *mean = nullptr, *disp = nullptr, *buf = nullptr;
+    double *rid = nullptr, *rslot = nullptr, *mean = nullptr, *disp = nullptr, *mom = nullptr, *buf = nullptr;
     HIPCK(hipMalloc((void **)&nn, sizeof(int32_t) * (size_t)n * k));
     HIPCK(hipMalloc((void **)&rid, sizeof(double) * n));
     HIPCK(hipMalloc((void **)&rslot, sizeof(double) * n));
     HIPCK(hipMalloc((void **)&mean, sizeof(double) * 3 * n));
-    if (want_disp) HIPCK(hipMalloc((void **)&disp, sizeof(double) * 9 * n));
+    if (need_disp) HIPCK(hipMalloc((void **)&disp, sizeof(double) * 9 * n));
+    if (moments) HIPCK(hipMalloc((void **)&mom, sizeof(double) * 6 * n));
     HIPCK(hipMalloc((void **)&buf, sizeof(double) * 9 * n));
     int rc = density_impl(t, k, NBX_DENS_SYMMETRIC, 0, rid, nullptr, nn, st);
     hipError_t e = hipSuccess;
     if (rc == NBX_OK) {
         if (rho_by_id) e = hipMemcpy(rid, rho_by_id, sizeof(double) * n, hipMemcpyHostToDevice);
+        if (e == hipSuccess) hipLaunchKernelGGL(k_gather_by_id, dim3(nblk(n)), dim3(256), 0, st, rid, t->perm, n, rslot);
         if (e == hipSuccess) {
-            hipLaunchKernelGGL(k_gather_by_id, dim3(nblk(n)), dim3(256), 0, st, rid, t->perm, n, rslot);
-            if (mean_in_by_id) {
-                // n x 3 by ID -> [3][n] by slot
-                std::vector<double> soa((size_t)3 * n);
-                for (int64_t i = 0; i < n; ++i)
-                    for (int d = 0; d < 3; ++d) soa[(size_t)d * n + i] = mean_in_by_id[3 * i + d];
-                e = hipMemcpy(buf, soa.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice);
-                for (int d = 0; d < 3 && e == hipSuccess; ++d)
-                    hipLaunchKernelGGL(k_gather_by_id, dim3(nblk(n)), dim3(256), 0, st, buf + d * n, t->perm, n,
-                                       mean + d * n);
-            } else {
-                hipLaunchKernelGGL(k_fill_f64, dim3(nblk(3 * n)), dim3(256), 0, st, mean, 0.0, 3 * n);
-            }
-            if (disp) hipLaunchKernelGGL(k_fill_f64, dim3(nblk(9 * n)), dim3(256), 0, st, disp, 0.0, 9 * n);
+            if (mean_in) e = upload_rows(t, mean_in, 3, buf, mean);
+            else hipLaunchKernelGGL(k_fill_f64, dim3(nblk(3 * n)), dim3(256), 0, st, mean, 0.0, 3 * n);
         }
+        if (e == hipSuccess && need_disp) {
+            if (disp_in) e = upload_rows(t, disp_in, 9, buf, disp);
+            else hipLaunchKernelGGL(k_fill_f64, dim3(nblk(9 * n)), dim3(256), 0, st, disp, 0.0, 9 * n);
+        }
+        if (e == hipSuccess && moments)
+            hipLaunchKernelGGL(k_fill_f64, dim3(nblk(6 * n)), dim3(256), 0, st, mom, 0.0, 6 * n);
         if (e == hipSuccess) {
-            SmoothArgs A = {nn, k, rslot, mean, disp};
-            e = nbx_launch_smooth(t->view(), A, mean_in_by_id == nullptr, st);
+            SmoothArgs A = {nn, k, rslot, mean, disp, moments ? mom : nullptr, moments ? mom + 3 * n : nullptr};
+            e = nbx_launch_smooth(t->view(), A, mean_in == nullptr, need_disp && disp_in == nullptr, st);
         }
-        if (e == hipSuccess && smvel_by_id) {
-            hipLaunchKernelGGL(k_smooth_out, dim3(nblk(n)), dim3(256), 0, st, mean, t->perm, n, 3, buf);
-            e = hipMemcpy(smvel_by_id, buf, sizeof(double) * 3 * n, hipMemcpyDeviceToHost);
+        struct Out {
+            double *dev;
+            int ncomp;
+            double *host;
+        } outs[4] = {{mean, 3, mean_out}, {disp, 9, disp_out}, {mom, 3, skew_out}, {mom + 3 * n, 3, kurt_out}};
+        for (const Out &o : outs) {
+            if (e != hipSuccess || !o.host) continue;
+            hipLaunchKernelGGL(k_smooth_out, dim3(nblk(n)), dim3(256), 0, st, o.dev, t->perm, n, o.ncomp, buf);
+            e = hipMemcpy(o.host, buf, sizeof(double) * o.ncomp * n, hipMemcpyDeviceToHost);
         }
-        if (e == hipSuccess && disp) {
-            hipLaunchKernelGGL(k_smooth_out, dim3(nblk(n)), dim3(256), 0, st, disp, t->perm, n, 9, buf);
-            e = hipMemcpy(disp_by_id, buf, sizeof(double) * 9 * n, hipMemcpyDeviceToHost);
-        }
-        if (e != hipSuccess) rc = hipfail(e, "nbx_smooth_vel");
+        if (e != hipSuccess) rc = hipfail(e, "smoothed velocities");
     }
     hipFree(nn);
     hipFree(rid);
     hipFree(rslot);
     hipFree(mean);
     if (disp) hipFree(disp);
+    if (mom) hipFree(mom);
     hipFree(buf);
     return rc;
+}
+
+extern "C" int nbx_smooth_vel(nbx_tree *t, int32_t k, const double *rho_by_id, const double *mean_in_by_id,
+                              double *smvel_by_id, double *disp_by_id) {
+    if (!t || (!smvel_by_id && !disp_by_id)) return fail(NBX_ERR_ARG, "NULL argument");
+    return smooth_impl(t, k, rho_by_id, mean_in_by_id, nullptr, smvel_by_id, disp_by_id, nullptr, nullptr);
+}
+
+extern "C" int nbx_smooth_vel_moments(nbx_tree *t, int32_t k, const double *rho_by_id, const double *mean_in_by_id,
+                                      const double *disp_in_by_id, double *skew_by_id, double *kurt_by_id) {
+    if (!t || (!skew_by_id && !kurt_by_id)) return fail(NBX_ERR_ARG, "NULL argument");
+    return smooth_impl(t, k, rho_by_id, mean_in_by_id, disp_in_by_id, nullptr, nullptr, skew_by_id, kurt_by_id);
 }
 
 extern "C" int nbx_density(nbx_tree *t, int32_t k, int32_t variant, int32_t nsearch, double *rho_by_id,

@@ -107,8 +107,12 @@ struct SmoothArgs {
     const double *rho;  // densities by slot
     double *mean;       // smoothed velocity by slot, [3][n]; zeroed by the caller
     double *disp;       // dispersion by slot, [9][n] (upper triangle used) or null; zeroed by the caller
+    double *skew;       // skewness by slot, [3][n], or null; zeroed by the caller
+    double *kurt;       // kurtosis by slot, [3][n] (with skew); zeroed by the caller
 };
-hipError_t nbx_launch_smooth(const DevTree &T, const SmoothArgs &A, bool with_mean, hipStream_t st);
+// with_mean / with_disp: compute those first (else A.mean / A.disp hold the
+// caller's); the moments run when A.skew is set.
+hipError_t nbx_launch_smooth(const DevTree &T, const SmoothArgs &A, bool with_mean, bool with_disp, hipStream_t st);
 __global__ void k_smooth_out(const double *in, const int32_t *perm, int64_t n, int ncomp, double *out);
 __global__ void k_gather_by_id(const double *by_id, const int32_t *perm, int64_t n, double *by_slot);
 

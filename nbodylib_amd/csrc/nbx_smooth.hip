@@ -1,7 +1,8 @@
 // nbx_smooth.hip -- smoothed velocity mean and dispersion on gfx950.
 //
-// Replaces KDTree::CalcSmoothVel (KDCalcSmoothQuantities.cxx:480-540) and
-// CalcSmoothVelDisp (:542-614) on TPHYS trees.  For every particle i the
+// Replaces KDTree::CalcSmoothVel (KDCalcSmoothQuantities.cxx:480-540),
+// CalcSmoothVelDisp (:542-614), CalcSmoothVelSkew (:617-690) and
+// CalcSmoothVelKurtosis (:692-767) on TPHYS trees.  For every particle i the
 // reference takes the non-periodic target kNN of CalcDensity, h_i = r_k / 2,
 // and for each neighbour j adds half-weight shares to both ends:
 //   mean_i += W_ij / rho_j * m_j * v_j,   mean_j += W_ij / rho_i * m_i * v_i,
@@ -143,6 +144,71 @@ __global__ __launch_bounds__(SM_B) void k_smooth_disp(DevTree T, SmoothArgs A) {
     }
 }
 
+// Third pass: per-component skewness and kurtosis (CalcSmoothVelSkew,
+// KDCalcSmoothQuantities.cxx:617-690; CalcSmoothVelKurtosis, :692-767) about
+// the receiving particle's mean, scaled by its dispersion diagonal s:
+// share * dv^3 / s^1.5 and share * dv^4 / (s * s) - 3, the 3 taken off every
+// share as the reference does.
+__global__ __launch_bounds__(SM_B) void k_smooth_moments(DevTree T, SmoothArgs A) {
+    __shared__ double lm[6][SM_B];
+    const int tid = threadIdx.x;
+    const int64_t i0 = (int64_t)blockIdx.x * SM_B, i = i0 + tid;
+    const int64_t n = T.n;
+    const bool valid = i < n;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) lm[c][tid] = 0.0;
+    __syncthreads();
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (valid) {
+        const int k = A.k;
+        const int32_t *row = A.nn + i * k;
+        double xi[3], vi[3], ui[3], si[3], pi[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            xi[d] = T.x[d][i];
+            vi[d] = T.x[3 + d][i];
+            ui[d] = A.mean[d * n + i];
+            si[d] = A.disp[4 * d * n + i];
+            pi[d] = pow(si[d], 1.5);
+        }
+        const double hi = smooth_h(T, xi, row, k);
+        const double norm = 1.0 / pow(hi, 3.0);
+        const double rhoi = A.rho[i], mi = T.mass[i];
+        for (int t = 0; t < k; ++t) {
+            const int32_t j = row[t];
+            const double rij = sqrt(smooth_d2(T, xi, j));
+            const double w = 0.5 * wsm_at(rij, hi, T.kern, T.kernres) * norm;
+            const double tj = w / A.rho[j] * T.mass[j];
+            const double ti = w / rhoi * mi;
+            const uint32_t r = (uint32_t)(j - i0);
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const double a = T.x[3 + d][j] - ui[d];
+                acc[d] += tj * a * a * a / pi[d];
+                acc[3 + d] += tj * a * a * a * a / (si[d] * si[d]) - 3.;
+                const double b = vi[d] - A.mean[d * n + j], sj = A.disp[4 * d * n + j];
+                const double vs = ti * b * b * b / pow(sj, 1.5);
+                const double vk = ti * b * b * b * b / (sj * sj) - 3.;
+                if (r < SM_B) {
+                    atomicAdd(&lm[d][r], vs);
+                    atomicAdd(&lm[3 + d][r], vk);
+                } else {
+                    unsafeAtomicAdd(A.skew + d * n + j, vs);
+                    unsafeAtomicAdd(A.kurt + d * n + j, vk);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (valid) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            unsafeAtomicAdd(A.skew + d * n + i, acc[d] + lm[d][tid]);
+            unsafeAtomicAdd(A.kurt + d * n + i, acc[3 + d] + lm[3 + d][tid]);
+        }
+    }
+}
+
 // Slot-major SoA to ID-major rows: out[id * ncomp + c] = in[c * n + slot];
 // the lower triangle of a 3x3 dispersion (ncomp 9) mirrors the upper.
 __global__ __launch_bounds__(256) void k_smooth_out(const double *in, const int32_t *perm, int64_t n, int ncomp,
@@ -165,18 +231,24 @@ __global__ __launch_bounds__(256) void k_gather_by_id(const double *by_id, const
 
 static inline int nblk(int64_t n) { return (int)((n + 255) / 256); }
 
-// with_mean: compute the means first (else A.mean holds the caller's)
-hipError_t nbx_launch_smooth(const DevTree &T, const SmoothArgs &A, bool with_mean, hipStream_t st) {
+hipError_t nbx_launch_smooth(const DevTree &T, const SmoothArgs &A, bool with_mean, bool with_disp, hipStream_t st) {
     const int64_t n = T.n;
     if (n == 0) return hipSuccess;
     if (T.nd != 3 || !T.x[3] || A.k < 1) return hipErrorInvalidValue;
+    if ((with_disp || A.skew) && !A.disp) return hipErrorInvalidValue;
+    if (A.skew && !A.kurt) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)((n + SM_B - 1) / SM_B));
     if (with_mean) {
         ProfScope ps("k_smooth_vel", st);
-        hipLaunchKernelGGL(k_smooth_vel, dim3((n + SM_B - 1) / SM_B), dim3(SM_B), 0, st, T, A);
+        hipLaunchKernelGGL(k_smooth_vel, grid, dim3(SM_B), 0, st, T, A);
     }
-    if (A.disp) {
+    if (with_disp) {
         ProfScope ps("k_smooth_disp", st);
-        hipLaunchKernelGGL(k_smooth_disp, dim3((n + SM_B - 1) / SM_B), dim3(SM_B), 0, st, T, A);
+        hipLaunchKernelGGL(k_smooth_disp, grid, dim3(SM_B), 0, st, T, A);
+    }
+    if (A.skew) {
+        ProfScope ps("k_smooth_moments", st);
+        hipLaunchKernelGGL(k_smooth_moments, grid, dim3(SM_B), 0, st, T, A);
     }
     return hipGetLastError();
 }

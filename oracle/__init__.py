@@ -63,6 +63,8 @@ def lib():
         L.orc_calc_density_position.restype = _ci
         L.orc_smooth_vel.argtypes = [_vp, _ci, ctypes.c_void_p, _dp, ctypes.c_void_p]
         L.orc_smooth_vel.restype = _ci
+        L.orc_smooth_moments.argtypes = [_vp, _ci, ctypes.c_void_p, _dp, _dp, _dp, _dp]
+        L.orc_smooth_moments.restype = _ci
         L.orc_fof_tree.argtypes = [_vp, _cd, _ci, _ci, _ip]
         L.orc_fof_tree.restype = _i64
         L.orc_knn_brute.argtypes = [_dp, _i64, _ci, _ci, _ci, ctypes.c_void_p, _ip, ctypes.c_void_p, _ci]
@@ -199,6 +201,19 @@ class Tree:
         if rc != 0:
             raise RuntimeError("CalcSmoothVel failed")
         return (mean, dd.reshape(self.n, 3, 3)) if disp else mean
+
+    def smooth_moments(self, k, mean, disp, rho=None):
+        """CalcSmoothVelSkew / CalcSmoothVelKurtosis given means (n, 3) and dispersions (n, 3, 3) by ID:
+        (skew, kurt), each (n, 3)."""
+        skew = np.zeros((self.n, 3), dtype=np.float64)
+        kurt = np.zeros((self.n, 3), dtype=np.float64)
+        r = None if rho is None else np.ascontiguousarray(rho, dtype=np.float64)
+        mn = np.ascontiguousarray(mean, dtype=np.float64).reshape(self.n, 3)
+        dp = np.ascontiguousarray(disp, dtype=np.float64).reshape(self.n, 9)
+        rc = lib().orc_smooth_moments(self.h, k, _ptr(r), mn, dp, skew, kurt)
+        if rc != 0:
+            raise RuntimeError("CalcSmoothVelSkew failed")
+        return skew, kurt
 
     def calc_veldensity(self, nsmooth, nsearch, nthreads=0):
         rho = np.zeros(self.n, dtype=np.float64)

@@ -665,6 +665,60 @@ int orc_smooth_vel(void *h, int k, const double *rho_by_id, double *mean, double
     return rc;
 }
 
+/* CalcSmoothVelSkew (KDCalcSmoothQuantities.cxx:617-690) and
+ * CalcSmoothVelKurtosis (:692-767) given the means (n x 3) and dispersions
+ * (n x 9) by ID, serial as shipped: per component, share * dv^3 /
+ * pow(s, 1.5) and share * dv^4 / (s * s) - 3 for every share, with s the
+ * receiving particle's dispersion diagonal.  rho_by_id as orc_smooth_vel.
+ * skew, kurt: n x 3 by ID. */
+int orc_smooth_moments(void *h, int k, const double *rho_by_id, const double *mean, const double *disp, double *skew,
+                       double *kurt) {
+    otree *t = (otree *)h;
+    const int64_t n = t->n;
+    if (rho_by_id) {
+        for (int64_t i = 0; i < n; i++) t->p[i].rho = rho_by_id[t->p[i].id];
+    } else {
+        double *tmp = (double *)malloc(sizeof(double) * (size_t)n);
+        int rc = orc_calc_density(h, k, tmp);
+        free(tmp);
+        if (rc) return rc;
+    }
+    for (int64_t i = 0; i < 3 * n; i++) skew[i] = kurt[i] = 0;
+    pq_t pq;
+    pq_init(&pq, k);
+    int rc = 0;
+    for (int64_t i = 0; i < n && rc == 0; i++) {
+        double off[6] = {0, 0, 0, 0, 0, 0};
+        pq.n = 0;
+        for (int j = 0; j < k; j++) pq_push(&pq, -1, ORC_MAXVALUE);
+        nn_target(t, t->root, 0.0, &pq, off, i, 0, 3);
+        double hi = 0.5 * sqrt(pq_top(&pq));
+        double norm = 1.0 / pow(hi, 3.0);
+        for (int j = 0; j < k; j++) {
+            if (pq_topq(&pq) == -1) { rc = -1; break; }
+            double rij = sqrt(pq_top(&pq));
+            double Wij = 0.5 * wsm_at(rij, hi, t->kernres, t->kern) * norm;
+            int32_t x = pq_topq(&pq);
+            const int64_t id1 = t->p[i].id, id2 = t->p[x].id;
+            double temp = Wij / t->p[x].rho * t->p[x].m;
+            for (int d = 0; d < 3; d++) {
+                double dv = t->p[x].ph[3 + d] - mean[id1 * 3 + d], s = disp[id1 * 9 + 4 * d];
+                skew[id1 * 3 + d] += temp * dv * dv * dv / pow(s, 1.5);
+                kurt[id1 * 3 + d] += temp * dv * dv * dv * dv / (s * s) - 3.;
+            }
+            temp = Wij / t->p[i].rho * t->p[i].m;
+            for (int d = 0; d < 3; d++) {
+                double dv = t->p[i].ph[3 + d] - mean[id2 * 3 + d], s = disp[id2 * 9 + 4 * d];
+                skew[id2 * 3 + d] += temp * dv * dv * dv / pow(s, 1.5);
+                kurt[id2 * 3 + d] += temp * dv * dv * dv * dv / (s * s) - 3.;
+            }
+            pq_pop(&pq);
+        }
+    }
+    pq_free(&pq);
+    return rc;
+}
+
 /* CalcDensityParticle for every particle, KDCalcSmoothQuantities.cxx:768-844
  * (gather form, no 1/2, no scatter).  Also the all-cores CPU restatement. */
 int orc_calc_density_gather(void *h, int k, double *rho_by_id, double *h_by_id, int nthreads) {

@@ -151,3 +151,9 @@ def test_smooth_vel_vs_oracle(gpu, form, kind):
     assert np.max(np.abs(2.0 * m2 - om)) < 1e-12 * np.max(np.abs(om))
     _, d2 = t.CalcSmoothVelDisp(k, mean=om)
     assert np.max(np.abs(d2 - od)) < 1e-11 * np.max(np.abs(od))
+    osk, oku = ot.smooth_moments(k, om, od)
+    sk, ku = t.CalcSmoothVelMoments(k)
+    assert np.max(np.abs(sk - osk)) < 1e-9 * np.max(np.abs(osk))
+    assert np.max(np.abs(ku - oku)) < 1e-9 * np.max(np.abs(oku))
+    sk2 = t.CalcSmoothVelSkew(k, mean=om, disp=od)
+    assert np.max(np.abs(sk2 - osk)) < 1e-10 * np.max(np.abs(osk))

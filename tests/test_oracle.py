@@ -290,6 +290,29 @@ def test_smooth_vel_mean_with_unit_fields_is_calc_density():
     assert np.max(np.abs(disp - want)) <= 1e-12 * np.max(np.abs(want))
 
 
+def test_smooth_moments_closed_forms():
+    """Skewness and kurtosis with a constant velocity c and rho = 1 given:
+    every share into particle i has dv = c (1 - rho_i) and the shares sum to
+    rho_i (CalcDensity's value), so skew_i = sign(c (1 - rho_i)) / sqrt(rho_i)
+    and kurt_i = 1 / rho_i - 3 * (the number of shares into i: k plus the
+    lists that hold i), the 3 coming off every share as in the reference."""
+    n, k = 3000, 32
+    pos = datagen.uniform(n, 1.0, 45)
+    m = datagen.lognormal_masses(n, 45)
+    c = np.array([1.0, -2.0, 0.5])
+    t = oracle.Tree(pos, mass=m, vel=np.tile(c, (n, 1)), b=16, kerntype=oracle.KSPH)
+    rho = oracle.Tree(pos, mass=m, b=16, kerntype=oracle.KSPH).calc_density(k)
+    ones = np.ones(n)
+    mean, disp = t.smooth_vel(k, rho=ones, disp=True)
+    skew, kurt = t.smooth_moments(k, mean, disp, rho=ones)
+    want_s = np.sign(c[None, :] * (1 - rho)[:, None]) / np.sqrt(rho)[:, None]
+    assert np.max(np.abs(skew - want_s)) < 1e-10
+    ids, _ = oracle.knn_brute(pos, k, mode=0)
+    deg = k + np.bincount(ids.ravel(), minlength=n)
+    want_k = (1.0 / rho - 3.0 * deg)[:, None]
+    assert np.max(np.abs(kurt - want_k)) < 1e-9 * np.max(np.abs(want_k))
+
+
 def test_smooth_vel_vs_numpy_lists():
     """CalcSmoothVel and CalcSmoothVelDisp on the oracle tree against a NumPy
     restatement over brute-force neighbour lists, with the computed densities
@@ -317,6 +340,16 @@ def test_smooth_vel_vs_numpy_lists():
     np.add.at(dp, rows, (own[..., None, None] * a[..., :, None] * a[..., None, :]).reshape(-1, 3, 3))
     np.add.at(dp, ids.ravel(), oth.ravel()[:, None, None] * b[:, :, None] * b[:, None, :])
     assert np.max(np.abs(disp - dp)) < 1e-11 * np.max(np.abs(dp))
+    skew, kurt = t.smooth_moments(k, mean, disp)
+    sd = np.stack([dp[:, 0, 0], dp[:, 1, 1], dp[:, 2, 2]], 1)
+    sk = np.zeros((n, 3))
+    ku = np.zeros((n, 3))
+    np.add.at(sk, rows, (own[..., None] * a ** 3 / sd[:, None, :] ** 1.5).reshape(-1, 3))
+    np.add.at(sk, ids.ravel(), oth.ravel()[:, None] * b ** 3 / sd[ids.ravel()] ** 1.5)
+    np.add.at(ku, rows, (own[..., None] * a ** 4 / sd[:, None, :] ** 2 - 3).reshape(-1, 3))
+    np.add.at(ku, ids.ravel(), oth.ravel()[:, None] * b ** 4 / sd[ids.ravel()] ** 2 - 3)
+    assert np.max(np.abs(skew - sk)) < 1e-10 * np.max(np.abs(sk))
+    assert np.max(np.abs(kurt - ku)) < 1e-10 * np.max(np.abs(ku))
 
 
 def _veldens_numpy(pos, vel, nsmooth, nsearch):
