@@ -151,6 +151,14 @@ int nbx_knn_density_dev(nbx_tree *t, int32_t k, int32_t *nn_by_slot_dev, double 
  * Replaces SearchBallPosTagged(Double_t*x, fdist2, Int_t*tagged)
  * KDFindNearest.cxx:618-688 (period-aware).  out: capacity cap; *nout = total found. */
 int nbx_ball_points(nbx_tree *t, const double *x, double r2, int32_t *out, int64_t cap, int64_t *nout);
+/* Ball search around m positions (q: 3*m SoA) at once: SearchBallPosTagged
+ * (x, fdist2, tagged) KDFindNearest.cxx:618-688 for each row (period-aware
+ * when the tree is, with a radius below half the period).  offsets[m+1]: CSR
+ * offsets, offsets[m] = total hits; ids (IDs) and d2 (nullable) are written
+ * only when total <= cap, else call again with a larger cap.  Order within a
+ * row is unspecified (the reference's is its traversal order). */
+int nbx_ball_points_many(nbx_tree *t, const double *q, int64_t m, double r2, int64_t *offsets, int32_t *ids,
+                         double *d2, int64_t cap);
 /* Ball search that marks: group_by_id[id] = imark and d2_by_id[id] = d2 for
  * every particle with d2 < r2 around x, except the particle with ID
  * exclude_id (pass -1 to exclude none); other entries are left as they are.

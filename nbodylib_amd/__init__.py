@@ -31,7 +31,7 @@ NBX_ERR_ARG, NBX_ERR_HIP, NBX_ERR_UNSUPPORTED, NBX_ERR_NOMEM, NBX_ERR_SEARCH = -
 EXPORTS = [
     "nbx_params_default", "nbx_tree_create", "nbx_tree_create_dev", "nbx_tree_destroy", "nbx_tree_order",
     "nbx_tree_size", "nbx_tree_kernel", "nbx_knn", "nbx_knn_dev", "nbx_knn_points", "nbx_density_points", "nbx_smooth_vel", "nbx_smooth_vel_moments", "nbx_density",
-    "nbx_density_dev", "nbx_knn_density_dev", "nbx_ball_points", "nbx_ball_mark", "nbx_fof", "nbx_fof_dev", "nbx_fof_criterion", "nbx_fof_criterion_dev",
+    "nbx_density_dev", "nbx_knn_density_dev", "nbx_ball_points", "nbx_ball_points_many", "nbx_ball_mark", "nbx_fof", "nbx_fof_dev", "nbx_fof_criterion", "nbx_fof_criterion_dev",
     "nbx_sph_density_fwd_dev", "nbx_sph_density_adjoint_dev", "nbx_prof_enable", "nbx_prof_reset", "nbx_prof_get",
     "nbx_tree_last_fallbacks", "nbx_last_error",
 ]
@@ -86,6 +86,7 @@ def lib():
     L.nbx_knn_density_dev.argtypes = [vp, i32, vp, vp, vp, vp]
     L.nbx_ball_points.argtypes = [vp, vp, dbl, vp, i64, P(i64)]
     L.nbx_ball_mark.argtypes = [vp, vp, i64, dbl, i32, vp, vp]
+    L.nbx_ball_points_many.argtypes = [vp, vp, i64, dbl, vp, vp, vp, i64]
     L.nbx_fof.argtypes = [vp, dbl, i32, vp, P(i64)]
     L.nbx_fof_dev.argtypes = [vp, dbl, i32, vp, P(i64), vp]
     L.nbx_fof_criterion.argtypes = [vp, i32, vp, i32, vp, P(i64)]
@@ -315,6 +316,21 @@ class KDTree:
         nout = ctypes.c_int64()
         _check(lib().nbx_ball_points(self._h, _p(x), float(fdist2), _p(out), cap, ctypes.byref(nout)))
         return out[: min(nout.value, cap)].copy()
+
+    def SearchBallPosTaggedMany(self, x, fdist2):
+        """SearchBallPosTagged(x, fdist2) for every row of x (m, 3) in one call: (offsets (m+1,), ids, d2),
+        row i's IDs being ids[offsets[i]:offsets[i+1]] (unordered)."""
+        x = np.asarray(x, dtype=np.float64).reshape(-1, 3)
+        m = x.shape[0]
+        q = np.ascontiguousarray(x.T)
+        off = np.zeros(m + 1, dtype=np.int64)
+        _check(lib().nbx_ball_points_many(self._h, _p(q), m, float(fdist2), _p(off), None, None, 0))
+        total = int(off[-1])
+        ids = np.zeros(max(total, 1), dtype=np.int32)
+        d2 = np.zeros(max(total, 1), dtype=np.float64)
+        if total:
+            _check(lib().nbx_ball_points_many(self._h, _p(q), m, float(fdist2), _p(off), _p(ids), _p(d2), total))
+        return off, ids[:total], d2[:total]
 
     def SearchBallPos(self, x, fdist2, imark, nn, dist2, exclude_id=-1):
         """SearchBallPos(x, fdist2, imark, nn, dist2) (KDFindNearest.cxx:575-582): nn[id] = imark and

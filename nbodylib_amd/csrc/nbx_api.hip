@@ -811,6 +811,85 @@ extern "C" int nbx_fof(nbx_tree *t, double fdist, int32_t minnum, int32_t *label
     return rc;
 }
 
+// ------------------------------------------------------------------ ball search, many positions
+extern "C" int nbx_ball_points_many(nbx_tree *t, const double *q, int64_t m, double r2, int64_t *offsets,
+                                    int32_t *ids, double *d2, int64_t cap) {
+    if (!t || (m > 0 && !q) || !offsets) return fail(NBX_ERR_ARG, "NULL argument");
+    if (t->nd != 3) return fail(NBX_ERR_UNSUPPORTED, "ball search needs a TPHYS tree");
+    if (m < 0 || cap < 0 || !(r2 >= 0)) return fail(NBX_ERR_ARG, "bad sizes or radius");
+    offsets[0] = 0;
+    if (m == 0) return NBX_OK;
+    const bool per = t->p.periodic && t->n > t->p.bucket_size;
+    if (per)
+        for (int d = 0; d < 3; ++d)
+            if (!(4.0 * r2 < t->p.period[d] * t->p.period[d]))
+                return fail(NBX_ERR_UNSUPPORTED, "periodic ball search needs a radius below half the period");
+    if (m > INT32_MAX) return fail(NBX_ERR_ARG, "too many positions (m < 2^31)");
+    double *dq = nullptr;
+    int64_t *cnt = nullptr, *doff = nullptr;
+    uint64_t *keys = nullptr;
+    int32_t *idx = nullptr;
+    void *tmp = nullptr;
+    HIPCK(hipMalloc((void **)&dq, sizeof(double) * 3 * m));
+    HIPCK(hipMemcpy(dq, q, sizeof(double) * 3 * m, hipMemcpyHostToDevice));
+    HIPCK(hipMalloc((void **)&cnt, sizeof(int64_t) * m));
+    // walk order: Morton order of the positions over the root box
+    double rb[6];
+    HIPCK(hipMalloc((void **)&keys, sizeof(uint64_t) * 2 * m));
+    HIPCK(hipMalloc((void **)&idx, sizeof(int32_t) * 2 * m));
+    HIPCK(hipMemcpy(rb, t->box + 6, sizeof(rb), hipMemcpyDeviceToHost));
+    double sc[3];
+    for (int d = 0; d < 3; ++d) sc[d] = rb[3 + d] > rb[d] ? 2097151.0 / (rb[3 + d] - rb[d]) : 0.0;
+    hipLaunchKernelGGL(k_morton_keys, dim3(nblk(m)), dim3(256), 0, nullptr, dq, dq + m, dq + 2 * m, m, rb[0], rb[1],
+                       rb[2], sc[0], sc[1], sc[2], keys, idx);
+    size_t tb = 0;
+    hipError_t e = nbx_sort_pairs_u64(nullptr, tb, keys, keys + m, idx, idx + m, m, nullptr);
+    if (e == hipSuccess) e = hipMalloc(&tmp, tb > 0 ? tb : 1);
+    if (e == hipSuccess) e = nbx_sort_pairs_u64(tmp, tb, keys, keys + m, idx, idx + m, m, nullptr);
+    BallManyArgs A;
+    memset(&A, 0, sizeof(A));
+    for (int d = 0; d < 3; ++d) { A.q[d] = dq + d * m; A.period[d] = t->p.period[d]; }
+    A.m = m;
+    A.r2 = r2;
+    A.periodic = per;
+    A.count = cnt;
+    A.order = idx + m;
+    DevTree T = t->view();
+    if (e == hipSuccess) e = nbx_launch_ball_many(T, A, false, nullptr);
+    std::vector<int64_t> hc((size_t)m);
+    if (e == hipSuccess) e = hipMemcpy(hc.data(), cnt, sizeof(int64_t) * m, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) {
+        for (int64_t i = 0; i < m; ++i) offsets[i + 1] = offsets[i] + hc[i];
+        const int64_t total = offsets[m];
+        if (total > 0 && total <= cap && ids) {
+            int32_t *dids = nullptr;
+            double *dd2 = nullptr;
+            e = hipMalloc((void **)&doff, sizeof(int64_t) * m);
+            if (e == hipSuccess) e = hipMemcpy(doff, offsets, sizeof(int64_t) * m, hipMemcpyHostToDevice);
+            if (e == hipSuccess) e = hipMalloc((void **)&dids, sizeof(int32_t) * total);
+            if (e == hipSuccess && d2) e = hipMalloc((void **)&dd2, sizeof(double) * total);
+            if (e == hipSuccess) {
+                A.off = doff;
+                A.ids = dids;
+                A.d2 = dd2;
+                e = nbx_launch_ball_many(T, A, true, nullptr);
+            }
+            if (e == hipSuccess) e = hipMemcpy(ids, dids, sizeof(int32_t) * total, hipMemcpyDeviceToHost);
+            if (e == hipSuccess && d2) e = hipMemcpy(d2, dd2, sizeof(double) * total, hipMemcpyDeviceToHost);
+            if (dids) hipFree(dids);
+            if (dd2) hipFree(dd2);
+        }
+    }
+    hipFree(dq);
+    hipFree(cnt);
+    hipFree(keys);
+    hipFree(idx);
+    if (tmp) hipFree(tmp);
+    if (doff) hipFree(doff);
+    if (e != hipSuccess) return hipfail(e, "nbx_ball_points_many");
+    return NBX_OK;
+}
+
 // ------------------------------------------------------------------ ball search
 // IDs (and d2, when d2out is given) of the particles with d2 < r2 around x,
 // through the images of SearchBallPosPeriodicTagged (KDSplitNode.cxx:1388-1414).

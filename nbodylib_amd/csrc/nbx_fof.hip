@@ -386,6 +386,146 @@ __global__ __launch_bounds__(256) void k_fof_out(const int32_t *P, const int32_t
 
 static inline int nblk(int64_t n) { return (int)((n + 255) / 256); }
 
+// ---------------------------------------------------------------- ball search, many positions
+// SearchBallPosTagged(x, r2) (KDFindNearest.cxx:618-688) for m positions at
+// once: one wave per 64 positions walks the union of the nodes any lane's
+// ball reaches (per-lane FP64 box bounds, the reference's strict d2 < r2 with
+// x -> z sums), then its periodic images (SearchBallPosPeriodicTagged,
+// KDSplitNode.cxx:1388-1414: image when r2 > s^2, the position shifted
+// first).  FILL 0 counts hits per position; FILL 1 writes each position's IDs
+// (and d2) from its offset.  Positions are taken in the given walk order
+// (Morton order of the positions, so a wave's balls are neighbours).
+template <int FILL>
+__device__ __forceinline__ void ball_leaf(const DevTree &T, int leaf, const double *q, bool act, double r2,
+                                          const BallManyArgs &A, int64_t &at) {
+    const int base = leaf * NBX_LEAF;
+    typedef double dxc __attribute__((ext_vector_type(8)));
+    typedef const __attribute__((address_space(4))) dxc *cdcptr;
+#pragma unroll
+    for (int jj = 0; jj < NBX_LEAF; jj += 8) {
+        dxc cv[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) cv[d] = *(cdcptr)(T.x[d] + base + jj);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            double s = 0.0;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const double t = q[d] - cv[d][u];
+                s += t * t;
+            }
+            if (act && s < r2) {  // padding slots are 1e300 away
+                if (FILL) {
+                    A.ids[at] = T.perm[base + jj + u];
+                    if (A.d2) A.d2[at] = s;
+                }
+                ++at;
+            }
+        }
+    }
+}
+
+template <int FILL>
+__device__ __forceinline__ void ball_walk(const DevTree &T, const double *q, bool act, double r2,
+                                          const BallManyArgs &A, int64_t &at) {
+    int node = 1;
+    bool down = true;
+    for (;;) {
+        if (down) {
+            cdptr bx = (cdptr)node_box(T, node, 3);
+            const bool want = act && box_lb<3>(q, bx) < r2;
+            if (ballot(want)) {
+                if (node >= T.nleafp) {
+                    ball_leaf<FILL>(T, node - T.nleafp, q, want, r2, A, at);
+                    down = false;
+                } else {
+                    node = 2 * node;
+                    continue;
+                }
+            } else {
+                down = false;
+            }
+        }
+        if (node == 1) break;
+        if (!(node & 1)) {
+            node ^= 1;
+            down = true;
+        } else {
+            node >>= 1;
+        }
+    }
+}
+
+__device__ __forceinline__ uint64_t morton_spread(uint64_t v) {
+    v &= 0x1fffffULL;
+    v = (v | (v << 32)) & 0x1f00000000ffffULL;
+    v = (v | (v << 16)) & 0x1f0000ff0000ffULL;
+    v = (v | (v << 8)) & 0x100f00f00f00f00fULL;
+    v = (v | (v << 4)) & 0x10c30c30c30c30c3ULL;
+    v = (v | (v << 2)) & 0x1249249249249249ULL;
+    return v;
+}
+__global__ __launch_bounds__(256) void k_morton_keys(const double *qx, const double *qy, const double *qz, int64_t m,
+                                                     double lo0, double lo1, double lo2, double s0, double s1,
+                                                     double s2, uint64_t *keys, int32_t *idx) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= m) return;
+    const double q[3] = {qx[i], qy[i], qz[i]}, lo[3] = {lo0, lo1, lo2}, sc[3] = {s0, s1, s2};
+    uint64_t key = 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const double t = fmin(fmax((q[d] - lo[d]) * sc[d], 0.0), 2097151.0);
+        key |= morton_spread((uint64_t)t) << (2 - d);
+    }
+    keys[i] = key;
+    idx[i] = (int32_t)i;
+}
+
+template <int FILL>
+__global__ __launch_bounds__(64) void k_ball_many(DevTree T, BallManyArgs A) {
+    const int64_t r = (int64_t)blockIdx.x * NBX_WAVE + threadIdx.x;
+    const bool valid = r < A.m;
+    const int64_t qi = valid ? (A.order ? (int64_t)A.order[r] : r) : 0;
+    double q0[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) q0[d] = valid ? A.q[d][qi] : 0.0;
+    int64_t at = (FILL && valid) ? A.off[qi] : 0;
+    ball_walk<FILL>(T, q0, valid, A.r2, A, at);
+    if (A.periodic) {
+        const double *per = A.period;
+        for (int pass = 1; pass < 8; ++pass) {
+            double xp[3] = {q0[0], q0[1], q0[2]};
+            double sval = 0.0;
+            if (pass <= 3) {
+                const int k = pass - 1;
+                if (q0[k] < per[k] / 2.0) { xp[k] = q0[k] + per[k]; sval = q0[k]; }
+                else { xp[k] = q0[k] - per[k]; sval = -xp[k]; }
+            } else {
+                const int k1 = (pass == 6) ? 1 : 0, k2 = (pass == 4) ? 1 : 2;
+                for (int k = 0; k < 3; ++k) {
+                    if (!(pass == 7 || k == k1 || k == k2)) continue;
+                    if (q0[k] < per[k] / 2.0) { xp[k] = q0[k] + per[k]; sval += q0[k] * q0[k]; }
+                    else { xp[k] = q0[k] - per[k]; sval += xp[k] * xp[k]; }
+                }
+                sval = sqrt(sval);
+            }
+            const bool act = valid && A.r2 > sval * sval;
+            if (!ballot(act)) continue;
+            ball_walk<FILL>(T, xp, act, A.r2, A, at);
+        }
+    }
+    if (!FILL && valid) A.count[qi] = at;
+}
+
+hipError_t nbx_launch_ball_many(const DevTree &T, const BallManyArgs &A, bool fill, hipStream_t st) {
+    const int blocks = (int)((A.m + NBX_WAVE - 1) / NBX_WAVE);
+    if (blocks == 0) return hipSuccess;
+    ProfScope ps("k_ball_many", st);
+    if (fill) hipLaunchKernelGGL(k_ball_many<1>, dim3(blocks), dim3(NBX_WAVE), 0, st, T, A);
+    else hipLaunchKernelGGL(k_ball_many<0>, dim3(blocks), dim3(NBX_WAVE), 0, st, T, A);
+    return hipGetLastError();
+}
+
 template <int ND, int MT>
 static void launch_fof_t(const DevTree &T, const FofArgs &A, uint8_t *conn, hipStream_t st) {
     {

@@ -100,6 +100,25 @@ hipError_t nbx_launch_fof(const DevTree &T, const FofArgs &A, hipStream_t st);
 hipError_t nbx_launch_fof_finish(const DevTree &T, int32_t *parent, int32_t *count, int32_t *minid, int64_t n,
                                  hipStream_t st);
 
+// ------------------------------------------------------------------ ball search, many positions
+struct BallManyArgs {
+    const double *q[3];  // positions, SoA
+    int64_t m;
+    double r2;
+    int32_t periodic;
+    double period[3];
+    int64_t *count;      // counting pass: hits per position
+    const int64_t *off;  // filling pass: first output index per position
+    int32_t *ids;        // filling pass
+    double *d2;          // filling pass, nullable
+    const int32_t *order;  // positions in walk order (nullable: as given)
+};
+hipError_t nbx_launch_ball_many(const DevTree &T, const BallManyArgs &A, bool fill, hipStream_t st);
+// Morton keys (21 bits a dimension over the box lo + [0, 2^21 / s)) and
+// their indices, for ordering query positions along the space-filling curve
+__global__ void k_morton_keys(const double *qx, const double *qy, const double *qz, int64_t m, double lo0, double lo1,
+                              double lo2, double s0, double s1, double s2, uint64_t *keys, int32_t *idx);
+
 // ------------------------------------------------------------------ smoothed velocities
 struct SmoothArgs {
     const int32_t *nn;  // n x k neighbour slots, rows by slot

@@ -306,3 +306,64 @@ def test_ball_search(gpu):
             d2 = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
             exp.extend(np.nonzero(d2 < 0.01)[0].tolist())
         assert np.array_equal(got, np.sort(np.array(exp, dtype=np.int32)))
+
+
+def _ball_brute(pos, x, r2, period):
+    """IDs with d2 < r2 around x and its images (image when r2 > s^2, the
+    position shifted first: SearchBallPosPeriodicTagged), with their d2."""
+    pts = [np.array(x, dtype=np.float64)]
+    if period is not None:
+        x0 = pts[0]
+        for ax in ([0], [1], [2], [0, 1], [0, 2], [1, 2], [0, 1, 2]):
+            xp = x0.copy()
+            s = 0.0
+            for a in ax:
+                if x0[a] < period / 2.0:
+                    xp[a] = x0[a] + period
+                    s += x0[a] * x0[a]
+                else:
+                    xp[a] = x0[a] - period
+                    s += xp[a] * xp[a]
+            s = np.sqrt(s) if len(ax) > 1 else (x0[ax[0]] if x0[ax[0]] < period / 2.0 else -xp[ax[0]])
+            if r2 > s * s:
+                pts.append(xp)
+    ids, d2s = [], []
+    for q in pts:
+        d = q[None, :] - pos
+        d2 = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
+        hit = np.nonzero(d2 < r2)[0]
+        ids.extend(hit.tolist())
+        d2s.extend(d2[hit].tolist())
+    return np.array(ids, dtype=np.int32), np.array(d2s)
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_ball_search_many(gpu, periodic):
+    """SearchBallPosTagged for 700 positions in one call (inside, outside and
+    on the box edges, and on particles): every row's IDs and d2 equal a
+    brute force with the reference's image rule; an empty radius gives empty
+    rows; too small a cap writes only the offsets."""
+    nb = _nb()
+    n, m = 5000, 700
+    pos = datagen.uniform(n, 1.0, 46)
+    rng = np.random.default_rng(46)
+    x = np.concatenate([rng.uniform(-0.1, 1.1, (400, 3)), pos[:200], rng.uniform(0, 1, (100, 3)) * [1, 0, 1]])
+    per = 1.0 if periodic else None
+    say(f"ball many n={n} m={m} periodic={periodic}")
+    t = nb.KDTree(pos, kerntype=nb.KSPH, period=per)
+    r2 = 0.006
+    off, ids, d2 = t.SearchBallPosTaggedMany(x, r2)
+    assert off[0] == 0 and np.all(np.diff(off) >= 0)
+    for i in range(m):
+        bi, bd = _ball_brute(pos, x[i], r2, per)
+        row, rd = ids[off[i]:off[i + 1]], d2[off[i]:off[i + 1]]
+        o1, o2 = np.argsort(row, kind="stable"), np.argsort(bi, kind="stable")
+        assert np.array_equal(row[o1], bi[o2]), i
+        assert np.array_equal(rd[o1], bd[o2]), i
+    off0, ids0, _ = t.SearchBallPosTaggedMany(x, 0.0)
+    assert off0[-1] == 0 and ids0.size == 0
+    o = np.zeros(m + 1, dtype=np.int64)
+    got = np.full(4, -7, dtype=np.int32)
+    qq = np.ascontiguousarray(x.T)
+    nb._check(nb.lib().nbx_ball_points_many(t._h, nb._p(qq), m, r2, nb._p(o), nb._p(got), None, 3))
+    assert np.array_equal(o, off) and np.all(got == -7)
