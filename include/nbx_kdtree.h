@@ -28,6 +28,7 @@
 #ifndef NBX_KDTREE_H
 #define NBX_KDTREE_H
 
+#include <cstdio>
 #include <mutex>
 #include <vector>
 
@@ -82,6 +83,42 @@ class Particle {
     Double_t GetPotential() const { return phi; }
     void SetPotential(const Double_t &p) { phi = p; }
 };
+
+// The reference driver's raw particle file (src/tests/test_kdtree.cxx:60-73,
+// :181-208): six doubles per particle, GetPhase(0..5), no header.  Reading
+// sets mass 1 and ID / PID = input index, as the driver does; n = 0 reads
+// every record.  Return false on a short or unreadable file.
+inline bool ReadPartsOut(const char *fname, std::vector<Particle> &parts, size_t n = 0) {
+    std::FILE *f = std::fopen(fname, "rb");
+    if (!f) return false;
+    if (n == 0) {
+        std::fseek(f, 0, SEEK_END);
+        n = (size_t)std::ftell(f) / (6 * sizeof(Double_t));
+        std::fseek(f, 0, SEEK_SET);
+    }
+    parts.assign(n, Particle());
+    bool ok = true;
+    for (size_t i = 0; i < n && ok; ++i) {
+        Double_t x[6];
+        ok = std::fread(x, sizeof(Double_t), 6, f) == 6;
+        for (int j = 0; j < 6 && ok; ++j) parts[i].SetPhase(j, x[j]);
+        parts[i].SetMass(1.0);
+        parts[i].SetID((Int_t)i);
+        parts[i].SetPID((Int_t)i);
+    }
+    std::fclose(f);
+    return ok;
+}
+inline bool WritePartsOut(const char *fname, const std::vector<Particle> &parts) {
+    std::FILE *f = std::fopen(fname, "wb");
+    if (!f) return false;
+    bool ok = true;
+    for (const Particle &p : parts) {
+        const Double_t x[6] = {p.GetPhase(0), p.GetPhase(1), p.GetPhase(2), p.GetPhase(3), p.GetPhase(4), p.GetPhase(5)};
+        ok = ok && std::fwrite(x, sizeof(Double_t), 6, f) == 6;
+    }
+    return std::fclose(f) == 0 && ok;
+}
 
 // The results of CalcSmoothVel and CalcSmoothVelDisp: the forms of
 // NBody::Coordinate (src/Math/Coordinate.h:46-78) and the 3x3 NBody::Matrix
