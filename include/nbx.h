@@ -186,6 +186,16 @@ int nbx_fof_dev(nbx_tree *t, double fdist, int32_t minnum, int32_t *labels_by_id
  * dv_j^2 / params[7]) < 1.  The search scales must be >= the comparator's
  * (else the reference's groups depend on its traversal); NBX_ERR_UNSUPPORTED
  * otherwise and for any other comparator.  Labels as nbx_fof. */
+/* FOF's linked-list outputs (KDTree::FOF's pHead, pNext, pTail, pLen,
+ * KDFOF.cxx:29-31, :52-55) built from canonical labels on the host: arrays of n
+ * by tree index (slot; slot_to_id from nbx_tree_order).  Members of group g
+ * form one list in slot order: head[s] = its first slot, next[s] = the next
+ * slot or -1, tail[head] = its last slot; a particle in no group is a list
+ * of its own (head = tail = itself).  len[g] = the size of group g (1..G),
+ * other entries 0.  Any output may be NULL.  The reference's lists run in
+ * its search order. */
+int nbx_fof_lists(const int32_t *labels_by_id, const int32_t *slot_to_id, int64_t n, int64_t ngroups, int32_t *head,
+                  int32_t *next, int32_t *tail, int32_t *len);
 #define NBX_FOF_CMP_3D 1
 #define NBX_FOF_CMP_6D 2
 int nbx_fof_criterion(nbx_tree *t, int32_t comparator, const double *params, int32_t minnum, int32_t *labels_by_id,

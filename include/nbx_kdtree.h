@@ -247,9 +247,10 @@ class KDTree {
     Int_t *FOF(Double_t fdist, Int_t &numgroup, Int_t minnum = 8, int order = 0, Int_tree_t *pHead = nullptr,
                Int_tree_t *pNext = nullptr, Int_tree_t *pTail = nullptr, Int_tree_t *pLen = nullptr);
     // KDTree.h:503, KDFOF.cxx:157-265: cmp = FOF3d (TPHYS tree) or FOF6d (TPHS
-    // tree) with params as the reference's; any other comparator, a check
-    // function other than Pnocheck, or the linked-list outputs are reported
-    // as unsupported.  Labels are canonical, as FOF's.
+    // tree) with params as the reference's; any other comparator or a check
+    // function other than Pnocheck is reported as unsupported.  Labels are
+    // canonical, as FOF's; the linked lists, when given, are nbx_fof_lists'
+    // (members in tree-index order).
     Int_t *FOFCriterion(FOFcompfunc cmp, Double_t *params, Int_t &numgroups, Int_t minnum = 8, int order = 0,
                         int ipcheckflag = 0, FOFcheckfunc check = Pnocheck, Int_tree_t *pHead = nullptr,
                         Int_tree_t *pNext = nullptr, Int_tree_t *pTail = nullptr, Int_tree_t *pLen = nullptr);

@@ -31,7 +31,7 @@ NBX_ERR_ARG, NBX_ERR_HIP, NBX_ERR_UNSUPPORTED, NBX_ERR_NOMEM, NBX_ERR_SEARCH = -
 EXPORTS = [
     "nbx_params_default", "nbx_tree_create", "nbx_tree_create_dev", "nbx_tree_destroy", "nbx_tree_order",
     "nbx_tree_size", "nbx_tree_kernel", "nbx_knn", "nbx_knn_dev", "nbx_knn_points", "nbx_density_points", "nbx_smooth_vel", "nbx_smooth_vel_moments", "nbx_density",
-    "nbx_density_dev", "nbx_knn_density_dev", "nbx_ball_points", "nbx_ball_points_many", "nbx_ball_mark", "nbx_fof", "nbx_fof_dev", "nbx_fof_criterion", "nbx_fof_criterion_dev",
+    "nbx_density_dev", "nbx_knn_density_dev", "nbx_ball_points", "nbx_ball_points_many", "nbx_ball_mark", "nbx_fof", "nbx_fof_dev", "nbx_fof_criterion", "nbx_fof_criterion_dev", "nbx_fof_lists",
     "nbx_sph_density_fwd_dev", "nbx_sph_density_adjoint_dev", "nbx_prof_enable", "nbx_prof_reset", "nbx_prof_get",
     "nbx_tree_last_fallbacks", "nbx_last_error",
 ]
@@ -90,6 +90,7 @@ def lib():
     L.nbx_fof.argtypes = [vp, dbl, i32, vp, P(i64)]
     L.nbx_fof_dev.argtypes = [vp, dbl, i32, vp, P(i64), vp]
     L.nbx_fof_criterion.argtypes = [vp, i32, vp, i32, vp, P(i64)]
+    L.nbx_fof_lists.argtypes = [vp, vp, i64, i64, vp, vp, vp, vp]
     L.nbx_fof_criterion_dev.argtypes = [vp, i32, vp, i32, vp, P(i64), vp]
     L.nbx_sph_density_fwd_dev.argtypes = [vp, vp, vp, i64, i32, vp, vp, vp]
     L.nbx_sph_density_adjoint_dev.argtypes = [vp, vp, vp, i64, i32, vp, vp, vp, vp]
@@ -362,6 +363,17 @@ class KDTree:
         ng = ctypes.c_int64()
         _check(lib().nbx_fof_criterion(self._h, int(comparator), _p(p), minnum, _p(lab), ctypes.byref(ng)))
         return lab, ng.value
+
+
+def fof_lists(labels_by_id, slot_to_id, ngroups):
+    """FOF's pHead/pNext/pTail/pLen (KDFOF.cxx:29-31) from canonical labels, by tree index (nbx_fof_lists):
+    members of each group linked in slot order; host only."""
+    lab = np.ascontiguousarray(labels_by_id, dtype=np.int32)
+    order = np.ascontiguousarray(slot_to_id, dtype=np.int32)
+    n = lab.shape[0]
+    head, nxt, tail, ln = (np.zeros(n, dtype=np.int32) for _ in range(4))
+    _check(lib().nbx_fof_lists(_p(lab), _p(order), n, int(ngroups), _p(head), _p(nxt), _p(tail), _p(ln)))
+    return head, nxt, tail, ln
 
 
 def prof_enable(on=True):

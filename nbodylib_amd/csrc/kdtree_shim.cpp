@@ -322,13 +322,23 @@ Coordinate *KDTree::CalcSmoothVelKurtosis(Coordinate *smvel, Matrix *smveldisp, 
 }
 
 // ------------------------------------------------------------------ FOF
-Int_t *KDTree::FOF(Double_t fdist, Int_t &numgroup, Int_t minnum, int, Int_tree_t *, Int_tree_t *, Int_tree_t *,
-                   Int_tree_t *) {
+// pHead/pNext/pTail/pLen (KDFOF.cxx:29-31): filled when given, by tree index
+static void fof_lists(nbx_tree *tree, Int_t n, const Int_t *pGroup, int64_t ng, Int_tree_t *pHead, Int_tree_t *pNext,
+                      Int_tree_t *pTail, Int_tree_t *pLen) {
+    if (!pHead && !pNext && !pTail && !pLen) return;
+    std::vector<int32_t> order(n);
+    check(nbx_tree_order(tree, order.data()), "FOF");
+    check(nbx_fof_lists(pGroup, order.data(), n, ng, pHead, pNext, pTail, pLen), "FOF");
+}
+
+Int_t *KDTree::FOF(Double_t fdist, Int_t &numgroup, Int_t minnum, int, Int_tree_t *pHead, Int_tree_t *pNext,
+                   Int_tree_t *pTail, Int_tree_t *pLen) {
     std::lock_guard<std::mutex> lk(mu);
     Int_t *pGroup = new Int_t[numparts];
     int64_t ng = 0;
     check(nbx_fof(tree, fdist, minnum, pGroup, &ng), "FOF");
     numgroup = (Int_t)ng;
+    fof_lists(tree, numparts, pGroup, ng, pHead, pNext, pTail, pLen);
     return pGroup;
 }
 
@@ -341,7 +351,6 @@ Int_t *KDTree::FOFCriterion(FOFcompfunc cmp, Double_t *params, Int_t &numgroups,
     std::string why;
     if (!c) why = "only the built-in comparators FOF3d and FOF6d are provided";
     else if (ipcheckflag && chk != &Pnocheck) why = "check functions other than Pnocheck are not provided";
-    else if (pHead || pNext || pTail || pLen) why = "the linked-list outputs are not provided";
     if (!why.empty()) {
         if (g_throw) throw std::runtime_error("FOFCriterion: " + why);
         std::fprintf(stderr, "FOFCriterion: %s\n", why.c_str());
@@ -353,6 +362,7 @@ Int_t *KDTree::FOFCriterion(FOFcompfunc cmp, Double_t *params, Int_t &numgroups,
     if (rc < 0) delete[] pGroup;
     check(rc, "FOFCriterion");
     numgroups = (Int_t)ng;
+    fof_lists(tree, numparts, pGroup, ng, pHead, pNext, pTail, pLen);
     return pGroup;
 }
 

@@ -5,6 +5,7 @@ import ctypes
 import os
 import re
 
+import numpy as np
 import pytest
 
 import nbodylib_amd as nb
@@ -85,3 +86,33 @@ def test_python_mirror_fails_loudly_without_library(tmp_path, monkeypatch):
     monkeypatch.undo()
     importlib.reload(nb)
     nb.lib()
+
+
+def test_fof_lists_from_labels():
+    """nbx_fof_lists (host only): each group's members linked once, in slot
+    order, from its head; tail of the head is the last member; len[g] is the
+    group size; particles in no group are lists of one."""
+    import oracle
+    from nbodylib_amd import datagen
+
+    n = 3000
+    pos = datagen.uniform(n, 1.0, 53)
+    lab, ng = oracle.fof_brute(pos, 0.6 * datagen.mean_spacing(n, 1.0), 3)
+    assert ng > 5
+    order = np.random.default_rng(53).permutation(n).astype(np.int32)  # slot -> ID
+    head, nxt, tail, ln = nb.fof_lists(lab, order, ng)
+    g_of_slot = lab[order]
+    for g in range(1, ng + 1):
+        slots = np.nonzero(g_of_slot == g)[0]
+        assert ln[g] == slots.size
+        walk, s = [], slots[0]
+        while s != -1:
+            walk.append(s)
+            s = nxt[s]
+        assert walk == slots.tolist()
+        assert np.all(head[slots] == slots[0]) and tail[slots[0]] == slots[-1]
+    lone = np.nonzero(g_of_slot == 0)[0]
+    assert np.all(head[lone] == lone) and np.all(tail[lone] == lone) and np.all(nxt[lone] == -1)
+    assert ln[0] == 0 and not ln[ng + 1:].any()
+    with pytest.raises(nb.NbxError):
+        nb.fof_lists(lab, order[::-1] * 0 + n, ng)
