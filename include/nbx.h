@@ -142,6 +142,22 @@ int nbx_smooth_vel(nbx_tree *t, int32_t k, const double *rho_by_id, const double
  * skew_by_id, kurt_by_id: n x 3 by ID, either may be NULL (not both). */
 int nbx_smooth_vel_moments(nbx_tree *t, int32_t k, const double *rho_by_id, const double *mean_in_by_id,
                            const double *disp_in_by_id, double *skew_by_id, double *kurt_by_id);
+/* Local mean and dispersion of a per-particle scalar (TPHYS trees), the
+ * gather form with the full weight W(r/h) h^-3: mean = sum_j W/rho_j * m_j *
+ * w_j and disp = sum_j W/rho_j * m_j * (w_j - mean_in)^2 over the k nearest.
+ * nbx_smooth_local replaces CalcSmoothLocalMeanParticle(target, weight,
+ * Nsmooth, densityset) KDCalcSmoothQuantities.cxx:1539-1577 and
+ * CalcSmoothLocalDispParticle :1579-1617 for every particle (CalcDensity's
+ * neighbours); nbx_smooth_local_points replaces the Position forms
+ * :1619-1693 for m positions (q: 3*m SoA; the non-periodic search from x, no
+ * exclusion).  weight_by_id: n values; rho_by_id: the densities divided by
+ * (densityset = 1), or NULL to run CalcDensity(k) first; mean_in: the mean
+ * each dispersion is taken about (by ID, or by position row), needed only
+ * for the dispersion.  Outputs by ID (or by row), each nullable. */
+int nbx_smooth_local(nbx_tree *t, int32_t k, const double *weight_by_id, const double *rho_by_id,
+                     const double *mean_in_by_id, double *mean_by_id, double *disp_by_id);
+int nbx_smooth_local_points(nbx_tree *t, const double *q, int64_t m, int32_t k, const double *weight_by_id,
+                            const double *rho_by_id, const double *mean_in, double *mean_out, double *disp_out);
 /* Fused: kNN (NBX_KNN_NONPERIODIC semantics) + symmetric density in one pass,
  * neighbour IDs by slot.  The batch form of FindNearest + CalcDensity. */
 int nbx_knn_density_dev(nbx_tree *t, int32_t k, int32_t *nn_by_slot_dev, double *rho_by_id_dev,

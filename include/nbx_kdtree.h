@@ -239,6 +239,15 @@ class KDTree {
                                   int meanvelset = 1, int veldispset = 1);
     Coordinate *CalcSmoothVelKurtosis(Coordinate *smvel, Matrix *smveldisp, Int_t Nsmooth = 64, int densityset = 1,
                                       int meanvelset = 1, int veldispset = 1);
+    // KDTree.h:454-459, KDCalcSmoothQuantities.cxx:1539-1693: weight by ID.
+    // Each call runs one GPU pass (every particle, or the one position); for
+    // many targets call nbx_smooth_local / nbx_smooth_local_points once.
+    Double_t CalcSmoothLocalMeanParticle(Int_t target, Double_t *weight, Int_t Nsmooth = 64, int densityset = 1);
+    Double_t CalcSmoothLocalDispParticle(Int_t target, Double_t *weight, Double_t localmean, Int_t Nsmooth = 64,
+                                         int densityset = 1);
+    Double_t CalcSmoothLocalMeanPosition(Double_t *x, Double_t *weight, Int_t Nsmooth = 64, int densityset = 1);
+    Double_t CalcSmoothLocalDispPosition(Double_t *x, Double_t *weight, Double_t localmean, Int_t Nsmooth = 64,
+                                         int densityset = 1);
     void CalcVelDensity(Int_t Nsmooth = 64, Int_t Nsearch = 64);
     Double_t CalcDensityParticle(Int_t target, Int_t Nsmooth = 64);
     // KDTree.h:432 (TPHYS trees; v is unused there); KDCalcSmoothQuantities.cxx:1092-1149
@@ -265,6 +274,8 @@ class KDTree {
 
   private:
     void density_to_bucket(Int_t Nsmooth);
+    Double_t smooth_local_one(Int_t target, const Double_t *x, Double_t *weight, const Double_t *mean, Int_t Nsmooth,
+                              int densityset);
     Coordinate *smooth_moment(int which, Coordinate *smvel, Matrix *smveldisp, Int_t Nsmooth, int densityset,
                               int meanvelset, int veldispset);
     std::vector<double> rho_by_id();

@@ -30,7 +30,7 @@ NBX_ERR_ARG, NBX_ERR_HIP, NBX_ERR_UNSUPPORTED, NBX_ERR_NOMEM, NBX_ERR_SEARCH = -
 
 EXPORTS = [
     "nbx_params_default", "nbx_tree_create", "nbx_tree_create_dev", "nbx_tree_destroy", "nbx_tree_order",
-    "nbx_tree_size", "nbx_tree_kernel", "nbx_knn", "nbx_knn_dev", "nbx_knn_points", "nbx_density_points", "nbx_smooth_vel", "nbx_smooth_vel_moments", "nbx_density",
+    "nbx_tree_size", "nbx_tree_kernel", "nbx_knn", "nbx_knn_dev", "nbx_knn_points", "nbx_density_points", "nbx_smooth_vel", "nbx_smooth_vel_moments", "nbx_smooth_local", "nbx_smooth_local_points", "nbx_density",
     "nbx_density_dev", "nbx_knn_density_dev", "nbx_ball_points", "nbx_ball_points_many", "nbx_ball_mark", "nbx_fof", "nbx_fof_dev", "nbx_fof_criterion", "nbx_fof_criterion_dev", "nbx_fof_lists",
     "nbx_sph_density_fwd_dev", "nbx_sph_density_adjoint_dev", "nbx_prof_enable", "nbx_prof_reset", "nbx_prof_get",
     "nbx_tree_last_fallbacks", "nbx_last_error",
@@ -81,6 +81,8 @@ def lib():
     L.nbx_density_points.argtypes = [vp, vp, i64, i32, vp, vp]
     L.nbx_smooth_vel.argtypes = [vp, i32, vp, vp, vp, vp]
     L.nbx_smooth_vel_moments.argtypes = [vp, i32, vp, vp, vp, vp, vp]
+    L.nbx_smooth_local.argtypes = [vp, i32, vp, vp, vp, vp, vp]
+    L.nbx_smooth_local_points.argtypes = [vp, vp, i64, i32, vp, vp, vp, vp, vp]
     L.nbx_density.argtypes = [vp, i32, i32, i32, vp, vp]
     L.nbx_density_dev.argtypes = [vp, i32, i32, i32, vp, vp, vp]
     L.nbx_knn_density_dev.argtypes = [vp, i32, vp, vp, vp, vp]
@@ -285,6 +287,44 @@ class KDTree:
     def CalcSmoothVelKurtosis(self, Nsmooth=64, rho=None, mean=None, disp=None):
         """CalcSmoothVelKurtosis (KDCalcSmoothQuantities.cxx:692-767): (n, 3) by ID."""
         return self.CalcSmoothVelMoments(Nsmooth, rho, mean, disp)[1]
+
+    def CalcSmoothLocalMeanAll(self, weight, Nsmooth=64, rho=None):
+        """CalcSmoothLocalMeanParticle(i, weight, Nsmooth) (KDCalcSmoothQuantities.cxx:1539-1577) for every
+        particle: (n,) by ID.  weight: one value per particle (by ID); rho: the densities to divide by, or
+        None to compute CalcDensity first."""
+        return self._local(weight, Nsmooth, rho, None)[0]
+
+    def CalcSmoothLocalDispAll(self, weight, mean, Nsmooth=64, rho=None):
+        """CalcSmoothLocalDispParticle(i, weight, mean[i], Nsmooth) (:1579-1617) for every particle."""
+        return self._local(weight, Nsmooth, rho, mean)[1]
+
+    def CalcSmoothLocalMeanPosition(self, x, weight, Nsmooth=64, rho=None):
+        """CalcSmoothLocalMeanPosition(x, weight, Nsmooth) (:1619-1655) at every row of x (m, 3)."""
+        return self._local(weight, Nsmooth, rho, None, x)[0]
+
+    def CalcSmoothLocalDispPosition(self, x, weight, mean, Nsmooth=64, rho=None):
+        """CalcSmoothLocalDispPosition(x, weight, mean, Nsmooth) (:1657-1693) at every row of x (m, 3); mean:
+        one value per row."""
+        return self._local(weight, Nsmooth, rho, mean, x)[1]
+
+    def _local(self, weight, k, rho, mean_in, x=None):
+        w = np.ascontiguousarray(weight, dtype=np.float64)
+        if w.shape != (self.n,):
+            raise ValueError("weight must have one value per particle")
+        r = None if rho is None else np.ascontiguousarray(rho, dtype=np.float64)
+        mi = None if mean_in is None else np.ascontiguousarray(mean_in, dtype=np.float64)
+        if x is None:
+            mean = np.zeros(self.n)
+            disp = np.zeros(self.n) if mi is not None else None
+            _check(lib().nbx_smooth_local(self._h, k, _p(w), _p(r), _p(mi), _p(mean), _p(disp)))
+        else:
+            x = np.asarray(x, dtype=np.float64).reshape(-1, 3)
+            mm = x.shape[0]
+            q = np.ascontiguousarray(x.T)
+            mean = np.zeros(mm)
+            disp = np.zeros(mm) if mi is not None else None
+            _check(lib().nbx_smooth_local_points(self._h, _p(q), mm, k, _p(w), _p(r), _p(mi), _p(mean), _p(disp)))
+        return mean, disp
 
     def _smooth(self, k, rho, with_disp, mean_in=None):
         mean = np.zeros((self.n, 3), dtype=np.float64)

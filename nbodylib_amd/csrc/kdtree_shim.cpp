@@ -321,6 +321,44 @@ Coordinate *KDTree::CalcSmoothVelKurtosis(Coordinate *smvel, Matrix *smveldisp, 
     return smooth_moment(1, smvel, smveldisp, Nsmooth, densityset, meanvelset, veldispset);
 }
 
+// One local mean or dispersion: at particle `target` (x null) or position x.
+// The weights are by ID, as the reference indexes them (weight[GetID()]).
+Double_t KDTree::smooth_local_one(Int_t target, const Double_t *x, Double_t *weight, const Double_t *mean,
+                                  Int_t Nsmooth, int densityset) {
+    if (densityset != 1) CalcDensity(Nsmooth);
+    std::lock_guard<std::mutex> lk(mu);
+    const std::vector<double> rho = rho_by_id();
+    double out = 0.0;
+    if (x) {
+        const double q[3] = {x[0], x[1], x[2]};
+        check(nbx_smooth_local_points(tree, q, 1, Nsmooth, weight, rho.data(), mean, mean ? nullptr : &out,
+                                      mean ? &out : nullptr),
+              "CalcSmoothLocalMean failed");
+        return out;
+    }
+    std::vector<double> res(numparts), mi;
+    const Int_t id = bucket[target].GetID();
+    if (mean) mi.assign(numparts, *mean);
+    check(nbx_smooth_local(tree, Nsmooth, weight, rho.data(), mean ? mi.data() : nullptr, mean ? nullptr : res.data(),
+                           mean ? res.data() : nullptr),
+          "CalcSmoothLocalMean failed");
+    return res[id];
+}
+Double_t KDTree::CalcSmoothLocalMeanParticle(Int_t target, Double_t *weight, Int_t Nsmooth, int densityset) {
+    return smooth_local_one(target, nullptr, weight, nullptr, Nsmooth, densityset);
+}
+Double_t KDTree::CalcSmoothLocalDispParticle(Int_t target, Double_t *weight, Double_t localmean, Int_t Nsmooth,
+                                             int densityset) {
+    return smooth_local_one(target, nullptr, weight, &localmean, Nsmooth, densityset);
+}
+Double_t KDTree::CalcSmoothLocalMeanPosition(Double_t *x, Double_t *weight, Int_t Nsmooth, int densityset) {
+    return smooth_local_one(0, x, weight, nullptr, Nsmooth, densityset);
+}
+Double_t KDTree::CalcSmoothLocalDispPosition(Double_t *x, Double_t *weight, Double_t localmean, Int_t Nsmooth,
+                                             int densityset) {
+    return smooth_local_one(0, x, weight, &localmean, Nsmooth, densityset);
+}
+
 // ------------------------------------------------------------------ FOF
 // pHead/pNext/pTail/pLen (KDFOF.cxx:29-31): filled when given, by tree index
 static void fof_lists(nbx_tree *tree, Int_t n, const Int_t *pGroup, int64_t ng, Int_tree_t *pHead, Int_tree_t *pNext,

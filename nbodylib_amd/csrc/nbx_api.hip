@@ -705,6 +705,117 @@ extern "C" int nbx_smooth_vel_moments(nbx_tree *t, int32_t k, const double *rho_
     return smooth_impl(t, k, rho_by_id, mean_in_by_id, disp_in_by_id, nullptr, nullptr, skew_by_id, kurt_by_id);
 }
 
+// CalcSmoothLocalMeanParticle / CalcSmoothLocalDispParticle for every
+// particle: CalcDensity's neighbour rows (fused pass), then the gather sums.
+extern "C" int nbx_smooth_local(nbx_tree *t, int32_t k, const double *weight_by_id, const double *rho_by_id,
+                                const double *mean_in_by_id, double *mean_by_id, double *disp_by_id) {
+    if (!t || !weight_by_id || (!mean_by_id && !disp_by_id)) return fail(NBX_ERR_ARG, "NULL argument");
+    if (disp_by_id && !mean_in_by_id) return fail(NBX_ERR_ARG, "the dispersion needs the means (mean_in_by_id)");
+    if (t->nd != 3) return fail(NBX_ERR_UNSUPPORTED, "CalcSmoothLocalMean here needs a TPHYS tree");
+    if (k < 1 || k > 64) return fail(NBX_ERR_UNSUPPORTED, "Nsmooth out of range (1..64)");
+    const int64_t n = t->n;
+    hipStream_t st = nullptr;
+    int32_t *nn = nullptr;
+    double *b[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    HIPCK(hipMalloc((void **)&nn, sizeof(int32_t) * (size_t)n * k));
+    for (int i = 0; i < 8; ++i) HIPCK(hipMalloc((void **)&b[i], sizeof(double) * n));
+    double *rid = b[0], *rslot = b[1], *wid = b[2], *wslot = b[3], *mid = b[4], *mslot = b[5], *om = b[6], *od = b[7];
+    int rc = density_impl(t, k, NBX_DENS_SYMMETRIC, 0, rid, nullptr, nn, st);
+    hipError_t e = hipSuccess;
+    if (rc == NBX_OK) {
+        if (rho_by_id) e = hipMemcpy(rid, rho_by_id, sizeof(double) * n, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(wid, weight_by_id, sizeof(double) * n, hipMemcpyHostToDevice);
+        if (e == hipSuccess && mean_in_by_id) e = hipMemcpy(mid, mean_in_by_id, sizeof(double) * n, hipMemcpyHostToDevice);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(k_gather_by_id, dim3(nblk(n)), dim3(256), 0, st, rid, t->perm, n, rslot);
+            hipLaunchKernelGGL(k_gather_by_id, dim3(nblk(n)), dim3(256), 0, st, wid, t->perm, n, wslot);
+            if (mean_in_by_id) hipLaunchKernelGGL(k_gather_by_id, dim3(nblk(n)), dim3(256), 0, st, mid, t->perm, n, mslot);
+            LocalArgs A = {{nullptr, nullptr, nullptr}, n, nn, k, 0, rslot, wslot, mean_in_by_id ? mslot : nullptr,
+                           om, od};
+            e = nbx_launch_smooth_local(t->view(), A, st);
+        }
+        if (e == hipSuccess && mean_by_id) {
+            hipLaunchKernelGGL(k_scatter_by_id, dim3(nblk(n)), dim3(256), 0, st, om, t->perm, rid, n);
+            e = hipMemcpy(mean_by_id, rid, sizeof(double) * n, hipMemcpyDeviceToHost);
+        }
+        if (e == hipSuccess && disp_by_id) {
+            hipLaunchKernelGGL(k_scatter_by_id, dim3(nblk(n)), dim3(256), 0, st, od, t->perm, rid, n);
+            e = hipMemcpy(disp_by_id, rid, sizeof(double) * n, hipMemcpyDeviceToHost);
+        }
+        if (e != hipSuccess) rc = hipfail(e, "nbx_smooth_local");
+    }
+    hipFree(nn);
+    for (int i = 0; i < 8; ++i) hipFree(b[i]);
+    return rc;
+}
+
+// CalcSmoothLocalMeanPosition / CalcSmoothLocalDispPosition for m positions:
+// the node search from x (non-periodic, no exclusion) and the gather sums.
+extern "C" int nbx_smooth_local_points(nbx_tree *t, const double *q, int64_t m, int32_t k, const double *weight_by_id,
+                                       const double *rho_by_id, const double *mean_in, double *mean_out,
+                                       double *disp_out) {
+    if (!t || !q || !weight_by_id || (!mean_out && !disp_out)) return fail(NBX_ERR_ARG, "NULL argument");
+    if (disp_out && !mean_in) return fail(NBX_ERR_ARG, "the dispersion needs the means (mean_in)");
+    if (t->nd != 3) return fail(NBX_ERR_UNSUPPORTED, "CalcSmoothLocalMeanPosition here needs a TPHYS tree");
+    if (k < 1 || k > 64) return fail(NBX_ERR_UNSUPPORTED, "Nsmooth out of range (1..64)");
+    if (k > t->n) return fail(NBX_ERR_SEARCH, "CalcSmoothLocalMean failed: fewer than Nsmooth particles");
+    if (m <= 0) return NBX_OK;
+    if (m > INT32_MAX) return fail(NBX_ERR_ARG, "too many positions (m < 2^31)");
+    const int64_t n = t->n;
+    hipStream_t st = nullptr;
+    double *dq = nullptr, *rid = nullptr, *rslot = nullptr, *wid = nullptr, *wslot = nullptr, *dmi = nullptr,
+           *dm = nullptr, *dd = nullptr;
+    int32_t *dnn = nullptr;
+    HIPCK(hipMalloc((void **)&dq, sizeof(double) * 3 * m));
+    HIPCK(hipMemcpy(dq, q, sizeof(double) * 3 * m, hipMemcpyHostToDevice));
+    HIPCK(hipMalloc((void **)&dnn, sizeof(int32_t) * m * k));
+    HIPCK(hipMalloc((void **)&rid, sizeof(double) * n));
+    HIPCK(hipMalloc((void **)&rslot, sizeof(double) * n));
+    HIPCK(hipMalloc((void **)&wid, sizeof(double) * n));
+    HIPCK(hipMalloc((void **)&wslot, sizeof(double) * n));
+    HIPCK(hipMalloc((void **)&dm, sizeof(double) * m));
+    HIPCK(hipMalloc((void **)&dd, sizeof(double) * m));
+    if (mean_in) {
+        HIPCK(hipMalloc((void **)&dmi, sizeof(double) * m));
+        HIPCK(hipMemcpy(dmi, mean_in, sizeof(double) * m, hipMemcpyHostToDevice));
+    }
+    int rc = NBX_OK;
+    if (!rho_by_id) rc = density_impl(t, k, NBX_DENS_SYMMETRIC, 0, rid, nullptr, nullptr, st);
+    hipError_t e = hipSuccess;
+    if (rc == NBX_OK) {
+        if (rho_by_id) e = hipMemcpy(rid, rho_by_id, sizeof(double) * n, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(wid, weight_by_id, sizeof(double) * n, hipMemcpyHostToDevice);
+        KnnArgs K;
+        memset(&K, 0, sizeof(K));
+        K.qsrc = NBX_Q_POINTS;
+        K.kh = k;
+        K.kout = k;
+        K.m = (int32_t)m;
+        for (int d = 0; d < 3; ++d) K.q[d] = dq + d * m;
+        K.nn = dnn;
+        if (e == hipSuccess) e = nbx_launch_knn(t->view(), K, (int)m, st);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(k_gather_by_id, dim3(nblk(n)), dim3(256), 0, st, rid, t->perm, n, rslot);
+            hipLaunchKernelGGL(k_gather_by_id, dim3(nblk(n)), dim3(256), 0, st, wid, t->perm, n, wslot);
+            LocalArgs A = {{dq, dq + m, dq + 2 * m}, m, dnn, k, 1, rslot, wslot, dmi, dm, dd};
+            e = nbx_launch_smooth_local(t->view(), A, st);
+        }
+        if (e == hipSuccess && mean_out) e = hipMemcpy(mean_out, dm, sizeof(double) * m, hipMemcpyDeviceToHost);
+        if (e == hipSuccess && disp_out) e = hipMemcpy(disp_out, dd, sizeof(double) * m, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = hipfail(e, "nbx_smooth_local_points");
+    }
+    hipFree(dq);
+    hipFree(dnn);
+    hipFree(rid);
+    hipFree(rslot);
+    hipFree(wid);
+    hipFree(wslot);
+    hipFree(dm);
+    hipFree(dd);
+    if (dmi) hipFree(dmi);
+    return rc;
+}
+
 extern "C" int nbx_density(nbx_tree *t, int32_t k, int32_t variant, int32_t nsearch, double *rho_by_id,
                            double *h_by_id) {
     if (!t || !rho_by_id) return fail(NBX_ERR_ARG, "NULL argument");

@@ -132,6 +132,22 @@ struct SmoothArgs {
 // with_mean / with_disp: compute those first (else A.mean / A.disp hold the
 // caller's); the moments run when A.skew is set.
 hipError_t nbx_launch_smooth(const DevTree &T, const SmoothArgs &A, bool with_mean, bool with_disp, hipStream_t st);
+// Local mean / dispersion of a per-particle scalar (gather form): row r of
+// nn holds the k neighbours of query r, whose coordinates are q (position
+// queries) or the particle at slot r (q[0] null).
+struct LocalArgs {
+    const double *q[3];     // query coordinates, or null for particle rows
+    int64_t m;              // rows
+    const int32_t *nn;      // m x k neighbours
+    int32_t k;
+    int32_t nn_ids;         // entries are IDs (else slots)
+    const double *rho;      // by slot
+    const double *w;        // weights by slot
+    const double *mean_in;  // by row, for the dispersion (null: no dispersion)
+    double *mean;           // by row, nullable
+    double *disp;           // by row, nullable
+};
+hipError_t nbx_launch_smooth_local(const DevTree &T, const LocalArgs &A, hipStream_t st);
 __global__ void k_smooth_out(const double *in, const int32_t *perm, int64_t n, int ncomp, double *out);
 __global__ void k_gather_by_id(const double *by_id, const int32_t *perm, int64_t n, double *by_slot);
 

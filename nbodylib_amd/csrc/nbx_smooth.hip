@@ -2,7 +2,8 @@
 //
 // Replaces KDTree::CalcSmoothVel (KDCalcSmoothQuantities.cxx:480-540),
 // CalcSmoothVelDisp (:542-614), CalcSmoothVelSkew (:617-690) and
-// CalcSmoothVelKurtosis (:692-767) on TPHYS trees.  For every particle i the
+// CalcSmoothVelKurtosis (:692-767) on TPHYS trees, and the local mean and
+// dispersion of a per-particle scalar (k_smooth_local).  For every particle i the
 // reference takes the non-periodic target kNN of CalcDensity, h_i = r_k / 2,
 // and for each neighbour j adds half-weight shares to both ends:
 //   mean_i += W_ij / rho_j * m_j * v_j,   mean_j += W_ij / rho_i * m_i * v_i,
@@ -207,6 +208,47 @@ __global__ __launch_bounds__(SM_B) void k_smooth_moments(DevTree T, SmoothArgs A
             unsafeAtomicAdd(A.kurt + d * n + i, acc[3 + d] + lm[3 + d][tid]);
         }
     }
+}
+
+// CalcSmoothLocalMeanParticle / CalcSmoothLocalDispParticle (:1539-1617) and
+// the Position forms (:1619-1693): the gather sum over a row with the full
+// weight W(r/h) h^-3 (no half, no scatter), each neighbour counting
+// W / rho_j * m_j * w_j, and for the dispersion W / rho_j * m_j * (w_j - mean)^2.
+__global__ __launch_bounds__(256) void k_smooth_local(DevTree T, LocalArgs A) {
+    const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (r >= A.m) return;
+    const int k = A.k;
+    const int32_t *row = A.nn + r * k;
+    double xi[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) xi[d] = A.q[0] ? A.q[d][r] : T.x[d][r];
+    double m2 = 0.0;
+    for (int t = 0; t < k; ++t) {
+        const int32_t j = A.nn_ids ? T.inv[row[t]] : row[t];
+        m2 = fmax(m2, smooth_d2(T, xi, j));
+    }
+    const double hi = 0.5 * sqrt(m2);
+    const double norm = 1.0 / pow(hi, 3.0);
+    const double mu = A.mean_in ? A.mean_in[r] : 0.0;
+    double mean = 0.0, disp = 0.0;
+    for (int t = 0; t < k; ++t) {
+        const int32_t j = A.nn_ids ? T.inv[row[t]] : row[t];
+        const double rij = sqrt(smooth_d2(T, xi, j));
+        const double w = wsm_at(rij, hi, T.kern, T.kernres) * norm;
+        const double temp = w / A.rho[j] * T.mass[j];
+        mean += temp * A.w[j];
+        const double delta = A.w[j] - mu;
+        disp += temp * delta * delta;
+    }
+    if (A.mean) A.mean[r] = mean;
+    if (A.disp && A.mean_in) A.disp[r] = disp;
+}
+
+hipError_t nbx_launch_smooth_local(const DevTree &T, const LocalArgs &A, hipStream_t st) {
+    if (A.m == 0) return hipSuccess;
+    ProfScope ps("k_smooth_local", st);
+    hipLaunchKernelGGL(k_smooth_local, dim3((unsigned)((A.m + 255) / 256)), dim3(256), 0, st, T, A);
+    return hipGetLastError();
 }
 
 // Slot-major SoA to ID-major rows: out[id * ncomp + c] = in[c * n + slot];

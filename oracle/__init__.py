@@ -65,6 +65,9 @@ def lib():
         L.orc_smooth_vel.restype = _ci
         L.orc_smooth_moments.argtypes = [_vp, _ci, ctypes.c_void_p, _dp, _dp, _dp, _dp]
         L.orc_smooth_moments.restype = _ci
+        L.orc_smooth_local.argtypes = [_vp, ctypes.c_void_p, _i64, _ci, _dp, ctypes.c_void_p, ctypes.c_void_p,
+                                       ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_smooth_local.restype = _ci
         L.orc_fof_tree.argtypes = [_vp, _cd, _ci, _ci, _ip]
         L.orc_fof_tree.restype = _i64
         L.orc_knn_brute.argtypes = [_dp, _i64, _ci, _ci, _ci, ctypes.c_void_p, _ip, ctypes.c_void_p, _ci]
@@ -214,6 +217,21 @@ class Tree:
         if rc != 0:
             raise RuntimeError("CalcSmoothVelSkew failed")
         return skew, kurt
+
+    def smooth_local(self, k, weight, rho=None, mean_in=None, q=None):
+        """CalcSmoothLocalMean/DispParticle for every particle (q None; by ID) or the Position forms at the rows
+        of q (m, 3): (mean, disp); disp only when mean_in is given."""
+        w = np.ascontiguousarray(weight, dtype=np.float64)
+        r = None if rho is None else np.ascontiguousarray(rho, dtype=np.float64)
+        qq = None if q is None else np.ascontiguousarray(q, dtype=np.float64).reshape(-1, 3)
+        rows = self.n if qq is None else qq.shape[0]
+        mi = None if mean_in is None else np.ascontiguousarray(mean_in, dtype=np.float64)
+        mean = np.zeros(rows)
+        disp = np.zeros(rows) if mi is not None else None
+        rc = lib().orc_smooth_local(self.h, _ptr(qq), rows, k, w, _ptr(r), _ptr(mi), _ptr(mean), _ptr(disp))
+        if rc != 0:
+            raise RuntimeError("CalcSmoothLocalMean failed")
+        return mean, disp
 
     def calc_veldensity(self, nsmooth, nsearch, nthreads=0):
         rho = np.zeros(self.n, dtype=np.float64)

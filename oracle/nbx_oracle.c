@@ -719,6 +719,55 @@ int orc_smooth_moments(void *h, int k, const double *rho_by_id, const double *me
     return rc;
 }
 
+/* CalcSmoothLocalMeanParticle / CalcSmoothLocalDispParticle
+ * (KDCalcSmoothQuantities.cxx:1539-1617) for every particle (q == NULL), or
+ * the Position forms (:1619-1693) for m positions q (3 per row): the k
+ * nearest (target search, or the non-periodic search from x), h = sqrt(top)/2,
+ * in pop order temp = W(r/h) h^-3 / rho_j * m_j, mean += temp * w_j and
+ * disp += temp * (w_j - mean_in)^2.  Weights, rho and (particle form) means by
+ * ID; rho NULL runs CalcDensity(k) first; outputs by ID or by row. */
+int orc_smooth_local(void *h, const double *q, int64_t m, int k, const double *weight, const double *rho_by_id,
+                     const double *mean_in, double *mean_out, double *disp_out) {
+    otree *t = (otree *)h;
+    const int64_t n = t->n;
+    if (rho_by_id) {
+        for (int64_t i = 0; i < n; i++) t->p[i].rho = rho_by_id[t->p[i].id];
+    } else {
+        double *tmp = (double *)malloc(sizeof(double) * (size_t)n);
+        int rc = orc_calc_density(h, k, tmp);
+        free(tmp);
+        if (rc) return rc;
+    }
+    const int64_t rows = q ? m : n;
+    pq_t pq;
+    pq_init(&pq, k);
+    int rc = 0;
+    for (int64_t r = 0; r < rows && rc == 0; r++) {
+        double off[6] = {0, 0, 0, 0, 0, 0};
+        pq.n = 0;
+        for (int j = 0; j < k; j++) pq_push(&pq, -1, ORC_MAXVALUE);
+        if (q) nn_pos(t, t->root, 0.0, &pq, off, q + 3 * r, 0, 3);
+        else nn_target(t, t->root, 0.0, &pq, off, r, 0, 3);
+        const int64_t out = q ? r : t->p[r].id;
+        const double mu = mean_in ? mean_in[out] : 0.0;
+        double hi = 0.5 * sqrt(pq_top(&pq)), norm = 1.0 / pow(hi, 3.0), mean = 0., disp = 0.;
+        for (int j = 0; j < k; j++) {
+            if (pq_topq(&pq) == -1) { rc = -1; break; }
+            double rij = sqrt(pq_top(&pq));
+            double Wij = wsm_at(rij, hi, t->kernres, t->kern) * norm;
+            int32_t x = pq_topq(&pq);
+            double temp = Wij / t->p[x].rho * t->p[x].m, wj = weight[t->p[x].id];
+            mean += temp * wj;
+            disp += temp * (wj - mu) * (wj - mu);
+            pq_pop(&pq);
+        }
+        if (mean_out) mean_out[out] = mean;
+        if (disp_out) disp_out[out] = disp;
+    }
+    pq_free(&pq);
+    return rc;
+}
+
 /* CalcDensityParticle for every particle, KDCalcSmoothQuantities.cxx:768-844
  * (gather form, no 1/2, no scatter).  Also the all-cores CPU restatement. */
 int orc_calc_density_gather(void *h, int k, double *rho_by_id, double *h_by_id, int nthreads) {

@@ -157,3 +157,37 @@ def test_smooth_vel_vs_oracle(gpu, form, kind):
     assert np.max(np.abs(ku - oku)) < 1e-9 * np.max(np.abs(oku))
     sk2 = t.CalcSmoothVelSkew(k, mean=om, disp=od)
     assert np.max(np.abs(sk2 - osk)) < 1e-10 * np.max(np.abs(osk))
+
+
+@pytest.mark.parametrize("form", ["plain", "mixed"])
+def test_smooth_local_vs_oracle(gpu, form):
+    """CalcSmoothLocalMean/Disp, Particle forms for every particle and
+    Position forms at positions inside and outside the box, against the
+    oracle (1e-12 of the largest value); given rho = 1 and w = 1 the means
+    are the gather densities."""
+    import nbodylib_amd as nb
+
+    n, k = 6000, 32
+    say(f"smooth local form={form} n={n} k={k}")
+    pos, _ = datagen.clustered(n, 1.0, nhalo=8, seed=64)
+    m = datagen.lognormal_masses(n, 64)
+    w = np.random.default_rng(64).normal(size=n)
+    fm = nb.FORM_PLAIN if form == "plain" else nb.FORM_MIXED
+    t = nb.KDTree(pos, mass=m, kerntype=nb.KSPH, form=fm)
+    ot = oracle.Tree(pos, mass=m, b=16, kerntype=oracle.KSPH)
+    mu = t.CalcSmoothLocalMeanAll(w, k)
+    om, _ = ot.smooth_local(k, w)
+    assert np.max(np.abs(mu - om)) < 1e-12 * np.max(np.abs(om))
+    dd = t.CalcSmoothLocalDispAll(w, om, k)
+    _, od = ot.smooth_local(k, w, mean_in=om)
+    assert np.max(np.abs(dd - od)) < 1e-12 * np.max(np.abs(od))
+    x = np.random.default_rng(65).uniform(-0.1, 1.1, (400, 3))
+    mp = t.CalcSmoothLocalMeanPosition(x, w, k)
+    omp, _ = ot.smooth_local(k, w, q=x)
+    assert np.max(np.abs(mp - omp)) < 1e-12 * np.max(np.abs(omp))
+    dp = t.CalcSmoothLocalDispPosition(x, w, omp, k)
+    _, odp = ot.smooth_local(k, w, mean_in=omp, q=x)
+    assert np.max(np.abs(dp - odp)) < 1e-12 * np.max(np.abs(odp))
+    g, _ = t.CalcDensityParticleAll(k)
+    g1 = t.CalcSmoothLocalMeanAll(np.ones(n), k, rho=np.ones(n))
+    assert np.max(np.abs(g1 / g - 1)) < 1e-12

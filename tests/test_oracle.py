@@ -313,6 +313,41 @@ def test_smooth_moments_closed_forms():
     assert np.max(np.abs(kurt - want_k)) < 1e-9 * np.max(np.abs(want_k))
 
 
+def test_smooth_local_identities_and_numpy():
+    """CalcSmoothLocalMean{Particle,Position} with w = 1 and rho = 1 given add
+    exactly CalcDensityParticle's / CalcDensityPosition's terms in their order
+    (bit-identical); a constant field about its own value has zero dispersion
+    and about 0 the dispersion is c^2 times that sum; with real rho and random
+    weights both forms match a NumPy restatement over brute-force lists."""
+    n, k = 2000, 32
+    pos = datagen.uniform(n, 1.0, 62)
+    m = datagen.lognormal_masses(n, 62)
+    t = oracle.Tree(pos, mass=m, b=16, kerntype=oracle.KSPH)
+    ones = np.ones(n)
+    g, _ = t.calc_density_gather(k)
+    mean, _ = t.smooth_local(k, ones, rho=ones)
+    assert np.array_equal(mean, g)
+    x = datagen.uniform(300, 1.0, 63) * 1.2 - 0.1
+    cp, _ = t.calc_density_position(x, k)
+    mp, _ = t.smooth_local(k, ones, rho=ones, q=x)
+    assert np.array_equal(mp, cp)
+    c = 3.0
+    _, d0 = t.smooth_local(k, np.full(n, c), rho=ones, mean_in=np.full(n, c))
+    assert not d0.any()
+    _, dc = t.smooth_local(k, np.full(n, c), rho=ones, mean_in=np.zeros(n))
+    assert np.max(np.abs(dc / (c * c * g) - 1)) < 1e-14
+    # real densities, random weights
+    w = np.random.default_rng(62).normal(size=n)
+    rho = t.calc_density(k)
+    mu, _ = t.smooth_local(k, w)
+    _, dd = t.smooth_local(k, w, mean_in=mu)
+    ids, d2 = oracle.knn_brute(pos, k, mode=0)
+    h = 0.5 * np.sqrt(d2[:, -1])
+    tmp = _wtab(np.sqrt(d2) / h[:, None]) / h[:, None] ** 3 / rho[ids] * m[ids]
+    assert np.max(np.abs(mu - (tmp * w[ids]).sum(1))) < 1e-12 * np.max(np.abs(mu))
+    assert np.max(np.abs(dd - (tmp * (w[ids] - mu[:, None]) ** 2).sum(1))) < 1e-12 * np.max(dd)
+
+
 def test_smooth_vel_vs_numpy_lists():
     """CalcSmoothVel and CalcSmoothVelDisp on the oracle tree against a NumPy
     restatement over brute-force neighbour lists, with the computed densities
