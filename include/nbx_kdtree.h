@@ -142,6 +142,8 @@ class Matrix {
     const Double_t &operator()(int i, int j) const { return m[3 * i + j]; }
 };
 
+class PriorityQueue;  // src/KDTree/PriorityQueue.h; only passed through as workspace here
+
 // FOFFunc.h:24-55: comparator and check-function types and the built-in
 // functions.  A comparator links a and b when it returns nonzero; params[6]
 // and params[7] are the position and velocity scales (squared linking
@@ -250,6 +252,13 @@ class KDTree {
                                          int densityset = 1);
     void CalcVelDensity(Int_t Nsmooth = 64, Int_t Nsearch = 64);
     Double_t CalcDensityParticle(Int_t target, Int_t Nsmooth = 64);
+    // KDTree.h:425, KDCalcSmoothQuantities.cxx:845-921 (TPHYS trees with
+    // velocities): served like CalcDensityParticle from one batched
+    // CalcVelDensity per (Nsmooth, Nsearch); the workspace arguments (iflag,
+    // pq, pq2, nnIDs, vdist) are accepted and not needed.
+    Double_t CalcVelDensityParticle(Int_t target, Int_t Nsmooth = 64, Int_t Nsearch = 64, int iflag = 0,
+                                    PriorityQueue *pq = nullptr, PriorityQueue *pq2 = nullptr, Int_t *nnIDs = nullptr,
+                                    Double_t *vdist = nullptr);
     // KDTree.h:432 (TPHYS trees; v is unused there); KDCalcSmoothQuantities.cxx:1092-1149
     Double_t CalcDensityPosition(Double_t *x, Int_t Nsmooth = 64, Double_t *v = NULL);
 
@@ -296,6 +305,8 @@ class KDTree {
     std::vector<Double_t> cache_d2;
     int dcache_k;
     std::vector<Double_t> dcache;
+    int vcache_k, vcache_s;
+    std::vector<Double_t> vcache;
 
     void init(Particle *p, Int_t n, Int_t bucket_size, int TreeType, int KernType, int KernRes, int Splitting,
               int Aniso, int ScaleSpace, Double_t *Period);

@@ -41,7 +41,7 @@ void KDTree::init(Particle *p, Int_t n, Int_t bucket_size, int TreeType, int Ker
     kernfunctype = KernType;
     iresetorder = true;
     tree = nullptr;
-    cache_variant = cache_k = dcache_k = -1;
+    cache_variant = cache_k = dcache_k = vcache_k = vcache_s = -1;
     periodic = Period != nullptr;
     for (int k = 0; k < 3; ++k) period[k] = periodic ? Period[k] : 0.0;
     nbx_params prm;
@@ -272,6 +272,18 @@ Double_t KDTree::CalcDensityParticle(Int_t target, Int_t Nsmooth) {
         dcache_k = Nsmooth;
     }
     return dcache[bucket[target].GetID()];
+}
+Double_t KDTree::CalcVelDensityParticle(Int_t target, Int_t Nsmooth, Int_t Nsearch, int, PriorityQueue *,
+                                        PriorityQueue *, Int_t *, Double_t *) {
+    std::lock_guard<std::mutex> lk(mu);
+    if (Nsmooth > Nsearch) Nsmooth = Nsearch;  // as the reference does (:855-858)
+    if (vcache_k != Nsmooth || vcache_s != Nsearch) {
+        vcache.assign(numparts, 0.0);
+        check(nbx_density(tree, Nsmooth, NBX_DENS_VEL, Nsearch, vcache.data(), nullptr), "CalcDensity failed");
+        vcache_k = Nsmooth;
+        vcache_s = Nsearch;
+    }
+    return vcache[bucket[target].GetID()];
 }
 Double_t KDTree::CalcDensityPosition(Double_t *x, Int_t Nsmooth, Double_t *) {
     std::lock_guard<std::mutex> lk(mu);
