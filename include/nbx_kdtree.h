@@ -241,6 +241,11 @@ class KDTree {
                                   int meanvelset = 1, int veldispset = 1);
     Coordinate *CalcSmoothVelKurtosis(Coordinate *smvel, Matrix *smveldisp, Int_t Nsmooth = 64, int densityset = 1,
                                       int meanvelset = 1, int veldispset = 1);
+    // KDTree.h:427, :435, KDCalcSmoothQuantities.cxx:992-1035, :1209-1253: the
+    // gather form of the smoothed velocity at one particle or position (full
+    // weight, no scatter), i.e. the local mean of each velocity component.
+    Coordinate CalcSmoothVelParticle(Int_t target, Int_t Nsmooth = 64, int densityset = 1);
+    Coordinate CalcSmoothVelPosition(Double_t *x, Int_t Nsmooth = 64, int densityset = 1);
     // KDTree.h:454-459, KDCalcSmoothQuantities.cxx:1539-1693: weight by ID.
     // Each call runs one GPU pass (every particle, or the one position); for
     // many targets call nbx_smooth_local / nbx_smooth_local_points once.

@@ -356,6 +356,24 @@ Double_t KDTree::smooth_local_one(Int_t target, const Double_t *x, Double_t *wei
           "CalcSmoothLocalMean failed");
     return res[id];
 }
+Coordinate KDTree::CalcSmoothVelParticle(Int_t target, Int_t Nsmooth, int densityset) {
+    std::vector<Double_t> w(numparts);
+    Coordinate out;
+    for (int d = 0; d < 3; ++d) {
+        for (Int_t s = 0; s < numparts; ++s) w[bucket[s].GetID()] = bucket[s].GetVelocity(d);
+        out[d] = smooth_local_one(target, nullptr, w.data(), nullptr, Nsmooth, d == 0 ? densityset : 1);
+    }
+    return out;
+}
+Coordinate KDTree::CalcSmoothVelPosition(Double_t *x, Int_t Nsmooth, int densityset) {
+    std::vector<Double_t> w(numparts);
+    Coordinate out;
+    for (int d = 0; d < 3; ++d) {
+        for (Int_t s = 0; s < numparts; ++s) w[bucket[s].GetID()] = bucket[s].GetVelocity(d);
+        out[d] = smooth_local_one(0, x, w.data(), nullptr, Nsmooth, d == 0 ? densityset : 1);
+    }
+    return out;
+}
 Double_t KDTree::CalcSmoothLocalMeanParticle(Int_t target, Double_t *weight, Int_t Nsmooth, int densityset) {
     return smooth_local_one(target, nullptr, weight, nullptr, Nsmooth, densityset);
 }
