@@ -11,7 +11,7 @@ import oracle
 from nbodylib_amd import datagen
 
 pytestmark = pytest.mark.gpu
-MIXED_TOL = 2e-6  # north-star bound; FP64 weights give ~1e-15 in practice
+MIXED_TOL = 2e-6  # the precision form's stated density bound; FP64 weights give ~1e-15 in practice
 
 
 def say(msg):
@@ -118,3 +118,33 @@ def test_mixed_phase_density(gpu):
     rel = np.max(np.abs(rho / ro - 1.0))
     say(f"  max rel {rel:.3e} fallbacks={t.last_fallbacks()}")
     assert rel < MIXED_TOL
+
+
+def test_mixed_sets_full_size():
+    """At the headline size (16.8M clustered particles, periodic L = 100,
+    k = 32, CalcDensity's search) the precision form's neighbour rows equal
+    the FP64 form's exactly, and so do the smoothing lengths; the densities
+    agree to the stated bound."""
+    import nbodylib_amd as nb
+    import torch
+    from nbodylib_amd.device import DeviceTree
+
+    n, k, L = 1 << 24, 32, 100.0
+    say(f"mixed vs plain sets n={n} k={k}")
+    pos, _ = datagen.clustered(n, L, nhalo=2000, seed=20261203)
+    dev = torch.device("cuda:0")
+    xs = torch.from_numpy(np.ascontiguousarray(pos.T)).to(dev)
+    out = {}
+    for form in (nb.FORM_PLAIN, nb.FORM_MIXED):
+        t = DeviceTree(xs, bucket_size=16, kerntype=nb.KSPH, period=L, form=form)
+        nn = torch.empty((n, k), dtype=torch.int32, device=dev)
+        rho = torch.empty(n, dtype=torch.float64, device=dev)
+        hh = torch.empty(n, dtype=torch.float64, device=dev)
+        t.knn_density(k, nn, rho, hh)
+        out[form] = (torch.sort(nn, dim=1).values.cpu(), rho.cpu(), hh.cpu(), t.last_fallbacks())
+        say(f"  form {form}: fallbacks {out[form][3]}")
+        del t, nn
+    (np_, rp, hp, _), (nm, rm, hm, _) = out[nb.FORM_PLAIN], out[nb.FORM_MIXED]
+    assert torch.equal(np_, nm)
+    assert torch.equal(hp, hm)
+    assert float(torch.max(torch.abs(rm / rp - 1.0))) < MIXED_TOL
