@@ -37,6 +37,23 @@ static int hipfail(hipError_t e, const char *where) {
 
 extern "C" const char *nbx_last_error(void) { return g_err.c_str(); }
 
+// Temporary device buffers of one host call, freed when the call returns on
+// any path (the HIPCK early returns included).
+struct DevBufs {
+    std::vector<void *> held;
+    template <class T>
+    hipError_t alloc(T **ptr, size_t bytes) {
+        *ptr = nullptr;
+        hipError_t e = hipMalloc((void **)ptr, bytes > 0 ? bytes : 1);
+        if (e == hipSuccess) held.push_back((void *)*ptr);
+        else *ptr = nullptr;
+        return e;
+    }
+    ~DevBufs() {
+        for (void *x : held) hipFree(x);
+    }
+};
+
 // ------------------------------------------------------------------ profiling
 struct ProfRec {
     std::string name;
@@ -491,17 +508,18 @@ extern "C" int nbx_knn(nbx_tree *t, int32_t k, int32_t variant, int32_t layout, 
 // of the period-aware search.
 static int points_impl(nbx_tree *t, const double *q, int64_t m, int32_t k, bool periodic, int32_t *nn, double *d2,
                        double *rho, double *h, const char *what) {
+    DevBufs bufs;
     if (m <= 0) return NBX_OK;
     if (m > INT32_MAX) return fail(NBX_ERR_ARG, "too many positions (m < 2^31)");
     double *dq = nullptr, *dd2 = nullptr, *drho = nullptr, *dh = nullptr;
     int32_t *dnn = nullptr, *err = nullptr;
-    HIPCK(hipMalloc((void **)&dq, sizeof(double) * 3 * m));
+    HIPCK(bufs.alloc(&dq, sizeof(double) * 3 * m));
     HIPCK(hipMemcpy(dq, q, sizeof(double) * 3 * m, hipMemcpyHostToDevice));
-    if (nn) HIPCK(hipMalloc((void **)&dnn, sizeof(int32_t) * m * k));
-    if (d2) HIPCK(hipMalloc((void **)&dd2, sizeof(double) * m * k));
-    if (rho) HIPCK(hipMalloc((void **)&drho, sizeof(double) * m));
-    if (h) HIPCK(hipMalloc((void **)&dh, sizeof(double) * m));
-    HIPCK(hipMalloc((void **)&err, sizeof(int32_t)));
+    if (nn) HIPCK(bufs.alloc(&dnn, sizeof(int32_t) * m * k));
+    if (d2) HIPCK(bufs.alloc(&dd2, sizeof(double) * m * k));
+    if (rho) HIPCK(bufs.alloc(&drho, sizeof(double) * m));
+    if (h) HIPCK(bufs.alloc(&dh, sizeof(double) * m));
+    HIPCK(bufs.alloc(&err, sizeof(int32_t)));
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, nullptr, err, 0, (int64_t)1);
     KnnArgs A;
     memset(&A, 0, sizeof(A));
@@ -525,12 +543,6 @@ static int points_impl(nbx_tree *t, const double *q, int64_t m, int32_t k, bool 
     if (e == hipSuccess && d2) e = hipMemcpy(d2, dd2, sizeof(double) * m * k, hipMemcpyDeviceToHost);
     if (e == hipSuccess && rho) e = hipMemcpy(rho, drho, sizeof(double) * m, hipMemcpyDeviceToHost);
     if (e == hipSuccess && h) e = hipMemcpy(h, dh, sizeof(double) * m, hipMemcpyDeviceToHost);
-    hipFree(dq);
-    if (dnn) hipFree(dnn);
-    if (dd2) hipFree(dd2);
-    if (drho) hipFree(drho);
-    if (dh) hipFree(dh);
-    hipFree(err);
     if (e != hipSuccess) return hipfail(e, what);
     if (herr) return fail(NBX_ERR_SEARCH, "CalcDensityPosition failed: fewer than Nsmooth particles");
     return NBX_OK;
@@ -636,6 +648,7 @@ static hipError_t upload_rows(const nbx_tree *t, const double *rows, int ncomp, 
 
 static int smooth_impl(nbx_tree *t, int32_t k, const double *rho_by_id, const double *mean_in, const double *disp_in,
                        double *mean_out, double *disp_out, double *skew_out, double *kurt_out) {
+    DevBufs bufs;
     if (t->nd != 3 || !t->has_vel)
         return fail(NBX_ERR_UNSUPPORTED, "CalcSmoothVel needs a TPHYS tree built with velocities");
     if (k < 1 || k > 64) return fail(NBX_ERR_UNSUPPORTED, "Nsmooth out of range (1..64)");
@@ -645,13 +658,13 @@ static int smooth_impl(nbx_tree *t, int32_t k, const double *rho_by_id, const do
     hipStream_t st = nullptr;
     int32_t *nn = nullptr;
     double *rid = nullptr, *rslot = nullptr, *mean = nullptr, *disp = nullptr, *mom = nullptr, *buf = nullptr;
-    HIPCK(hipMalloc((void **)&nn, sizeof(int32_t) * (size_t)n * k));
-    HIPCK(hipMalloc((void **)&rid, sizeof(double) * n));
-    HIPCK(hipMalloc((void **)&rslot, sizeof(double) * n));
-    HIPCK(hipMalloc((void **)&mean, sizeof(double) * 3 * n));
-    if (need_disp) HIPCK(hipMalloc((void **)&disp, sizeof(double) * 9 * n));
-    if (moments) HIPCK(hipMalloc((void **)&mom, sizeof(double) * 6 * n));
-    HIPCK(hipMalloc((void **)&buf, sizeof(double) * 9 * n));
+    HIPCK(bufs.alloc(&nn, sizeof(int32_t) * (size_t)n * k));
+    HIPCK(bufs.alloc(&rid, sizeof(double) * n));
+    HIPCK(bufs.alloc(&rslot, sizeof(double) * n));
+    HIPCK(bufs.alloc(&mean, sizeof(double) * 3 * n));
+    if (need_disp) HIPCK(bufs.alloc(&disp, sizeof(double) * 9 * n));
+    if (moments) HIPCK(bufs.alloc(&mom, sizeof(double) * 6 * n));
+    HIPCK(bufs.alloc(&buf, sizeof(double) * 9 * n));
     int rc = density_impl(t, k, NBX_DENS_SYMMETRIC, 0, rid, nullptr, nn, st);
     hipError_t e = hipSuccess;
     if (rc == NBX_OK) {
@@ -683,13 +696,6 @@ static int smooth_impl(nbx_tree *t, int32_t k, const double *rho_by_id, const do
         }
         if (e != hipSuccess) rc = hipfail(e, "smoothed velocities");
     }
-    hipFree(nn);
-    hipFree(rid);
-    hipFree(rslot);
-    hipFree(mean);
-    if (disp) hipFree(disp);
-    if (mom) hipFree(mom);
-    hipFree(buf);
     return rc;
 }
 
@@ -709,6 +715,7 @@ extern "C" int nbx_smooth_vel_moments(nbx_tree *t, int32_t k, const double *rho_
 // particle: CalcDensity's neighbour rows (fused pass), then the gather sums.
 extern "C" int nbx_smooth_local(nbx_tree *t, int32_t k, const double *weight_by_id, const double *rho_by_id,
                                 const double *mean_in_by_id, double *mean_by_id, double *disp_by_id) {
+    DevBufs bufs;
     if (!t || !weight_by_id || (!mean_by_id && !disp_by_id)) return fail(NBX_ERR_ARG, "NULL argument");
     if (disp_by_id && !mean_in_by_id) return fail(NBX_ERR_ARG, "the dispersion needs the means (mean_in_by_id)");
     if (t->nd != 3) return fail(NBX_ERR_UNSUPPORTED, "CalcSmoothLocalMean here needs a TPHYS tree");
@@ -717,8 +724,8 @@ extern "C" int nbx_smooth_local(nbx_tree *t, int32_t k, const double *weight_by_
     hipStream_t st = nullptr;
     int32_t *nn = nullptr;
     double *b[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-    HIPCK(hipMalloc((void **)&nn, sizeof(int32_t) * (size_t)n * k));
-    for (int i = 0; i < 8; ++i) HIPCK(hipMalloc((void **)&b[i], sizeof(double) * n));
+    HIPCK(bufs.alloc(&nn, sizeof(int32_t) * (size_t)n * k));
+    for (int i = 0; i < 8; ++i) HIPCK(bufs.alloc(&b[i], sizeof(double) * n));
     double *rid = b[0], *rslot = b[1], *wid = b[2], *wslot = b[3], *mid = b[4], *mslot = b[5], *om = b[6], *od = b[7];
     int rc = density_impl(t, k, NBX_DENS_SYMMETRIC, 0, rid, nullptr, nn, st);
     hipError_t e = hipSuccess;
@@ -744,8 +751,6 @@ extern "C" int nbx_smooth_local(nbx_tree *t, int32_t k, const double *weight_by_
         }
         if (e != hipSuccess) rc = hipfail(e, "nbx_smooth_local");
     }
-    hipFree(nn);
-    for (int i = 0; i < 8; ++i) hipFree(b[i]);
     return rc;
 }
 
@@ -754,6 +759,7 @@ extern "C" int nbx_smooth_local(nbx_tree *t, int32_t k, const double *weight_by_
 extern "C" int nbx_smooth_local_points(nbx_tree *t, const double *q, int64_t m, int32_t k, const double *weight_by_id,
                                        const double *rho_by_id, const double *mean_in, double *mean_out,
                                        double *disp_out) {
+    DevBufs bufs;
     if (!t || !q || !weight_by_id || (!mean_out && !disp_out)) return fail(NBX_ERR_ARG, "NULL argument");
     if (disp_out && !mean_in) return fail(NBX_ERR_ARG, "the dispersion needs the means (mean_in)");
     if (t->nd != 3) return fail(NBX_ERR_UNSUPPORTED, "CalcSmoothLocalMeanPosition here needs a TPHYS tree");
@@ -766,17 +772,17 @@ extern "C" int nbx_smooth_local_points(nbx_tree *t, const double *q, int64_t m, 
     double *dq = nullptr, *rid = nullptr, *rslot = nullptr, *wid = nullptr, *wslot = nullptr, *dmi = nullptr,
            *dm = nullptr, *dd = nullptr;
     int32_t *dnn = nullptr;
-    HIPCK(hipMalloc((void **)&dq, sizeof(double) * 3 * m));
+    HIPCK(bufs.alloc(&dq, sizeof(double) * 3 * m));
     HIPCK(hipMemcpy(dq, q, sizeof(double) * 3 * m, hipMemcpyHostToDevice));
-    HIPCK(hipMalloc((void **)&dnn, sizeof(int32_t) * m * k));
-    HIPCK(hipMalloc((void **)&rid, sizeof(double) * n));
-    HIPCK(hipMalloc((void **)&rslot, sizeof(double) * n));
-    HIPCK(hipMalloc((void **)&wid, sizeof(double) * n));
-    HIPCK(hipMalloc((void **)&wslot, sizeof(double) * n));
-    HIPCK(hipMalloc((void **)&dm, sizeof(double) * m));
-    HIPCK(hipMalloc((void **)&dd, sizeof(double) * m));
+    HIPCK(bufs.alloc(&dnn, sizeof(int32_t) * m * k));
+    HIPCK(bufs.alloc(&rid, sizeof(double) * n));
+    HIPCK(bufs.alloc(&rslot, sizeof(double) * n));
+    HIPCK(bufs.alloc(&wid, sizeof(double) * n));
+    HIPCK(bufs.alloc(&wslot, sizeof(double) * n));
+    HIPCK(bufs.alloc(&dm, sizeof(double) * m));
+    HIPCK(bufs.alloc(&dd, sizeof(double) * m));
     if (mean_in) {
-        HIPCK(hipMalloc((void **)&dmi, sizeof(double) * m));
+        HIPCK(bufs.alloc(&dmi, sizeof(double) * m));
         HIPCK(hipMemcpy(dmi, mean_in, sizeof(double) * m, hipMemcpyHostToDevice));
     }
     int rc = NBX_OK;
@@ -804,15 +810,6 @@ extern "C" int nbx_smooth_local_points(nbx_tree *t, const double *q, int64_t m, 
         if (e == hipSuccess && disp_out) e = hipMemcpy(disp_out, dd, sizeof(double) * m, hipMemcpyDeviceToHost);
         if (e != hipSuccess) rc = hipfail(e, "nbx_smooth_local_points");
     }
-    hipFree(dq);
-    hipFree(dnn);
-    hipFree(rid);
-    hipFree(rslot);
-    hipFree(wid);
-    hipFree(wslot);
-    hipFree(dm);
-    hipFree(dd);
-    if (dmi) hipFree(dmi);
     return rc;
 }
 
@@ -960,6 +957,7 @@ extern "C" int nbx_fof_lists(const int32_t *labels_by_id, const int32_t *slot_to
 // ------------------------------------------------------------------ ball search, many positions
 extern "C" int nbx_ball_points_many(nbx_tree *t, const double *q, int64_t m, double r2, int64_t *offsets,
                                     int32_t *ids, double *d2, int64_t cap) {
+    DevBufs bufs;
     if (!t || (m > 0 && !q) || !offsets) return fail(NBX_ERR_ARG, "NULL argument");
     if (t->nd != 3) return fail(NBX_ERR_UNSUPPORTED, "ball search needs a TPHYS tree");
     if (m < 0 || cap < 0 || !(r2 >= 0)) return fail(NBX_ERR_ARG, "bad sizes or radius");
@@ -976,13 +974,13 @@ extern "C" int nbx_ball_points_many(nbx_tree *t, const double *q, int64_t m, dou
     uint64_t *keys = nullptr;
     int32_t *idx = nullptr;
     void *tmp = nullptr;
-    HIPCK(hipMalloc((void **)&dq, sizeof(double) * 3 * m));
+    HIPCK(bufs.alloc(&dq, sizeof(double) * 3 * m));
     HIPCK(hipMemcpy(dq, q, sizeof(double) * 3 * m, hipMemcpyHostToDevice));
-    HIPCK(hipMalloc((void **)&cnt, sizeof(int64_t) * m));
+    HIPCK(bufs.alloc(&cnt, sizeof(int64_t) * m));
     // walk order: Morton order of the positions over the root box
     double rb[6];
-    HIPCK(hipMalloc((void **)&keys, sizeof(uint64_t) * 2 * m));
-    HIPCK(hipMalloc((void **)&idx, sizeof(int32_t) * 2 * m));
+    HIPCK(bufs.alloc(&keys, sizeof(uint64_t) * 2 * m));
+    HIPCK(bufs.alloc(&idx, sizeof(int32_t) * 2 * m));
     HIPCK(hipMemcpy(rb, t->box + 6, sizeof(rb), hipMemcpyDeviceToHost));
     double sc[3];
     for (int d = 0; d < 3; ++d) sc[d] = rb[3 + d] > rb[d] ? 2097151.0 / (rb[3 + d] - rb[d]) : 0.0;
@@ -990,7 +988,7 @@ extern "C" int nbx_ball_points_many(nbx_tree *t, const double *q, int64_t m, dou
                        rb[2], sc[0], sc[1], sc[2], keys, idx);
     size_t tb = 0;
     hipError_t e = nbx_sort_pairs_u64(nullptr, tb, keys, keys + m, idx, idx + m, m, nullptr);
-    if (e == hipSuccess) e = hipMalloc(&tmp, tb > 0 ? tb : 1);
+    if (e == hipSuccess) e = bufs.alloc((char **)&tmp, tb > 0 ? tb : 1);
     if (e == hipSuccess) e = nbx_sort_pairs_u64(tmp, tb, keys, keys + m, idx, idx + m, m, nullptr);
     BallManyArgs A;
     memset(&A, 0, sizeof(A));
@@ -1010,10 +1008,10 @@ extern "C" int nbx_ball_points_many(nbx_tree *t, const double *q, int64_t m, dou
         if (total > 0 && total <= cap && ids) {
             int32_t *dids = nullptr;
             double *dd2 = nullptr;
-            e = hipMalloc((void **)&doff, sizeof(int64_t) * m);
+            e = bufs.alloc(&doff, sizeof(int64_t) * m);
             if (e == hipSuccess) e = hipMemcpy(doff, offsets, sizeof(int64_t) * m, hipMemcpyHostToDevice);
-            if (e == hipSuccess) e = hipMalloc((void **)&dids, sizeof(int32_t) * total);
-            if (e == hipSuccess && d2) e = hipMalloc((void **)&dd2, sizeof(double) * total);
+            if (e == hipSuccess) e = bufs.alloc(&dids, sizeof(int32_t) * total);
+            if (e == hipSuccess && d2) e = bufs.alloc(&dd2, sizeof(double) * total);
             if (e == hipSuccess) {
                 A.off = doff;
                 A.ids = dids;
@@ -1022,16 +1020,8 @@ extern "C" int nbx_ball_points_many(nbx_tree *t, const double *q, int64_t m, dou
             }
             if (e == hipSuccess) e = hipMemcpy(ids, dids, sizeof(int32_t) * total, hipMemcpyDeviceToHost);
             if (e == hipSuccess && d2) e = hipMemcpy(d2, dd2, sizeof(double) * total, hipMemcpyDeviceToHost);
-            if (dids) hipFree(dids);
-            if (dd2) hipFree(dd2);
         }
     }
-    hipFree(dq);
-    hipFree(cnt);
-    hipFree(keys);
-    hipFree(idx);
-    if (tmp) hipFree(tmp);
-    if (doff) hipFree(doff);
     if (e != hipSuccess) return hipfail(e, "nbx_ball_points_many");
     return NBX_OK;
 }
