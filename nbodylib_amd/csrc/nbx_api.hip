@@ -38,19 +38,30 @@ static int hipfail(hipError_t e, const char *where) {
 extern "C" const char *nbx_last_error(void) { return g_err.c_str(); }
 
 // Temporary device buffers of one host call, freed when the call returns on
-// any path (the HIPCK early returns included).
+// any path (the HIPCK early returns included).  Constructed with a stream, it
+// allocates and frees in that stream's order (hipMallocAsync / hipFreeAsync).
 struct DevBufs {
     std::vector<void *> held;
+    bool ordered = false;  // stream-ordered allocations on st
+    hipStream_t st = nullptr;
+    DevBufs() = default;
+    explicit DevBufs(hipStream_t s) : ordered(true), st(s) {}
+    DevBufs(const DevBufs &) = delete;
+    DevBufs &operator=(const DevBufs &) = delete;
     template <class T>
     hipError_t alloc(T **ptr, size_t bytes) {
         *ptr = nullptr;
-        hipError_t e = hipMalloc((void **)ptr, bytes > 0 ? bytes : 1);
+        const size_t b = bytes > 0 ? bytes : 1;
+        hipError_t e = ordered ? hipMallocAsync((void **)ptr, b, st) : hipMalloc((void **)ptr, b);
         if (e == hipSuccess) held.push_back((void *)*ptr);
         else *ptr = nullptr;
         return e;
     }
     ~DevBufs() {
-        for (void *x : held) hipFree(x);
+        for (void *x : held) {
+            if (ordered) hipFreeAsync(x, st);
+            else hipFree(x);
+        }
     }
 };
 
@@ -225,13 +236,13 @@ static int build_tree(nbx_tree *t, const double *pos, const double *vel, const d
     {
         const int nb = 1024;
         double *part = nullptr;
-        HIPCK(hipMallocAsync((void **)&part, sizeof(double) * 2 * nb * nd, st));
+        DevBufs pb(st);
+        HIPCK(pb.alloc(&part, sizeof(double) * 2 * nb * nd));
         for (int d = 0; d < nd; ++d)
             hipLaunchKernelGGL(k_bbox, dim3(nb), dim3(256), 0, st, c[d], n, part + 2 * nb * d);
         std::vector<double> hp(2 * nb * nd);
         HIPCK(hipMemcpyAsync(hp.data(), part, sizeof(double) * hp.size(), hipMemcpyDeviceToHost, st));
         HIPCK(hipStreamSynchronize(st));
-        HIPCK(hipFreeAsync(part, st));
         for (int d = 0; d < nd; ++d) {
             double mn = INFINITY, mx = -INFINITY;
             for (int b = 0; b < nb; ++b) { mn = std::min(mn, hp[2 * nb * d + 2 * b]); mx = std::max(mx, hp[2 * nb * d + 2 * b + 1]); }
@@ -245,7 +256,8 @@ static int build_tree(nbx_tree *t, const double *pos, const double *vel, const d
     while (t->nleafp < t->nleaf) { t->nleafp <<= 1; t->depth++; }
     t->nnodes = 2 * (int64_t)t->nleafp;
     int32_t *order = nullptr;
-    HIPCK(hipMallocAsync((void **)&order, sizeof(int32_t) * n, st));
+    DevBufs ob(st);  // order is freed (stream-ordered) when build_tree returns
+    HIPCK(ob.alloc(&order, sizeof(int32_t) * n));
     {
         ProfScope ps("k_build_kdorder", st);
         HIPCK(nbx_kd_order(c, nd, n, t->nleafp, order, st));
@@ -285,7 +297,6 @@ static int build_tree(nbx_tree *t, const double *pos, const double *vel, const d
         for (int d = 0; d < nd; ++d) g.xf[d] = t->xf[d];
         hipLaunchKernelGGL(k_gather_leaf, dim3(nblk(npad)), dim3(256), 0, st, g);
     }
-    HIPCK(hipFreeAsync(order, st));
     {
         ProfScope ps("k_build_boxes", st);
         if (t->nleafp > t->nleaf)
@@ -351,21 +362,18 @@ extern "C" int nbx_tree_create(const nbx_params *p, const double *pos, const dou
     int rc0 = check_params(p, n, vel != nullptr);
     if (rc0) return rc0;
     double *dpos = nullptr, *dvel = nullptr, *dm = nullptr;
-    HIPCK(hipMalloc((void **)&dpos, sizeof(double) * 3 * n));
+    DevBufs bufs;
+    HIPCK(bufs.alloc(&dpos, sizeof(double) * 3 * n));
     HIPCK(hipMemcpy(dpos, pos, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
     if (vel) {
-        HIPCK(hipMalloc((void **)&dvel, sizeof(double) * 3 * n));
+        HIPCK(bufs.alloc(&dvel, sizeof(double) * 3 * n));
         HIPCK(hipMemcpy(dvel, vel, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
     }
     if (mass) {
-        HIPCK(hipMalloc((void **)&dm, sizeof(double) * n));
+        HIPCK(bufs.alloc(&dm, sizeof(double) * n));
         HIPCK(hipMemcpy(dm, mass, sizeof(double) * n, hipMemcpyHostToDevice));
     }
-    int rc = nbx_tree_create_dev(p, dpos, dvel, dm, n, nullptr, out);
-    hipFree(dpos);
-    if (dvel) hipFree(dvel);
-    if (dm) hipFree(dm);
-    return rc;
+    return nbx_tree_create_dev(p, dpos, dvel, dm, n, nullptr, out);
 }
 
 extern "C" int nbx_tree_destroy(nbx_tree *t) {
@@ -489,8 +497,9 @@ extern "C" int nbx_knn(nbx_tree *t, int32_t k, int32_t variant, int32_t layout, 
     int32_t *dnn = nullptr;
     double *dd2 = nullptr;
     size_t cnt = (size_t)t->n * (size_t)k;
-    if (nn) HIPCK(hipMalloc((void **)&dnn, sizeof(int32_t) * cnt));
-    if (d2) HIPCK(hipMalloc((void **)&dd2, sizeof(double) * cnt));
+    DevBufs bufs;
+    if (nn) HIPCK(bufs.alloc(&dnn, sizeof(int32_t) * cnt));
+    if (d2) HIPCK(bufs.alloc(&dd2, sizeof(double) * cnt));
     int rc = nbx_knn_dev(t, k, variant, layout, dnn, dd2, nullptr);
     if (rc == NBX_OK) {
         hipError_t e = hipDeviceSynchronize();
@@ -498,8 +507,6 @@ extern "C" int nbx_knn(nbx_tree *t, int32_t k, int32_t variant, int32_t layout, 
         if (e == hipSuccess && d2) e = hipMemcpy(d2, dd2, sizeof(double) * cnt, hipMemcpyDeviceToHost);
         if (e != hipSuccess) rc = hipfail(e, "nbx_knn");
     }
-    if (dnn) hipFree(dnn);
-    if (dd2) hipFree(dd2);
     return rc;
 }
 
@@ -591,9 +598,10 @@ static int density_impl(nbx_tree *t, int32_t k, int32_t variant, int32_t nsearch
     const int64_t n = t->n;
     double *rho = nullptr, *h = nullptr;
     int32_t *err = nullptr;
-    HIPCK(hipMallocAsync((void **)&rho, sizeof(double) * n, st));
-    if (h_id) HIPCK(hipMallocAsync((void **)&h, sizeof(double) * n, st));
-    HIPCK(hipMallocAsync((void **)&err, sizeof(int32_t), st));
+    DevBufs bufs(st);
+    HIPCK(bufs.alloc(&rho, sizeof(double) * n));
+    if (h_id) HIPCK(bufs.alloc(&h, sizeof(double) * n));
+    HIPCK(bufs.alloc(&err, sizeof(int32_t)));
     hipLaunchKernelGGL(k_fill_f64, dim3(nblk(n)), dim3(256), 0, st, rho, 0.0, n);
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, st, err, 0, (int64_t)1);
     A.rho = rho;
@@ -609,9 +617,6 @@ static int density_impl(nbx_tree *t, int32_t k, int32_t variant, int32_t nsearch
     int32_t herr = 0;
     if (e == hipSuccess) e = hipMemcpyAsync(&herr, err, sizeof(int32_t), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-    hipFreeAsync(rho, st);
-    if (h) hipFreeAsync(h, st);
-    hipFreeAsync(err, st);
     if (e != hipSuccess) return hipfail(e, "density");
     if (herr) return fail(NBX_ERR_SEARCH, "CalcDensity failed: fewer than k neighbours");
     return NBX_OK;
@@ -817,16 +822,15 @@ extern "C" int nbx_density(nbx_tree *t, int32_t k, int32_t variant, int32_t nsea
                            double *h_by_id) {
     if (!t || !rho_by_id) return fail(NBX_ERR_ARG, "NULL argument");
     double *dr = nullptr, *dh = nullptr;
-    HIPCK(hipMalloc((void **)&dr, sizeof(double) * t->n));
-    if (h_by_id) HIPCK(hipMalloc((void **)&dh, sizeof(double) * t->n));
+    DevBufs bufs;
+    HIPCK(bufs.alloc(&dr, sizeof(double) * t->n));
+    if (h_by_id) HIPCK(bufs.alloc(&dh, sizeof(double) * t->n));
     int rc = density_impl(t, k, variant, nsearch, dr, dh, nullptr, nullptr);
     if (rc == NBX_OK) {
         hipError_t e = hipMemcpy(rho_by_id, dr, sizeof(double) * t->n, hipMemcpyDeviceToHost);
         if (e == hipSuccess && h_by_id) e = hipMemcpy(h_by_id, dh, sizeof(double) * t->n, hipMemcpyDeviceToHost);
         if (e != hipSuccess) rc = hipfail(e, "nbx_density");
     }
-    hipFree(dr);
-    if (dh) hipFree(dh);
     return rc;
 }
 
@@ -836,7 +840,8 @@ static int fof_impl(nbx_tree *t, int metric, const FofLink &link, int32_t minnum
     const int64_t n = t->n;
     int32_t *P = nullptr;
     size_t bytes = sizeof(int32_t) * n + (size_t)t->nnodes + 16;
-    HIPCK(hipMallocAsync((void **)&P, bytes, st));
+    DevBufs bufs(st);
+    HIPCK(bufs.alloc(&P, bytes));
     FofArgs A;
     memset(&A, 0, sizeof(A));
     A.link = link;
@@ -848,7 +853,6 @@ static int fof_impl(nbx_tree *t, int metric, const FofLink &link, int32_t minnum
     hipError_t e = nbx_launch_fof(T, A, st);
     if (e == hipSuccess) e = nbx_fof_canonical(T, P, minnum, labels_by_id_dev, ngroups, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-    hipFreeAsync(P, st);
     if (e != hipSuccess) return hipfail(e, "fof");
     return NBX_OK;
 }
@@ -896,26 +900,26 @@ extern "C" int nbx_fof_criterion(nbx_tree *t, int32_t comparator, const double *
                                  int32_t *labels_by_id, int64_t *ngroups) {
     if (!t || !labels_by_id || !ngroups) return fail(NBX_ERR_ARG, "NULL argument");
     int32_t *d = nullptr;
-    HIPCK(hipMalloc((void **)&d, sizeof(int32_t) * t->n));
+    DevBufs bufs;
+    HIPCK(bufs.alloc(&d, sizeof(int32_t) * t->n));
     int rc = nbx_fof_criterion_dev(t, comparator, params, minnum, d, ngroups, nullptr);
     if (rc == NBX_OK) {
         hipError_t e = hipMemcpy(labels_by_id, d, sizeof(int32_t) * t->n, hipMemcpyDeviceToHost);
         if (e != hipSuccess) rc = hipfail(e, "nbx_fof_criterion");
     }
-    hipFree(d);
     return rc;
 }
 
 extern "C" int nbx_fof(nbx_tree *t, double fdist, int32_t minnum, int32_t *labels_by_id, int64_t *ngroups) {
     if (!t || !labels_by_id || !ngroups) return fail(NBX_ERR_ARG, "NULL argument");
     int32_t *d = nullptr;
-    HIPCK(hipMalloc((void **)&d, sizeof(int32_t) * t->n));
+    DevBufs bufs;
+    HIPCK(bufs.alloc(&d, sizeof(int32_t) * t->n));
     int rc = nbx_fof_dev(t, fdist, minnum, d, ngroups, nullptr);
     if (rc == NBX_OK) {
         hipError_t e = hipMemcpy(labels_by_id, d, sizeof(int32_t) * t->n, hipMemcpyDeviceToHost);
         if (e != hipSuccess) rc = hipfail(e, "nbx_fof");
     }
-    hipFree(d);
     return rc;
 }
 
@@ -1037,9 +1041,10 @@ static int ball_impl(nbx_tree *t, const double *x, double r2, int32_t *out, doub
     double *dd2 = nullptr;
     unsigned long long *dc = nullptr;
     int64_t dcap = cap > 0 ? cap : 1;
-    HIPCK(hipMalloc((void **)&dout, sizeof(int32_t) * dcap));
-    if (d2out) HIPCK(hipMalloc((void **)&dd2, sizeof(double) * dcap));
-    HIPCK(hipMalloc((void **)&dc, sizeof(unsigned long long)));
+    DevBufs bufs;
+    HIPCK(bufs.alloc(&dout, sizeof(int32_t) * dcap));
+    if (d2out) HIPCK(bufs.alloc(&dd2, sizeof(double) * dcap));
+    HIPCK(bufs.alloc(&dc, sizeof(unsigned long long)));
     HIPCK(hipMemset(dc, 0, sizeof(unsigned long long)));
     DevTree T = t->view();
     // primary position and the images of SearchBallPosPeriodicTagged
@@ -1075,9 +1080,6 @@ static int ball_impl(nbx_tree *t, const double *x, double r2, int32_t *out, doub
     if (e == hipSuccess && cap > 0) e = hipMemcpy(out, dout, sizeof(int32_t) * std::min<int64_t>(cap, (int64_t)hc), hipMemcpyDeviceToHost);
     if (e == hipSuccess && cap > 0 && d2out)
         e = hipMemcpy(d2out, dd2, sizeof(double) * std::min<int64_t>(cap, (int64_t)hc), hipMemcpyDeviceToHost);
-    hipFree(dout);
-    if (dd2) hipFree(dd2);
-    hipFree(dc);
     if (e != hipSuccess) return hipfail(e, "ball");
     *nout = (int64_t)hc;
     return NBX_OK;
