@@ -33,8 +33,9 @@ extern "C" {
 #define NBX_ERR_NOMEM (-4)       /* device allocation failed */
 #define NBX_ERR_SEARCH (-5)      /* fewer than k neighbours ("CalcDensity failed") */
 
-/* KDTree::TPHYS/TPHS (KDTree.h:90-91) */
+/* KDTree::TPHYS/TVEL/TPHS (KDTree.h:90-91) */
 #define NBX_TPHYS 0
+#define NBX_TVEL 2 /* 3-D tree on the velocities (KDTree.cxx:1116-1122, ND = 3 at :1154) */
 #define NBX_TPHS 3
 /* KDTree::KSPH/KGAUSS/KEPAN/KTH (KDTree.h:97) */
 #define NBX_KSPH 0
@@ -50,7 +51,7 @@ typedef struct nbx_tree nbx_tree;
 /* Mirrors the KDTree constructor options (KDTree.h:229-245). */
 typedef struct nbx_params {
     int32_t bucket_size; /* bucket_size (default 16); only the N<=b edge case depends on it */
-    int32_t treetype;    /* NBX_TPHYS or NBX_TPHS */
+    int32_t treetype;    /* NBX_TPHYS, NBX_TVEL (non-periodic) or NBX_TPHS */
     int32_t kerntype;    /* NBX_KSPH ... (reference default KEPAN) */
     int32_t kernres;     /* kernel table size (default 1000; <100 clamps to 100) */
     int32_t splitting;   /* only KDTREE_SPLIT_SPREAD (0) */
@@ -68,7 +69,16 @@ void nbx_params_default(nbx_params *p);
 /* Builds the tree.  Replaces KDTree::KDTree(Particle*,N,...) KDTree.cxx:1238-1306
  * (BuildNodes :970-1055, KernelConstruction :1144-1183).
  * pos: 3*n SoA (x[0..n), y[..], z[..]); vel: 3*n SoA or NULL (needed for TPHS
- * and velocity density); mass: n or NULL (then 1/n each). */
+ * and velocity density); mass: n or NULL (then 1/n each).
+ * TVEL: the tree is built on vel (pos is not read and may be NULL); kNN
+ * (NBX_KNN_FINDNEAREST = FindNearestVel(tt), KDFindNearest.cxx:335-346,
+ * :258; nbx_knn_points = FindNearestVel(v), :530-541), CalcDensity,
+ * CalcDensityParticle and CalcDensityPosition then run on velocity
+ * coordinates as in the reference (KDCalcSmoothQuantities.cxx:228, :783,
+ * :1107).  Calls that need positions (FOF, ball searches, smoothing,
+ * CalcVelDensity, FindNearestPos(tt)) return NBX_ERR_UNSUPPORTED on it, and
+ * so does a periodic TVEL tree (the reference applies the period to
+ * velocity images there). */
 int nbx_tree_create(const nbx_params *p, const double *pos, const double *vel, const double *mass, int64_t n,
                     nbx_tree **out);
 int nbx_tree_create_dev(const nbx_params *p, const double *pos_dev, const double *vel_dev, const double *mass_dev,

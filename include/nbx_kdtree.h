@@ -130,6 +130,7 @@ class Coordinate {
     Coordinate(const Double_t x = 0, const Double_t y = 0, const Double_t z = 0) { c[0] = x; c[1] = y; c[2] = z; }
     Double_t &operator[](int i) { return c[i]; }
     const Double_t &operator[](int i) const { return c[i]; }
+    Double_t *GetCoord() { return c; }
 };
 class Matrix {
     Double_t m[9];
@@ -217,6 +218,14 @@ class KDTree {
     void FindNearestPhase(Int_t tt, Int_t *nn, Double_t *dist2, Int_t Nsearch = 64);
     // one position
     void FindNearestPos(Double_t *x, Int_t *nn, Double_t *dist2, Int_t Nsearch = 64);
+    // velocity space, TVEL trees only (KDTree.h FindNearestVel; KDFindNearest.cxx:335-346,
+    // :451-453, :530-541, :559-561): every particle, one particle, one velocity
+    void FindNearestVel(Int_t **nn, Double_t **dist2, Int_t Nsearch = 64);
+    void FindNearestVel(Int_t tt, Int_t *nn, Double_t *dist2, Int_t Nsearch = 64);
+    void FindNearestVel(Double_t *v, Int_t *nn, Double_t *dist2, Int_t Nsearch = 64);
+    void FindNearestVel(Coordinate v, Int_t *nn, Double_t *dist2, Int_t Nsearch = 64) {
+        FindNearestVel(v.GetCoord(), nn, dist2, Nsearch);
+    }
 
     // tree indices of all particles with d2 < fdist2 around a position / particle
     // nn[id] = imark, dist2[id] = d2 for every particle with d2 < fdist2

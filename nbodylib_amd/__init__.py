@@ -147,17 +147,23 @@ class KDTree:
     """NBody::KDTree over host numpy arrays (the constructor of KDTree.h:229-245).
 
     pos, vel: (n, 3) or (3, n) float64; mass: (n,) or None (equal masses 1/n).
+    treetype=TVEL builds the tree on vel (pos may be None); its searches and
+    densities then run in velocity space, as the reference's TVEL tree does.
     """
 
     def __init__(self, pos, mass=None, vel=None, bucket_size=16, treetype=TPHYS, kerntype=KEPAN, kernres=1000,
                  splitting=0, aniso=0, scalespace=0, period=None, form=FORM_PLAIN, recheck_rel=1e-5):
-        pos = np.asarray(pos, dtype=np.float64)
-        n = pos.shape[0] if pos.ndim == 2 and pos.shape[1] == 3 else pos.shape[-1]
+        src = vel if treetype == TVEL and vel is not None else pos
+        if src is None:
+            raise ValueError("pos is required (vel for a TVEL tree)")
+        src = np.asarray(src, dtype=np.float64)
+        n = src.shape[0] if src.ndim == 2 and src.shape[1] == 3 else src.shape[-1]
         self.n = int(n)
+        self.treetype = treetype
         self.period = period
         self._params = params(bucket_size, treetype, kerntype, kernres, splitting, aniso, scalespace, period, form,
                               recheck_rel)
-        spos = _soa(pos, n, "pos")
+        spos = None if pos is None or treetype == TVEL else _soa(pos, n, "pos")
         svel = None if vel is None else _soa(vel, n, "vel")
         m = None if mass is None else np.ascontiguousarray(mass, dtype=np.float64)
         h = ctypes.c_void_p()
@@ -217,6 +223,19 @@ class KDTree:
     def FindNearestNonPeriodic(self, Nsearch=64, layout=LAYOUT_BY_ID):
         """The search CalcDensity makes (KDCalcSmoothQuantities.cxx:227)."""
         return self._knn(Nsearch, KNN_NONPERIODIC, layout)
+
+    def FindNearestVel(self, Nsearch=64, layout=LAYOUT_BY_ID):
+        """FindNearestVel(Int_t**,...) on a TVEL tree (KDFindNearest.cxx:335-346, :451-453): the
+        target search in velocity space, self and zero distances excluded."""
+        if self.treetype != TVEL:
+            raise NbxError(NBX_ERR_UNSUPPORTED, "FindNearestVel needs a TVEL tree")
+        return self._knn(Nsearch, KNN_FINDNEAREST, layout)
+
+    def FindNearestVelPoints(self, v, Nsearch=64):
+        """FindNearestVel(Double_t *v, ...) (KDFindNearest.cxx:530-541) for many velocities v (m, 3), TVEL tree."""
+        if self.treetype != TVEL:
+            raise NbxError(NBX_ERR_UNSUPPORTED, "FindNearestVel needs a TVEL tree")
+        return self.FindNearestPosPoints(v, Nsearch)
 
     def FindNearestPosPoints(self, x, Nsearch=64):
         """FindNearestPos(Double_t *x, ...) for many positions x (m, 3)."""

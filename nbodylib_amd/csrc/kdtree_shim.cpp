@@ -24,12 +24,15 @@ static bool g_mixed = false;
 void KDTree::SetThrowOnError(bool on) { g_throw = on; }
 void KDTree::SetPrecisionForm(bool mixed) { g_mixed = mixed; }
 
-static void check(int rc, const char *what) {
-    if (rc >= 0) return;
-    std::string msg = std::string(what) + ": " + nbx_last_error();
+static void die(const std::string &msg) {
     if (g_throw) throw std::runtime_error(msg);
     std::fprintf(stderr, "%s\n", msg.c_str());
     std::exit(1);
+}
+
+static void check(int rc, const char *what) {
+    if (rc >= 0) return;
+    die(std::string(what) + ": " + nbx_last_error());
 }
 
 void KDTree::init(Particle *p, Int_t n, Int_t bucket_size, int TreeType, int KernType, int KernRes, int Splitting,
@@ -175,6 +178,25 @@ void KDTree::FindNearestPos(Double_t *x, Int_t *nn, Double_t *dist2, Int_t Nsear
     double q[3] = {x[0], x[1], x[2]};
     std::vector<int32_t> ids(Nsearch);
     check(nbx_knn_points(tree, q, 1, Nsearch, ids.data(), dist2), "FindNearestPos");
+    for (Int_t j = 0; j < Nsearch; ++j) nn[j] = ids[j] >= 0 ? slot_of_id[ids[j]] : -1;
+}
+
+// velocity space: the TVEL tree's own coordinates (KDFindNearest.cxx:335-346,
+// :451-453, :530-541, :559-561)
+void KDTree::FindNearestVel(Int_t **nn, Double_t **dist2, Int_t Nsearch) {
+    if (treetype != TVEL) die("FindNearestVel needs a TVEL tree");
+    FindNearest(nn, dist2, Nsearch);
+}
+void KDTree::FindNearestVel(Int_t tt, Int_t *nn, Double_t *dist2, Int_t Nsearch) {
+    if (treetype != TVEL) die("FindNearestVel needs a TVEL tree");
+    FindNearest(tt, nn, dist2, Nsearch);
+}
+void KDTree::FindNearestVel(Double_t *v, Int_t *nn, Double_t *dist2, Int_t Nsearch) {
+    if (treetype != TVEL) die("FindNearestVel needs a TVEL tree");
+    std::lock_guard<std::mutex> lk(mu);
+    double q[3] = {v[0], v[1], v[2]};
+    std::vector<int32_t> ids(Nsearch);
+    check(nbx_knn_points(tree, q, 1, Nsearch, ids.data(), dist2), "FindNearestVel");
     for (Int_t j = 0; j < Nsearch; ++j) nn[j] = ids[j] >= 0 ? slot_of_id[ids[j]] : -1;
 }
 

@@ -131,6 +131,7 @@ struct nbx_tree {
     int64_t n;
     int nd;        // tree dimensionality (3 or 6)
     bool has_vel;  // TPHYS tree that also keeps sorted velocities (x[3..5])
+    bool tvel;     // TVEL: the tree's coordinates are the velocities
     double *xs[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     double *mass = nullptr;
     int32_t *perm = nullptr;
@@ -324,13 +325,16 @@ static int build_tree(nbx_tree *t, const double *pos, const double *vel, const d
 static int check_params(const nbx_params *p, int64_t n, bool has_vel) {
     if (!p) return fail(NBX_ERR_ARG, "params is NULL");
     if (n < 1 || n > 0x7fffffffLL - 4096) return fail(NBX_ERR_ARG, "number of particles out of range");
-    if (p->treetype != NBX_TPHYS && p->treetype != NBX_TPHS)
-        return fail(NBX_ERR_UNSUPPORTED, "Error in type of tree specified (TPHYS or TPHS supported)");
+    if (p->treetype != NBX_TPHYS && p->treetype != NBX_TVEL && p->treetype != NBX_TPHS)
+        return fail(NBX_ERR_UNSUPPORTED, "Error in type of tree specified (TPHYS, TVEL or TPHS supported)");
     if (p->splitting != 0) return fail(NBX_ERR_UNSUPPORTED, "only the spread splitting criterion is supported");
     if (p->scalespace != 0) return fail(NBX_ERR_UNSUPPORTED, "ScaleSpace is not supported; prescale coordinates");
     if (p->treetype == NBX_TPHS && p->aniso != -1)
         return fail(NBX_ERR_UNSUPPORTED, "TPHS trees need aniso = -1 (metric factor 1 on prescaled coordinates)");
     if (p->treetype == NBX_TPHS && !has_vel) return fail(NBX_ERR_ARG, "TPHS tree needs velocities");
+    if (p->treetype == NBX_TVEL && !has_vel) return fail(NBX_ERR_ARG, "TVEL tree needs velocities");
+    if (p->treetype == NBX_TVEL && p->periodic)
+        return fail(NBX_ERR_UNSUPPORTED, "periodic TVEL trees are not supported (the period applies to positions)");
     if (p->periodic)
         for (int k = 0; k < 3; ++k)
             if (!(p->period[k] > 0)) return fail(NBX_ERR_ARG, "period must be positive");
@@ -340,7 +344,9 @@ static int check_params(const nbx_params *p, int64_t n, bool has_vel) {
 
 extern "C" int nbx_tree_create_dev(const nbx_params *p, const double *pos, const double *vel, const double *mass,
                                    int64_t n, void *stream, nbx_tree **out) {
-    if (!out || !pos) return fail(NBX_ERR_ARG, "NULL argument");
+    if (!out || !p) return fail(NBX_ERR_ARG, "NULL argument");
+    const bool tvel = p->treetype == NBX_TVEL;
+    if (!(tvel ? vel : pos)) return fail(NBX_ERR_ARG, "NULL argument");
     int rc = check_params(p, n, vel != nullptr);
     if (rc) return rc;
     nbx_tree *t = new nbx_tree();
@@ -348,9 +354,12 @@ extern "C" int nbx_tree_create_dev(const nbx_params *p, const double *pos, const
     if (t->p.recheck_rel <= 0) t->p.recheck_rel = 1e-5;
     t->n = n;
     t->nd = p->treetype == NBX_TPHS ? 6 : 3;
-    t->has_vel = (t->nd == 3 && vel != nullptr);
+    t->tvel = tvel;
+    t->has_vel = (t->nd == 3 && !tvel && vel != nullptr);
     t->kernres = p->kernres;
-    rc = build_tree(t, pos, vel, mass, (hipStream_t)stream);
+    // a TVEL tree is the 3-D tree over the velocities
+    rc = tvel ? build_tree(t, vel, nullptr, mass, (hipStream_t)stream)
+              : build_tree(t, pos, vel, mass, (hipStream_t)stream);
     if (rc) { free_tree(t); return rc; }
     *out = t;
     return NBX_OK;
@@ -358,13 +367,16 @@ extern "C" int nbx_tree_create_dev(const nbx_params *p, const double *pos, const
 
 extern "C" int nbx_tree_create(const nbx_params *p, const double *pos, const double *vel, const double *mass, int64_t n,
                                nbx_tree **out) {
-    if (!out || !pos) return fail(NBX_ERR_ARG, "NULL argument");
+    if (!out || !p) return fail(NBX_ERR_ARG, "NULL argument");
+    if (!(p->treetype == NBX_TVEL ? vel : pos)) return fail(NBX_ERR_ARG, "NULL argument");
     int rc0 = check_params(p, n, vel != nullptr);
     if (rc0) return rc0;
     double *dpos = nullptr, *dvel = nullptr, *dm = nullptr;
     DevBufs bufs;
-    HIPCK(bufs.alloc(&dpos, sizeof(double) * 3 * n));
-    HIPCK(hipMemcpy(dpos, pos, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+    if (pos && p->treetype != NBX_TVEL) {
+        HIPCK(bufs.alloc(&dpos, sizeof(double) * 3 * n));
+        HIPCK(hipMemcpy(dpos, pos, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+    }
     if (vel) {
         HIPCK(bufs.alloc(&dvel, sizeof(double) * 3 * n));
         HIPCK(hipMemcpy(dvel, vel, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
@@ -457,7 +469,7 @@ static int setup_knn(nbx_tree *t, int32_t k, int32_t variant, KnnArgs &A, int &n
             else { A.qsrc = NBX_Q_PERIODIC; A.kh = k + 1; A.drop = NBX_DROP_SMALLEST; A.periodic = !rootleaf; }
             break;
         case NBX_KNN_FINDNEARESTPOS:
-            if (t->nd != 3) return fail(NBX_ERR_UNSUPPORTED, "FindNearestPos needs a TPHYS tree");
+            if (t->nd != 3 || t->tvel) return fail(NBX_ERR_UNSUPPORTED, "FindNearestPos needs a TPHYS tree");
             if (!per) { A.qsrc = NBX_Q_SLOTS; A.kh = k; A.drop = NBX_DROP_NONE; }
             else if (rootleaf) { A.qsrc = NBX_Q_SLOTS; A.kh = k + 1; A.drop = NBX_DROP_LARGEST; }
             else { A.qsrc = NBX_Q_PERIODIC; A.kh = k + 1; A.drop = NBX_DROP_LARGEST; A.periodic = 1; }
@@ -723,7 +735,7 @@ extern "C" int nbx_smooth_local(nbx_tree *t, int32_t k, const double *weight_by_
     DevBufs bufs;
     if (!t || !weight_by_id || (!mean_by_id && !disp_by_id)) return fail(NBX_ERR_ARG, "NULL argument");
     if (disp_by_id && !mean_in_by_id) return fail(NBX_ERR_ARG, "the dispersion needs the means (mean_in_by_id)");
-    if (t->nd != 3) return fail(NBX_ERR_UNSUPPORTED, "CalcSmoothLocalMean here needs a TPHYS tree");
+    if (t->nd != 3 || t->tvel) return fail(NBX_ERR_UNSUPPORTED, "CalcSmoothLocalMean here needs a TPHYS tree");
     if (k < 1 || k > 64) return fail(NBX_ERR_UNSUPPORTED, "Nsmooth out of range (1..64)");
     const int64_t n = t->n;
     hipStream_t st = nullptr;
@@ -767,7 +779,8 @@ extern "C" int nbx_smooth_local_points(nbx_tree *t, const double *q, int64_t m, 
     DevBufs bufs;
     if (!t || !q || !weight_by_id || (!mean_out && !disp_out)) return fail(NBX_ERR_ARG, "NULL argument");
     if (disp_out && !mean_in) return fail(NBX_ERR_ARG, "the dispersion needs the means (mean_in)");
-    if (t->nd != 3) return fail(NBX_ERR_UNSUPPORTED, "CalcSmoothLocalMeanPosition here needs a TPHYS tree");
+    if (t->nd != 3 || t->tvel)
+        return fail(NBX_ERR_UNSUPPORTED, "CalcSmoothLocalMeanPosition here needs a TPHYS tree");
     if (k < 1 || k > 64) return fail(NBX_ERR_UNSUPPORTED, "Nsmooth out of range (1..64)");
     if (k > t->n) return fail(NBX_ERR_SEARCH, "CalcSmoothLocalMean failed: fewer than Nsmooth particles");
     if (m <= 0) return NBX_OK;
@@ -861,6 +874,7 @@ extern "C" int nbx_fof_dev(nbx_tree *t, double fdist, int32_t minnum, int32_t *l
                            void *stream) {
     if (!t || !labels_by_id_dev || !ngroups) return fail(NBX_ERR_ARG, "NULL argument");
     if (!(fdist >= 0)) return fail(NBX_ERR_ARG, "linking length must be >= 0");
+    if (t->tvel) return fail(NBX_ERR_UNSUPPORTED, "FOF needs a TPHYS or TPHS tree");
     FofLink L = {fdist * fdist, 0.0, 0.0, 0.0, 0.0};
     return fof_impl(t, NBX_FOFM_PLAIN, L, minnum, labels_by_id_dev, ngroups, (hipStream_t)stream);
 }
@@ -877,7 +891,7 @@ extern "C" int nbx_fof_criterion_dev(nbx_tree *t, int32_t comparator, const doub
     L.r7 = 1.0 / L.s7;
     int metric;
     if (comparator == NBX_FOF_CMP_3D) {
-        if (t->nd != 3 || (int)params[0] != NBX_TPHYS)
+        if (t->nd != 3 || t->tvel || (int)params[0] != NBX_TPHYS)
             return fail(NBX_ERR_UNSUPPORTED, "FOF3d runs on a TPHYS tree with params[0] = TPHYS");
         if (!(params[6] > 0) || !(params[1] >= params[6]))
             return fail(NBX_ERR_UNSUPPORTED, "FOF3d needs params[6] > 0 and a search scale params[1] >= params[6]");
@@ -963,7 +977,7 @@ extern "C" int nbx_ball_points_many(nbx_tree *t, const double *q, int64_t m, dou
                                     int32_t *ids, double *d2, int64_t cap) {
     DevBufs bufs;
     if (!t || (m > 0 && !q) || !offsets) return fail(NBX_ERR_ARG, "NULL argument");
-    if (t->nd != 3) return fail(NBX_ERR_UNSUPPORTED, "ball search needs a TPHYS tree");
+    if (t->nd != 3 || t->tvel) return fail(NBX_ERR_UNSUPPORTED, "ball search needs a TPHYS tree");
     if (m < 0 || cap < 0 || !(r2 >= 0)) return fail(NBX_ERR_ARG, "bad sizes or radius");
     offsets[0] = 0;
     if (m == 0) return NBX_OK;
@@ -1036,7 +1050,7 @@ extern "C" int nbx_ball_points_many(nbx_tree *t, const double *q, int64_t m, dou
 static int ball_impl(nbx_tree *t, const double *x, double r2, int32_t *out, double *d2out, int64_t cap,
                      int64_t *nout) {
     if (!t || !x || !nout) return fail(NBX_ERR_ARG, "NULL argument");
-    if (t->nd != 3) return fail(NBX_ERR_UNSUPPORTED, "ball search needs a TPHYS tree");
+    if (t->nd != 3 || t->tvel) return fail(NBX_ERR_UNSUPPORTED, "ball search needs a TPHYS tree");
     int32_t *dout = nullptr;
     double *dd2 = nullptr;
     unsigned long long *dc = nullptr;

@@ -50,7 +50,7 @@ def _create(p, n=10, pos=True, vel=False):
 
 @pytest.mark.parametrize("field,value,code", [
     ("treetype", 7, nb.NBX_ERR_UNSUPPORTED),   # KDTree.cxx:1096-1099 "Error in type of tree specified"
-    ("treetype", nb.TVEL, nb.NBX_ERR_UNSUPPORTED),
+    ("treetype", nb.TPROJ, nb.NBX_ERR_UNSUPPORTED),
     ("splitting", 1, nb.NBX_ERR_UNSUPPORTED),
     ("scalespace", 1, nb.NBX_ERR_UNSUPPORTED),
 ])
@@ -65,6 +65,15 @@ def test_phase_tree_needs_velocities_and_aniso():
     p = nb.params(treetype=nb.TPHS, aniso=-1)
     assert _create(p, vel=False) == nb.NBX_ERR_ARG
     p = nb.params(treetype=nb.TPHS, aniso=0)
+    assert _create(p, vel=True) == nb.NBX_ERR_UNSUPPORTED
+
+
+def test_velocity_tree_needs_velocities_and_no_period():
+    """TVEL builds on the velocities; a periodic TVEL tree is rejected (the
+    reference would apply the period to velocity images)."""
+    p = nb.params(treetype=nb.TVEL)
+    assert _create(p, vel=False) == nb.NBX_ERR_ARG
+    p = nb.params(treetype=nb.TVEL, period=1.0)
     assert _create(p, vel=True) == nb.NBX_ERR_UNSUPPORTED
 
 

@@ -1,5 +1,5 @@
-"""GPU parity: 6-D phase space, velocity density, position queries and the
-differentiation form (adjoint)."""
+"""GPU parity: 6-D phase space, velocity density, velocity (TVEL) trees,
+position queries and the differentiation form (adjoint)."""
 import numpy as np
 import pytest
 
@@ -72,6 +72,46 @@ def test_knn_points_vs_oracle(gpu, periodic):
     oi, od = oracle.Tree(pos, b=16, period=per).knn_points(q, k)
     assert np.array_equal(np.sort(nn, 1), np.sort(oi, 1))
     assert np.array_equal(d2, od)
+
+
+@pytest.mark.parametrize("form", ["plain", "mixed"])
+def test_velocity_tree_vs_oracle(gpu, form):
+    """TVEL: the tree is built on the velocities and FindNearestVel, the
+    velocity-point search and both densities run in velocity space
+    (KDFindNearest.cxx:258, :335-346, :530-541; KDCalcSmoothQuantities.cxx:228,
+    :783).  Positions are passed too and must be ignored: the oracle here is
+    brute force and the oracle's tree, both over the velocities."""
+    import nbodylib_amd as nb
+
+    n, m, k = 20000, 500, 32
+    say(f"velocity tree n={n} k={k} form={form}")
+    pos, vel, _ = datagen.clustered(n, 100.0, nhalo=10, seed=36, with_vel=True)
+    mass = datagen.lognormal_masses(n, 36)
+    f = nb.FORM_PLAIN if form == "plain" else nb.FORM_MIXED
+    t = nb.KDTree(pos, mass=mass, vel=vel, treetype=nb.TVEL, kerntype=nb.KSPH, form=f)
+    nn, d2 = t.FindNearestVel(k)
+    bi, bd = oracle.knn_brute(vel, k, mode=0)
+    assert np.array_equal(np.sort(nn, 1), np.sort(bi, 1))
+    assert np.array_equal(d2, bd)
+    qv = vel[:m] + 7.0
+    ot = oracle.Tree(vel, mass=mass, b=16, kerntype=oracle.KSPH)
+    qi, qd = t.FindNearestVelPoints(qv, k)
+    oi, od = ot.knn_points(qv, k)
+    assert np.array_equal(np.sort(qi, 1), np.sort(oi, 1))
+    assert np.array_equal(qd, od)
+    rho = t.CalcDensity(k)
+    assert np.max(np.abs(rho / ot.calc_density(k) - 1.0)) < 1e-12
+    rg, hg = t.CalcDensityParticleAll(k)
+    og, ohg = ot.calc_density_gather(k)
+    assert np.array_equal(hg, ohg)
+    assert np.max(np.abs(rg / og - 1.0)) < 1e-12
+    # calls that need positions are refused on a velocity tree
+    for call in (lambda: t.FOF(1.0, 8), lambda: t.FindNearestPos(k)):
+        with pytest.raises(nb.NbxError):
+            call()
+    tp = nb.KDTree(pos, vel=vel, kerntype=nb.KSPH)
+    with pytest.raises(nb.NbxError):
+        tp.FindNearestVel(k)
 
 
 def test_adjoint_gradcheck_and_reference(gpu):
