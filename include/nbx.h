@@ -118,6 +118,16 @@ int nbx_knn_points(nbx_tree *t, const double *q, int64_t m, int32_t k, int32_t *
  * (the reference exits). */
 int nbx_density_points(nbx_tree *t, const double *q, int64_t m, int32_t k, double *rho, double *h);
 
+/* Velocity density at arbitrary phase points (x, v: 3*m SoA each), TPHYS
+ * trees built with velocities.  Replaces CalcVelDensityPosition(x, v,
+ * Nsmooth, Nsearch) KDCalcSmoothQuantities.cxx:1150-1208 for m points: the
+ * Nsearch nearest of x (non-periodic, nothing excluded), the Nsmooth of them
+ * nearest to v, h_v = max |v - v_j| / 2, rho = sum W(|v - v_j| / h_v) h_v^-3
+ * (no masses).  Nsmooth > Nsearch is taken as Nsearch.  rho[m]; hv[m]
+ * nullable.  NBX_ERR_SEARCH when the tree holds fewer than Nsearch particles. */
+int nbx_veldensity_points(nbx_tree *t, const double *x, const double *v, int64_t m, int32_t nsmooth, int32_t nsearch,
+                          double *rho, double *hv);
+
 /* SPH densities, results keyed by ID.
  * SYMMETRIC replaces KDTree::CalcDensity KDCalcSmoothQuantities.cxx:203-305;
  * GATHER replaces CalcDensityParticle :768-844 for every particle;

@@ -275,6 +275,11 @@ class KDTree {
                                     Double_t *vdist = nullptr);
     // KDTree.h:432 (TPHYS trees; v is unused there); KDCalcSmoothQuantities.cxx:1092-1149
     Double_t CalcDensityPosition(Double_t *x, Int_t Nsmooth = 64, Double_t *v = NULL);
+    // KDTree.h:433, :438 (TPHYS trees with velocities); KDCalcSmoothQuantities.cxx:1150-1208
+    Double_t CalcVelDensityPosition(Double_t *x, Double_t *v, Int_t Nsmooth = 64, Int_t Nsearch = 64);
+    Double_t CalcVelDensityPosition(Coordinate x, Coordinate v, Int_t Nsmooth = 64, Int_t Nsearch = 64) {
+        return CalcVelDensityPosition(x.GetCoord(), v.GetCoord(), Nsmooth, Nsearch);
+    }
 
     Int_t *FOF(Double_t fdist, Int_t &numgroup, Int_t minnum = 8, int order = 0, Int_tree_t *pHead = nullptr,
                Int_tree_t *pNext = nullptr, Int_tree_t *pTail = nullptr, Int_tree_t *pLen = nullptr);

@@ -313,6 +313,12 @@ Double_t KDTree::CalcDensityPosition(Double_t *x, Int_t Nsmooth, Double_t *) {
     check(nbx_density_points(tree, x, 1, Nsmooth, &den, nullptr), "CalcDensity failed");
     return den;
 }
+Double_t KDTree::CalcVelDensityPosition(Double_t *x, Double_t *v, Int_t Nsmooth, Int_t Nsearch) {
+    std::lock_guard<std::mutex> lk(mu);
+    double den = 0.0;
+    check(nbx_veldensity_points(tree, x, v, 1, Nsmooth, Nsearch, &den, nullptr), "CalcVelDensityPosition failed");
+    return den;
+}
 
 // which: 0 skewness, 1 kurtosis.  The reference's order of preparation:
 // densities (densityset), then means (meanvelset), then dispersions

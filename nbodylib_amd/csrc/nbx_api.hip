@@ -525,15 +525,22 @@ extern "C" int nbx_knn(nbx_tree *t, int32_t k, int32_t variant, int32_t layout, 
 // Position queries through the plain kernel: kNN (nn/d2, nullable) and/or the
 // gather density with h = r_k/2 (rho/h, nullable).  periodic: the image passes
 // of the period-aware search.
+// qv (velocities of the points) selects the velocity density over the k
+// nearest positions, with nsmooth of them in velocity (else the gather SPH
+// density when rho is given).
 static int points_impl(nbx_tree *t, const double *q, int64_t m, int32_t k, bool periodic, int32_t *nn, double *d2,
-                       double *rho, double *h, const char *what) {
+                       double *rho, double *h, const char *what, const double *qv = nullptr, int32_t nsmooth = 0) {
     DevBufs bufs;
     if (m <= 0) return NBX_OK;
     if (m > INT32_MAX) return fail(NBX_ERR_ARG, "too many positions (m < 2^31)");
-    double *dq = nullptr, *dd2 = nullptr, *drho = nullptr, *dh = nullptr;
+    double *dq = nullptr, *dqv = nullptr, *dd2 = nullptr, *drho = nullptr, *dh = nullptr;
     int32_t *dnn = nullptr, *err = nullptr;
     HIPCK(bufs.alloc(&dq, sizeof(double) * 3 * m));
     HIPCK(hipMemcpy(dq, q, sizeof(double) * 3 * m, hipMemcpyHostToDevice));
+    if (qv) {
+        HIPCK(bufs.alloc(&dqv, sizeof(double) * 3 * m));
+        HIPCK(hipMemcpy(dqv, qv, sizeof(double) * 3 * m, hipMemcpyHostToDevice));
+    }
     if (nn) HIPCK(bufs.alloc(&dnn, sizeof(int32_t) * m * k));
     if (d2) HIPCK(bufs.alloc(&dd2, sizeof(double) * m * k));
     if (rho) HIPCK(bufs.alloc(&drho, sizeof(double) * m));
@@ -550,7 +557,11 @@ static int points_impl(nbx_tree *t, const double *q, int64_t m, int32_t k, bool 
     for (int d = 0; d < 3; ++d) { A.period[d] = t->p.period[d]; A.q[d] = dq + d * m; }
     A.nn = dnn;
     A.d2 = dd2;
-    A.dens = rho ? NBX_DO_GATHER : NBX_DO_NONE;
+    A.dens = rho ? (qv ? NBX_DO_VEL : NBX_DO_GATHER) : NBX_DO_NONE;
+    if (qv) {
+        for (int d = 0; d < 3; ++d) A.qv[d] = dqv + d * m;
+        A.nsmooth = nsmooth;
+    }
     A.rho = drho;
     A.hout = dh;
     A.err = err;
@@ -563,7 +574,8 @@ static int points_impl(nbx_tree *t, const double *q, int64_t m, int32_t k, bool 
     if (e == hipSuccess && rho) e = hipMemcpy(rho, drho, sizeof(double) * m, hipMemcpyDeviceToHost);
     if (e == hipSuccess && h) e = hipMemcpy(h, dh, sizeof(double) * m, hipMemcpyDeviceToHost);
     if (e != hipSuccess) return hipfail(e, what);
-    if (herr) return fail(NBX_ERR_SEARCH, "CalcDensityPosition failed: fewer than Nsmooth particles");
+    if (herr) return fail(NBX_ERR_SEARCH, qv ? "CalcVelDensityPosition failed: fewer than Nsearch particles"
+                                             : "CalcDensityPosition failed: fewer than Nsmooth particles");
     return NBX_OK;
 }
 
@@ -580,6 +592,19 @@ extern "C" int nbx_density_points(nbx_tree *t, const double *q, int64_t m, int32
     if (k < 1 || k > 64) return fail(NBX_ERR_UNSUPPORTED, "Nsmooth out of range (1..64)");
     if (k > t->n) return fail(NBX_ERR_SEARCH, "CalcDensityPosition failed: fewer than Nsmooth particles");
     return points_impl(t, q, m, k, false, nullptr, nullptr, rho, h, "nbx_density_points");
+}
+
+extern "C" int nbx_veldensity_points(nbx_tree *t, const double *x, const double *v, int64_t m, int32_t nsmooth,
+                                     int32_t nsearch, double *rho, double *hv) {
+    if (!t || !x || !v || !rho) return fail(NBX_ERR_ARG, "NULL argument");
+    if (t->nd != 3 || !t->has_vel)
+        return fail(NBX_ERR_UNSUPPORTED, "CalcVelDensityPosition needs a TPHYS tree built with velocities");
+    if (nsmooth < 1 || nsearch < 1) return fail(NBX_ERR_ARG, "Nsmooth and Nsearch must be >= 1");
+    if (nsearch > 64) return fail(NBX_ERR_UNSUPPORTED, "Nsearch out of range (1..64)");
+    if (nsearch > t->n) return fail(NBX_ERR_SEARCH, "CalcVelDensityPosition failed: fewer than Nsearch particles");
+    // Nsmooth > Nsearch is taken as Nsearch (KDCalcSmoothQuantities.cxx:1160-1163)
+    return points_impl(t, x, m, nsearch, false, nullptr, nullptr, rho, hv, "nbx_veldensity_points", v,
+                       nsmooth > nsearch ? nsearch : nsmooth);
 }
 
 // ------------------------------------------------------------------ density

@@ -65,6 +65,7 @@ struct KnnArgs {
     int32_t periodic;
     double period[3];
     const double *q[3];  // external points (SoA)
+    const double *qv[3];  // external points: their velocities (NBX_DO_VEL)
     int32_t *nn;        // nullable
     double *d2;         // nullable
     double *rho;        // by slot (density); by query row for position queries

@@ -365,21 +365,27 @@ __global__ __launch_bounds__(64) void k_knn(DevTree T, KnnArgs A) {
     double racc_own = 0.0;
     if (own) racc[lane] = 0.0;
     __syncthreads();
-    // position queries take the gather form only (mass and slot of the
+    // position queries take the gather forms only (mass and slot of the
     // query are undefined there)
-    if (A.dens != NBX_DO_NONE && valid && (QSRC == NBX_Q_SLOTS || A.dens == NBX_DO_GATHER)) {
+    if (A.dens != NBX_DO_NONE && valid &&
+        (QSRC == NBX_Q_SLOTS || A.dens == NBX_DO_GATHER || (QSRC == NBX_Q_POINTS && A.dens == NBX_DO_VEL))) {
         bool bad = false;
         if (A.dens == NBX_DO_VEL) {
             // CalcVelDensity: |dv| of the Nsearch physical neighbours, the
             // Nsmooth smallest of them, h_v = max/2, sum W(|dv|/h_v) h_v^-3.
             // The kh entries move to positions 0..kh-1 first (sorted order).
+            // (CalcVelDensityPosition: the query's velocity comes with it)
             const int ns = A.nsmooth;
+            const bool pts = QSRC == NBX_Q_POINTS;
+            const double qvx = pts ? A.qv[0][qi] : T.x[3][qi];
+            const double qvy = pts ? A.qv[1][qi] : T.x[4][qi];
+            const double qvz = pts ? A.qv[2][qi] : T.x[5][qi];
             heap_sort(hd, hid, lane, hc);
             for (int p = 0; p < kh; ++p) {
                 const int32_t id = HI(p + nfill);
                 if (id == NBX_NOID) { bad = true; break; }
                 const int32_t s = T.inv[id];
-                double vx = T.x[3][qi] - T.x[3][s], vy = T.x[4][qi] - T.x[4][s], vz = T.x[5][qi] - T.x[5][s];
+                double vx = qvx - T.x[3][s], vy = qvy - T.x[4][s], vz = qvz - T.x[5][s];
                 HD(p) = sqrt(vx * vx + vy * vy + vz * vz);
                 HI(p) = id;
             }

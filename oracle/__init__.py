@@ -59,6 +59,8 @@ def lib():
         L.orc_calc_density_gather.restype = _ci
         L.orc_calc_veldensity.argtypes = [_vp, _ci, _ci, _dp, _ci]
         L.orc_calc_veldensity.restype = _ci
+        L.orc_calc_veldensity_position.argtypes = [_vp, _dp, _dp, ctypes.c_int64, _ci, _ci, _dp, _vp, _ci]
+        L.orc_calc_veldensity_position.restype = _ci
         L.orc_calc_density_position.argtypes = [_vp, _dp, _i64, _ci, _dp, ctypes.c_void_p, _ci]
         L.orc_calc_density_position.restype = _ci
         L.orc_smooth_vel.argtypes = [_vp, _ci, ctypes.c_void_p, _dp, ctypes.c_void_p]
@@ -239,6 +241,18 @@ class Tree:
         if rc != 0:
             raise RuntimeError("CalcVelDensity failed")
         return rho
+
+    def calc_veldensity_position(self, q, qv, nsmooth, nsearch, nthreads=0):
+        """CalcVelDensityPosition(x, v, Nsmooth, Nsearch) at each row of q, qv (m, 3): (rho, h_v)."""
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        qv = np.ascontiguousarray(qv, dtype=np.float64)
+        m = q.shape[0]
+        rho = np.zeros(m, dtype=np.float64)
+        h = np.zeros(m, dtype=np.float64)
+        rc = lib().orc_calc_veldensity_position(self.h, q, qv, m, nsmooth, nsearch, rho, _ptr(h), nthreads)
+        if rc != 0:
+            raise RuntimeError("CalcVelDensityPosition failed")
+        return rho, h
 
     def knn_density_sample(self, k, slots, nthreads=0):
         """kNN + gather density for the given tree slots (CPU baseline sample)."""

@@ -890,6 +890,59 @@ int orc_calc_veldensity(void *h, int nsmooth, int nsearch, double *rho_by_id, in
     return rc;
 }
 
+/* CalcVelDensityPosition(x, v, Nsmooth, Nsearch), KDCalcSmoothQuantities.cxx:1150-1208:
+ * the Nsearch nearest of position x (position search: nothing excluded, no
+ * period), their |v - v_j|, the Nsmooth smallest, h_v = max / 2 and the sum of
+ * W(|dv| / h_v) h_v^-3 without masses.  q: m rows of (x, y, z); qv: m rows of
+ * velocities.  A TPHYS tree built with velocities. */
+int orc_calc_veldensity_position(void *h, const double *q, const double *qv, int64_t m, int nsmooth, int nsearch,
+                                 double *rho, double *hout, int nthreads) {
+    otree *t = (otree *)h;
+    if (nsmooth > nsearch) nsmooth = nsearch;
+    int rc = 0;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel reduction(min : rc)
+    {
+        pq_t pq, pq2;
+        pq_init(&pq, nsearch);
+        pq_init(&pq2, nsmooth);
+        int32_t *nnid = (int32_t *)malloc(sizeof(int32_t) * (size_t)nsearch);
+        double *vdist = (double *)malloc(sizeof(double) * (size_t)nsearch);
+#pragma omp for schedule(guided)
+        for (int64_t i = 0; i < m; i++) {
+            double off[6] = {0, 0, 0, 0, 0, 0};
+            pq.n = 0;
+            for (int j = 0; j < nsearch; j++) pq_push(&pq, -1, ORC_MAXVALUE);
+            nn_pos(t, t->root, 0.0, &pq, off, q + 3 * i, 0, 3);
+            int bad = 0;
+            for (int j = 0; j < nsearch; j++) {
+                if (pq_topq(&pq) == -1) { bad = 1; break; }
+                nnid[j] = pq_topq(&pq);
+                vdist[j] = sqrt(dist_dim(qv + 3 * i, t->p[nnid[j]].ph + 3, 3));
+                pq_pop(&pq);
+            }
+            if (bad) { rc = -1; continue; }
+            pq2.n = 0;
+            for (int j = 0; j < nsmooth; j++) pq_push(&pq2, -1, ORC_MAXVALUE);
+            for (int j = 0; j < nsearch; j++)
+                if (vdist[j] < pq_top(&pq2)) { pq_pop(&pq2); pq_push(&pq2, nnid[j], vdist[j]); }
+            double hi = 0.5 * pq_top(&pq2);
+            double norm = 1.0 / pow(hi, 3.0);
+            double den = 0;
+            for (int j = 0; j < nsmooth; j++) {
+                den += wsm_at(pq_top(&pq2), hi, t->kernres, t->kern) * norm;
+                pq_pop(&pq2);
+            }
+            rho[i] = den;
+            if (hout) hout[i] = hi;
+        }
+        free(nnid); free(vdist); pq_free(&pq); pq_free(&pq2);
+    }
+    return rc;
+}
+
 /* --------------------------------------------------------------------- FOF */
 typedef struct {
     const otree *t;

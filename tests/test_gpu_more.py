@@ -58,6 +58,35 @@ def test_vel_density_vs_oracle(gpu):
     assert np.max(np.abs(rv / ro - 1.0)) < 1e-12
 
 
+def test_veldensity_position_vs_oracle(gpu):
+    """CalcVelDensityPosition at random phase points and at the particles' own
+    (x, v), against the oracle; Nsmooth > Nsearch clamps; a tree without
+    velocities and too few particles are errors."""
+    import nbodylib_amd as nb
+
+    n, m = 20000, 3000
+    say(f"vel density position n={n} m={m}")
+    pos, vel, _ = datagen.clustered(n, 100.0, nhalo=10, seed=38, with_vel=True)
+    rng = np.random.default_rng(38)
+    x = np.concatenate([rng.uniform(0.0, 100.0, (m, 3)), pos[:500]])
+    v = np.concatenate([rng.normal(0.0, 300.0, (m, 3)), vel[:500]])
+    t = nb.KDTree(pos, vel=vel, kerntype=nb.KSPH, period=100.0)
+    ot = oracle.Tree(pos, vel=vel, b=16, kerntype=oracle.KSPH, period=100.0)
+    ref = {}
+    for ns, nS in ((32, 64), (16, 16), (64, 32)):
+        rho, hv = t.CalcVelDensityPosition(x, v, ns, nS, return_h=True)
+        ro, ho = ot.calc_veldensity_position(x, v, ns, nS)
+        assert np.max(np.abs(hv / ho - 1.0)) < 1e-14
+        assert np.max(np.abs(rho / ro - 1.0)) < 1e-12
+        ref[(ns, nS)] = rho
+    one = t.CalcVelDensityPosition(x[7], v[7], 32, 64)
+    assert one == ref[(32, 64)][7]
+    with pytest.raises(nb.NbxError):
+        nb.KDTree(pos, kerntype=nb.KSPH).CalcVelDensityPosition(x[:2], v[:2], 8, 16)
+    with pytest.raises(nb.NbxError):
+        nb.KDTree(pos[:10], vel=vel[:10], kerntype=nb.KSPH).CalcVelDensityPosition(x[:2], v[:2], 8, 16)
+
+
 @pytest.mark.parametrize("periodic", [False, True])
 def test_knn_points_vs_oracle(gpu, periodic):
     import nbodylib_amd as nb

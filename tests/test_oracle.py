@@ -264,6 +264,45 @@ def test_density_position_at_particles_is_gather_plus_self():
     assert abs(rb.mean() - 1.0) < 0.03
 
 
+def _veldensity_position_brute(pos, vel, x, v, ns, nS):
+    """CalcVelDensityPosition restated over all pairs: the nS nearest of x
+    (nothing excluded), their |v - v_j|, the ns smallest, h_v = max / 2."""
+    rho = np.zeros(len(x))
+    hv = np.zeros(len(x))
+    for i in range(len(x)):
+        d2 = ((x[i] - pos) ** 2).sum(1)
+        near = np.argsort(d2, kind="stable")[:nS]
+        vd = np.sort(np.sqrt(((v[i] - vel[near]) ** 2).sum(1)))[:ns]
+        hv[i] = 0.5 * vd[-1]
+        rho[i] = (_wtab(vd / hv[i]) / hv[i] ** 3).sum()
+    return rho, hv
+
+
+def test_veldensity_position_pins():
+    """CalcVelDensityPosition: (1) at a particle's own (x_i, v_i) with Nsmooth
+    + 1 and Nsearch + 1 the position search finds the particle itself, whose
+    |dv| = 0 takes one smoothing slot, so the result is CalcVelDensity's plus
+    the self term W(0) h_v^-3 = h_v^-3 / pi; (2) at random phase points it
+    equals an all-pairs restatement.  Using the particles' velocities in place
+    of the query's, or an exclusion of d2 = 0, fails one of them."""
+    pos, vel, _ = datagen.clustered(3000, 10.0, nhalo=6, seed=37, with_vel=True)
+    t = oracle.Tree(pos, vel=vel, b=16, kerntype=oracle.KSPH)
+    rv = t.calc_veldensity(16, 48)
+    rp, hp = t.calc_veldensity_position(pos, vel, 17, 49)
+    assert np.max(np.abs(rp / (rv + (1.0 / math.pi) / hp ** 3) - 1)) < 1e-12
+    rng = np.random.default_rng(37)
+    x = rng.uniform(0.0, 10.0, (200, 3))
+    v = rng.normal(0.0, 300.0, (200, 3))
+    r1, h1 = t.calc_veldensity_position(x, v, 16, 48)
+    rb, hb = _veldensity_position_brute(pos, vel, x, v, 16, 48)
+    assert np.max(np.abs(h1 / hb - 1)) < 1e-15
+    assert np.max(np.abs(r1 / rb - 1)) < 1e-12
+    # Nsmooth > Nsearch is taken as Nsearch
+    r2, _ = t.calc_veldensity_position(x[:20], v[:20], 64, 32)
+    r3, _ = t.calc_veldensity_position(x[:20], v[:20], 32, 32)
+    assert np.array_equal(r2, r3)
+
+
 def _wtab(q):
     """The kernel table read as Wsm does (KDCalcSmoothQuantities.cxx:12-15), 3-D KSPH, kernres 1000."""
     _, tab = oracle.kernel(3, oracle.KSPH, 1000)
