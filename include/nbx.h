@@ -128,6 +128,18 @@ int nbx_density_points(nbx_tree *t, const double *q, int64_t m, int32_t k, doubl
 int nbx_veldensity_points(nbx_tree *t, const double *x, const double *v, int64_t m, int32_t nsmooth, int32_t nsearch,
                           double *rho, double *hv);
 
+/* Velocity density weighted by the physical overlap, every particle, keyed
+ * by ID.  Replaces CalcVelDensityWithPhysDensity(Nsmooth, Nsearch, densityset)
+ * KDCalcSmoothQuantities.cxx:393-476 (and, per particle,
+ * CalcVelDensityWithPhysDensityParticle :922-990): CalcVelDensity's selection
+ * (Nsearch physical neighbours, self and d2 == 0 excluded, no period; the
+ * Nsmooth nearest in velocity), each term W(|dv| / h_v) h_v^-3 times
+ * W(r / h) h^-3 / rho_i * m_i, where h = (d2 of the Nsearch-th neighbour) / 2
+ * -- the reference takes no square root there.  rho_by_id: the densities the
+ * reference reads from Particle::rho (CalcDensity(Nsmooth) when densityset
+ * != 1; the caller runs it).  TPHYS trees built with velocities. */
+int nbx_veldensity_phys(nbx_tree *t, int32_t nsmooth, int32_t nsearch, const double *rho_by_id, double *vden_by_id);
+
 /* SPH densities, results keyed by ID.
  * SYMMETRIC replaces KDTree::CalcDensity KDCalcSmoothQuantities.cxx:203-305;
  * GATHER replaces CalcDensityParticle :768-844 for every particle;

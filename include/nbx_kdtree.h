@@ -275,6 +275,11 @@ class KDTree {
                                     Double_t *vdist = nullptr);
     // KDTree.h:432 (TPHYS trees; v is unused there); KDCalcSmoothQuantities.cxx:1092-1149
     Double_t CalcDensityPosition(Double_t *x, Int_t Nsmooth = 64, Double_t *v = NULL);
+    // KDCalcSmoothQuantities.cxx:393-476 (new[] array by ID; the caller delete[]s it) and,
+    // per particle, :922-990, served from one batched call per (Nsmooth, Nsearch)
+    Double_t *CalcVelDensityWithPhysDensity(Int_t Nsmooth = 64, Int_t Nsearch = 64, int densityset = 0);
+    Double_t CalcVelDensityWithPhysDensityParticle(Int_t target, Int_t Nsmooth = 64, Int_t Nsearch = 64,
+                                                   int densityset = 0);
     // KDTree.h:433, :438 (TPHYS trees with velocities); KDCalcSmoothQuantities.cxx:1150-1208
     Double_t CalcVelDensityPosition(Double_t *x, Double_t *v, Int_t Nsmooth = 64, Int_t Nsearch = 64);
     Double_t CalcVelDensityPosition(Coordinate x, Coordinate v, Int_t Nsmooth = 64, Int_t Nsearch = 64) {

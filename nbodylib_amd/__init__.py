@@ -30,7 +30,7 @@ NBX_ERR_ARG, NBX_ERR_HIP, NBX_ERR_UNSUPPORTED, NBX_ERR_NOMEM, NBX_ERR_SEARCH = -
 
 EXPORTS = [
     "nbx_params_default", "nbx_tree_create", "nbx_tree_create_dev", "nbx_tree_destroy", "nbx_tree_order",
-    "nbx_tree_size", "nbx_tree_kernel", "nbx_knn", "nbx_knn_dev", "nbx_knn_points", "nbx_density_points", "nbx_veldensity_points", "nbx_smooth_vel", "nbx_smooth_vel_moments", "nbx_smooth_local", "nbx_smooth_local_points", "nbx_density",
+    "nbx_tree_size", "nbx_tree_kernel", "nbx_knn", "nbx_knn_dev", "nbx_knn_points", "nbx_density_points", "nbx_veldensity_points", "nbx_veldensity_phys", "nbx_smooth_vel", "nbx_smooth_vel_moments", "nbx_smooth_local", "nbx_smooth_local_points", "nbx_density",
     "nbx_density_dev", "nbx_knn_density_dev", "nbx_ball_points", "nbx_ball_points_many", "nbx_ball_mark", "nbx_fof", "nbx_fof_dev", "nbx_fof_criterion", "nbx_fof_criterion_dev", "nbx_fof_lists",
     "nbx_sph_density_fwd_dev", "nbx_sph_density_adjoint_dev", "nbx_prof_enable", "nbx_prof_reset", "nbx_prof_get",
     "nbx_tree_last_fallbacks", "nbx_last_error",
@@ -80,6 +80,7 @@ def lib():
     L.nbx_knn_points.argtypes = [vp, vp, i64, i32, vp, vp]
     L.nbx_density_points.argtypes = [vp, vp, i64, i32, vp, vp]
     L.nbx_veldensity_points.argtypes = [vp, vp, vp, i64, i32, i32, vp, vp]
+    L.nbx_veldensity_phys.argtypes = [vp, i32, i32, vp, vp]
     L.nbx_smooth_vel.argtypes = [vp, i32, vp, vp, vp, vp]
     L.nbx_smooth_vel_moments.argtypes = [vp, i32, vp, vp, vp, vp, vp]
     L.nbx_smooth_local.argtypes = [vp, i32, vp, vp, vp, vp, vp]
@@ -277,6 +278,18 @@ class KDTree:
         if one:
             return (float(rho[0]), float(h[0])) if return_h else float(rho[0])
         return (rho, h) if return_h else rho
+
+    def CalcVelDensityWithPhysDensity(self, Nsmooth=64, Nsearch=64, rho=None):
+        """CalcVelDensityWithPhysDensity(Nsmooth, Nsearch, densityset) (KDCalcSmoothQuantities.cxx:393-476),
+        keyed by ID.  rho (by ID) plays densityset = 1; None runs CalcDensity(Nsmooth) first."""
+        if rho is None:
+            rho = self.CalcDensity(Nsmooth)
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        if rho.shape != (self.n,):
+            raise ValueError("rho must have shape (n,)")
+        out = np.zeros(self.n, dtype=np.float64)
+        _check(lib().nbx_veldensity_phys(self._h, Nsmooth, Nsearch, _p(rho), _p(out)))
+        return out
 
     def CalcVelDensityPosition(self, x, v, Nsmooth=64, Nsearch=64, return_h=False):
         """CalcVelDensityPosition(x, v, Nsmooth, Nsearch) (KDCalcSmoothQuantities.cxx:1150-1208) at each

@@ -313,6 +313,21 @@ Double_t KDTree::CalcDensityPosition(Double_t *x, Int_t Nsmooth, Double_t *) {
     check(nbx_density_points(tree, x, 1, Nsmooth, &den, nullptr), "CalcDensity failed");
     return den;
 }
+Double_t *KDTree::CalcVelDensityWithPhysDensity(Int_t Nsmooth, Int_t Nsearch, int densityset) {
+    if (densityset != 1) CalcDensity(Nsmooth);
+    std::lock_guard<std::mutex> lk(mu);
+    std::vector<double> rho(numparts);
+    for (Int_t s = 0; s < numparts; ++s) rho[bucket[s].GetID()] = bucket[s].GetDensity();
+    Double_t *out = new Double_t[numparts];
+    check(nbx_veldensity_phys(tree, Nsmooth, Nsearch, rho.data(), out), "CalcVelDensitywithPhysDensity failed");
+    return out;
+}
+Double_t KDTree::CalcVelDensityWithPhysDensityParticle(Int_t target, Int_t Nsmooth, Int_t Nsearch, int densityset) {
+    Double_t *all = CalcVelDensityWithPhysDensity(Nsmooth, Nsearch, densityset);
+    Double_t v = all[bucket[target].GetID()];
+    delete[] all;
+    return v;
+}
 Double_t KDTree::CalcVelDensityPosition(Double_t *x, Double_t *v, Int_t Nsmooth, Int_t Nsearch) {
     std::lock_guard<std::mutex> lk(mu);
     double den = 0.0;

@@ -608,8 +608,11 @@ extern "C" int nbx_veldensity_points(nbx_tree *t, const double *x, const double 
 }
 
 // ------------------------------------------------------------------ density
+// variant DENS_VELPHYS (internal): CalcVelDensityWithPhysDensity over the
+// densities rho_in_id (device, by ID)
+static const int32_t DENS_VELPHYS = 100;
 static int density_impl(nbx_tree *t, int32_t k, int32_t variant, int32_t nsearch, double *rho_id, double *h_id,
-                        int32_t *nn_slot, hipStream_t st) {
+                        int32_t *nn_slot, hipStream_t st, const double *rho_in_id = nullptr) {
     if (!t) return fail(NBX_ERR_ARG, "NULL tree");
     if (k < 1) return fail(NBX_ERR_ARG, "k must be >= 1");
     KnnArgs A;
@@ -621,17 +624,17 @@ static int density_impl(nbx_tree *t, int32_t k, int32_t variant, int32_t nsearch
     A.layout = NBX_LAYOUT_BY_SLOT;
     if (variant == NBX_DENS_SYMMETRIC) A.dens = NBX_DO_SYM;
     else if (variant == NBX_DENS_GATHER) A.dens = NBX_DO_GATHER;
-    else if (variant == NBX_DENS_VEL) {
+    else if (variant == NBX_DENS_VEL || (variant == DENS_VELPHYS && rho_in_id)) {
         if (t->nd != 3 || !t->has_vel)
             return fail(NBX_ERR_UNSUPPORTED, "CalcVelDensity needs a TPHYS tree built with velocities");
-        A.dens = NBX_DO_VEL;
+        A.dens = variant == NBX_DENS_VEL ? NBX_DO_VEL : NBX_DO_VELPHYS;
         if (nsearch < 1) return fail(NBX_ERR_ARG, "Nsearch must be >= 1");
         A.kh = nsearch;
         A.nsmooth = k > nsearch ? nsearch : k;
     } else
         return fail(NBX_ERR_ARG, "unknown density variant");
     if (A.kh > heap_cap(t)) return fail(NBX_ERR_UNSUPPORTED, "k too large (k <= 64 supported)");
-    const bool mixed = t->p.form == NBX_FORM_MIXED && variant != NBX_DENS_VEL;
+    const bool mixed = t->p.form == NBX_FORM_MIXED && A.dens != NBX_DO_VEL && A.dens != NBX_DO_VELPHYS;
     const int64_t n = t->n;
     double *rho = nullptr, *h = nullptr;
     int32_t *err = nullptr;
@@ -641,6 +644,12 @@ static int density_impl(nbx_tree *t, int32_t k, int32_t variant, int32_t nsearch
     HIPCK(bufs.alloc(&err, sizeof(int32_t)));
     hipLaunchKernelGGL(k_fill_f64, dim3(nblk(n)), dim3(256), 0, st, rho, 0.0, n);
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, st, err, 0, (int64_t)1);
+    if (A.dens == NBX_DO_VELPHYS) {
+        double *rin = nullptr;
+        HIPCK(bufs.alloc(&rin, sizeof(double) * n));
+        hipLaunchKernelGGL(k_gather_by_id, dim3(nblk(n)), dim3(256), 0, st, rho_in_id, t->perm, n, rin);
+        A.rho_in = rin;
+    }
     A.rho = rho;
     A.hout = h;
     A.err = err;
@@ -868,6 +877,23 @@ extern "C" int nbx_density(nbx_tree *t, int32_t k, int32_t variant, int32_t nsea
         hipError_t e = hipMemcpy(rho_by_id, dr, sizeof(double) * t->n, hipMemcpyDeviceToHost);
         if (e == hipSuccess && h_by_id) e = hipMemcpy(h_by_id, dh, sizeof(double) * t->n, hipMemcpyDeviceToHost);
         if (e != hipSuccess) rc = hipfail(e, "nbx_density");
+    }
+    return rc;
+}
+
+extern "C" int nbx_veldensity_phys(nbx_tree *t, int32_t nsmooth, int32_t nsearch, const double *rho_by_id,
+                                   double *vden_by_id) {
+    if (!t || !rho_by_id || !vden_by_id) return fail(NBX_ERR_ARG, "NULL argument");
+    if (nsmooth < 1) return fail(NBX_ERR_ARG, "Nsmooth must be >= 1");
+    DevBufs bufs;
+    double *rin = nullptr, *dv = nullptr;
+    HIPCK(bufs.alloc(&rin, sizeof(double) * t->n));
+    HIPCK(bufs.alloc(&dv, sizeof(double) * t->n));
+    HIPCK(hipMemcpy(rin, rho_by_id, sizeof(double) * t->n, hipMemcpyHostToDevice));
+    int rc = density_impl(t, nsmooth, DENS_VELPHYS, nsearch, dv, nullptr, nullptr, nullptr, rin);
+    if (rc == NBX_OK) {
+        hipError_t e = hipMemcpy(vden_by_id, dv, sizeof(double) * t->n, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = hipfail(e, "nbx_veldensity_phys");
     }
     return rc;
 }

@@ -47,6 +47,7 @@ hipError_t nbx_sort_pairs_u64(void *temp, size_t &temp_bytes, uint64_t *kin, uin
 #define NBX_DO_SYM 1     // CalcDensity: 1/2 W scattered to both ends
 #define NBX_DO_GATHER 2  // CalcDensityParticle: W m_j gathered
 #define NBX_DO_VEL 3     // CalcVelDensity
+#define NBX_DO_VELPHYS 4  // CalcVelDensityWithPhysDensity: CalcVelDensity weighted by W(r) m_i / rho_i
 // Output drop rule for the k+1 periodic search
 #define NBX_DROP_NONE 0
 #define NBX_DROP_SMALLEST 1
@@ -66,6 +67,7 @@ struct KnnArgs {
     double period[3];
     const double *q[3];  // external points (SoA)
     const double *qv[3];  // external points: their velocities (NBX_DO_VEL)
+    const double *rho_in;  // NBX_DO_VELPHYS: densities by slot
     int32_t *nn;        // nullable
     double *d2;         // nullable
     double *rho;        // by slot (density); by query row for position queries

@@ -370,7 +370,7 @@ __global__ __launch_bounds__(64) void k_knn(DevTree T, KnnArgs A) {
     if (A.dens != NBX_DO_NONE && valid &&
         (QSRC == NBX_Q_SLOTS || A.dens == NBX_DO_GATHER || (QSRC == NBX_Q_POINTS && A.dens == NBX_DO_VEL))) {
         bool bad = false;
-        if (A.dens == NBX_DO_VEL) {
+        if (A.dens == NBX_DO_VEL || A.dens == NBX_DO_VELPHYS) {
             // CalcVelDensity: |dv| of the Nsearch physical neighbours, the
             // Nsmooth smallest of them, h_v = max/2, sum W(|dv|/h_v) h_v^-3.
             // The kh entries move to positions 0..kh-1 first (sorted order).
@@ -399,7 +399,24 @@ __global__ __launch_bounds__(64) void k_knn(DevTree T, KnnArgs A) {
                 double hv = 0.5 * HD(0);
                 double norm = 1.0 / pow(hv, 3.0);
                 double den = 0.0;
-                for (int p = 0; p < ns; ++p) den = den + wsm_at(HD(p), hv, T.kern, T.kernres) * norm;
+                if (A.dens == NBX_DO_VEL) {
+                    for (int p = 0; p < ns; ++p) den = den + wsm_at(HD(p), hv, T.kern, T.kernres) * norm;
+                } else {
+                    // CalcVelDensityWithPhysDensity (KDCalcSmoothQuantities.cxx:428-466):
+                    // each term times W(r_ij / h) h^-3 / rho_i * m_i, with
+                    // h = d2 of the Nsearch-th neighbour / 2 (no square root)
+                    const double hp = 0.5 * kth;
+                    const double pnorm = 1.0 / pow(hp, 3.0);
+                    const double rhoi = A.rho_in[qi], mi = T.mass[qi];
+                    for (int p = 0; p < ns; ++p) {
+                        const int32_t s = T.inv[HI(p)];
+                        double dx = T.x[0][qi] - T.x[0][s], dy = T.x[1][qi] - T.x[1][s], dz = T.x[2][qi] - T.x[2][s];
+                        double r = sqrt(dx * dx + dy * dy + dz * dz);
+                        double vW = wsm_at(HD(p), hv, T.kern, T.kernres) * norm;
+                        double W = wsm_at(r, hp, T.kern, T.kernres) * pnorm;
+                        den = den + vW * (W / rhoi * mi);
+                    }
+                }
                 A.rho[qi] = den;
                 if (A.hout) A.hout[qi] = hv;
             }
@@ -454,10 +471,10 @@ __global__ __launch_bounds__(64) void k_knn(DevTree T, KnnArgs A) {
     // ------------------------------------------------------------ outputs
     if (A.nn == nullptr && A.d2 == nullptr) return;
     // (the velocity density sorted already and reused the entries)
-    if (valid && A.dens != NBX_DO_VEL) heap_sort(hd, hid, lane, hc);
+    if (valid && A.dens != NBX_DO_VEL && A.dens != NBX_DO_VELPHYS) heap_sort(hd, hid, lane, hc);
     __syncthreads();
     const int kout = A.kout;
-    const int off = ((A.drop == NBX_DROP_SMALLEST) ? 1 : 0) + (A.dens == NBX_DO_VEL ? 0 : nfill);
+    const int off = ((A.drop == NBX_DROP_SMALLEST) ? 1 : 0) + ((A.dens == NBX_DO_VEL || A.dens == NBX_DO_VELPHYS) ? 0 : nfill);
     const int total = NBX_WAVE * kout;
     for (int e = lane; e < total; e += NBX_WAVE) {
         int r = e / kout, t = e - r * kout;

@@ -61,6 +61,8 @@ def lib():
         L.orc_calc_veldensity.restype = _ci
         L.orc_calc_veldensity_position.argtypes = [_vp, _dp, _dp, ctypes.c_int64, _ci, _ci, _dp, _vp, _ci]
         L.orc_calc_veldensity_position.restype = _ci
+        L.orc_calc_veldensity_phys.argtypes = [_vp, _ci, _ci, _dp, _dp, _ci]
+        L.orc_calc_veldensity_phys.restype = _ci
         L.orc_calc_density_position.argtypes = [_vp, _dp, _i64, _ci, _dp, ctypes.c_void_p, _ci]
         L.orc_calc_density_position.restype = _ci
         L.orc_smooth_vel.argtypes = [_vp, _ci, ctypes.c_void_p, _dp, ctypes.c_void_p]
@@ -241,6 +243,15 @@ class Tree:
         if rc != 0:
             raise RuntimeError("CalcVelDensity failed")
         return rho
+
+    def calc_veldensity_phys(self, nsmooth, nsearch, rho_by_id, nthreads=0):
+        """CalcVelDensityWithPhysDensity(Nsmooth, Nsearch) over the given densities; keyed by ID."""
+        out = np.zeros(self.n, dtype=np.float64)
+        rc = lib().orc_calc_veldensity_phys(self.h, nsmooth, nsearch,
+                                            np.ascontiguousarray(rho_by_id, dtype=np.float64), out, nthreads)
+        if rc != 0:
+            raise RuntimeError("CalcVelDensityWithPhysDensity failed")
+        return out
 
     def calc_veldensity_position(self, q, qv, nsmooth, nsearch, nthreads=0):
         """CalcVelDensityPosition(x, v, Nsmooth, Nsearch) at each row of q, qv (m, 3): (rho, h_v)."""

@@ -104,10 +104,14 @@ int orc_kernel(int nd, int kerntype, int kernres, double *kernnorm_out, double *
     return kernres;
 }
 
-/* Wsm, KDCalcSmoothQuantities.cxx:12-15 */
+/* Wsm, KDCalcSmoothQuantities.cxx:12-15.  The reference returns x[i] past
+ * the interpolation range, which reads beyond the table once r/h > 2; the
+ * searches that set h = r_k / 2 never go there (i <= size - 1), but
+ * CalcVelDensityWithPhysDensity's h without a square root does.  There the
+ * last entry W(2) = 0 is taken. */
 static inline double wsm(double r, int i, int size, double delta, const double *x) {
     if (i < size - 1) return (x[i] + (x[i + 1] - x[i]) * (r - delta * i) / delta);
-    return x[i];
+    return x[size - 1];
 }
 static inline double wsm_at(double rij, double hi, int kernres, const double *kern) {
     return wsm(rij / hi, (int)(rij / hi * 0.5 * (kernres - 1)), kernres, 2.0 / (double)(kernres - 1), kern);
@@ -884,6 +888,63 @@ int orc_calc_veldensity(void *h, int nsmooth, int nsearch, double *rho_by_id, in
                 pq_pop(&pq2);
             }
             rho_by_id[t->p[i].id] = den;
+        }
+        free(nnid); free(vdist); pq_free(&pq); pq_free(&pq2);
+    }
+    return rc;
+}
+
+/* CalcVelDensityWithPhysDensity(Nsmooth, Nsearch, densityset = 1),
+ * KDCalcSmoothQuantities.cxx:393-476: CalcVelDensity's selection, each term
+ * times W(r_ij / h) h^-3 / rho_i * m_i with h = 0.5 * (the Nsearch-th d2)
+ * (no square root, :428).  rho_by_id: the densities read from Particle::rho. */
+int orc_calc_veldensity_phys(void *h, int nsmooth, int nsearch, const double *rho_by_id, double *out_by_id,
+                             int nthreads) {
+    otree *t = (otree *)h;
+    if (nsmooth > nsearch) nsmooth = nsearch;
+    int rc = 0;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel reduction(min : rc)
+    {
+        pq_t pq, pq2;
+        pq_init(&pq, nsearch);
+        pq_init(&pq2, nsmooth);
+        int32_t *nnid = (int32_t *)malloc(sizeof(int32_t) * (size_t)nsearch);
+        double *vdist = (double *)malloc(sizeof(double) * (size_t)nsearch);
+#pragma omp for schedule(guided)
+        for (int64_t i = 0; i < t->n; i++) {
+            double off[6] = {0, 0, 0, 0, 0, 0};
+            pq.n = 0;
+            for (int j = 0; j < nsearch; j++) pq_push(&pq, -1, ORC_MAXVALUE);
+            nn_target(t, t->root, 0.0, &pq, off, i, 0, 3);
+            double hp = 0.5 * pq_top(&pq);
+            double pnorm = 1.0 / pow(hp, 3.0);
+            int bad = 0;
+            for (int j = 0; j < nsearch; j++) {
+                if (pq_topq(&pq) == -1) { bad = 1; break; }
+                nnid[j] = pq_topq(&pq);
+                vdist[j] = sqrt(dist_dim(t->p[i].ph + 3, t->p[nnid[j]].ph + 3, 3));
+                pq_pop(&pq);
+            }
+            if (bad) { rc = -1; continue; }
+            pq2.n = 0;
+            for (int j = 0; j < nsmooth; j++) pq_push(&pq2, -1, ORC_MAXVALUE);
+            for (int j = 0; j < nsearch; j++)
+                if (vdist[j] < pq_top(&pq2)) { pq_pop(&pq2); pq_push(&pq2, nnid[j], vdist[j]); }
+            double hv = 0.5 * pq_top(&pq2);
+            double vnorm = 1.0 / pow(hv, 3.0);
+            double rhoi = rho_by_id[t->p[i].id], den = 0;
+            for (int j = 0; j < nsmooth; j++) {
+                int32_t jj = pq_topq(&pq2);
+                double rij = sqrt(dist_dim(t->p[i].ph, t->p[jj].ph, 3));
+                double vW = wsm_at(pq_top(&pq2), hv, t->kernres, t->kern) * vnorm;
+                double W = wsm_at(rij, hp, t->kernres, t->kern) * pnorm;
+                den += vW * (W / rhoi * t->p[i].m);
+                pq_pop(&pq2);
+            }
+            out_by_id[t->p[i].id] = den;
         }
         free(nnid); free(vdist); pq_free(&pq); pq_free(&pq2);
     }

@@ -87,6 +87,30 @@ def test_veldensity_position_vs_oracle(gpu):
         nb.KDTree(pos[:10], vel=vel[:10], kerntype=nb.KSPH).CalcVelDensityPosition(x[:2], v[:2], 8, 16)
 
 
+@pytest.mark.parametrize("form", ["plain", "mixed"])
+def test_veldensity_phys_vs_oracle(gpu, form):
+    """CalcVelDensityWithPhysDensity over the tree's own CalcDensity and over
+    given densities, against the oracle (zeros, where every neighbour lies
+    beyond r = 2h, compared as zeros)."""
+    import nbodylib_amd as nb
+
+    n = 20000
+    say(f"vel density phys n={n} form={form}")
+    pos, vel, _ = datagen.clustered(n, 100.0, nhalo=10, seed=40, with_vel=True)
+    mass = datagen.lognormal_masses(n, 40)
+    f = nb.FORM_PLAIN if form == "plain" else nb.FORM_MIXED
+    t = nb.KDTree(pos, mass=mass, vel=vel, kerntype=nb.KSPH, form=f)
+    ot = oracle.Tree(pos, mass=mass, vel=vel, b=16, kerntype=oracle.KSPH)
+    given = datagen.lognormal_masses(n, 41) * 50.0
+    for ns, nS, rho in ((32, 64, None), (16, 48, given)):
+        got = t.CalcVelDensityWithPhysDensity(ns, nS, rho)
+        ro = ot.calc_density(ns) if rho is None else rho
+        want = ot.calc_veldensity_phys(ns, nS, ro)
+        z = want == 0
+        assert np.array_equal(got == 0, z)
+        assert np.max(np.abs(got[~z] / want[~z] - 1.0)) < 1e-12
+
+
 @pytest.mark.parametrize("periodic", [False, True])
 def test_knn_points_vs_oracle(gpu, periodic):
     import nbodylib_amd as nb
@@ -187,8 +211,10 @@ def test_adjoint_gradcheck_and_reference(gpu):
     sp = base_p[:, idx].clone().requires_grad_(True)
     sm = base_m[idx].clone().requires_grad_(True)
     # the adjoint accumulates with FP64 atomics, so repeated backward passes
-    # differ in the last bits (summation order): allow that, not more
-    assert torch.autograd.gradcheck(f, (sp, sm), eps=1e-6, atol=1e-5, rtol=1e-4, nondet_tol=1e-12)
+    # differ in the last bits (summation order).  nondet_tol is absolute and
+    # these gradients reach 1e2-1e3, where a few ulps of some 64 terms come
+    # to ~1e-11: 1e-9 allows that (1e-12 relative), not more
+    assert torch.autograd.gradcheck(f, (sp, sm), eps=1e-6, atol=1e-5, rtol=1e-4, nondet_tol=1e-9)
 
 
 @pytest.mark.parametrize("form,kind", [("plain", "uniform"), ("mixed", "uniform"), ("plain", "clustered")])

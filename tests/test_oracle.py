@@ -303,6 +303,33 @@ def test_veldensity_position_pins():
     assert np.array_equal(r2, r3)
 
 
+def test_veldensity_phys_vs_brute():
+    """CalcVelDensityWithPhysDensity against an all-pairs restatement over the
+    brute-force target lists (self and d2 = 0 excluded): h = (Nsearch-th d2)
+    / 2 with no square root, as the reference computes it; each term is
+    W(|dv| / h_v) h_v^-3 W(r / h) h^-3 / rho_i m_i.  Taking the square root
+    for h, or the neighbour's rho or mass in place of the particle's, fails."""
+    n, ns, nS = 1500, 16, 40
+    pos, vel, _ = datagen.clustered(n, 10.0, nhalo=4, seed=39, with_vel=True)
+    mass = datagen.lognormal_masses(n, 39)
+    t = oracle.Tree(pos, mass=mass, vel=vel, b=16, kerntype=oracle.KSPH)
+    rho = t.calc_density(ns)
+    got = t.calc_veldensity_phys(ns, nS, rho)
+    ids, d2 = oracle.knn_brute(pos, nS, mode=0)
+    want = np.zeros(n)
+    for i in range(n):
+        hp = 0.5 * d2[i, -1]
+        vd = np.sqrt(((vel[i] - vel[ids[i]]) ** 2).sum(1))
+        sel = np.argsort(vd, kind="stable")[:ns]
+        hv = 0.5 * vd[sel].max()
+        r = np.sqrt(((pos[i] - pos[ids[i][sel]]) ** 2).sum(1))
+        want[i] = (_wtab(vd[sel] / hv) / hv ** 3 * (_wtab(r / hp) / hp ** 3 / rho[i] * mass[i])).sum()
+    # particles whose every neighbour lies beyond r = 2h give exactly 0
+    z = want == 0
+    assert np.array_equal(got == 0, z) and 0 < z.sum() < n
+    assert np.max(np.abs(got[~z] / want[~z] - 1)) < 1e-12
+
+
 def _wtab(q):
     """The kernel table read as Wsm does (KDCalcSmoothQuantities.cxx:12-15), 3-D KSPH, kernres 1000."""
     _, tab = oracle.kernel(3, oracle.KSPH, 1000)
