@@ -110,6 +110,13 @@ int nbx_knn_dev(nbx_tree *t, int32_t k, int32_t variant, int32_t layout, int32_t
 /* kNN of arbitrary positions (q: 3*m SoA).  Replaces FindNearestPos(Double_t*x,...)
  * KDFindNearest.cxx:462-564 (period-aware like the reference).  Entries are IDs. */
 int nbx_knn_points(nbx_tree *t, const double *q, int64_t m, int32_t k, int32_t *nn, double *d2);
+/* kNN of arbitrary phase-space points (x, v: 3*m SoA each), TPHS trees.
+ * Replaces FindNearestPhase(Double_t *x, Double_t *v, ...) KDFindNearest.cxx:528-538
+ * and FindNearest(Double_t *x, ...) on a TPHS tree (:462-509, x[3..5] = v):
+ * 6-D distances, nothing excluded, and on a periodic tree every one of the 7
+ * spatial images searched (KDSplitNode.cxx:1153-1179).  Entries are IDs. */
+int nbx_knn_phase_points(nbx_tree *t, const double *x, const double *v, int64_t m, int32_t k, int32_t *nn,
+                         double *d2);
 /* Gather SPH density at arbitrary positions (q: 3*m SoA), TPHYS trees.
  * Replaces CalcDensityPosition(Double_t *x, Nsmooth, v) KDCalcSmoothQuantities.cxx:1092-1149
  * for m positions: non-periodic search with no exclusion (a particle at x is

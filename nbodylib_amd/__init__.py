@@ -30,7 +30,7 @@ NBX_ERR_ARG, NBX_ERR_HIP, NBX_ERR_UNSUPPORTED, NBX_ERR_NOMEM, NBX_ERR_SEARCH = -
 
 EXPORTS = [
     "nbx_params_default", "nbx_tree_create", "nbx_tree_create_dev", "nbx_tree_destroy", "nbx_tree_order",
-    "nbx_tree_size", "nbx_tree_kernel", "nbx_knn", "nbx_knn_dev", "nbx_knn_points", "nbx_density_points", "nbx_veldensity_points", "nbx_veldensity_phys", "nbx_smooth_vel", "nbx_smooth_vel_moments", "nbx_smooth_local", "nbx_smooth_local_points", "nbx_density",
+    "nbx_tree_size", "nbx_tree_kernel", "nbx_knn", "nbx_knn_dev", "nbx_knn_points", "nbx_knn_phase_points", "nbx_density_points", "nbx_veldensity_points", "nbx_veldensity_phys", "nbx_smooth_vel", "nbx_smooth_vel_moments", "nbx_smooth_local", "nbx_smooth_local_points", "nbx_density",
     "nbx_density_dev", "nbx_knn_density_dev", "nbx_ball_points", "nbx_ball_points_many", "nbx_ball_mark", "nbx_fof", "nbx_fof_dev", "nbx_fof_criterion", "nbx_fof_criterion_dev", "nbx_fof_lists",
     "nbx_sph_density_fwd_dev", "nbx_sph_density_adjoint_dev", "nbx_prof_enable", "nbx_prof_reset", "nbx_prof_get",
     "nbx_tree_last_fallbacks", "nbx_last_error",
@@ -78,6 +78,7 @@ def lib():
     L.nbx_knn.argtypes = [vp, i32, i32, i32, vp, vp]
     L.nbx_knn_dev.argtypes = [vp, i32, i32, i32, vp, vp, vp]
     L.nbx_knn_points.argtypes = [vp, vp, i64, i32, vp, vp]
+    L.nbx_knn_phase_points.argtypes = [vp, vp, vp, i64, i32, vp, vp]
     L.nbx_density_points.argtypes = [vp, vp, i64, i32, vp, vp]
     L.nbx_veldensity_points.argtypes = [vp, vp, vp, i64, i32, i32, vp, vp]
     L.nbx_veldensity_phys.argtypes = [vp, i32, i32, vp, vp]
@@ -249,6 +250,20 @@ class KDTree:
         _check(lib().nbx_knn_points(self._h, _p(q), m, Nsearch, _p(nn), _p(d2)))
         return nn, d2
 
+    def FindNearestPhasePoints(self, x, v, Nsearch=64):
+        """FindNearestPhase(Double_t *x, Double_t *v, ...) (KDFindNearest.cxx:528-538) for many phase
+        points (x, v: (m, 3) each), TPHS tree; entries are IDs."""
+        x = np.asarray(x, dtype=np.float64).reshape(-1, 3)
+        v = np.asarray(v, dtype=np.float64).reshape(-1, 3)
+        m = x.shape[0]
+        if v.shape[0] != m:
+            raise ValueError("x and v must have the same number of rows")
+        qx, qv = np.ascontiguousarray(x.T), np.ascontiguousarray(v.T)
+        nn = np.zeros((m, Nsearch), dtype=np.int32)
+        d2 = np.zeros((m, Nsearch), dtype=np.float64)
+        _check(lib().nbx_knn_phase_points(self._h, _p(qx), _p(qv), m, Nsearch, _p(nn), _p(d2)))
+        return nn, d2
+
     # -------------------------------------------------------------- densities
     def CalcDensity(self, Nsmooth=64, return_h=False):
         """KDTree::CalcDensity (KDCalcSmoothQuantities.cxx:203-305); rho by ID."""
@@ -281,7 +296,9 @@ class KDTree:
 
     def CalcVelDensityWithPhysDensity(self, Nsmooth=64, Nsearch=64, rho=None):
         """CalcVelDensityWithPhysDensity(Nsmooth, Nsearch, densityset) (KDCalcSmoothQuantities.cxx:393-476),
-        keyed by ID.  rho (by ID) plays densityset = 1; None runs CalcDensity(Nsmooth) first."""
+        keyed by ID.  rho (by ID) plays densityset = 1; None runs CalcDensity(Nsmooth) first, with
+        Nsmooth > Nsearch taken as Nsearch before it, as the reference does (:405-409)."""
+        Nsmooth = min(Nsmooth, Nsearch)
         if rho is None:
             rho = self.CalcDensity(Nsmooth)
         rho = np.ascontiguousarray(rho, dtype=np.float64)

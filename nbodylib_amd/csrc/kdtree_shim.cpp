@@ -35,6 +35,10 @@ static void check(int rc, const char *what) {
     die(std::string(what) + ": " + nbx_last_error());
 }
 
+// tagged ball rows are cached for blocks of this many consecutive tree indices
+static const int BALL_BLOCK_LOG2 = 16;
+static const size_t BALL_BLOCKS_KEPT = 64;
+
 void KDTree::init(Particle *p, Int_t n, Int_t bucket_size, int TreeType, int KernType, int KernRes, int Splitting,
                   int Aniso, int ScaleSpace, Double_t *Period) {
     bucket = p;
@@ -45,9 +49,11 @@ void KDTree::init(Particle *p, Int_t n, Int_t bucket_size, int TreeType, int Ker
     iresetorder = true;
     tree = nullptr;
     cache_variant = cache_k = dcache_k = vcache_k = vcache_s = -1;
+    pcache_k = pcache_s = pcache_set = -1;
+    ball_r2 = -1.0;
+    ball_tick = 0;
     periodic = Period != nullptr;
     for (int k = 0; k < 3; ++k) period[k] = periodic ? Period[k] : 0.0;
-    nbx_params prm;
     nbx_params_default(&prm);
     prm.bucket_size = bucket_size;
     prm.treetype = TreeType;
@@ -147,6 +153,7 @@ void KDTree::FindNearestPhase(Int_t **nn, Double_t **dist2, Int_t Nsearch) {
 
 void KDTree::ensure_cache(int variant, Int_t k) {
     if (cache_variant == variant && cache_k == k) return;
+    cache_variant = cache_k = -1;
     cache_nn.assign((size_t)numparts * k, -1);
     cache_d2.assign((size_t)numparts * k, 0.0);
     batch_knn(variant, k, cache_nn.data(), cache_d2.data());
@@ -173,11 +180,27 @@ void KDTree::FindNearestPhase(Int_t tt, Int_t *nn, Double_t *dist2, Int_t Nsearc
     std::copy(cache_d2.begin() + (size_t)tt * Nsearch, cache_d2.begin() + (size_t)(tt + 1) * Nsearch, dist2);
 }
 
-void KDTree::FindNearestPos(Double_t *x, Int_t *nn, Double_t *dist2, Int_t Nsearch) {
+// one point in the tree's own 3-D space (positions, or velocities on TVEL)
+void KDTree::FindNearest(Double_t *x, Int_t *nn, Double_t *dist2, Int_t Nsearch) {
+    if (treetype == TPHS) {  // x[0..5] is the phase point (KDFindNearest.cxx:475-478, :500-503)
+        FindNearestPhase(x, x + 3, nn, dist2, Nsearch);
+        return;
+    }
     std::lock_guard<std::mutex> lk(mu);
     double q[3] = {x[0], x[1], x[2]};
     std::vector<int32_t> ids(Nsearch);
-    check(nbx_knn_points(tree, q, 1, Nsearch, ids.data(), dist2), "FindNearestPos");
+    check(nbx_knn_points(tree, q, 1, Nsearch, ids.data(), dist2), "FindNearest");
+    for (Int_t j = 0; j < Nsearch; ++j) nn[j] = ids[j] >= 0 ? slot_of_id[ids[j]] : -1;
+}
+void KDTree::FindNearestPos(Double_t *x, Int_t *nn, Double_t *dist2, Int_t Nsearch) {
+    if (treetype != TPHYS) die("FindNearestPos(x) here needs a TPHYS tree");
+    FindNearest(x, nn, dist2, Nsearch);
+}
+void KDTree::FindNearestPhase(Double_t *x, Double_t *v, Int_t *nn, Double_t *dist2, Int_t Nsearch) {
+    std::lock_guard<std::mutex> lk(mu);
+    double qx[3] = {x[0], x[1], x[2]}, qv[3] = {v[0], v[1], v[2]};
+    std::vector<int32_t> ids(Nsearch);
+    check(nbx_knn_phase_points(tree, qx, qv, 1, Nsearch, ids.data(), dist2), "FindNearestPhase");
     for (Int_t j = 0; j < Nsearch; ++j) nn[j] = ids[j] >= 0 ? slot_of_id[ids[j]] : -1;
 }
 
@@ -193,26 +216,73 @@ void KDTree::FindNearestVel(Int_t tt, Int_t *nn, Double_t *dist2, Int_t Nsearch)
 }
 void KDTree::FindNearestVel(Double_t *v, Int_t *nn, Double_t *dist2, Int_t Nsearch) {
     if (treetype != TVEL) die("FindNearestVel needs a TVEL tree");
-    std::lock_guard<std::mutex> lk(mu);
-    double q[3] = {v[0], v[1], v[2]};
-    std::vector<int32_t> ids(Nsearch);
-    check(nbx_knn_points(tree, q, 1, Nsearch, ids.data(), dist2), "FindNearestVel");
-    for (Int_t j = 0; j < Nsearch; ++j) nn[j] = ids[j] >= 0 ? slot_of_id[ids[j]] : -1;
+    FindNearest(v, nn, dist2, Nsearch);
 }
 
 // ------------------------------------------------------------------ ball search
+// The rows of the block of tree indices holding tt, for radius^2 fdist2:
+// one batched search around every position of the block (the caller holds
+// the mutex).  Rows hold every particle with d2 < fdist2, the block's own
+// particle included.
+const KDTree::BallBlock &KDTree::ball_block(Int_t tt, Double_t fdist2) {
+    if (fdist2 != ball_r2) {
+        ball_cache.clear();
+        ball_r2 = fdist2;
+    }
+    const Int_t blk = tt >> BALL_BLOCK_LOG2;
+    auto it = ball_cache.find(blk);
+    if (it != ball_cache.end()) {
+        it->second->used = ++ball_tick;
+        return *it->second;
+    }
+    if (ball_cache.size() >= BALL_BLOCKS_KEPT) {
+        auto old = ball_cache.begin();
+        for (auto j = ball_cache.begin(); j != ball_cache.end(); ++j)
+            if (j->second->used < old->second->used) old = j;
+        ball_cache.erase(old);
+    }
+    const Int_t s0 = blk << BALL_BLOCK_LOG2;
+    const int64_t m = std::min<int64_t>((int64_t)1 << BALL_BLOCK_LOG2, (int64_t)numparts - s0);
+    std::vector<double> q(3 * (size_t)m);
+    for (int64_t i = 0; i < m; ++i)
+        for (int d = 0; d < 3; ++d) q[(size_t)d * m + i] = bucket[s0 + i].GetPosition(d);
+    std::unique_ptr<BallBlock> B(new BallBlock());
+    B->off.assign(m + 1, 0);
+    check(nbx_ball_points_many(tree, q.data(), m, fdist2, B->off.data(), nullptr, nullptr, 0), "SearchBallPosTagged");
+    const int64_t total = B->off[m];
+    std::vector<int32_t> ids(total > 0 ? total : 1);
+    B->d2.assign(total > 0 ? total : 1, 0.0);
+    if (total > 0)
+        check(nbx_ball_points_many(tree, q.data(), m, fdist2, B->off.data(), ids.data(), B->d2.data(), total),
+              "SearchBallPosTagged");
+    B->idx.resize(total);
+    for (int64_t e = 0; e < total; ++e) B->idx[e] = slot_of_id[ids[e]];
+    B->used = ++ball_tick;
+    const BallBlock &ref = *B;
+    ball_cache[blk] = std::move(B);
+    return ref;
+}
+
 void KDTree::SearchBallPos(Double_t *x, Double_t fdist2, Int_t imark, Int_t *nn, Double_t *dist2) {
     std::lock_guard<std::mutex> lk(mu);
     check(nbx_ball_mark(tree, x, -1, fdist2, imark, nn, dist2), "SearchBallPos");
 }
 void KDTree::SearchBallPos(Int_t tt, Double_t fdist2, Int_t imark, Int_t *nn, Double_t *dist2) {
-    Double_t x[3] = {bucket[tt].GetPosition(0), bucket[tt].GetPosition(1), bucket[tt].GetPosition(2)};
+    if (treetype != TPHYS) die("SearchBallPos needs a TPHYS tree");
     std::lock_guard<std::mutex> lk(mu);
-    check(nbx_ball_mark(tree, x, bucket[tt].GetID(), fdist2, imark, nn, dist2), "SearchBallPos");
+    const BallBlock &B = ball_block(tt, fdist2);
+    const Int_t r = tt & ((1 << BALL_BLOCK_LOG2) - 1);
+    for (int64_t e = B.off[r]; e < B.off[r + 1]; ++e) {
+        const Int_t i = B.idx[e];
+        if (!periodic && i == tt) continue;
+        const Int_t id = bucket[i].GetID();
+        nn[id] = imark;
+        dist2[id] = B.d2[e];
+    }
 }
 std::vector<Int_t> KDTree::SearchBallPosTagged(Double_t *x, Double_t fdist2) {
     std::lock_guard<std::mutex> lk(mu);
-    int64_t cap = numparts, nout = 0;
+    int64_t cap = std::min<int64_t>(numparts, 1 << 16), nout = 0;
     std::vector<int32_t> ids;
     for (;;) {
         ids.resize(cap);
@@ -224,21 +294,37 @@ std::vector<Int_t> KDTree::SearchBallPosTagged(Double_t *x, Double_t fdist2) {
     for (int64_t i = 0; i < nout; ++i) out[i] = slot_of_id[ids[i]];
     return out;
 }
+std::vector<Int_t> KDTree::SearchBallPosTagged(Int_t tt, Double_t fdist2) {
+    if (treetype != TPHYS) die("SearchBallPosTagged needs a TPHYS tree");
+    std::lock_guard<std::mutex> lk(mu);
+    const BallBlock &B = ball_block(tt, fdist2);
+    const Int_t r = tt & ((1 << BALL_BLOCK_LOG2) - 1);
+    std::vector<Int_t> out;
+    out.reserve(B.off[r + 1] - B.off[r]);
+    for (int64_t e = B.off[r]; e < B.off[r + 1]; ++e)
+        if (periodic || B.idx[e] != tt) out.push_back(B.idx[e]);
+    return out;
+}
 Int_t KDTree::SearchBallPosTagged(Double_t *x, Double_t fdist2, Int_t *tagged) {
     std::vector<Int_t> v = SearchBallPosTagged(x, fdist2);
     std::copy(v.begin(), v.end(), tagged);
     return (Int_t)v.size();
 }
 Int_t KDTree::SearchBallPosTagged(Int_t tt, Double_t fdist2, Int_t *tagged) {
-    Double_t x[3] = {bucket[tt].GetPosition(0), bucket[tt].GetPosition(1), bucket[tt].GetPosition(2)};
-    return SearchBallPosTagged(x, fdist2, tagged);
+    std::vector<Int_t> v = SearchBallPosTagged(tt, fdist2);
+    std::copy(v.begin(), v.end(), tagged);
+    return (Int_t)v.size();
 }
 
 // ------------------------------------------------------------------ densities
+// caches that read Particle::rho (the caller holds the mutex)
+void KDTree::drop_density_caches() { pcache_k = pcache_s = pcache_set = -1; }
+
 void KDTree::density_to_bucket(Int_t Nsmooth) {
     std::vector<double> rho(numparts);
     check(nbx_density(tree, Nsmooth, NBX_DENS_SYMMETRIC, 0, rho.data(), nullptr), "CalcDensity failed");
     for (Int_t s = 0; s < numparts; ++s) bucket[s].SetDensity(rho[bucket[s].GetID()]);
+    drop_density_caches();
 }
 std::vector<double> KDTree::rho_by_id() {
     std::vector<double> rho(numparts);
@@ -285,10 +371,12 @@ void KDTree::CalcVelDensity(Int_t Nsmooth, Int_t Nsearch) {
     std::vector<double> rho(numparts);
     check(nbx_density(tree, Nsmooth, NBX_DENS_VEL, Nsearch, rho.data(), nullptr), "CalcVelDensity failed");
     for (Int_t s = 0; s < numparts; ++s) bucket[s].SetDensity(rho[bucket[s].GetID()]);
+    drop_density_caches();
 }
 Double_t KDTree::CalcDensityParticle(Int_t target, Int_t Nsmooth) {
     std::lock_guard<std::mutex> lk(mu);
     if (dcache_k != Nsmooth) {
+        dcache_k = -1;
         dcache.assign(numparts, 0.0);
         check(nbx_density(tree, Nsmooth, NBX_DENS_GATHER, 0, dcache.data(), nullptr), "CalcDensity failed");
         dcache_k = Nsmooth;
@@ -300,6 +388,7 @@ Double_t KDTree::CalcVelDensityParticle(Int_t target, Int_t Nsmooth, Int_t Nsear
     std::lock_guard<std::mutex> lk(mu);
     if (Nsmooth > Nsearch) Nsmooth = Nsearch;  // as the reference does (:855-858)
     if (vcache_k != Nsmooth || vcache_s != Nsearch) {
+        vcache_k = vcache_s = -1;
         vcache.assign(numparts, 0.0);
         check(nbx_density(tree, Nsmooth, NBX_DENS_VEL, Nsearch, vcache.data(), nullptr), "CalcDensity failed");
         vcache_k = Nsmooth;
@@ -308,25 +397,38 @@ Double_t KDTree::CalcVelDensityParticle(Int_t target, Int_t Nsmooth, Int_t Nsear
     return vcache[bucket[target].GetID()];
 }
 Double_t KDTree::CalcDensityPosition(Double_t *x, Int_t Nsmooth, Double_t *) {
+    if (treetype == TPHS) die("CalcDensityPosition here needs a TPHYS or TVEL tree");
     std::lock_guard<std::mutex> lk(mu);
     double den = 0.0;
     check(nbx_density_points(tree, x, 1, Nsmooth, &den, nullptr), "CalcDensity failed");
     return den;
 }
 Double_t *KDTree::CalcVelDensityWithPhysDensity(Int_t Nsmooth, Int_t Nsearch, int densityset) {
-    if (densityset != 1) CalcDensity(Nsmooth);
+    if (Nsmooth > Nsearch) Nsmooth = Nsearch;  // before the densities, as the reference (:405-409)
     std::lock_guard<std::mutex> lk(mu);
-    std::vector<double> rho(numparts);
-    for (Int_t s = 0; s < numparts; ++s) rho[bucket[s].GetID()] = bucket[s].GetDensity();
+    if (densityset != 1) density_to_bucket(Nsmooth);
+    const std::vector<double> rho = rho_by_id();
     Double_t *out = new Double_t[numparts];
-    check(nbx_veldensity_phys(tree, Nsmooth, Nsearch, rho.data(), out), "CalcVelDensitywithPhysDensity failed");
+    const int rc = nbx_veldensity_phys(tree, Nsmooth, Nsearch, rho.data(), out);
+    if (rc < 0) delete[] out;
+    check(rc, "CalcVelDensitywithPhysDensity failed");
     return out;
 }
 Double_t KDTree::CalcVelDensityWithPhysDensityParticle(Int_t target, Int_t Nsmooth, Int_t Nsearch, int densityset) {
-    Double_t *all = CalcVelDensityWithPhysDensity(Nsmooth, Nsearch, densityset);
-    Double_t v = all[bucket[target].GetID()];
-    delete[] all;
-    return v;
+    if (Nsmooth > Nsearch) Nsmooth = Nsearch;  // :933-936
+    const int set = densityset == 1 ? 1 : 0;
+    std::lock_guard<std::mutex> lk(mu);
+    if (pcache_k != Nsmooth || pcache_s != Nsearch || pcache_set != set) {
+        if (!set) density_to_bucket(Nsmooth);  // :937
+        pcache.assign(numparts, 0.0);
+        const std::vector<double> rho = rho_by_id();
+        check(nbx_veldensity_phys(tree, Nsmooth, Nsearch, rho.data(), pcache.data()),
+              "CalcVelDensitywithPhysDensity failed");
+        pcache_k = Nsmooth;
+        pcache_s = Nsearch;
+        pcache_set = set;
+    }
+    return pcache[bucket[target].GetID()];
 }
 Double_t KDTree::CalcVelDensityPosition(Double_t *x, Double_t *v, Int_t Nsmooth, Int_t Nsearch) {
     std::lock_guard<std::mutex> lk(mu);
@@ -433,48 +535,92 @@ Double_t KDTree::CalcSmoothLocalDispPosition(Double_t *x, Double_t *weight, Doub
 }
 
 // ------------------------------------------------------------------ FOF
-// pHead/pNext/pTail/pLen (KDFOF.cxx:29-31): filled when given, by tree index
-static void fof_lists(nbx_tree *tree, Int_t n, const Int_t *pGroup, int64_t ng, Int_tree_t *pHead, Int_tree_t *pNext,
-                      Int_tree_t *pTail, Int_tree_t *pLen) {
-    if (!pHead && !pNext && !pTail && !pLen) return;
-    std::vector<int32_t> order(n);
-    check(nbx_tree_order(tree, order.data()), "FOF");
-    check(nbx_fof_lists(pGroup, order.data(), n, ng, pHead, pNext, pTail, pLen), "FOF");
-}
-
-Int_t *KDTree::FOF(Double_t fdist, Int_t &numgroup, Int_t minnum, int, Int_tree_t *pHead, Int_tree_t *pNext,
-                   Int_tree_t *pTail, Int_tree_t *pLen) {
+// One FOF run (cmp 0: FOF with linking length fdist; else FOFCriterion with
+// that built-in comparator).  With ipcheckflag, the particles whose check is
+// nonzero take no part: the links are those of a tree over the others, built
+// with this tree's options and the same periodic rule (a reference tree of
+// N <= bucket_size particles searches without images, KDLeafNode.cxx:841-844).
+Int_t *KDTree::fof_run(int cmp, Double_t fdist, Double_t *params, Int_t &numgroup, Int_t minnum, Int_tree_t *pHead,
+                       Int_tree_t *pNext, Int_tree_t *pTail, Int_tree_t *pLen, int ipcheckflag, FOFcheckfunc chk,
+                       const char *what) {
     std::lock_guard<std::mutex> lk(mu);
-    Int_t *pGroup = new Int_t[numparts];
-    int64_t ng = 0;
-    check(nbx_fof(tree, fdist, minnum, pGroup, &ng), "FOF");
-    numgroup = (Int_t)ng;
-    fof_lists(tree, numparts, pGroup, ng, pHead, pNext, pTail, pLen);
-    return pGroup;
-}
-
-Int_t *KDTree::FOFCriterion(FOFcompfunc cmp, Double_t *params, Int_t &numgroups, Int_t minnum, int,
-                            int ipcheckflag, FOFcheckfunc chk, Int_tree_t *pHead, Int_tree_t *pNext, Int_tree_t *pTail,
-                            Int_tree_t *pLen) {
-    std::lock_guard<std::mutex> lk(mu);
-    const int32_t c = cmp == &FOF3d ? NBX_FOF_CMP_3D : cmp == &FOF6d ? NBX_FOF_CMP_6D : 0;
-    int rc = NBX_OK;
-    std::string why;
-    if (!c) why = "only the built-in comparators FOF3d and FOF6d are provided";
-    else if (ipcheckflag && chk != &Pnocheck) why = "check functions other than Pnocheck are not provided";
-    if (!why.empty()) {
-        if (g_throw) throw std::runtime_error("FOFCriterion: " + why);
-        std::fprintf(stderr, "FOFCriterion: %s\n", why.c_str());
-        std::exit(1);
+    std::vector<int> val;  // check values by ID (empty: every particle takes part)
+    if (ipcheckflag && chk != &Pnocheck) {
+        val.assign(numparts, 0);
+        bool any = false;
+        for (Int_t s = 0; s < numparts; ++s) {
+            const int c = chk(bucket[s], params);
+            val[bucket[s].GetID()] = c;
+            any = any || c != 0;
+            if (cmp && c > 0)
+                die(std::string(what) + ": check functions returning a positive value are not supported "
+                                        "(the reference lets such particles join groups but not start one)");
+        }
+        if (!any) val.clear();
     }
     Int_t *pGroup = new Int_t[numparts];
     int64_t ng = 0;
-    rc = nbx_fof_criterion(tree, c, params, minnum, pGroup, &ng);
+    int rc = NBX_OK;
+    if (val.empty()) {
+        rc = cmp ? nbx_fof_criterion(tree, cmp, params, minnum, pGroup, &ng) : nbx_fof(tree, fdist, minnum, pGroup, &ng);
+    } else {
+        std::vector<Int_t> act;  // IDs taking part, ascending (so canonical ties follow the IDs)
+        for (Int_t i = 0; i < numparts; ++i)
+            if (val[i] == 0) act.push_back(i);
+        for (Int_t i = 0; i < numparts; ++i) pGroup[i] = 0;
+        const int64_t m = (int64_t)act.size();
+        if (m > 0) {
+            std::vector<double> pos(3 * (size_t)m), vel(3 * (size_t)m), mass(m);
+            for (int64_t j = 0; j < m; ++j) {
+                const Particle &p = bucket[slot_of_id[act[j]]];
+                for (int d = 0; d < 3; ++d) {
+                    pos[d * (size_t)m + j] = p.GetPosition(d);
+                    vel[d * (size_t)m + j] = p.GetVelocity(d);
+                }
+                mass[j] = p.GetMass();
+            }
+            nbx_params sp = prm;
+            sp.form = NBX_FORM_PLAIN;
+            sp.periodic = periodic && numparts > b ? 1 : 0;
+            sp.bucket_size = 1;
+            nbx_tree *sub = nullptr;
+            rc = nbx_tree_create(&sp, pos.data(), vel.data(), mass.data(), m, &sub);
+            std::vector<int32_t> lab(m, 0);
+            if (rc == NBX_OK)
+                rc = cmp ? nbx_fof_criterion(sub, cmp, params, minnum, lab.data(), &ng)
+                         : nbx_fof(sub, fdist, minnum, lab.data(), &ng);
+            if (sub) nbx_tree_destroy(sub);
+            if (rc == NBX_OK)
+                for (int64_t j = 0; j < m; ++j) pGroup[act[j]] = lab[j];
+        }
+    }
     if (rc < 0) delete[] pGroup;
-    check(rc, "FOFCriterion");
-    numgroups = (Int_t)ng;
-    fof_lists(tree, numparts, pGroup, ng, pHead, pNext, pTail, pLen);
+    check(rc, what);
+    numgroup = (Int_t)ng;
+    // pHead/pNext/pTail/pLen (KDFOF.cxx:29-31), by tree index, from the groups
+    if (pHead || pNext || pTail || pLen) {
+        std::vector<int32_t> order(numparts);
+        check(nbx_tree_order(tree, order.data()), what);
+        check(nbx_fof_lists(pGroup, order.data(), numparts, ng, pHead, pNext, pTail, pLen), what);
+    }
+    // particles left out keep their check value, -1 becoming 0 (KDFOF.cxx:63-64, :107)
+    if (!val.empty())
+        for (Int_t i = 0; i < numparts; ++i)
+            if (val[i] != 0) pGroup[i] = val[i] == -1 ? 0 : val[i];
     return pGroup;
+}
+
+Int_t *KDTree::FOF(Double_t fdist, Int_t &numgroup, Int_t minnum, int, Int_tree_t *pHead, Int_tree_t *pNext,
+                   Int_tree_t *pTail, Int_tree_t *pLen, int ipcheckflag, FOFcheckfunc chk, Double_t *params) {
+    return fof_run(0, fdist, params, numgroup, minnum, pHead, pNext, pTail, pLen, ipcheckflag, chk, "FOF");
+}
+
+Int_t *KDTree::FOFCriterion(FOFcompfunc cmp, Double_t *params, Int_t &numgroups, Int_t minnum, int, int ipcheckflag,
+                            FOFcheckfunc chk, Int_tree_t *pHead, Int_tree_t *pNext, Int_tree_t *pTail,
+                            Int_tree_t *pLen) {
+    const int32_t c = cmp == &FOF3d ? NBX_FOF_CMP_3D : cmp == &FOF6d ? NBX_FOF_CMP_6D : 0;
+    if (!c) die("FOFCriterion: only the built-in comparators FOF3d and FOF6d are provided");
+    return fof_run(c, 0.0, params, numgroups, minnum, pHead, pNext, pTail, pLen, ipcheckflag, chk, "FOFCriterion");
 }
 
 }  // namespace NBody

@@ -36,6 +36,7 @@ static int hipfail(hipError_t e, const char *where) {
     } while (0)
 
 extern "C" const char *nbx_last_error(void) { return g_err.c_str(); }
+extern "C" int nbx_host_fail(int code, const char *msg) { return fail(code, msg); }
 
 // Temporary device buffers of one host call, freed when the call returns on
 // any path (the HIPCK early returns included).  Constructed with a stream, it
@@ -586,6 +587,17 @@ extern "C" int nbx_knn_points(nbx_tree *t, const double *q, int64_t m, int32_t k
     return points_impl(t, q, m, k, true, nn, d2, nullptr, nullptr, "nbx_knn_points");
 }
 
+// FindNearestPhase(x, v) (KDFindNearest.cxx:528-538) and, on a TPHS tree,
+// FindNearest(x) (:462-509): the 6-D search from phase points, no exclusion,
+// every periodic image taken (KDSplitNode.cxx:1153-1179)
+extern "C" int nbx_knn_phase_points(nbx_tree *t, const double *x, const double *v, int64_t m, int32_t k, int32_t *nn,
+                                    double *d2) {
+    if (!t || !x || !v || !nn) return fail(NBX_ERR_ARG, "NULL argument");
+    if (t->nd != 6) return fail(NBX_ERR_UNSUPPORTED, "phase-space point queries need a TPHS tree");
+    if (k < 1 || k > heap_cap(t)) return fail(NBX_ERR_UNSUPPORTED, "k out of range");
+    return points_impl(t, x, m, k, true, nn, d2, nullptr, nullptr, "nbx_knn_phase_points", v, 0);
+}
+
 extern "C" int nbx_density_points(nbx_tree *t, const double *q, int64_t m, int32_t k, double *rho, double *h) {
     if (!t || !q || !rho) return fail(NBX_ERR_ARG, "NULL argument");
     if (t->nd != 3) return fail(NBX_ERR_UNSUPPORTED, "CalcDensityPosition here needs a TPHYS tree");
@@ -986,41 +998,6 @@ extern "C" int nbx_fof(nbx_tree *t, double fdist, int32_t minnum, int32_t *label
         if (e != hipSuccess) rc = hipfail(e, "nbx_fof");
     }
     return rc;
-}
-
-// FOF's linked-list outputs from canonical labels (host only).  Lists run in
-// tree-index (slot) order; the reference's run in its search order.
-extern "C" int nbx_fof_lists(const int32_t *labels_by_id, const int32_t *slot_to_id, int64_t n, int64_t ngroups,
-                             int32_t *head, int32_t *next, int32_t *tail, int32_t *len) {
-    if (!labels_by_id || !slot_to_id || n < 0 || ngroups < 0) return fail(NBX_ERR_ARG, "bad argument");
-    std::vector<int32_t> first((size_t)ngroups + 1, -1), last((size_t)ngroups + 1, -1), cnt((size_t)ngroups + 1, 0);
-    for (int64_t s = 0; s < n; ++s) {
-        const int32_t id = slot_to_id[s];
-        if (id < 0 || id >= n) return fail(NBX_ERR_ARG, "slot_to_id out of range");
-        const int32_t g = labels_by_id[id];
-        if (g < 0 || g > ngroups) return fail(NBX_ERR_ARG, "label out of range");
-        if (next) next[s] = -1;
-        if (g == 0) {  // in no group: a list of its own
-            if (head) head[s] = (int32_t)s;
-            if (tail) tail[s] = (int32_t)s;
-            continue;
-        }
-        if (first[g] < 0) first[g] = (int32_t)s;
-        else if (next) next[last[g]] = (int32_t)s;
-        last[g] = (int32_t)s;
-        cnt[g]++;
-    }
-    for (int64_t s = 0; s < n; ++s) {
-        const int32_t g = labels_by_id[slot_to_id[s]];
-        if (g == 0) continue;
-        if (head) head[s] = first[g];
-        if (tail) tail[s] = s == first[g] ? last[g] : (int32_t)s;
-    }
-    if (len) {
-        for (int64_t i = 0; i < n; ++i) len[i] = 0;
-        for (int64_t g = 1; g <= ngroups && g < n; ++g) len[g] = cnt[g];
-    }
-    return NBX_OK;
 }
 
 // ------------------------------------------------------------------ ball search, many positions

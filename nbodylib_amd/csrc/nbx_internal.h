@@ -155,14 +155,18 @@ __device__ __forceinline__ double readfirst_d(double v) {
 }
 
 // Wsm, KDCalcSmoothQuantities.cxx:12-15: linear interpolation in the table,
-// indexed as (int)(r/h*0.5*(kernres-1)).
+// indexed as (int)(r/h*0.5*(kernres-1)).  Past the table's end (r/h >= 2,
+// which CalcVelDensityWithPhysDensity's h without a square root reaches) the
+// last entry is taken; that is decided on the double before any conversion,
+// so a huge r/h never meets an out-of-range float-to-int conversion.
 __device__ __forceinline__ double wsm_at(double rij, double hi, const double *kern, int kernres) {
-    double r = rij / hi;
-    int i = (int)(rij / hi * 0.5 * (kernres - 1));
+    const double r = rij / hi;
+    const double fi = rij / hi * 0.5 * (kernres - 1);
+    if (!(fi < (double)(kernres - 1))) return kern[kernres - 1];
+    int i = (int)fi;
     if (i < 0) i = 0;
-    double delta = 2.0 / (double)(kernres - 1);
-    if (i < kernres - 1) return kern[i] + (kern[i + 1] - kern[i]) * (r - delta * i) / delta;
-    return kern[kernres - 1];
+    const double delta = 2.0 / (double)(kernres - 1);
+    return kern[i] + (kern[i + 1] - kern[i]) * (r - delta * i) / delta;
 }
 
 // Lock-free union-find over slots (hook the larger root under the smaller).

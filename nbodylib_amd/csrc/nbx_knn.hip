@@ -321,8 +321,9 @@ __global__ __launch_bounds__(64) void k_knn(DevTree T, KnnArgs A) {
     const int qi = listed ? (valid ? A.qlist[gi] : 0) : gi;
     double q0[ND];
     if (QSRC == NBX_Q_POINTS) {
+        // phase points (ND = 6) bring their velocities in qv
 #pragma unroll
-        for (int d = 0; d < ND; ++d) q0[d] = valid ? A.q[d][qi] : 0.0;
+        for (int d = 0; d < ND; ++d) q0[d] = valid ? (d < 3 ? A.q[d][qi] : A.qv[d - 3][qi]) : 0.0;
     } else {
 #pragma unroll
         for (int d = 0; d < ND; ++d) q0[d] = valid ? T.x[d][qi] : 0.0;

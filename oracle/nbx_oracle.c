@@ -114,7 +114,10 @@ static inline double wsm(double r, int i, int size, double delta, const double *
     return x[size - 1];
 }
 static inline double wsm_at(double rij, double hi, int kernres, const double *kern) {
-    return wsm(rij / hi, (int)(rij / hi * 0.5 * (kernres - 1)), kernres, 2.0 / (double)(kernres - 1), kern);
+    const double fi = rij / hi * 0.5 * (kernres - 1);
+    /* past the end: decided before the conversion, which a huge r/h would overflow */
+    if (!(fi < (double)(kernres - 1))) return kern[kernres - 1];
+    return wsm(rij / hi, (int)fi, kernres, 2.0 / (double)(kernres - 1), kern);
 }
 
 /* ---------------------------------------------------------- priority queue */

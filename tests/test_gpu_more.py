@@ -107,8 +107,12 @@ def test_veldensity_phys_vs_oracle(gpu, form):
         ro = ot.calc_density(ns) if rho is None else rho
         want = ot.calc_veldensity_phys(ns, nS, ro)
         z = want == 0
+        assert (~z).any()
         assert np.array_equal(got == 0, z)
         assert np.max(np.abs(got[~z] / want[~z] - 1.0)) < 1e-12
+    # positions of order 1e-9: r/h far past the table (index beyond 2^31), every term W(2) = 0
+    tiny = nb.KDTree(pos * 1e-10, mass=mass, vel=vel, kerntype=nb.KSPH, form=f)
+    assert np.array_equal(tiny.CalcVelDensityWithPhysDensity(16, 32, np.ones(n)), np.zeros(n))
 
 
 @pytest.mark.parametrize("periodic", [False, True])

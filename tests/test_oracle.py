@@ -160,6 +160,39 @@ def test_phase_knn_vs_brute():
     assert np.array_equal(d2, bd)
 
 
+def _periodic_phase_set(n, seed):
+    rng = np.random.default_rng(seed)
+    return rng.random((n, 3)), rng.normal(0.0, 0.04, (n, 3))
+
+
+def test_phase_periodic_knn_vs_brute_and_scipy():
+    """Periodic FindNearestPhase(tt) (KDFindNearest.cxx:347-361 with
+    KDSplitNode.cxx:1153-1179): a k+1 search over the primary point and all 7
+    spatial images with nothing excluded, the smallest (the particle itself)
+    left in the heap.  Pinned against the brute-force union over images and,
+    independently of both, against scipy's periodic kd-tree (boxsize on the
+    three spatial axes only): with r_k far below half the box the union over
+    the 8 images is the minimum-image search."""
+    from scipy.spatial import cKDTree
+
+    n, k = 3000, 32
+    x, v = _periodic_phase_set(n, 21)
+    t = oracle.Tree(x, vel=v, b=16, treetype=oracle.TPHS, period=1.0)
+    ids, d2 = t.knn(k, 2)
+    bi, bd = oracle.knn_brute(x, k, mode=1, period=1.0, vel=v, nd=6)
+    assert np.array_equal(np.sort(ids, 1), np.sort(bi, 1))
+    assert np.array_equal(d2, bd)
+    ph = np.concatenate([x, v + 10.0], axis=1)  # velocities moved into [0, box) for scipy
+    sd, si = cKDTree(ph, boxsize=[1.0, 1.0, 1.0, 1e6, 1e6, 1e6]).query(ph, k=k + 1)
+    assert np.array_equal(si[:, 0], np.arange(n))
+    assert np.array_equal(np.sort(ids, 1), np.sort(si[:, 1:], 1))
+    assert np.allclose(d2, sd[:, 1:] ** 2, rtol=1e-12, atol=0)
+    # the union needs the images: particles near a face find neighbours across it
+    near = (np.minimum(x, 1 - x) < 0.02).any(1)
+    nid, _ = oracle.Tree(x, vel=v, b=16, treetype=oracle.TPHS).knn(k, 2)
+    assert not np.array_equal(np.sort(nid[near], 1), np.sort(ids[near], 1))
+
+
 # ---------------------------------------------------------------- density
 def test_calc_density_vs_lists():
     pos = datagen.uniform(4000, 1.0, 11)
@@ -328,6 +361,23 @@ def test_veldensity_phys_vs_brute():
     z = want == 0
     assert np.array_equal(got == 0, z) and 0 < z.sum() < n
     assert np.max(np.abs(got[~z] / want[~z] - 1)) < 1e-12
+
+
+def test_veldensity_phys_tiny_coordinates():
+    """With positions of order 1e-9 the reference's h = d2 / 2 (no square
+    root) is of order 1e-20, so r/h reaches ~1e11 and the table index
+    r/h * 0.5 * (kernres - 1) would overflow a 32-bit int.  Past the table's
+    end the last entry W(2) = 0 is taken, so every term is 0 and nothing is
+    read outside the table."""
+    n, ns, nS = 600, 8, 16
+    pos, vel, _ = datagen.clustered(n, 10.0, nhalo=3, seed=44, with_vel=True)
+    pos = pos * 1e-10
+    t = oracle.Tree(pos, vel=vel, b=16, kerntype=oracle.KSPH)
+    _, d2 = oracle.knn_brute(pos, nS, mode=0)
+    r = np.sqrt(d2[:, :ns])
+    assert (r / (0.5 * d2[:, -1:]) * 0.5 * 999 > 2**31).all()
+    got = t.calc_veldensity_phys(ns, nS, np.ones(n))
+    assert np.array_equal(got, np.zeros(n))
 
 
 def _wtab(q):
