@@ -202,6 +202,18 @@ int nbx_smooth_local_points(nbx_tree *t, const double *q, int64_t m, int32_t k, 
 int nbx_knn_density_dev(nbx_tree *t, int32_t k, int32_t *nn_by_slot_dev, double *rho_by_id_dev,
                         double *h_by_id_dev, void *stream);
 
+/* nbx_knn_density_dev for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid allocation).  No
+ * counterpart in the reference.  Same arguments and results; the call makes
+ * no hipMalloc / hipFree or other allocation: its temporaries come from a
+ * 256 MiB device array that each GPU the library runs on holds until the
+ * process ends.  n particles need 8n bytes (16n with h) plus, for the
+ * precision form, 4n bytes and the search scratch (about 50 MB); larger
+ * trees return NBX_ERR_UNSUPPORTED before anything runs.  Calls that overlap
+ * on one GPU share that array and race. */
+int nbx_knn_density_noalloc(nbx_tree *t, int32_t k, int32_t *nn_by_slot_dev, double *rho_by_id_dev,
+                            double *h_by_id_dev, void *stream);
+
 /* Ball search around a position: IDs of all particles with d2 < r2 (strict).
  * Replaces SearchBallPosTagged(Double_t*x, fdist2, Int_t*tagged)
  * KDFindNearest.cxx:618-688 (period-aware).  out: capacity cap; *nout = total found. */

@@ -31,7 +31,7 @@ NBX_ERR_ARG, NBX_ERR_HIP, NBX_ERR_UNSUPPORTED, NBX_ERR_NOMEM, NBX_ERR_SEARCH = -
 EXPORTS = [
     "nbx_params_default", "nbx_tree_create", "nbx_tree_create_dev", "nbx_tree_destroy", "nbx_tree_order",
     "nbx_tree_size", "nbx_tree_kernel", "nbx_knn", "nbx_knn_dev", "nbx_knn_points", "nbx_knn_phase_points", "nbx_density_points", "nbx_veldensity_points", "nbx_veldensity_phys", "nbx_smooth_vel", "nbx_smooth_vel_moments", "nbx_smooth_local", "nbx_smooth_local_points", "nbx_density",
-    "nbx_density_dev", "nbx_knn_density_dev", "nbx_ball_points", "nbx_ball_points_many", "nbx_ball_mark", "nbx_fof", "nbx_fof_dev", "nbx_fof_criterion", "nbx_fof_criterion_dev", "nbx_fof_lists",
+    "nbx_density_dev", "nbx_knn_density_dev", "nbx_knn_density_noalloc", "nbx_ball_points", "nbx_ball_points_many", "nbx_ball_mark", "nbx_fof", "nbx_fof_dev", "nbx_fof_criterion", "nbx_fof_criterion_dev", "nbx_fof_lists",
     "nbx_sph_density_fwd_dev", "nbx_sph_density_adjoint_dev", "nbx_prof_enable", "nbx_prof_reset", "nbx_prof_get",
     "nbx_tree_last_fallbacks", "nbx_last_error",
 ]
@@ -89,6 +89,7 @@ def lib():
     L.nbx_density.argtypes = [vp, i32, i32, i32, vp, vp]
     L.nbx_density_dev.argtypes = [vp, i32, i32, i32, vp, vp, vp]
     L.nbx_knn_density_dev.argtypes = [vp, i32, vp, vp, vp, vp]
+    L.nbx_knn_density_noalloc.argtypes = [vp, i32, vp, vp, vp, vp]
     L.nbx_ball_points.argtypes = [vp, vp, dbl, vp, i64, P(i64)]
     L.nbx_ball_mark.argtypes = [vp, vp, i64, dbl, i32, vp, vp]
     L.nbx_ball_points_many.argtypes = [vp, vp, i64, dbl, vp, vp, vp, i64]

@@ -66,6 +66,21 @@ struct DevBufs {
     }
 };
 
+// Device memory handed out from one caller-owned block (the no-alloc entry
+// point's scratch): every carve is 256-byte aligned; carve() returns null
+// once the block is exhausted.
+struct Workspace {
+    char *base;
+    size_t cap, used;
+    template <class T>
+    T *carve(size_t bytes) {
+        const size_t a = (used + 255) & ~(size_t)255;
+        if (a + bytes > cap) return nullptr;
+        used = a + bytes;
+        return (T *)(base + a);
+    }
+};
+
 // ------------------------------------------------------------------ profiling
 struct ProfRec {
     std::string name;
@@ -412,18 +427,29 @@ extern "C" int nbx_tree_kernel(const nbx_tree *t, double *kernnorm, double *tabl
 // FP32 search with the FP64 certificate; queries whose certificate fails are
 // re-run by the FP64 kernel from a list, so results equal the plain form's
 // neighbour sets.  A.dens / A.rho / A.nn / A.layout as for nbx_launch_knn.
-static hipError_t run_mixed(nbx_tree *t, KnnArgs A, int k, hipStream_t st) {
+static hipError_t run_mixed(nbx_tree *t, KnnArgs A, int k, hipStream_t st, Workspace *ws = nullptr) {
     const int64_t n = t->n;
     int32_t *fb = nullptr;
-    hipError_t e = hipMallocAsync((void **)&fb, sizeof(int32_t) * (n + 1), st);
-    if (e != hipSuccess) return e;
-    int32_t *cnt = fb + n;
+    void *mwork = nullptr;
+    size_t mbytes = 0;
+    hipError_t e = hipSuccess;
+    const int64_t nfb = n;
+    if (ws) {
+        mbytes = nbx_mixed_scratch_bytes(t->nd, k + 1);
+        fb = ws->carve<int32_t>(sizeof(int32_t) * (nfb + 1));
+        mwork = ws->carve<void>(mbytes);
+        if (!fb || !mwork) return hipErrorInvalidValue;
+    } else {
+        e = hipMallocAsync((void **)&fb, sizeof(int32_t) * (nfb + 1), st);
+        if (e != hipSuccess) return e;
+    }
+    int32_t *cnt = fb + nfb;
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, st, cnt, 0, (int64_t)1);
     KnnArgs M = A;
     M.kh = k + 1;
     M.kout = k;
     DevTree T = t->view();
-    e = nbx_launch_knn_mixed(T, M, t->ext, fb, cnt, st);
+    e = nbx_launch_knn_mixed(T, M, t->ext, fb, cnt, st, mwork, mbytes);
     int32_t hc = 0;
     if (e == hipSuccess) e = hipMemcpyAsync(&hc, cnt, sizeof(int32_t), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
@@ -438,7 +464,7 @@ static hipError_t run_mixed(nbx_tree *t, KnnArgs A, int k, hipStream_t st) {
         e = nbx_launch_knn(T, F, hc, st);
     }
     t->last_fallbacks = hc;
-    hipError_t e2 = hipFreeAsync(fb, st);
+    hipError_t e2 = ws ? hipSuccess : hipFreeAsync(fb, st);
     return e != hipSuccess ? e : e2;
 }
 
@@ -624,7 +650,7 @@ extern "C" int nbx_veldensity_points(nbx_tree *t, const double *x, const double 
 // densities rho_in_id (device, by ID)
 static const int32_t DENS_VELPHYS = 100;
 static int density_impl(nbx_tree *t, int32_t k, int32_t variant, int32_t nsearch, double *rho_id, double *h_id,
-                        int32_t *nn_slot, hipStream_t st, const double *rho_in_id = nullptr) {
+                        int32_t *nn_slot, hipStream_t st, const double *rho_in_id = nullptr, Workspace *ws = nullptr) {
     if (!t) return fail(NBX_ERR_ARG, "NULL tree");
     if (k < 1) return fail(NBX_ERR_ARG, "k must be >= 1");
     KnnArgs A;
@@ -651,9 +677,17 @@ static int density_impl(nbx_tree *t, int32_t k, int32_t variant, int32_t nsearch
     double *rho = nullptr, *h = nullptr;
     int32_t *err = nullptr;
     DevBufs bufs(st);
-    HIPCK(bufs.alloc(&rho, sizeof(double) * n));
-    if (h_id) HIPCK(bufs.alloc(&h, sizeof(double) * n));
-    HIPCK(bufs.alloc(&err, sizeof(int32_t)));
+    if (ws) {
+        if (A.dens == NBX_DO_VELPHYS) return fail(NBX_ERR_UNSUPPORTED, "no workspace form of this density");
+        rho = ws->carve<double>(sizeof(double) * n);
+        if (h_id) h = ws->carve<double>(sizeof(double) * n);
+        err = ws->carve<int32_t>(sizeof(int32_t));
+        if (!rho || (h_id && !h) || !err) return fail(NBX_ERR_ARG, "scratch too small");
+    } else {
+        HIPCK(bufs.alloc(&rho, sizeof(double) * n));
+        if (h_id) HIPCK(bufs.alloc(&h, sizeof(double) * n));
+        HIPCK(bufs.alloc(&err, sizeof(int32_t)));
+    }
     hipLaunchKernelGGL(k_fill_f64, dim3(nblk(n)), dim3(256), 0, st, rho, 0.0, n);
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, st, err, 0, (int64_t)1);
     if (A.dens == NBX_DO_VELPHYS) {
@@ -666,7 +700,7 @@ static int density_impl(nbx_tree *t, int32_t k, int32_t variant, int32_t nsearch
     A.hout = h;
     A.err = err;
     A.nn = nn_slot;
-    hipError_t e = mixed ? run_mixed(t, A, k, st) : nbx_launch_knn(t->view(), A, (int)n, st);
+    hipError_t e = mixed ? run_mixed(t, A, k, st, ws) : nbx_launch_knn(t->view(), A, (int)n, st);
     if (e == hipSuccess) {
         hipLaunchKernelGGL(k_scatter_by_id, dim3(nblk(n)), dim3(256), 0, st, rho, t->perm, rho_id, n);
         if (h_id) hipLaunchKernelGGL(k_scatter_by_id, dim3(nblk(n)), dim3(256), 0, st, h, t->perm, h_id, n);
@@ -690,6 +724,62 @@ extern "C" int nbx_knn_density_dev(nbx_tree *t, int32_t k, int32_t *nn_by_slot_d
                                    double *h_by_id_dev, void *stream) {
     if (!rho_by_id_dev) return fail(NBX_ERR_ARG, "NULL output");
     return density_impl(t, k, NBX_DENS_SYMMETRIC, 0, rho_by_id_dev, h_by_id_dev, nn_by_slot_dev, (hipStream_t)stream);
+}
+
+// ------------------------------------------------------------------ entry point that allocates nothing
+// nbx_knn_density_noalloc: nbx_knn_density_dev with every temporary (rho and
+// h by slot, the error flag, and for the precision form the fallback list,
+// the per-workgroup search scratch and the queue counters) carved from one
+// file-scope device array instead of allocated.  Each GPU holds its own copy
+// of the array from the library's load until the process ends; host code
+// reaches it only through the address hipGetSymbolAddress gives on that
+// device.  Every call on a GPU shares that GPU's copy: it clears what must
+// start at zero itself (fill kernels), and calls that overlap on one GPU race.
+#define NBX_NOALLOC_BYTES ((size_t)256 << 20)
+static __device__ __align__(256) unsigned char scratch[NBX_NOALLOC_BYTES];
+#define NBX_MAX_DEVICES 64
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[NBX_MAX_DEVICES];  // per device: the address of `scratch` there
+
+static int noalloc_scratch(void **ptr) {
+    int dev = 0;
+    HIPCK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NBX_MAX_DEVICES) return fail(NBX_ERR_ARG, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        HIPCK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = p;
+    }
+    *ptr = g_scratch_ptr[dev];
+    return NBX_OK;
+}
+
+extern "C" int nbx_knn_density_noalloc(nbx_tree *t, int32_t k, int32_t *nn_by_slot_dev, double *rho_by_id_dev,
+                                       double *h_by_id_dev, void *stream) {
+    if (!t || !rho_by_id_dev) return fail(NBX_ERR_ARG, "NULL argument");
+    if (k < 1 || k > heap_cap(t)) return fail(NBX_ERR_UNSUPPORTED, "k out of range");
+    // the carves density_impl and run_mixed will make, in their order
+    const int64_t n = t->n;
+    const bool mixed = t->p.form == NBX_FORM_MIXED;
+    Workspace dry = {nullptr, ~(size_t)0, 0};
+    dry.carve<char>(sizeof(double) * n);
+    if (h_by_id_dev) dry.carve<char>(sizeof(double) * n);
+    dry.carve<char>(sizeof(int32_t));
+    if (mixed) {
+        dry.carve<char>(sizeof(int32_t) * (n + 1));
+        dry.carve<char>(nbx_mixed_scratch_bytes(t->nd, k + 1));
+    }
+    if (dry.used > NBX_NOALLOC_BYTES)
+        return fail(NBX_ERR_UNSUPPORTED, "nbx_knn_density_noalloc: " + std::to_string(n) + " particles need " +
+                                             std::to_string(dry.used >> 20) + " MiB of scratch; the maximum is " +
+                                             std::to_string(NBX_NOALLOC_BYTES >> 20) + " MiB");
+    void *base = nullptr;
+    int rc = noalloc_scratch(&base);
+    if (rc) return rc;
+    Workspace ws = {(char *)base, NBX_NOALLOC_BYTES, 0};
+    return density_impl(t, k, NBX_DENS_SYMMETRIC, 0, rho_by_id_dev, h_by_id_dev, nn_by_slot_dev, (hipStream_t)stream,
+                        nullptr, &ws);
 }
 
 // Smoothed velocity mean, dispersion and moments: the fused kNN + CalcDensity

@@ -77,8 +77,12 @@ struct KnnArgs {
 };
 
 hipError_t nbx_launch_knn(const DevTree &T, const KnnArgs &A, int nqueries, hipStream_t st);
+// work (nullable): device memory of at least nbx_mixed_scratch_bytes(nd, kh)
+// bytes for the per-workgroup scratch and the queue counters; without it the
+// launcher allocates them on the stream.
 hipError_t nbx_launch_knn_mixed(const DevTree &T, const KnnArgs &A, const double *ext, int32_t *fallback_list,
-                                int32_t *fallback_count, hipStream_t st);
+                                int32_t *fallback_count, hipStream_t st, void *work = nullptr, size_t work_bytes = 0);
+size_t nbx_mixed_scratch_bytes(int nd, int kh);
 
 // ------------------------------------------------------------------ FOF
 // Link rule of a FOF run: pairs link when their value is below lim.  Plain

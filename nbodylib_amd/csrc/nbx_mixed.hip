@@ -783,18 +783,33 @@ static int mx_grid() {
     return grid;
 }
 
+static int mx_launch_grid(int nd, int km) {
+    return nd == 3 ? (km == 33 ? mx_grid<3, 33>() : mx_grid<3, 65>())
+                   : (km == 33 ? mx_grid<6, 33>() : mx_grid<6, 65>());
+}
+
+size_t nbx_mixed_scratch_bytes(int nd, int kh) {
+    const int km = kh <= 33 ? 33 : 65;
+    return (size_t)mx_launch_grid(nd, km) * MX_BLKW(km) * sizeof(uint32_t) + 8 * MX_CTR_STRIDE * sizeof(int32_t);
+}
+
 hipError_t nbx_launch_knn_mixed(const DevTree &T, const KnnArgs &A, const double *ext, int32_t *fallback_list,
-                                int32_t *fallback_count, hipStream_t st) {
+                                int32_t *fallback_count, hipStream_t st, void *work, size_t work_bytes) {
     const int nw = (T.n + NBX_WAVE - 1) / NBX_WAVE;
     if (nw == 0) return hipSuccess;
     if ((T.nd != 3 && T.nd != 6) || A.kh > 65 || A.kh < 2 || !T.kernf || !T.org || !ext) return hipErrorInvalidValue;
     const int km = A.kh <= 33 ? 33 : 65;
-    const int grid = T.nd == 3 ? (km == 33 ? mx_grid<3, 33>() : mx_grid<3, 65>())
-                               : (km == 33 ? mx_grid<6, 33>() : mx_grid<6, 65>());
+    const int grid = mx_launch_grid(T.nd, km);
     const size_t nscr = (size_t)grid * MX_BLKW(km);
-    void *mem = nullptr;
-    hipError_t e = hipMallocAsync(&mem, nscr * sizeof(uint32_t) + 8 * MX_CTR_STRIDE * sizeof(int32_t), st);
-    if (e != hipSuccess) return e;
+    const size_t need = nscr * sizeof(uint32_t) + 8 * MX_CTR_STRIDE * sizeof(int32_t);
+    void *mem = work;
+    hipError_t e = hipSuccess;
+    if (mem) {
+        if (work_bytes < need) return hipErrorInvalidValue;
+    } else {
+        e = hipMallocAsync(&mem, need, st);
+        if (e != hipSuccess) return e;
+    }
     uint32_t *scr = (uint32_t *)mem;
     int32_t *ctr = (int32_t *)(scr + nscr);
     hipLaunchKernelGGL(k_mx_zero, dim3(1), dim3(8 * MX_CTR_STRIDE), 0, st, ctr, 8 * MX_CTR_STRIDE);
@@ -813,6 +828,6 @@ hipError_t nbx_launch_knn_mixed(const DevTree &T, const KnnArgs &A, const double
 #undef MIXL
     }
     e = hipGetLastError();
-    hipFreeAsync(mem, st);
+    if (!work) hipFreeAsync(mem, st);
     return e;
 }
