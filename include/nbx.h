@@ -282,6 +282,21 @@ int nbx_sph_density_fwd_dev(const double *pos_dev, const double *mass_dev, const
 int nbx_sph_density_adjoint_dev(const double *pos_dev, const double *mass_dev, const int32_t *nbr_dev, int64_t n,
                                 int32_t k, const double *g_dev, double *dpos_dev, double *dmass_dev, void *stream);
 
+/* The same forward and adjoint over the tree's own neighbour lists, with no
+ * list of all n kept: each particle's k nearest under CalcDensity's search
+ * (KDCalcSmoothQuantities.cxx:227: self and d2 == 0 excluded, no period) are
+ * recomputed for one block of block_rows tree slots at a time (rounded up
+ * to a multiple of 1024; <= 0 takes every slot at once) into a buffer of
+ * block_rows * k * 4 bytes and consumed there.  The list memory is that
+ * buffer instead of n * k * 4 bytes.  TPHYS trees, either form (the
+ * precision form's neighbour sets are the plain form's).  All by ID, device
+ * pointers: rho_by_id, h_by_id (nullable), g_by_id = dL/drho, dpos_by_id
+ * (3*n SoA) and dmass_by_id, the last two written (not accumulated). */
+int nbx_sph_density_fwd_tree_dev(nbx_tree *t, int32_t k, int64_t block_rows, double *rho_by_id_dev,
+                                 double *h_by_id_dev, void *stream);
+int nbx_sph_density_adjoint_tree_dev(nbx_tree *t, int32_t k, int64_t block_rows, const double *g_by_id_dev,
+                                     double *dpos_by_id_dev, double *dmass_by_id_dev, void *stream);
+
 /* Precision-form statistics of the last kNN/density call on this tree:
  * queries whose FP32 certificate failed and were recomputed in FP64. */
 int nbx_tree_last_fallbacks(const nbx_tree *t, int64_t *fallbacks);

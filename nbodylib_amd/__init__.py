@@ -32,7 +32,8 @@ EXPORTS = [
     "nbx_params_default", "nbx_tree_create", "nbx_tree_create_dev", "nbx_tree_destroy", "nbx_tree_order",
     "nbx_tree_size", "nbx_tree_kernel", "nbx_knn", "nbx_knn_dev", "nbx_knn_points", "nbx_knn_phase_points", "nbx_density_points", "nbx_veldensity_points", "nbx_veldensity_phys", "nbx_smooth_vel", "nbx_smooth_vel_moments", "nbx_smooth_local", "nbx_smooth_local_points", "nbx_density",
     "nbx_density_dev", "nbx_knn_density_dev", "nbx_knn_density_noalloc", "nbx_ball_points", "nbx_ball_points_many", "nbx_ball_mark", "nbx_fof", "nbx_fof_dev", "nbx_fof_criterion", "nbx_fof_criterion_dev", "nbx_fof_lists",
-    "nbx_sph_density_fwd_dev", "nbx_sph_density_adjoint_dev", "nbx_prof_enable", "nbx_prof_reset", "nbx_prof_get",
+    "nbx_sph_density_fwd_dev", "nbx_sph_density_adjoint_dev", "nbx_sph_density_fwd_tree_dev",
+    "nbx_sph_density_adjoint_tree_dev", "nbx_prof_enable", "nbx_prof_reset", "nbx_prof_get",
     "nbx_tree_last_fallbacks", "nbx_last_error",
 ]
 
@@ -100,6 +101,8 @@ def lib():
     L.nbx_fof_criterion_dev.argtypes = [vp, i32, vp, i32, vp, P(i64), vp]
     L.nbx_sph_density_fwd_dev.argtypes = [vp, vp, vp, i64, i32, vp, vp, vp]
     L.nbx_sph_density_adjoint_dev.argtypes = [vp, vp, vp, i64, i32, vp, vp, vp, vp]
+    L.nbx_sph_density_fwd_tree_dev.argtypes = [vp, i32, i64, vp, vp, vp]
+    L.nbx_sph_density_adjoint_tree_dev.argtypes = [vp, i32, i64, vp, vp, vp, vp]
     L.nbx_prof_enable.argtypes = [ctypes.c_int]
     L.nbx_prof_enable.restype = None
     L.nbx_prof_reset.argtypes = []

@@ -38,20 +38,22 @@ __device__ __forceinline__ double dw_cubic(double q) {
 #define ADJ_G 8   // neighbours whose gathers are issued together (forward)
 #define ADJ_GB 4  // (adjoint: more live values per neighbour)
 
-__global__ __launch_bounds__(ADJ_B) void k_sph_fwd(const double *__restrict__ pos, const double *__restrict__ mass,
-                                                   const int32_t *__restrict__ nbr, int64_t n, int k, double *rho,
-                                                   double *hout) {
+// Rows i in [r0, r0 + nr) of the n particles: row i of the list is
+// nbr[(i - r0) k ...]; positions X, Y, Z and the other arrays have all n.
+__global__ __launch_bounds__(ADJ_B) void k_sph_fwd(const double *__restrict__ X, const double *__restrict__ Y,
+                                                   const double *__restrict__ Z, const double *__restrict__ mass,
+                                                   const int32_t *__restrict__ nbr, int64_t n, int64_t r0, int64_t nr,
+                                                   int k, double *rho, double *hout) {
     __shared__ double lrho[ADJ_B];
     const int t = threadIdx.x;
-    const int64_t i0 = (int64_t)blockIdx.x * ADJ_B;
+    const int64_t i0 = r0 + (int64_t)blockIdx.x * ADJ_B;
     const int64_t i = i0 + t;
     lrho[t] = 0.0;
     __syncthreads();
     double acc = 0.0;
-    if (i < n) {
-        const double *X = pos, *Y = pos + n, *Z = pos + 2 * n;
+    if (i < r0 + nr) {
         const double xi = X[i], yi = Y[i], zi = Z[i], mi = mass[i];
-        const int32_t *row = nbr + i * k;
+        const int32_t *row = nbr + (i - r0) * k;
         const int64_t K = row[k - 1];
         const double dx = xi - X[K], dy = yi - Y[K], dz = zi - Z[K];
         const double h = 0.5 * sqrt(dx * dx + dy * dy + dz * dz);
@@ -83,24 +85,24 @@ __global__ __launch_bounds__(ADJ_B) void k_sph_fwd(const double *__restrict__ po
     if (i < n) unsafeAtomicAdd(rho + i, acc + lrho[t]);
 }
 
-__global__ __launch_bounds__(ADJ_B) void k_sph_adj(const double *__restrict__ pos, const double *__restrict__ mass,
-                                                   const int32_t *__restrict__ nbr, int64_t n, int k,
-                                                   const double *__restrict__ g, double *dpos, double *dmass) {
+__global__ __launch_bounds__(ADJ_B) void k_sph_adj(const double *__restrict__ X, const double *__restrict__ Y,
+                                                   const double *__restrict__ Z, const double *__restrict__ mass,
+                                                   const int32_t *__restrict__ nbr, int64_t n, int64_t r0, int64_t nr,
+                                                   int k, const double *__restrict__ g, double *DX, double *DY,
+                                                   double *DZ, double *dmass) {
     __shared__ double lx[ADJ_B], ly[ADJ_B], lz[ADJ_B], lm[ADJ_B];
     const int t = threadIdx.x;
-    const int64_t i0 = (int64_t)blockIdx.x * ADJ_B;
+    const int64_t i0 = r0 + (int64_t)blockIdx.x * ADJ_B;
     const int64_t i = i0 + t;
     lx[t] = 0.0;
     ly[t] = 0.0;
     lz[t] = 0.0;
     lm[t] = 0.0;
     __syncthreads();
-    double *DX = dpos, *DY = dpos + n, *DZ = dpos + 2 * n;
     double gx = 0.0, gy = 0.0, gz = 0.0, gm = 0.0;
-    if (i < n) {
-        const double *X = pos, *Y = pos + n, *Z = pos + 2 * n;
+    if (i < r0 + nr) {
         const double xi = X[i], yi = Y[i], zi = Z[i], mi = mass[i], gi = g[i];
-        const int32_t *row = nbr + i * k;
+        const int32_t *row = nbr + (i - r0) * k;
         const int64_t K = row[k - 1];
         const double kx = xi - X[K], ky = yi - Y[K], kz = zi - Z[K];
         const double rk = sqrt(kx * kx + ky * ky + kz * kz);
@@ -179,20 +181,76 @@ __global__ __launch_bounds__(ADJ_B) void k_sph_adj(const double *__restrict__ po
     }
 }
 
-hipError_t nbx_launch_sph_fwd(const double *pos, const double *mass, const int32_t *nbr, int64_t n, int k, double *rho,
-                              double *h, hipStream_t st) {
-    int blocks = (int)((n + ADJ_B - 1) / ADJ_B);
+hipError_t nbx_launch_sph_fwd_rows(const double *const *X, const double *mass, const int32_t *nbr, int64_t n,
+                                   int64_t r0, int64_t nr, int k, double *rho, double *h, hipStream_t st) {
+    int blocks = (int)((nr + ADJ_B - 1) / ADJ_B);
     if (!blocks) return hipSuccess;
     ProfScope ps("k_sph_fwd", st);
-    hipLaunchKernelGGL(k_sph_fwd, dim3(blocks), dim3(ADJ_B), 0, st, pos, mass, nbr, n, k, rho, h);
+    hipLaunchKernelGGL(k_sph_fwd, dim3(blocks), dim3(ADJ_B), 0, st, X[0], X[1], X[2], mass, nbr, n, r0, nr, k, rho, h);
     return hipGetLastError();
+}
+
+hipError_t nbx_launch_sph_adj_rows(const double *const *X, const double *mass, const int32_t *nbr, int64_t n,
+                                   int64_t r0, int64_t nr, int k, const double *g, double *const *D, double *dmass,
+                                   hipStream_t st) {
+    int blocks = (int)((nr + ADJ_B - 1) / ADJ_B);
+    if (!blocks) return hipSuccess;
+    ProfScope ps("k_sph_adj", st);
+    hipLaunchKernelGGL(k_sph_adj, dim3(blocks), dim3(ADJ_B), 0, st, X[0], X[1], X[2], mass, nbr, n, r0, nr, k, g, D[0],
+                       D[1], D[2], dmass);
+    return hipGetLastError();
+}
+
+hipError_t nbx_launch_sph_fwd(const double *pos, const double *mass, const int32_t *nbr, int64_t n, int k, double *rho,
+                              double *h, hipStream_t st) {
+    const double *X[3] = {pos, pos + n, pos + 2 * n};
+    return nbx_launch_sph_fwd_rows(X, mass, nbr, n, 0, n, k, rho, h, st);
 }
 
 hipError_t nbx_launch_sph_adj(const double *pos, const double *mass, const int32_t *nbr, int64_t n, int k,
                               const double *g, double *dpos, double *dmass, hipStream_t st) {
-    int blocks = (int)((n + ADJ_B - 1) / ADJ_B);
+    const double *X[3] = {pos, pos + n, pos + 2 * n};
+    double *D[3] = {dpos, dpos + n, dpos + 2 * n};
+    return nbx_launch_sph_adj_rows(X, mass, nbr, n, 0, n, k, g, D, dmass, st);
+}
+
+// Move each row's k-th neighbour -- its largest (d2, ID), the one that sets
+// h -- into the last column (rows of slots from a search whose rows are not
+// in exact FP64 order).  d2 summed x -> z without FMA, as the search's.
+// A row with fewer than k neighbours (entries -1, possible when exact
+// duplicates are excluded) raises *err and its missing entries become the
+// row's own slot, so no later pass reads outside the arrays.
+__global__ void k_kth_last(const double *__restrict__ X, const double *__restrict__ Y, const double *__restrict__ Z,
+                           const int32_t *__restrict__ perm, int32_t *nbr, int64_t r0, int64_t nr, int k,
+                           int32_t *err) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nr) return;
+    const int64_t i = r0 + r;
+    int32_t *row = nbr + r * k;
+    const double xi = X[i], yi = Y[i], zi = Z[i];
+    int best = 0;
+    double bd = -1.0;
+    int32_t bid = -1;
+    for (int j = 0; j < k; ++j) {
+        if (row[j] < 0) {
+            row[j] = (int32_t)i;
+            atomicOr(err, 1);
+        }
+        const int32_t s = row[j];
+        const double ex = xi - X[s], ey = yi - Y[s], ez = zi - Z[s];
+        const double d = ex * ex + ey * ey + ez * ez;
+        const int32_t id = perm[s];
+        if (d > bd || (d == bd && id > bid)) { best = j; bd = d; bid = id; }
+    }
+    const int32_t tmp = row[k - 1];
+    row[k - 1] = row[best];
+    row[best] = tmp;
+}
+
+hipError_t nbx_launch_kth_last(const double *const *X, const int32_t *perm, int32_t *nbr, int64_t r0, int64_t nr, int k,
+                               int32_t *err, hipStream_t st) {
+    const int blocks = (int)((nr + 255) / 256);
     if (!blocks) return hipSuccess;
-    ProfScope ps("k_sph_adj", st);
-    hipLaunchKernelGGL(k_sph_adj, dim3(blocks), dim3(ADJ_B), 0, st, pos, mass, nbr, n, k, g, dpos, dmass);
+    hipLaunchKernelGGL(k_kth_last, dim3(blocks), dim3(256), 0, st, X[0], X[1], X[2], perm, nbr, r0, nr, k, err);
     return hipGetLastError();
 }

@@ -433,7 +433,8 @@ static hipError_t run_mixed(nbx_tree *t, KnnArgs A, int k, hipStream_t st, Works
     void *mwork = nullptr;
     size_t mbytes = 0;
     hipError_t e = hipSuccess;
-    const int64_t nfb = n;
+    // the fallback list holds at most the queries of the launch
+    const int64_t nfb = A.wcnt > 0 ? std::min<int64_t>(n, (int64_t)A.wcnt * NBX_WAVE) : n;
     if (ws) {
         mbytes = nbx_mixed_scratch_bytes(t->nd, k + 1);
         fb = ws->carve<int32_t>(sizeof(int32_t) * (nfb + 1));
@@ -461,6 +462,7 @@ static hipError_t run_mixed(nbx_tree *t, KnnArgs A, int k, hipStream_t st, Works
         F.excl_zero = 1;
         F.qlist = fb;
         F.m = hc;
+        F.wbeg = F.wcnt = 0;
         e = nbx_launch_knn(T, F, hc, st);
     }
     t->last_fallbacks = hc;
@@ -1248,6 +1250,102 @@ extern "C" int nbx_ball_mark(nbx_tree *t, const double *x, int64_t exclude_id, d
         d2_by_id[ids[i]] = d2[i];
     }
     return NBX_OK;
+}
+
+// ------------------------------------------------------------------ differentiation form over the tree
+// The frozen list is the tree's own: each particle's k nearest under
+// CalcDensity's search (KDCalcSmoothQuantities.cxx:227; self and d2 = 0
+// excluded, no period), recomputed for one block of tree slots at a time
+// into a buffer of that block and consumed there, so no list of all n is
+// kept.  Forward and adjoint are row-separable (every row adds its own
+// terms), so the blocks sum to the whole.  Work runs in slot order on the
+// tree's sorted coordinates; results are scattered back by ID.
+static int sph_tree_impl(nbx_tree *t, int32_t k, int64_t block, const double *g_id, double *rho_id, double *h_id,
+                         double *dpos_id, double *dmass_id, hipStream_t st) {
+    if (!t) return fail(NBX_ERR_ARG, "NULL tree");
+    if (t->nd != 3 || t->tvel) return fail(NBX_ERR_UNSUPPORTED, "the differentiation form needs a TPHYS tree");
+    const bool mixed = t->p.form == NBX_FORM_MIXED;
+    if (k < 2 || (mixed ? k + 1 : k) > heap_cap(t)) return fail(NBX_ERR_UNSUPPORTED, "k out of range");
+    const int64_t n = t->n;
+    if (n <= k) return fail(NBX_ERR_SEARCH, "fewer than k + 1 particles");
+    if (block <= 0) block = n;
+    block = (block + 1023) / 1024 * 1024;  // whole waves and whole 1024-row windows
+    if (block > n) block = (n + 1023) / 1024 * 1024;
+    DevBufs bufs(st);
+    int32_t *nbr = nullptr, *err = nullptr;
+    HIPCK(bufs.alloc(&nbr, sizeof(int32_t) * (size_t)block * k));
+    HIPCK(bufs.alloc(&err, sizeof(int32_t)));
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, st, err, 0, (int64_t)1);
+    const double *X[3] = {t->xs[0], t->xs[1], t->xs[2]};
+    double *rs = nullptr, *hs = nullptr, *gs = nullptr, *ds = nullptr, *ms = nullptr;
+    const bool adj = g_id != nullptr;
+    if (adj) {
+        HIPCK(bufs.alloc(&gs, sizeof(double) * n));
+        HIPCK(bufs.alloc(&ds, sizeof(double) * 3 * n));
+        HIPCK(bufs.alloc(&ms, sizeof(double) * n));
+        hipLaunchKernelGGL(k_gather_by_id, dim3(nblk(n)), dim3(256), 0, st, g_id, t->perm, n, gs);
+        hipLaunchKernelGGL(k_fill_f64, dim3(nblk(3 * n)), dim3(256), 0, st, ds, 0.0, 3 * n);
+        hipLaunchKernelGGL(k_fill_f64, dim3(nblk(n)), dim3(256), 0, st, ms, 0.0, n);
+    } else {
+        HIPCK(bufs.alloc(&rs, sizeof(double) * n));
+        if (h_id) HIPCK(bufs.alloc(&hs, sizeof(double) * n));
+        hipLaunchKernelGGL(k_fill_f64, dim3(nblk(n)), dim3(256), 0, st, rs, 0.0, n);
+    }
+    // the search's temporaries, reused by every block
+    Workspace ws = {nullptr, 0, 0};
+    if (mixed) {
+        ws.cap = 256 + sizeof(int32_t) * (block + 1) + 256 + nbx_mixed_scratch_bytes(t->nd, k + 1);
+        HIPCK(bufs.alloc(&ws.base, ws.cap));
+    }
+    double *D[3] = {ds, ds ? ds + n : nullptr, ds ? ds + 2 * n : nullptr};
+    for (int64_t r0 = 0; r0 < n; r0 += block) {
+        const int64_t nr = std::min<int64_t>(block, n - r0);
+        KnnArgs A;
+        memset(&A, 0, sizeof(A));
+        A.qsrc = NBX_Q_SLOTS;
+        A.excl_zero = 1;
+        A.kh = k;
+        A.kout = k;
+        A.layout = NBX_LAYOUT_BY_SLOT;
+        A.dens = NBX_DO_NONE;
+        A.nn = nbr;
+        A.wbeg = (int32_t)(r0 / NBX_WAVE);
+        A.wcnt = (int32_t)((nr + NBX_WAVE - 1) / NBX_WAVE);
+        A.row0 = (int32_t)r0;
+        ws.used = 0;
+        hipError_t e = mixed ? run_mixed(t, A, k, st, &ws) : nbx_launch_knn(t->view(), A, A.wcnt * NBX_WAVE, st);
+        if (e == hipSuccess) e = nbx_launch_kth_last(X, t->perm, nbr, r0, nr, k, err, st);
+        if (e == hipSuccess)
+            e = adj ? nbx_launch_sph_adj_rows(X, t->mass, nbr, n, r0, nr, k, gs, D, ms, st)
+                    : nbx_launch_sph_fwd_rows(X, t->mass, nbr, n, r0, nr, k, rs, hs, st);
+        if (e != hipSuccess) return hipfail(e, "differentiation form");
+    }
+    if (adj) {
+        for (int c = 0; c < 3; ++c)
+            hipLaunchKernelGGL(k_scatter_by_id, dim3(nblk(n)), dim3(256), 0, st, ds + c * n, t->perm, dpos_id + c * n, n);
+        hipLaunchKernelGGL(k_scatter_by_id, dim3(nblk(n)), dim3(256), 0, st, ms, t->perm, dmass_id, n);
+    } else {
+        hipLaunchKernelGGL(k_scatter_by_id, dim3(nblk(n)), dim3(256), 0, st, rs, t->perm, rho_id, n);
+        if (h_id) hipLaunchKernelGGL(k_scatter_by_id, dim3(nblk(n)), dim3(256), 0, st, hs, t->perm, h_id, n);
+    }
+    int32_t herr = 0;
+    HIPCK(hipGetLastError());
+    HIPCK(hipMemcpyAsync(&herr, err, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    HIPCK(hipStreamSynchronize(st));
+    if (herr) return fail(NBX_ERR_SEARCH, "differentiation form: fewer than k neighbours for some particle");
+    return NBX_OK;
+}
+
+extern "C" int nbx_sph_density_fwd_tree_dev(nbx_tree *t, int32_t k, int64_t block_rows, double *rho_by_id_dev,
+                                            double *h_by_id_dev, void *stream) {
+    if (!rho_by_id_dev) return fail(NBX_ERR_ARG, "NULL output");
+    return sph_tree_impl(t, k, block_rows, nullptr, rho_by_id_dev, h_by_id_dev, nullptr, nullptr, (hipStream_t)stream);
+}
+extern "C" int nbx_sph_density_adjoint_tree_dev(nbx_tree *t, int32_t k, int64_t block_rows, const double *g_by_id_dev,
+                                                double *dpos_by_id_dev, double *dmass_by_id_dev, void *stream) {
+    if (!g_by_id_dev || !dpos_by_id_dev || !dmass_by_id_dev) return fail(NBX_ERR_ARG, "NULL argument");
+    return sph_tree_impl(t, k, block_rows, g_by_id_dev, nullptr, nullptr, dpos_by_id_dev, dmass_by_id_dev,
+                         (hipStream_t)stream);
 }
 
 // ------------------------------------------------------------------ differentiation form

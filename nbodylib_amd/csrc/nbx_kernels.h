@@ -74,6 +74,9 @@ struct KnnArgs {
     double *hout;       // as rho (nullable)
     int32_t *err;       // set to 1 if a row has fewer than kh neighbours
     const int32_t *qlist;  // nullable: query slots (FP64 re-run of mixed-form fallbacks)
+    // particle queries of a run of whole waves only: slots [64 wbeg, 64 (wbeg + wcnt)) (wcnt 0: every slot)
+    int32_t wbeg, wcnt;
+    int32_t row0;  // subtracted from the slot of a row written in the by-slot layout
 };
 
 hipError_t nbx_launch_knn(const DevTree &T, const KnnArgs &A, int nqueries, hipStream_t st);
@@ -169,6 +172,17 @@ hipError_t nbx_launch_sph_fwd(const double *pos, const double *mass, const int32
                               double *h, hipStream_t st);
 hipError_t nbx_launch_sph_adj(const double *pos, const double *mass, const int32_t *nbr, int64_t n, int k,
                               const double *g, double *dpos, double *dmass, hipStream_t st);
+// The same over rows [r0, r0 + nr) of n particles, row i of the list at
+// nbr[(i - r0) k]; X[3] / D[3] are the coordinate / gradient arrays of all n.
+hipError_t nbx_launch_sph_fwd_rows(const double *const *X, const double *mass, const int32_t *nbr, int64_t n,
+                                   int64_t r0, int64_t nr, int k, double *rho, double *h, hipStream_t st);
+hipError_t nbx_launch_sph_adj_rows(const double *const *X, const double *mass, const int32_t *nbr, int64_t n,
+                                   int64_t r0, int64_t nr, int k, const double *g, double *const *D, double *dmass,
+                                   hipStream_t st);
+// Each row's largest (d2, ID) neighbour moved into its last column; rows
+// with a missing entry (-1) set *err and have it replaced by their own slot.
+hipError_t nbx_launch_kth_last(const double *const *X, const int32_t *perm, int32_t *nbr, int64_t r0, int64_t nr, int k,
+                               int32_t *err, hipStream_t st);
 
 // ------------------------------------------------------------------ profiling
 struct ProfScope {

@@ -310,12 +310,14 @@ __global__ __launch_bounds__(64) void k_knn(DevTree T, KnnArgs A) {
     __shared__ int32_t cid[NBX_LEAF];
     __shared__ double racc[NBX_WAVE];  // symmetric scatter into the wave's own queries
     const int lane = threadIdx.x;
-    const int g = xcd_wave(blockIdx.x, gridDim.x);
+    const int g = xcd_wave(blockIdx.x, gridDim.x) + A.wbeg;
     const int kh = A.kh;
     const int hc = heap_cap4(kh);
     const int nfill = hc - kh;  // filler entries, first after sorting
     const bool listed = (QSRC == NBX_Q_SLOTS) && A.qlist != nullptr;
-    const int nq = (QSRC == NBX_Q_POINTS || listed) ? A.m : T.n;
+    const int wend = A.wbeg + A.wcnt;
+    const int nq = (QSRC == NBX_Q_POINTS || listed) ? A.m
+                   : (A.wcnt > 0 && wend < (T.n + NBX_WAVE - 1) / NBX_WAVE ? wend * NBX_WAVE : T.n);
     const int gi = g * NBX_WAVE + lane;
     const bool valid = gi < nq;
     const int qi = listed ? (valid ? A.qlist[gi] : 0) : gi;
@@ -483,7 +485,8 @@ __global__ __launch_bounds__(64) void k_knn(DevTree T, KnnArgs A) {
         if (qr >= nq) break;
         if (listed) qr = A.qlist[qr];
         int64_t row;
-        if (QSRC == NBX_Q_POINTS || A.layout == 1) row = qr;
+        if (QSRC == NBX_Q_POINTS) row = qr;
+        else if (A.layout == 1) row = qr - A.row0;
         else row = T.perm[qr];
         const int32_t id = hid[(t + off) * NBX_WAVE + r];
         if (A.nn) {

@@ -710,7 +710,7 @@ __device__ __forceinline__ void mix_task(const DevTree &T, const KnnArgs &A, con
             s = (int32_t)(vt[h >> 5] * NBX_LEAF + (h & 31u));
         }
         else s = (int32_t)slb[t * NBX_WAVE + r];
-        int64_t row = A.layout == 1 ? qr : T.perm[qr];
+        int64_t row = A.layout == 1 ? qr - A.row0 : T.perm[qr];
         A.nn[row * k + t] = A.layout == 1 ? s : T.perm[s];
     }
 }
@@ -745,7 +745,7 @@ __global__ __launch_bounds__(64) void k_knn_mixed(DevTree T, KnnArgs A, const do
     const int lane = threadIdx.x;
     uint32_t *sl = scratch + (size_t)blockIdx.x * MX_BLKW(KMAX);
     uint32_t *slb = sl + KMAX * NBX_WAVE, *skc = slb + KMAX * NBX_WAVE;
-    const int nw = (T.n + NBX_WAVE - 1) / NBX_WAVE;
+    const int nw = A.wcnt > 0 ? A.wcnt : (T.n + NBX_WAVE - 1) / NBX_WAVE;
     const int x = blockIdx.x & 7, q8 = nw >> 3, r8 = nw & 7;
     for (int s = 0; s < 8; ++s) {
         const int y = (x + s) & 7;
@@ -755,7 +755,7 @@ __global__ __launch_bounds__(64) void k_knn_mixed(DevTree T, KnnArgs A, const do
             if (lane == 0) t = atomicAdd(ctr + y * MX_CTR_STRIDE, 1);
             t = __builtin_amdgcn_readfirstlane(__shfl(t, 0));
             if (t >= ycnt) break;
-            mix_task<ND, KMAX>(T, A, ext, fb_list, fb_count, hk, cx, racc, sl, slb, skc, ybeg + t, lane);
+            mix_task<ND, KMAX>(T, A, ext, fb_list, fb_count, hk, cx, racc, sl, slb, skc, A.wbeg + ybeg + t, lane);
         }
     }
 }
@@ -795,7 +795,7 @@ size_t nbx_mixed_scratch_bytes(int nd, int kh) {
 
 hipError_t nbx_launch_knn_mixed(const DevTree &T, const KnnArgs &A, const double *ext, int32_t *fallback_list,
                                 int32_t *fallback_count, hipStream_t st, void *work, size_t work_bytes) {
-    const int nw = (T.n + NBX_WAVE - 1) / NBX_WAVE;
+    const int nw = A.wcnt > 0 ? A.wcnt : (T.n + NBX_WAVE - 1) / NBX_WAVE;
     if (nw == 0) return hipSuccess;
     if ((T.nd != 3 && T.nd != 6) || A.kh > 65 || A.kh < 2 || !T.kernf || !T.org || !ext) return hipErrorInvalidValue;
     const int km = A.kh <= 33 ? 33 : 65;

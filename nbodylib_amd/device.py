@@ -81,6 +81,22 @@ class DeviceTree:
         _need(h_out, "h_out", torch.float64, (self.n,))
         _check(lib().nbx_knn_density_dev(self._h, k, _ptr(nn_out), _ptr(rho_out), _ptr(h_out), _stream()))
 
+    def sph_density(self, k, rho_out, h_out=None, block_rows=1 << 22):
+        """Differentiation-form forward over the tree's own k-nearest lists, recomputed per block of
+        block_rows tree slots (nbx_sph_density_fwd_tree_dev); rho / h by ID."""
+        _need(rho_out, "rho_out", torch.float64, (self.n,))
+        _need(h_out, "h_out", torch.float64, (self.n,))
+        _check(lib().nbx_sph_density_fwd_tree_dev(self._h, k, int(block_rows), _ptr(rho_out), _ptr(h_out),
+                                                  _stream()))
+
+    def sph_density_adjoint(self, k, g, dpos_out, dmass_out, block_rows=1 << 22):
+        """Its adjoint (nbx_sph_density_adjoint_tree_dev): g = dL/drho by ID; dpos (3, n), dmass (n,) by ID."""
+        _need(g, "g", torch.float64, (self.n,))
+        _need(dpos_out, "dpos_out", torch.float64, (3, self.n))
+        _need(dmass_out, "dmass_out", torch.float64, (self.n,))
+        _check(lib().nbx_sph_density_adjoint_tree_dev(self._h, k, int(block_rows), _ptr(g), _ptr(dpos_out),
+                                                      _ptr(dmass_out), _stream()))
+
     def density(self, k, rho_out, h_out=None, variant=0, nsearch=0):
         _need(rho_out, "rho_out", torch.float64, (self.n,))
         _need(h_out, "h_out", torch.float64, (self.n,))
