@@ -15,6 +15,12 @@ plain-FP64 rate, FOF Mparticles/s, build time, the roofline of the dominant
 kernel (k_knn, HIP events on its launch stream) and the CPU baseline (the
 oracle's OpenMP restatement of the reference, timed on a bounded sample).
 
+The same line also carries, so that one default run checks every target:
+the differentiation form at N=1M (forward/backward over a stored list and
+with the list recomputed per block of tree slots), the 6-D phase-space
+workload at N=16.8M (k=64, precision and plain forms, two steps each), the
+host CPU model, and the serial CalcDensity rate of the oracle on a sample.
+
 Other measured workloads (--workload), each printing its own JSON line:
   phase6d  N=16.8M clustered particles with velocities, metric (x/sigma_x,
            v/sigma_v): 6-D phase-space tree, k=64 kNN + 6-D SPH density,
@@ -24,7 +30,8 @@ Other measured workloads (--workload), each printing its own JSON line:
            list and its hand-written adjoint (d sum log rho / d x, d m), in
            input (ID) order and in tree order; list memory vs recompute time.
   large    N=134M clustered, lognormal masses: build, mixed kNN + density
-           (k=32), adjoint over the stored list (tree order) and FOF.
+           (k=32), the density adjoint with the list recomputed per block of
+           tree slots (and, for comparison, over a stored list), and FOF.
 """
 import argparse
 import json
@@ -65,6 +72,8 @@ def main():
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic.json"),
                     help="measured HBM bytes per launch of the dominant kernel (from rocprofv3 --pmc runs)")
     ap.add_argument("--workload", default="knn3d", choices=["knn3d", "phase6d", "adjoint", "large"])
+    ap.add_argument("--no-extras", action="store_true",
+                    help="headline workload only (skip the adjoint and 6-D sections of the default line)")
     args = ap.parse_args()
 
     if args.copy_probe:
@@ -210,12 +219,31 @@ def main():
         nth = oracle.num_threads()
         log(f"cpu baseline: kNN+density on {len(slots)} sampled queries, {nth} threads")
         t0 = time.perf_counter()
-        ot.knn_density_sample(k, slots)
+        rho_cpu = ot.knn_density_sample(k, slots, nth)
         cpu_t = time.perf_counter() - t0
+        model, ncpu, naff = cpu_info()
         cpu = {"value": round(len(slots) / cpu_t / 1e6, 4), "unit": "Mqueries/s", "cores": nth, "kind": "port",
                "sample": f"kNN k={k} + gather SPH density for every {stride}-th tree slot "
                          f"({len(slots)} of {n} queries) of the same clustered set, OpenMP schedule(guided); "
-                         f"oracle tree build {cpu_build:.1f} s not included"}
+                         f"oracle tree build {cpu_build:.1f} s not included",
+               "cpu_model": model, "host_cpus": ncpu, "cpus_available": naff, "oracle_flags": "-O3 -fopenmp"}
+        # the same sample against the GPU's gather density (CalcDensityParticle for every particle)
+        ids = ot.order()[slots]
+        rho_g = torch.empty(n, dtype=torch.float64, device=dev)
+        plain_tree.density(k, rho_g, variant=nb.DENS_GATHER)
+        rg = rho_g.cpu().numpy()[ids]
+        cpu["sample_rho_max_rel_vs_gpu_gather"] = float(np.max(np.abs(rg / rho_cpu - 1.0)))
+        # serial as shipped: the library's CalcDensity has no OpenMP
+        # (KDCalcSmoothQuantities.cxx:203-305); the same per-query work on one core
+        sslots = np.arange(0, n, 64, dtype=np.int64)
+        log(f"cpu baseline: serial kNN+density on {len(sslots)} sampled queries, 1 thread")
+        t0 = time.perf_counter()
+        ot.knn_density_sample(k, sslots, 1)
+        ser_t = time.perf_counter() - t0
+        cpu["serial_as_shipped"] = {"value": round(len(sslots) / ser_t / 1e6, 4), "unit": "Mqueries/s", "cores": 1,
+                                    "sample": f"every 64-th tree slot ({len(sslots)} queries), one thread"}
+        log(f"cpu baseline: {cpu['value']} Mq/s on {nth} threads, serial {cpu['serial_as_shipped']['value']} Mq/s, "
+            f"sample rho vs GPU gather max rel {cpu['sample_rho_max_rel_vs_gpu_gather']:.2e}")
         del ot
 
     out = {
@@ -247,7 +275,31 @@ def main():
         "cpu_baseline": cpu,
         "gpu_vs_cpu": round(value / cpu["value"], 2) if cpu else None,
     }
+    if not args.no_extras:
+        del plain_tree
+        torch.cuda.empty_cache()
+        out["c3_adjoint"] = adjoint_section(args, steps=max(args.steps, 3))
+        torch.cuda.empty_cache()
+        out["c2_phase6d"] = phase6d_section(args, steps=2, warmup=1)
     print(json.dumps(out), flush=True)
+
+
+def cpu_info():
+    """(model name, logical CPUs of the host, CPUs this process may use)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        naff = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        naff = None
+    return model, os.cpu_count(), naff
 
 
 class Timer:
@@ -283,7 +335,9 @@ def _line(metric, value, unit, args, ms, dtype, workload, extra, higher=True):
     print(json.dumps(out), flush=True)
 
 
-def run_phase6d(args):
+def phase6d_section(args, steps, warmup):
+    """6-D phase-space workload (N=16.8M, k=64): precision and plain forms,
+    3-D FOF and FOFCriterion(FOF6d) on the same particles."""
     import torch
 
     import nbodylib_amd as nb
@@ -294,11 +348,12 @@ def run_phase6d(args):
     n = args.n
     k = 64 if args.k == 32 else args.k
     L = 100.0
-    log(f"generating clustered N={n} with velocities")
+    log(f"phase6d: generating clustered N={n} with velocities")
     pos, vel, _ = datagen.clustered(n, L, nhalo=2000, seed=args.seed + 2, with_vel=True)
     sx, sv = pos.std(axis=0), vel.std(axis=0)
     xs = torch.from_numpy(np.ascontiguousarray((pos / sx).T)).to(dev)
     vs = torch.from_numpy(np.ascontiguousarray((vel / sv).T)).to(dev)
+    del vel
     rho = torch.empty(n, dtype=torch.float64, device=dev)
     hh = torch.empty(n, dtype=torch.float64, device=dev)
 
@@ -309,13 +364,13 @@ def run_phase6d(args):
             with Timer(torch) as tb:
                 t6 = DeviceTree(xs, vel=vs, bucket_size=16, treetype=nb.TPHS, kerntype=nb.KSPH, aniso=-1, form=form)
             builds.append(tb.ms)
-        log(f"{label}: 6-D tree build {builds[-1]:.1f} ms; warmup {args.warmup}")
-        for _ in range(args.warmup):
+        log(f"phase6d {label}: 6-D tree build {builds[-1]:.1f} ms; warmup {warmup}")
+        for _ in range(warmup):
             t6.density(k, rho, hh)
         nb.prof_reset()
         nb.prof_enable(True)
         times = []
-        for _ in range(args.steps):
+        for _ in range(steps):
             with Timer(torch) as tm:
                 t6.density(k, rho, hh)
             times.append(tm.ms)
@@ -324,7 +379,7 @@ def run_phase6d(args):
         kms, kl = nb.prof_get(kname)
         ms = _median(times)
         fb = t6.last_fallbacks()
-        log(f"{label}: 6-D k={k} kNN + density {ms:.2f} ms ({kname} {kms / max(kl, 1):.2f} ms/launch, "
+        log(f"phase6d {label}: 6-D k={k} kNN + density {ms:.2f} ms ({kname} {kms / max(kl, 1):.2f} ms/launch, "
             f"fallbacks {fb})")
         del t6
         return ms, kms / max(kl, 1), builds[-1], kname, fb
@@ -333,6 +388,7 @@ def run_phase6d(args):
     rho_plain = rho.clone()
     ms, kern_ms, build_ms, kname, fb = timed(nb.FORM_MIXED, "mixed")
     rel = float(torch.max(torch.abs(rho / rho_plain - 1.0)))
+    del rho_plain
     # 3-D FOF at b = 0.2 mean spacing on the same particles
     pd = torch.from_numpy(np.ascontiguousarray(pos.T)).to(dev)
     t3 = DeviceTree(pd, bucket_size=16, kerntype=nb.KSPH, period=L)
@@ -345,8 +401,8 @@ def run_phase6d(args):
             ng = t3.fof(b, 8, lab)
         fts.append(tf.ms)
     fms = _median(fts)
-    log(f"3-D FOF b={b:.5f}: {fms:.1f} ms, {ng} groups")
-    del t3
+    log(f"phase6d: 3-D FOF b={b:.5f}: {fms:.1f} ms, {ng} groups")
+    del t3, pd
     # FOFCriterion with FOF6d on the scaled phase-space tree: 0.2 mean spacing
     # in the scaled positions, 0.1 in the unit-variance velocities
     t6 = DeviceTree(xs, vel=vs, bucket_size=16, treetype=nb.TPHS, kerntype=nb.KSPH, aniso=-1)
@@ -359,22 +415,27 @@ def run_phase6d(args):
             ng6 = t6.fof_criterion(nb.FOF_CMP_6D, prm6, 8, lab)
         f6.append(tf.ms)
     f6ms = _median(f6)
-    log(f"FOF6d (FOFCriterion) params[6]={s6:.3e} params[7]=0.01: {f6ms:.1f} ms, {ng6} groups")
+    log(f"phase6d: FOF6d (FOFCriterion) params[6]={s6:.3e} params[7]=0.01: {f6ms:.1f} ms, {ng6} groups")
     del t6
     bpq = 48 + 8 + 4 * k + 16
     ach = bpq * n / (kern_ms / 1e3) / 1e9
-    _line(f"Mqueries/s 6-D kNN (k={k}) + 6-D SPH density, N={n}, precision form", n / (ms / 1e3) / 1e6,
-          "Mqueries/s", args, ms, "f32+f64",
-          f"6-D phase space (x/sigma_x, v/sigma_v), clustered NFW N={n}, k={k} CalcDensity on a TPHS tree; "
-          f"3-D FOF b=0.2 mean spacing", {
-              "plain_fp64_mqps": round(n / (plain_ms / 1e3) / 1e6, 3), "mixed_vs_plain": round(plain_ms / ms, 3),
-              "mixed_density_max_rel_vs_fp64": rel, "mixed_fp64_fallback_queries": fb,
-              "build_ms": round(build_ms, 2), "fof_ms": round(fms, 2), "fof_mparticles_s": round(n / fms / 1e3, 3),
-              "fof_groups": ng, "fof6d_criterion_ms": round(f6ms, 2), "fof6d_criterion_groups": ng6,
-              "roofline": {"bound": "hbm", "achieved": round(ach, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                           "frac": round(ach / PEAK_HBM_GBS, 5), "traffic": None, "kernel": kname,
-                           "kernel_ms": round(kern_ms, 3), "bytes_per_query": bpq},
-              "cpu_baseline": None})
+    return {"workload": f"6-D phase space (x/sigma_x, v/sigma_v), clustered NFW N={n}, k={k} CalcDensity on a "
+                        f"TPHS tree; 3-D FOF b=0.2 mean spacing; steps={steps}",
+            "n": n, "k": k, "mixed_mqps": round(n / (ms / 1e3) / 1e6, 3), "mixed_ms": round(ms, 2),
+            "plain_fp64_mqps": round(n / (plain_ms / 1e3) / 1e6, 3), "plain_fp64_ms": round(plain_ms, 2),
+            "mixed_vs_plain": round(plain_ms / ms, 3), "mixed_density_max_rel_vs_fp64": rel,
+            "mixed_fp64_fallback_queries": fb, "build_ms": round(build_ms, 2), "fof_ms": round(fms, 2),
+            "fof_mparticles_s": round(n / fms / 1e3, 3), "fof_groups": ng, "fof6d_criterion_ms": round(f6ms, 2),
+            "fof6d_criterion_groups": ng6,
+            "roofline": {"bound": "hbm", "achieved": round(ach, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                         "frac": round(ach / PEAK_HBM_GBS, 5), "kernel": kname, "kernel_ms": round(kern_ms, 3),
+                         "bytes_per_query": bpq}}
+
+
+def run_phase6d(args):
+    r = phase6d_section(args, args.steps, args.warmup)
+    _line(f"Mqueries/s 6-D kNN (k={r['k']}) + 6-D SPH density, N={r['n']}, precision form", r["mixed_mqps"],
+          "Mqueries/s", args, r["mixed_ms"], "f32+f64", r["workload"], dict(r, cpu_baseline=None))
 
 
 def _adjoint_pass(torch, nb, pos, mass, nbr, reps):
@@ -399,7 +460,10 @@ def _adjoint_pass(torch, nb, pos, mass, nbr, reps):
     return _median(fw), _median(bw)
 
 
-def run_adjoint(args):
+def adjoint_section(args, steps):
+    """Differentiation form at N=1M (clustered, lognormal masses, k=32):
+    forward / backward over a stored list in input (ID) and tree order, and
+    with the list recomputed per block of tree slots."""
     import torch
 
     import nbodylib_amd as nb
@@ -410,18 +474,21 @@ def run_adjoint(args):
     n = 1 << 20 if args.n == 256 ** 3 else args.n
     k = args.k
     L = 100.0
-    log(f"generating clustered N={n}, lognormal masses")
+    log(f"adjoint: generating clustered N={n}, lognormal masses")
     pos, _ = datagen.clustered(n, L, nhalo=2000, seed=args.seed + 3)
     m = datagen.lognormal_masses(n, seed=args.seed + 3)
     pd = torch.from_numpy(np.ascontiguousarray(pos.T)).to(dev)
     md = torch.from_numpy(m).to(dev)
     tree = DeviceTree(pd, mass=md, bucket_size=16, kerntype=nb.KSPH, form=nb.FORM_MIXED)
-    # frozen list in input (ID) order: rows and entries are IDs
+    # frozen list in input (ID) order: rows and entries are IDs, ascending
     nn_id = torch.empty((n, k), dtype=torch.int32, device=dev)
+    d2 = torch.empty((n, k), dtype=torch.float64, device=dev)
+    tree.knn(k, nn_id, d2, variant=nb.KNN_NONPERIODIC, layout=nb.LAYOUT_BY_ID)
+    del d2
     with Timer(torch) as tk:
-        tree.knn(k, nn_id, layout=nb.LAYOUT_BY_ID)
+        tree.knn(k, torch.empty_like(nn_id), variant=nb.KNN_NONPERIODIC, layout=nb.LAYOUT_BY_ID)
     knn_ms = tk.ms
-    fw_id, bw_id = _adjoint_pass(torch, nb, pd, md, nn_id, max(args.steps, 3))
+    fw_id, bw_id = _adjoint_pass(torch, nb, pd, md, nn_id, steps)
     # the same list in tree order: positions/masses permuted by the tree
     order = tree.order().long()
     inv = torch.empty_like(order)
@@ -429,24 +496,49 @@ def run_adjoint(args):
     pt = pd[:, order].contiguous()
     mt = md[order].contiguous()
     nn_t = inv[nn_id[order].long()].to(torch.int32).contiguous()
-    fw_t, bw_t = _adjoint_pass(torch, nb, pt, mt, nn_t, max(args.steps, 3))
-    log(f"ID order: fwd {fw_id:.2f} ms bwd {bw_id:.2f} ms; tree order: fwd {fw_t:.2f} ms bwd {bw_t:.2f} ms; "
-        f"kNN to rebuild the list {knn_ms:.2f} ms")
+    fw_t, bw_t = _adjoint_pass(torch, nb, pt, mt, nn_t, steps)
+    del pt, mt, nn_t
+    # the list recomputed per block of tree slots (no list of all n kept)
+    block = 1 << 18
+    rho = torch.empty(n, dtype=torch.float64, device=dev)
+    hh = torch.empty(n, dtype=torch.float64, device=dev)
+    tree.sph_density(k, rho, hh, block_rows=block)
+    g = (1.0 / rho).contiguous()
+    dp = torch.empty_like(pd)
+    dm = torch.empty_like(md)
+    rf, rb = [], []
+    for _ in range(steps):
+        with Timer(torch) as tf:
+            tree.sph_density(k, rho, hh, block_rows=block)
+        rf.append(tf.ms)
+        with Timer(torch) as tb:
+            tree.sph_density_adjoint(k, g, dp, dm, block_rows=block)
+        rb.append(tb.ms)
+    fw_r, bw_r = _median(rf), _median(rb)
+    log(f"adjoint: ID order fwd {fw_id:.2f} / bwd {bw_id:.2f} ms; tree order fwd {fw_t:.2f} / bwd {bw_t:.2f} ms; "
+        f"recomputed per {block}-slot block fwd {fw_r:.2f} / bwd {bw_r:.2f} ms; kNN for the list {knn_ms:.2f} ms")
     bpp = 24 + 8 + 8 + 8 + 4 * k + 32
     ach = bpp * n / (bw_t / 1e3) / 1e9
-    _line(f"Mparticles/s SPH density adjoint over a frozen k={k} list, N={n}", n / (bw_t / 1e3) / 1e6,
-          "Mparticles/s", args, bw_t, "f64",
-          f"clustered NFW N={n}, lognormal masses, frozen k={k} list; backward = hand-written adjoint of "
-          f"sum log rho", {
-              "fwd_ms_tree_order": round(fw_t, 3), "bwd_ms_tree_order": round(bw_t, 3),
-              "bwd_over_fwd_tree_order": round(bw_t / fw_t, 3),
-              "fwd_ms_id_order": round(fw_id, 3), "bwd_ms_id_order": round(bw_id, 3),
-              "bwd_over_fwd_id_order": round(bw_id / fw_id, 3),
-              "list_bytes_stored": n * k * 4, "list_recompute_ms": round(knn_ms, 3),
-              "roofline": {"bound": "hbm", "achieved": round(ach, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                           "frac": round(ach / PEAK_HBM_GBS, 5), "traffic": None, "kernel": "k_sph_adj",
-                           "kernel_ms": round(bw_t, 3), "bytes_per_particle": bpp},
-              "cpu_baseline": None})
+    return {"workload": f"clustered NFW N={n}, lognormal masses, frozen k={k} list; backward = hand-written "
+                        f"adjoint of sum log rho", "n": n, "k": k,
+            "fwd_ms_tree_order": round(fw_t, 3), "bwd_ms_tree_order": round(bw_t, 3),
+            "bwd_over_fwd_tree_order": round(bw_t / fw_t, 3),
+            "fwd_ms_id_order": round(fw_id, 3), "bwd_ms_id_order": round(bw_id, 3),
+            "bwd_over_fwd_id_order": round(bw_id / fw_id, 3),
+            "recompute_block_rows": block, "fwd_ms_recompute": round(fw_r, 3), "bwd_ms_recompute": round(bw_r, 3),
+            "bwd_over_fwd_recompute": round(bw_r / fw_r, 3),
+            "list_bytes_stored": n * k * 4, "list_bytes_recompute": block * k * 4,
+            "list_recompute_ms": round(knn_ms, 3),
+            "roofline": {"bound": "hbm", "achieved": round(ach, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                         "frac": round(ach / PEAK_HBM_GBS, 5), "kernel": "k_sph_adj", "kernel_ms": round(bw_t, 3),
+                         "bytes_per_particle": bpp}}
+
+
+def run_adjoint(args):
+    r = adjoint_section(args, max(args.steps, 3))
+    _line(f"Mparticles/s SPH density adjoint over a frozen k={r['k']} list, N={r['n']}",
+          r["n"] / (r["bwd_ms_tree_order"] / 1e3) / 1e6, "Mparticles/s", args, r["bwd_ms_tree_order"], "f64",
+          r["workload"], dict(r, cpu_baseline=None))
 
 
 def run_large(args):
@@ -490,8 +582,20 @@ def run_large(args):
     mt = md[order].contiguous()
     del order
     fw, bw = _adjoint_pass(torch, nb, pt, mt, nn, 2)
-    log(f"adjoint fwd {fw:.1f} ms bwd {bw:.1f} ms")
-    del pt, mt
+    log(f"adjoint over the stored list: fwd {fw:.1f} ms bwd {bw:.1f} ms")
+    del pt, mt, nn
+    torch.cuda.empty_cache()
+    # adjoint with the list recomputed per block of tree slots: no list of all n
+    block = 1 << 22
+    g = (1.0 / rho).contiguous()
+    dp = torch.empty_like(pd)
+    dm = torch.empty_like(md)
+    with Timer(torch) as trf:
+        tree.sph_density(k, rho, hh, block_rows=block)
+    with Timer(torch) as trb:
+        tree.sph_density_adjoint(k, g, dp, dm, block_rows=block)
+    log(f"adjoint with the list recomputed per {block}-slot block: fwd {trf.ms:.1f} ms bwd {trb.ms:.1f} ms")
+    del g, dp, dm
     b = 0.2 * datagen.mean_spacing(n, L)
     lab = torch.empty(n, dtype=torch.int32, device=dev)
     with Timer(torch) as tf:
@@ -500,10 +604,15 @@ def run_large(args):
     mem = torch.cuda.max_memory_allocated() / 2 ** 30
     _line(f"Mqueries/s kNN+SPH density (k={k}, N={n}) mixed", n / (kms / 1e3) / 1e6, "Mqueries/s", args, kms,
           "f32+f64", f"clustered NFW N={n}, lognormal masses, periodic L=100: build, mixed kNN+density k={k}, "
-                     f"adjoint over the stored list, FOF b=0.2 mean spacing", {
+                     f"adjoint with the list recomputed per block (and over the stored list), FOF b=0.2 mean "
+                     f"spacing", {
               "build_ms": round(tb.ms, 2), "mixed_fp64_fallback_queries": fb,
-              "adjoint_fwd_ms": round(fw, 2), "adjoint_bwd_ms": round(bw, 2), "fof_ms": round(tf.ms, 2),
-              "fof_groups": ng, "torch_peak_gib": round(mem, 2), "cpu_baseline": None})
+              "adjoint_stored_fwd_ms": round(fw, 2), "adjoint_stored_bwd_ms": round(bw, 2),
+              "adjoint_recompute_block_rows": block, "adjoint_recompute_fwd_ms": round(trf.ms, 2),
+              "adjoint_recompute_bwd_ms": round(trb.ms, 2),
+              "list_bytes_stored": n * k * 4, "list_bytes_recompute": block * k * 4,
+              "fof_ms": round(tf.ms, 2), "fof_groups": ng, "torch_peak_gib_after_list_freed": round(mem, 2),
+              "cpu_baseline": None})
 
 
 if __name__ == "__main__":

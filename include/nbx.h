@@ -107,6 +107,13 @@ int nbx_tree_kernel(const nbx_tree *t, double *kernnorm, double *table, int32_t 
 int nbx_knn(nbx_tree *t, int32_t k, int32_t variant, int32_t layout, int32_t *nn, double *d2);
 int nbx_knn_dev(nbx_tree *t, int32_t k, int32_t variant, int32_t layout, int32_t *nn_dev, double *d2_dev,
                 void *stream);
+/* kNN of a list of particles (ids: m IDs).  Replaces m per-particle calls
+ * FindNearest(tt) / FindNearestPos(tt) / FindNearestPhase(tt)
+ * (KDFindNearest.cxx:247-361) with the variant's rules as in nbx_knn; row r
+ * belongs to particle ids[r], entries are IDs (NBX_LAYOUT_BY_ID) or slots
+ * (NBX_LAYOUT_BY_SLOT).  nn: m*k; d2: m*k or NULL. */
+int nbx_knn_targets(nbx_tree *t, const int32_t *ids, int64_t m, int32_t k, int32_t variant, int32_t layout,
+                    int32_t *nn, double *d2);
 /* kNN of arbitrary positions (q: 3*m SoA).  Replaces FindNearestPos(Double_t*x,...)
  * KDFindNearest.cxx:462-564 (period-aware like the reference).  Entries are IDs. */
 int nbx_knn_points(nbx_tree *t, const double *q, int64_t m, int32_t k, int32_t *nn, double *d2);

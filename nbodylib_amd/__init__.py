@@ -30,7 +30,7 @@ NBX_ERR_ARG, NBX_ERR_HIP, NBX_ERR_UNSUPPORTED, NBX_ERR_NOMEM, NBX_ERR_SEARCH = -
 
 EXPORTS = [
     "nbx_params_default", "nbx_tree_create", "nbx_tree_create_dev", "nbx_tree_destroy", "nbx_tree_order",
-    "nbx_tree_size", "nbx_tree_kernel", "nbx_knn", "nbx_knn_dev", "nbx_knn_points", "nbx_knn_phase_points", "nbx_density_points", "nbx_veldensity_points", "nbx_veldensity_phys", "nbx_smooth_vel", "nbx_smooth_vel_moments", "nbx_smooth_local", "nbx_smooth_local_points", "nbx_density",
+    "nbx_tree_size", "nbx_tree_kernel", "nbx_knn", "nbx_knn_dev", "nbx_knn_points", "nbx_knn_targets", "nbx_knn_phase_points", "nbx_density_points", "nbx_veldensity_points", "nbx_veldensity_phys", "nbx_smooth_vel", "nbx_smooth_vel_moments", "nbx_smooth_local", "nbx_smooth_local_points", "nbx_density",
     "nbx_density_dev", "nbx_knn_density_dev", "nbx_knn_density_noalloc", "nbx_ball_points", "nbx_ball_points_many", "nbx_ball_mark", "nbx_fof", "nbx_fof_dev", "nbx_fof_criterion", "nbx_fof_criterion_dev", "nbx_fof_lists",
     "nbx_sph_density_fwd_dev", "nbx_sph_density_adjoint_dev", "nbx_sph_density_fwd_tree_dev",
     "nbx_sph_density_adjoint_tree_dev", "nbx_prof_enable", "nbx_prof_reset", "nbx_prof_get",
@@ -80,6 +80,7 @@ def lib():
     L.nbx_knn_dev.argtypes = [vp, i32, i32, i32, vp, vp, vp]
     L.nbx_knn_points.argtypes = [vp, vp, i64, i32, vp, vp]
     L.nbx_knn_phase_points.argtypes = [vp, vp, vp, i64, i32, vp, vp]
+    L.nbx_knn_targets.argtypes = [vp, vp, i64, i32, i32, i32, vp, vp]
     L.nbx_density_points.argtypes = [vp, vp, i64, i32, vp, vp]
     L.nbx_veldensity_points.argtypes = [vp, vp, vp, i64, i32, i32, vp, vp]
     L.nbx_veldensity_phys.argtypes = [vp, i32, i32, vp, vp]
@@ -252,6 +253,16 @@ class KDTree:
         nn = np.zeros((m, Nsearch), dtype=np.int32)
         d2 = np.zeros((m, Nsearch), dtype=np.float64)
         _check(lib().nbx_knn_points(self._h, _p(q), m, Nsearch, _p(nn), _p(d2)))
+        return nn, d2
+
+    def FindNearestTargets(self, ids, Nsearch=64, variant=KNN_FINDNEAREST, layout=LAYOUT_BY_ID):
+        """FindNearest(tt) (or the variant's per-particle call) for the particles with the given IDs;
+        row r belongs to ids[r]."""
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        m = ids.shape[0]
+        nn = np.zeros((m, Nsearch), dtype=np.int32)
+        d2 = np.zeros((m, Nsearch), dtype=np.float64)
+        _check(lib().nbx_knn_targets(self._h, _p(ids), m, Nsearch, variant, layout, _p(nn), _p(d2)))
         return nn, d2
 
     def FindNearestPhasePoints(self, x, v, Nsearch=64):

@@ -551,6 +551,43 @@ extern "C" int nbx_knn(nbx_tree *t, int32_t k, int32_t variant, int32_t layout, 
     return rc;
 }
 
+// FindNearest(tt) / FindNearestPos(tt) / FindNearestPhase(tt) for a list of
+// particles at once: the batch variant's rules, rows in the list's order.
+extern "C" int nbx_knn_targets(nbx_tree *t, const int32_t *ids, int64_t m, int32_t k, int32_t variant, int32_t layout,
+                               int32_t *nn, double *d2) {
+    KnnArgs A;
+    int nq;
+    int rc = setup_knn(t, k, variant, A, nq);
+    if (rc) return rc;
+    if (!nn || (m > 0 && !ids)) return fail(NBX_ERR_ARG, "NULL argument");
+    if (layout != NBX_LAYOUT_BY_ID && layout != NBX_LAYOUT_BY_SLOT) return fail(NBX_ERR_ARG, "unknown layout");
+    if (m < 0 || m > INT32_MAX) return fail(NBX_ERR_ARG, "number of targets out of range");
+    if (m == 0) return NBX_OK;
+    for (int64_t i = 0; i < m; ++i)
+        if (ids[i] < 0 || ids[i] >= t->n) return fail(NBX_ERR_ARG, "target ID out of range");
+    DevBufs bufs;
+    int32_t *did = nullptr, *dsl = nullptr, *dnn = nullptr;
+    double *dd2 = nullptr;
+    HIPCK(bufs.alloc(&did, sizeof(int32_t) * m));
+    HIPCK(bufs.alloc(&dsl, sizeof(int32_t) * m));
+    HIPCK(bufs.alloc(&dnn, sizeof(int32_t) * m * k));
+    if (d2) HIPCK(bufs.alloc(&dd2, sizeof(double) * m * k));
+    HIPCK(hipMemcpy(did, ids, sizeof(int32_t) * m, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_ids_to_slots, dim3(nblk(m)), dim3(256), 0, nullptr, did, t->inv, m, dsl);
+    A.layout = layout;
+    A.nn = dnn;
+    A.d2 = dd2;
+    A.qlist = dsl;
+    A.m = (int32_t)m;
+    A.list_rows = 1;
+    hipError_t e = nbx_launch_knn(t->view(), A, (int)m, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(nn, dnn, sizeof(int32_t) * m * k, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && d2) e = hipMemcpy(d2, dd2, sizeof(double) * m * k, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hipfail(e, "nbx_knn_targets");
+    return NBX_OK;
+}
+
 // Position queries through the plain kernel: kNN (nn/d2, nullable) and/or the
 // gather density with h = r_k/2 (rho/h, nullable).  periodic: the image passes
 // of the period-aware search.

@@ -628,6 +628,13 @@ __global__ __launch_bounds__(256) void k_iota(int32_t *p, int64_t n) {
     if (i < n) p[i] = (int32_t)i;
 }
 
+// tree slots of the particles with the given IDs (-1 for an ID out of range)
+__global__ __launch_bounds__(256) void k_ids_to_slots(const int32_t *ids, const int32_t *inv, int64_t m,
+                                                      int32_t *slots) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) slots[i] = inv[ids[i]];
+}
+
 // ------------------------------------------------------------------ ball search
 // SearchBallPosTagged(x) (KDFindNearest.cxx:618-688, KDLeafNode.cxx:336-356)
 // and SearchBallPos(x) (KDFindNearest.cxx:575-582, KDLeafNode.cxx:279-300):

@@ -77,6 +77,7 @@ struct KnnArgs {
     // particle queries of a run of whole waves only: slots [64 wbeg, 64 (wbeg + wcnt)) (wcnt 0: every slot)
     int32_t wbeg, wcnt;
     int32_t row0;  // subtracted from the slot of a row written in the by-slot layout
+    int32_t list_rows;  // with qlist: row r of the output is the query qlist[r]
 };
 
 hipError_t nbx_launch_knn(const DevTree &T, const KnnArgs &A, int nqueries, hipStream_t st);
@@ -166,6 +167,7 @@ __global__ void k_scatter_by_id(const double *src, const int32_t *perm, double *
 __global__ void k_fill_i32(int32_t *p, int32_t v, int64_t n);
 __global__ void k_fill_f64(double *p, double v, int64_t n);
 __global__ void k_iota(int32_t *p, int64_t n);
+__global__ void k_ids_to_slots(const int32_t *ids, const int32_t *inv, int64_t m, int32_t *slots);
 
 // ------------------------------------------------------------------ adjoint
 hipError_t nbx_launch_sph_fwd(const double *pos, const double *mass, const int32_t *nbr, int64_t n, int k, double *rho,

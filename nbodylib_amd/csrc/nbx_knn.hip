@@ -314,7 +314,7 @@ __global__ __launch_bounds__(64) void k_knn(DevTree T, KnnArgs A) {
     const int kh = A.kh;
     const int hc = heap_cap4(kh);
     const int nfill = hc - kh;  // filler entries, first after sorting
-    const bool listed = (QSRC == NBX_Q_SLOTS) && A.qlist != nullptr;
+    const bool listed = (QSRC != NBX_Q_POINTS) && A.qlist != nullptr;
     const int wend = A.wbeg + A.wcnt;
     const int nq = (QSRC == NBX_Q_POINTS || listed) ? A.m
                    : (A.wcnt > 0 && wend < (T.n + NBX_WAVE - 1) / NBX_WAVE ? wend * NBX_WAVE : T.n);
@@ -483,9 +483,11 @@ __global__ __launch_bounds__(64) void k_knn(DevTree T, KnnArgs A) {
         int r = e / kout, t = e - r * kout;
         int qr = g * NBX_WAVE + r;
         if (qr >= nq) break;
+        const int lr = qr;
         if (listed) qr = A.qlist[qr];
         int64_t row;
         if (QSRC == NBX_Q_POINTS) row = qr;
+        else if (listed && A.list_rows) row = lr;
         else if (A.layout == 1) row = qr - A.row0;
         else row = T.perm[qr];
         const int32_t id = hid[(t + off) * NBX_WAVE + r];
