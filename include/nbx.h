@@ -60,7 +60,9 @@ typedef struct nbx_params {
     int32_t periodic;    /* 0 or 1 */
     double period[3];    /* Period[3] when periodic */
     int32_t form;        /* NBX_FORM_PLAIN or NBX_FORM_MIXED */
-    double recheck_rel;  /* mixed form: relative FP64 re-check band (default 1e-5) */
+    double recheck_rel;  /* mixed form: least relative half-width of the FP64 re-check band around the
+                          * k-th distance (default 1e-5; the proven FP32 error bound is used where wider);
+                          * a wider band re-checks more candidates and never changes a result */
 } nbx_params;
 
 /* Defaults of KDTree::KDTree (KDTree.h:229-245). */

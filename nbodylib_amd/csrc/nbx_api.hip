@@ -449,6 +449,7 @@ static hipError_t run_mixed(nbx_tree *t, KnnArgs A, int k, hipStream_t st, Works
     KnnArgs M = A;
     M.kh = k + 1;
     M.kout = k;
+    M.band_rel = t->p.recheck_rel;
     DevTree T = t->view();
     e = nbx_launch_knn_mixed(T, M, t->ext, fb, cnt, st, mwork, mbytes);
     int32_t hc = 0;
@@ -477,7 +478,12 @@ extern "C" int nbx_tree_last_fallbacks(const nbx_tree *t, int64_t *fallbacks) {
 }
 
 // ------------------------------------------------------------------ kNN
-static int heap_cap(const nbx_tree *t) { return t->nd == 6 ? 65 : 65; }
+// Largest heap (kept entries per query) the FP64 kernel holds: k + 1 for
+// the periodic searches, so k <= 128 there and k <= 129 otherwise.  The
+// precision-form kernel keeps k + 1 entries up to 65 (k <= 64); larger k
+// run on the FP64 kernel, whose neighbour sets the precision form equals.
+static int heap_cap(const nbx_tree *) { return 129; }
+static bool mixed_ok(const nbx_tree *t, int32_t k) { return t->p.form == NBX_FORM_MIXED && k + 1 <= 65; }
 
 static int setup_knn(nbx_tree *t, int32_t k, int32_t variant, KnnArgs &A, int &nq) {
     memset(&A, 0, sizeof(A));
@@ -509,7 +515,7 @@ static int setup_knn(nbx_tree *t, int32_t k, int32_t variant, KnnArgs &A, int &n
         default:
             return fail(NBX_ERR_ARG, "unknown kNN variant");
     }
-    if (A.kh > heap_cap(t)) return fail(NBX_ERR_UNSUPPORTED, "k too large (k <= 64 supported)");
+    if (A.kh > heap_cap(t)) return fail(NBX_ERR_UNSUPPORTED, "k too large (k <= 128 supported)");
     if (A.periodic) for (int d = 0; d < 3; ++d) A.period[d] = t->p.period[d];
     return NBX_OK;
 }
@@ -525,8 +531,14 @@ extern "C" int nbx_knn_dev(nbx_tree *t, int32_t k, int32_t variant, int32_t layo
     A.nn = nn_dev;
     A.d2 = d2_dev;
     hipStream_t st = (hipStream_t)stream;
-    if (t->p.form == NBX_FORM_MIXED && A.qsrc == NBX_Q_SLOTS && A.kh == k && d2_dev == nullptr) {
+    if (mixed_ok(t, k) && A.qsrc == NBX_Q_SLOTS && A.kh == k) {
+        // FP32 search; when distances are asked for, the rows' FP64 d2 and
+        // their FP64 order come from a row pass over the selected entries
+        DevBufs bufs(st);
+        if (d2_dev && !A.nn) HIPCK(bufs.alloc(&A.nn, sizeof(int32_t) * (size_t)t->n * k));
+        A.d2 = nullptr;
         HIPCK(run_mixed(t, A, k, st));
+        if (d2_dev) HIPCK(nbx_launch_rows_fp64(t->view(), A.nn, d2_dev, k, layout, st));
         return NBX_OK;
     }
     HIPCK(nbx_launch_knn(t->view(), A, nq, st));
@@ -648,7 +660,7 @@ static int points_impl(nbx_tree *t, const double *q, int64_t m, int32_t k, bool 
 extern "C" int nbx_knn_points(nbx_tree *t, const double *q, int64_t m, int32_t k, int32_t *nn, double *d2) {
     if (!t || !q || !nn) return fail(NBX_ERR_ARG, "NULL argument");
     if (t->nd != 3) return fail(NBX_ERR_UNSUPPORTED, "position queries need a TPHYS tree");
-    if (k < 1 || k > 64) return fail(NBX_ERR_UNSUPPORTED, "k out of range (1..64)");
+    if (k < 1 || k > heap_cap(t)) return fail(NBX_ERR_UNSUPPORTED, "k out of range (1..129)");
     return points_impl(t, q, m, k, true, nn, d2, nullptr, nullptr, "nbx_knn_points");
 }
 
@@ -666,7 +678,7 @@ extern "C" int nbx_knn_phase_points(nbx_tree *t, const double *x, const double *
 extern "C" int nbx_density_points(nbx_tree *t, const double *q, int64_t m, int32_t k, double *rho, double *h) {
     if (!t || !q || !rho) return fail(NBX_ERR_ARG, "NULL argument");
     if (t->nd != 3) return fail(NBX_ERR_UNSUPPORTED, "CalcDensityPosition here needs a TPHYS tree");
-    if (k < 1 || k > 64) return fail(NBX_ERR_UNSUPPORTED, "Nsmooth out of range (1..64)");
+    if (k < 1 || k > heap_cap(t)) return fail(NBX_ERR_UNSUPPORTED, "Nsmooth out of range (1..129)");
     if (k > t->n) return fail(NBX_ERR_SEARCH, "CalcDensityPosition failed: fewer than Nsmooth particles");
     return points_impl(t, q, m, k, false, nullptr, nullptr, rho, h, "nbx_density_points");
 }
@@ -677,7 +689,7 @@ extern "C" int nbx_veldensity_points(nbx_tree *t, const double *x, const double 
     if (t->nd != 3 || !t->has_vel)
         return fail(NBX_ERR_UNSUPPORTED, "CalcVelDensityPosition needs a TPHYS tree built with velocities");
     if (nsmooth < 1 || nsearch < 1) return fail(NBX_ERR_ARG, "Nsmooth and Nsearch must be >= 1");
-    if (nsearch > 64) return fail(NBX_ERR_UNSUPPORTED, "Nsearch out of range (1..64)");
+    if (nsearch > heap_cap(t)) return fail(NBX_ERR_UNSUPPORTED, "Nsearch out of range (1..129)");
     if (nsearch > t->n) return fail(NBX_ERR_SEARCH, "CalcVelDensityPosition failed: fewer than Nsearch particles");
     // Nsmooth > Nsearch is taken as Nsearch (KDCalcSmoothQuantities.cxx:1160-1163)
     return points_impl(t, x, m, nsearch, false, nullptr, nullptr, rho, hv, "nbx_veldensity_points", v,
@@ -710,8 +722,8 @@ static int density_impl(nbx_tree *t, int32_t k, int32_t variant, int32_t nsearch
         A.nsmooth = k > nsearch ? nsearch : k;
     } else
         return fail(NBX_ERR_ARG, "unknown density variant");
-    if (A.kh > heap_cap(t)) return fail(NBX_ERR_UNSUPPORTED, "k too large (k <= 64 supported)");
-    const bool mixed = t->p.form == NBX_FORM_MIXED && A.dens != NBX_DO_VEL && A.dens != NBX_DO_VELPHYS;
+    if (A.kh > heap_cap(t)) return fail(NBX_ERR_UNSUPPORTED, "k too large (k <= 129 supported)");
+    const bool mixed = mixed_ok(t, k) && A.dens != NBX_DO_VEL && A.dens != NBX_DO_VELPHYS;
     const int64_t n = t->n;
     double *rho = nullptr, *h = nullptr;
     int32_t *err = nullptr;
@@ -800,7 +812,7 @@ extern "C" int nbx_knn_density_noalloc(nbx_tree *t, int32_t k, int32_t *nn_by_sl
     if (k < 1 || k > heap_cap(t)) return fail(NBX_ERR_UNSUPPORTED, "k out of range");
     // the carves density_impl and run_mixed will make, in their order
     const int64_t n = t->n;
-    const bool mixed = t->p.form == NBX_FORM_MIXED;
+    const bool mixed = mixed_ok(t, k);
     Workspace dry = {nullptr, ~(size_t)0, 0};
     dry.carve<char>(sizeof(double) * n);
     if (h_by_id_dev) dry.carve<char>(sizeof(double) * n);
@@ -843,7 +855,7 @@ static int smooth_impl(nbx_tree *t, int32_t k, const double *rho_by_id, const do
     DevBufs bufs;
     if (t->nd != 3 || !t->has_vel)
         return fail(NBX_ERR_UNSUPPORTED, "CalcSmoothVel needs a TPHYS tree built with velocities");
-    if (k < 1 || k > 64) return fail(NBX_ERR_UNSUPPORTED, "Nsmooth out of range (1..64)");
+    if (k < 1 || k > heap_cap(t)) return fail(NBX_ERR_UNSUPPORTED, "Nsmooth out of range (1..129)");
     const int64_t n = t->n;
     const bool moments = skew_out || kurt_out;
     const bool need_disp = disp_out || moments;
@@ -911,7 +923,7 @@ extern "C" int nbx_smooth_local(nbx_tree *t, int32_t k, const double *weight_by_
     if (!t || !weight_by_id || (!mean_by_id && !disp_by_id)) return fail(NBX_ERR_ARG, "NULL argument");
     if (disp_by_id && !mean_in_by_id) return fail(NBX_ERR_ARG, "the dispersion needs the means (mean_in_by_id)");
     if (t->nd != 3 || t->tvel) return fail(NBX_ERR_UNSUPPORTED, "CalcSmoothLocalMean here needs a TPHYS tree");
-    if (k < 1 || k > 64) return fail(NBX_ERR_UNSUPPORTED, "Nsmooth out of range (1..64)");
+    if (k < 1 || k > heap_cap(t)) return fail(NBX_ERR_UNSUPPORTED, "Nsmooth out of range (1..129)");
     const int64_t n = t->n;
     hipStream_t st = nullptr;
     int32_t *nn = nullptr;
@@ -956,7 +968,7 @@ extern "C" int nbx_smooth_local_points(nbx_tree *t, const double *q, int64_t m, 
     if (disp_out && !mean_in) return fail(NBX_ERR_ARG, "the dispersion needs the means (mean_in)");
     if (t->nd != 3 || t->tvel)
         return fail(NBX_ERR_UNSUPPORTED, "CalcSmoothLocalMeanPosition here needs a TPHYS tree");
-    if (k < 1 || k > 64) return fail(NBX_ERR_UNSUPPORTED, "Nsmooth out of range (1..64)");
+    if (k < 1 || k > heap_cap(t)) return fail(NBX_ERR_UNSUPPORTED, "Nsmooth out of range (1..129)");
     if (k > t->n) return fail(NBX_ERR_SEARCH, "CalcSmoothLocalMean failed: fewer than Nsmooth particles");
     if (m <= 0) return NBX_OK;
     if (m > INT32_MAX) return fail(NBX_ERR_ARG, "too many positions (m < 2^31)");
@@ -1301,8 +1313,8 @@ static int sph_tree_impl(nbx_tree *t, int32_t k, int64_t block, const double *g_
                          double *dpos_id, double *dmass_id, hipStream_t st) {
     if (!t) return fail(NBX_ERR_ARG, "NULL tree");
     if (t->nd != 3 || t->tvel) return fail(NBX_ERR_UNSUPPORTED, "the differentiation form needs a TPHYS tree");
-    const bool mixed = t->p.form == NBX_FORM_MIXED;
-    if (k < 2 || (mixed ? k + 1 : k) > heap_cap(t)) return fail(NBX_ERR_UNSUPPORTED, "k out of range");
+    const bool mixed = mixed_ok(t, k);
+    if (k < 2 || k > heap_cap(t)) return fail(NBX_ERR_UNSUPPORTED, "k out of range");
     const int64_t n = t->n;
     if (n <= k) return fail(NBX_ERR_SEARCH, "fewer than k + 1 particles");
     if (block <= 0) block = n;

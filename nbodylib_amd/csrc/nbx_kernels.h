@@ -78,6 +78,7 @@ struct KnnArgs {
     int32_t wbeg, wcnt;
     int32_t row0;  // subtracted from the slot of a row written in the by-slot layout
     int32_t list_rows;  // with qlist: row r of the output is the query qlist[r]
+    double band_rel;    // precision form: least relative half-width of the FP64 re-check band
 };
 
 hipError_t nbx_launch_knn(const DevTree &T, const KnnArgs &A, int nqueries, hipStream_t st);
@@ -87,6 +88,10 @@ hipError_t nbx_launch_knn(const DevTree &T, const KnnArgs &A, int nqueries, hipS
 hipError_t nbx_launch_knn_mixed(const DevTree &T, const KnnArgs &A, const double *ext, int32_t *fallback_list,
                                 int32_t *fallback_count, hipStream_t st, void *work = nullptr, size_t work_bytes = 0);
 size_t nbx_mixed_scratch_bytes(int nd, int kh);
+// FP64 d2 of every entry of the n rows nn (by ID, layout 0, or by slot,
+// layout 1) written by the precision-form kernel, rows re-sorted ascending
+// by (d2, ID); k <= 64.
+hipError_t nbx_launch_rows_fp64(const DevTree &T, int32_t *nn, double *d2, int k, int layout, hipStream_t st);
 
 // ------------------------------------------------------------------ FOF
 // Link rule of a FOF run: pairs link when their value is below lim.  Plain

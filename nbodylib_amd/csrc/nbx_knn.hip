@@ -60,8 +60,9 @@ __device__ __forceinline__ void max4(const double *d, const int32_t *id, double 
 // PriorityQueue, PriorityQueue.h:47-84, with three dependent levels instead
 // of five for k = 32).  Replace the root of the full heap of hc entries by
 // (dv, iv) for the lanes with ins set and sift it down; the other lanes
-// rewrite the root with itself.  Three unrolled levels cover hc <= 85 and
-// nothing branches.  (kth, kid) is the root afterwards.
+// rewrite the root with itself.  LV unrolled levels cover hc <= 85 (LV = 3)
+// or hc <= 341 (LV = 4) and nothing branches.  (kth, kid) is the root afterwards.
+template <int LV>
 __device__ __forceinline__ void heap_replace_root(double *hd, int32_t *hid, int lane, int hc, bool ins, double dn,
                                                   int32_t in, double &kth, int32_t &kid) {
     const double dv = ins ? dn : kth;
@@ -71,7 +72,7 @@ __device__ __forceinline__ void heap_replace_root(double *hd, int32_t *hid, int 
     int p = 0;
     bool go = ins;
 #pragma unroll
-    for (int lv = 0; lv < 3; ++lv) {
+    for (int lv = 0; lv < LV; ++lv) {
         const int c = 4 * p + 1;
         go = go && c < hc;
         const int cc = go ? c : 1;
@@ -97,11 +98,12 @@ __device__ __forceinline__ void heap_replace_root(double *hd, int32_t *hid, int 
 
 // Sift (dv, iv) down from the root of a heap of m <= hc entries; children
 // at positions >= m read as (-inf, min) and never outrank an entry.
+template <int LV>
 __device__ __forceinline__ void heap_sift_bounded(double *hd, int32_t *hid, int lane, int m, double dv, int32_t iv) {
     int p = 0;
     bool go = true;
 #pragma unroll
-    for (int lv = 0; lv < 3; ++lv) {
+    for (int lv = 0; lv < LV; ++lv) {
         const int c = 4 * p + 1;
         go = go && c < m;
         const int cc = go ? c : 1;
@@ -150,15 +152,23 @@ __device__ __forceinline__ void heap_sift(double *hd, int32_t *hid, int lane, in
 }
 
 // Heap sort in place: positions 0..hc-1 ascending afterwards (fillers first).
+template <int LV>
 __device__ __forceinline__ void heap_sort(double *hd, int32_t *hid, int lane, int hc) {
     for (int m = hc - 1; m > 0; --m) {
         const double dt = HD(0), dm = HD(m);
         const int32_t it = HI(0), im = HI(m);
         HD(m) = dt;
         HI(m) = it;
-        heap_sift_bounded(hd, hid, lane, m, dm, im);
+        heap_sift_bounded<LV>(hd, hid, lane, m, dm, im);
     }
 }
+
+// heap levels below the root that a capacity of KMAX entries needs
+template <int KMAX>
+struct HeapLv {
+    static constexpr int LV = KMAX <= 85 ? 3 : 4;
+    static_assert(KMAX <= 341, "four levels hold at most 341 entries");
+};
 
 template <int ND>
 struct LeafCtx {
@@ -170,7 +180,7 @@ struct LeafCtx {
 };
 
 // Consume one leaf for every active lane (two-phase insertion, see header).
-template <int ND>
+template <int ND, int LV>
 __device__ __forceinline__ void knn_leaf(const DevTree &T, int leaf, LeafCtx<ND> &c, double *hd, int32_t *hid,
                                          double *cx, int32_t *cid, int lane, int hc) {
     const int base = leaf * NBX_LEAF;
@@ -218,7 +228,7 @@ __device__ __forceinline__ void knn_leaf(const DevTree &T, int leaf, LeafCtx<ND>
             d2 += t * t;
         }
         const int32_t id = cid[j];
-        heap_replace_root(hd, hid, lane, hc, has && kgt(c.kth, c.kid, d2, id), d2, id, c.kth, c.kid);
+        heap_replace_root<LV>(hd, hid, lane, hc, has && kgt(c.kth, c.kid, d2, id), d2, id, c.kth, c.kid);
     }
     __syncthreads();
 }
@@ -231,7 +241,7 @@ __device__ __forceinline__ void knn_leaf(const DevTree &T, int leaf, LeafCtx<ND>
 // opened when any active lane has box_lb <= its current k-th distance.
 // State is uniform (SGPRs): node, depth and one "right child first" bit per
 // depth.
-template <int ND>
+template <int ND, int LV>
 __device__ __forceinline__ void knn_walk(const DevTree &T, LeafCtx<ND> &c, const float *qc, int anc, double *hd,
                                          int32_t *hid, double *cx, int32_t *cid, int lane, int hc) {
     const int D = T.depth;
@@ -244,7 +254,7 @@ __device__ __forceinline__ void knn_walk(const DevTree &T, LeafCtx<ND> &c, const
             bool want = c.act && box_lb<ND>(c.q, bx) <= c.kth;
             if (ballot(want)) {
                 if (dep == D) {
-                    knn_leaf<ND>(T, node - T.nleafp, c, hd, hid, cx, cid, lane, hc);
+                    knn_leaf<ND, LV>(T, node - T.nleafp, c, hd, hid, cx, cid, lane, hc);
                     down = false;
                 } else {
                     int c0 = 2 * node;
@@ -356,7 +366,7 @@ __global__ __launch_bounds__(64) void k_knn(DevTree T, KnnArgs A) {
 #pragma unroll
         for (int d = 0; d < ND; ++d) qc[d] = (float)readfirst_d(c.q[d]);
         const int anc = (pass == 0 && QSRC != NBX_Q_POINTS && !listed) ? T.nleafp + 2 * g : -1;
-        knn_walk<ND>(T, c, qc, anc, hd, hid, cx, cid, lane, hc);
+        knn_walk<ND, HeapLv<KMAX>::LV>(T, c, qc, anc, hd, hid, cx, cid, lane, hc);
     }
     const double kth = c.kth;
 
@@ -383,7 +393,7 @@ __global__ __launch_bounds__(64) void k_knn(DevTree T, KnnArgs A) {
             const double qvx = pts ? A.qv[0][qi] : T.x[3][qi];
             const double qvy = pts ? A.qv[1][qi] : T.x[4][qi];
             const double qvz = pts ? A.qv[2][qi] : T.x[5][qi];
-            heap_sort(hd, hid, lane, hc);
+            heap_sort<HeapLv<KMAX>::LV>(hd, hid, lane, hc);
             for (int p = 0; p < kh; ++p) {
                 const int32_t id = HI(p + nfill);
                 if (id == NBX_NOID) { bad = true; break; }
@@ -474,7 +484,7 @@ __global__ __launch_bounds__(64) void k_knn(DevTree T, KnnArgs A) {
     // ------------------------------------------------------------ outputs
     if (A.nn == nullptr && A.d2 == nullptr) return;
     // (the velocity density sorted already and reused the entries)
-    if (valid && A.dens != NBX_DO_VEL && A.dens != NBX_DO_VELPHYS) heap_sort(hd, hid, lane, hc);
+    if (valid && A.dens != NBX_DO_VEL && A.dens != NBX_DO_VELPHYS) heap_sort<HeapLv<KMAX>::LV>(hd, hid, lane, hc);
     __syncthreads();
     const int kout = A.kout;
     const int off = ((A.drop == NBX_DROP_SMALLEST) ? 1 : 0) + ((A.dens == NBX_DO_VEL || A.dens == NBX_DO_VELPHYS) ? 0 : nfill);
@@ -516,10 +526,12 @@ hipError_t nbx_launch_knn(const DevTree &T, const KnnArgs &A, int nqueries, hipS
     ProfScope ps(A.qlist ? "k_knn_fallback" : "k_knn_fp64", st);
     if (T.nd == 6) {
         if (A.kh <= 65) launch_q<6, 65>(T, A, blocks, st);
+        else if (A.kh <= 129) launch_q<6, 129>(T, A, blocks, st);
         else return hipErrorInvalidValue;
     } else {
         if (A.kh <= 33) launch_q<3, 33>(T, A, blocks, st);
         else if (A.kh <= 65) launch_q<3, 65>(T, A, blocks, st);
+        else if (A.kh <= 129) launch_q<3, 129>(T, A, blocks, st);
         else return hipErrorInvalidValue;
     }
     return hipGetLastError();

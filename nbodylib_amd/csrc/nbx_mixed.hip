@@ -525,8 +525,10 @@ __device__ __forceinline__ void mix_task(const DevTree &T, const KnnArgs &A, con
         }
         if ((SKC(kh - 1) & ~GShape<KMAX>::TM) == (MX_EMPTY & ~GShape<KMAX>::TM)) ok = false;  // < k+1 candidates
     }
-    // FP32 search error plus the key's position bits
-    const double eu = (12.0 + (double)(1 << GShape<KMAX>::TB)) * MX_U, ea = 5.0 * MX_U * c.emax;
+    // FP32 search error plus the key's position bits, or the caller's
+    // re-check band where that is wider (a wider band only sends more
+    // entries to the FP64 re-check and the exact verification)
+    const double eu = fmax((12.0 + (double)(1 << GShape<KMAX>::TB)) * MX_U, A.band_rel), ea = 5.0 * MX_U * c.emax;
     int i0 = k - 1;
     int32_t bs[4];
     uint32_t bk[4];
@@ -830,4 +832,61 @@ hipError_t nbx_launch_knn_mixed(const DevTree &T, const KnnArgs &A, const double
     e = hipGetLastError();
     if (!work) hipFreeAsync(mem, st);
     return e;
+}
+
+// FP64 distances for rows written by the precision-form kernel, whose rows
+// are in FP32 key order: each row's entries get their FP64 d2 (summed over
+// the dimensions in order, no FMA, as every FP64 distance here) and are
+// sorted ascending by (d2, ID) -- the LoadNN order of the FP64 form
+// (KDFindNearest.cxx:11-24) with ties to the smaller ID.  Rows and entries
+// are by ID (layout 0) or by slot (layout 1).  One thread per row.
+template <int ND, int KM>
+__global__ __launch_bounds__(256) void k_rows_fp64(DevTree T, int32_t *nn, double *d2, int k, int layout) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= T.n) return;
+    const int32_t qs = layout == 0 ? T.inv[r] : (int32_t)r;
+    double q[ND];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) q[d] = T.x[d][qs];
+    double dv[KM];
+    int32_t iv[KM], ev[KM];
+    int32_t *row = nn + r * k;
+    for (int j = 0; j < k; ++j) {
+        const int32_t e = row[j];
+        // a missing neighbour (-1) keeps d2 = MAXVALUE and sorts last
+        const bool miss = e < 0;
+        const int32_t s = miss ? 0 : (layout == 0 ? T.inv[e] : e);
+        const double v = miss ? NBX_MAXVALUE : d64<ND>(T, q, s);
+        const int32_t id = miss ? 0x7fffffff : (layout == 0 ? e : T.perm[e]);
+        // insertion into the sorted prefix (the rows come nearly sorted)
+        int p = j;
+        while (p > 0 && (dv[p - 1] > v || (dv[p - 1] == v && iv[p - 1] > id))) {
+            dv[p] = dv[p - 1];
+            iv[p] = iv[p - 1];
+            ev[p] = ev[p - 1];
+            --p;
+        }
+        dv[p] = v;
+        iv[p] = id;
+        ev[p] = e;
+    }
+    for (int j = 0; j < k; ++j) {
+        row[j] = ev[j];
+        d2[r * k + j] = dv[j];
+    }
+}
+
+hipError_t nbx_launch_rows_fp64(const DevTree &T, int32_t *nn, double *d2, int k, int layout, hipStream_t st) {
+    if (T.n == 0) return hipSuccess;
+    if (k < 1 || k > 64) return hipErrorInvalidValue;
+    const int blocks = (int)((T.n + 255) / 256);
+    ProfScope ps("k_rows_fp64", st);
+    if (T.nd == 3) {
+        if (k <= 32) hipLaunchKernelGGL((k_rows_fp64<3, 32>), dim3(blocks), dim3(256), 0, st, T, nn, d2, k, layout);
+        else hipLaunchKernelGGL((k_rows_fp64<3, 64>), dim3(blocks), dim3(256), 0, st, T, nn, d2, k, layout);
+    } else {
+        if (k <= 32) hipLaunchKernelGGL((k_rows_fp64<6, 32>), dim3(blocks), dim3(256), 0, st, T, nn, d2, k, layout);
+        else hipLaunchKernelGGL((k_rows_fp64<6, 64>), dim3(blocks), dim3(256), 0, st, T, nn, d2, k, layout);
+    }
+    return hipGetLastError();
 }
