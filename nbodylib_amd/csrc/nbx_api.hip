@@ -1056,9 +1056,9 @@ static int fof_impl(nbx_tree *t, int metric, const FofLink &link, int32_t minnum
                     int64_t *ngroups, hipStream_t st) {
     const int64_t n = t->n;
     int32_t *P = nullptr;
-    size_t bytes = sizeof(int32_t) * n + (size_t)t->nnodes + 16;
+    DevTree T = t->view();
     DevBufs bufs(st);
-    HIPCK(bufs.alloc(&P, bytes));
+    HIPCK(bufs.alloc(&P, nbx_fof_work_bytes(T)));
     FofArgs A;
     memset(&A, 0, sizeof(A));
     A.link = link;
@@ -1066,7 +1066,6 @@ static int fof_impl(nbx_tree *t, int metric, const FofLink &link, int32_t minnum
     A.periodic = t->p.periodic && n > t->p.bucket_size;
     for (int d = 0; d < 3; ++d) A.period[d] = t->p.period[d];
     A.parent = P;
-    DevTree T = t->view();
     hipError_t e = nbx_launch_fof(T, A, st);
     if (e == hipSuccess) e = nbx_fof_canonical(T, P, minnum, labels_by_id_dev, ngroups, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);

@@ -112,7 +112,9 @@ struct FofArgs {
     double period[3];
     int32_t *parent;
 };
+// A.parent: nbx_fof_work_bytes(T) bytes (parents, node flags, queue counters)
 hipError_t nbx_launch_fof(const DevTree &T, const FofArgs &A, hipStream_t st);
+size_t nbx_fof_work_bytes(const DevTree &T);
 hipError_t nbx_launch_fof_finish(const DevTree &T, int32_t *parent, int32_t *count, int32_t *minid, int64_t n,
                                  hipStream_t st);
 
