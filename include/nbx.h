@@ -85,7 +85,10 @@ int nbx_tree_create(const nbx_params *p, const double *pos, const double *vel, c
                     nbx_tree **out);
 int nbx_tree_create_dev(const nbx_params *p, const double *pos_dev, const double *vel_dev, const double *mass_dev,
                         int64_t n, void *stream, nbx_tree **out);
-/* Replaces KDTree::~KDTree (KDTree.cxx:1340-1356); caller-side reorder is the shim's job. */
+/* Replaces KDTree::~KDTree (KDTree.cxx:1340-1356); caller-side reorder is the
+ * shim's job.  The tree's arrays go back with hipFree, which waits for work
+ * still running on the device, so no launch on any stream may still be using
+ * the tree. */
 int nbx_tree_destroy(nbx_tree *t);
 /* slot -> ID map (host array of n).  The C++ shim uses it to reorder the
  * caller's Particle array as the reference does in BuildNodes. */
@@ -269,7 +272,10 @@ int nbx_fof_dev(nbx_tree *t, double fdist, int32_t minnum, int32_t *labels_by_id
  * slot or -1, tail[head] = its last slot; a particle in no group is a list
  * of its own (head = tail = itself).  len[g] = the size of group g (1..G),
  * other entries 0.  Any output may be NULL.  The reference's lists run in
- * its search order. */
+ * its search order.  The reference also keeps the members of groups pruned
+ * below minnum linked (KDFOF.cxx:98-104): pass the labels of a minnum-1 run
+ * for those lists; its groups of >= minnum members are its first labels
+ * (canonical order is by size), the rest become 0. */
 int nbx_fof_lists(const int32_t *labels_by_id, const int32_t *slot_to_id, int64_t n, int64_t ngroups, int32_t *head,
                   int32_t *next, int32_t *tail, int32_t *len);
 #define NBX_FOF_CMP_3D 1

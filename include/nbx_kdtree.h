@@ -379,8 +379,11 @@ class KDTree {
     // size, descending; ties by smallest member ID).  With ipcheckflag, the
     // particles whose check(p, params) is nonzero take no part in the links
     // and keep that value as their label, -1 becoming 0 (KDFOF.cxx:63-64,
-    // :107).  The linked lists, when given, hold the groups' members in
-    // tree-index order; a particle in no group is a list of its own.
+    // :107).  The linked lists, when given, hold every linked component's
+    // members in tree-index order, including those of components pruned
+    // below minnum, as the reference leaves them (KDFOF.cxx:98-104); a
+    // particle linked to none is a list of its own.  pLen[g] is the size of
+    // group g for g = 1..numgroup and 0 after it.
     Int_t *FOF(Double_t fdist, Int_t &numgroup, Int_t minnum = 8, int order = 0, Int_tree_t *pHead = nullptr,
                Int_tree_t *pNext = nullptr, Int_tree_t *pTail = nullptr, Int_tree_t *pLen = nullptr,
                int ipcheckflag = 0, FOFcheckfunc check = Pnocheck, Double_t *params = nullptr);

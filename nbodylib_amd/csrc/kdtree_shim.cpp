@@ -558,11 +558,17 @@ Int_t *KDTree::fof_run(int cmp, Double_t fdist, Double_t *params, Int_t &numgrou
         }
         if (!any) val.clear();
     }
+    // With lists requested every component is found (minnum 1): the
+    // reference leaves the members of groups pruned below minnum linked
+    // (KDFOF.cxx:98-104).  Canonical labels run by size descending, so the
+    // groups of >= minnum members are the first ones and keep their labels.
+    const bool lists = pHead || pNext || pTail || pLen;
+    const Int_t mrun = lists ? 1 : minnum;
     Int_t *pGroup = new Int_t[numparts];
     int64_t ng = 0;
     int rc = NBX_OK;
     if (val.empty()) {
-        rc = cmp ? nbx_fof_criterion(tree, cmp, params, minnum, pGroup, &ng) : nbx_fof(tree, fdist, minnum, pGroup, &ng);
+        rc = cmp ? nbx_fof_criterion(tree, cmp, params, mrun, pGroup, &ng) : nbx_fof(tree, fdist, mrun, pGroup, &ng);
     } else {
         std::vector<Int_t> act;  // IDs taking part, ascending (so canonical ties follow the IDs)
         for (Int_t i = 0; i < numparts; ++i)
@@ -587,8 +593,8 @@ Int_t *KDTree::fof_run(int cmp, Double_t fdist, Double_t *params, Int_t &numgrou
             rc = nbx_tree_create(&sp, pos.data(), vel.data(), mass.data(), m, &sub);
             std::vector<int32_t> lab(m, 0);
             if (rc == NBX_OK)
-                rc = cmp ? nbx_fof_criterion(sub, cmp, params, minnum, lab.data(), &ng)
-                         : nbx_fof(sub, fdist, minnum, lab.data(), &ng);
+                rc = cmp ? nbx_fof_criterion(sub, cmp, params, mrun, lab.data(), &ng)
+                         : nbx_fof(sub, fdist, mrun, lab.data(), &ng);
             if (sub) nbx_tree_destroy(sub);
             if (rc == NBX_OK)
                 for (int64_t j = 0; j < m; ++j) pGroup[act[j]] = lab[j];
@@ -596,13 +602,24 @@ Int_t *KDTree::fof_run(int cmp, Double_t fdist, Double_t *params, Int_t &numgrou
     }
     if (rc < 0) delete[] pGroup;
     check(rc, what);
-    numgroup = (Int_t)ng;
-    // pHead/pNext/pTail/pLen (KDFOF.cxx:29-31), by tree index, from the groups
-    if (pHead || pNext || pTail || pLen) {
+    // pHead/pNext/pTail/pLen (KDFOF.cxx:29-31), by tree index, from every
+    // component; then the groups below minnum are dropped from the labels
+    // and pLen (the reference zeroes a pruned group's pLen, KDFOF.cxx:103)
+    if (lists) {
         std::vector<int32_t> order(numparts);
         check(nbx_tree_order(tree, order.data()), what);
         check(nbx_fof_lists(pGroup, order.data(), numparts, ng, pHead, pNext, pTail, pLen), what);
+        std::vector<int64_t> size((size_t)ng + 1, 0);
+        for (Int_t i = 0; i < numparts; ++i) size[pGroup[i]]++;
+        int64_t kept = 0;
+        while (kept < ng && size[kept + 1] >= minnum) ++kept;
+        for (Int_t i = 0; i < numparts; ++i)
+            if (pGroup[i] > kept) pGroup[i] = 0;
+        if (pLen)
+            for (int64_t g = kept + 1; g <= ng && g < numparts; ++g) pLen[g] = 0;
+        ng = kept;
     }
+    numgroup = (Int_t)ng;
     // particles left out keep their check value, -1 becoming 0 (KDFOF.cxx:63-64, :107)
     if (!val.empty())
         for (Int_t i = 0; i < numparts; ++i)

@@ -156,6 +156,7 @@ struct nbx_tree {
     float *plane = nullptr;
     double *org = nullptr;
     float *xf[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    float4 *xq = nullptr;  // precision form, 3-D: FP32 offsets and their squared norms, interleaved
     double *kern = nullptr;
     float *kernf = nullptr;
     double *ext = nullptr;
@@ -172,6 +173,7 @@ struct nbx_tree {
         T.nleaf = nleaf;
         T.nd = nd;
         for (int d = 0; d < 6; ++d) { T.x[d] = xs[d]; T.xf[d] = xf[d]; }
+        T.xq = xq;
         T.mass = mass;
         T.perm = perm;
         T.inv = inv;
@@ -238,6 +240,7 @@ static void free_tree(nbx_tree *t) {
     if (t->kern) hipFree(t->kern);
     if (t->kernf) hipFree(t->kernf);
     if (t->ext) hipFree(t->ext);
+    if (t->xq) hipFree(t->xq);
     delete t;
 }
 
@@ -289,7 +292,8 @@ static int build_tree(nbx_tree *t, const double *pos, const double *vel, const d
     if (t->p.form == NBX_FORM_MIXED) {
         HIPCK(hipMalloc((void **)&t->org, sizeof(double) * nd * (size_t)t->nleaf));
         HIPCK(hipMalloc((void **)&t->ext, sizeof(double) * (size_t)t->nleaf));
-        for (int d = 0; d < nd; ++d) HIPCK(hipMalloc((void **)&t->xf[d], sizeof(float) * npad));
+        if (nd == 3) HIPCK(hipMalloc((void **)&t->xq, sizeof(float4) * npad));
+        else for (int d = 0; d < nd; ++d) HIPCK(hipMalloc((void **)&t->xf[d], sizeof(float) * npad));
     }
     {
         ProfScope ps("k_build_gather", st);
@@ -312,6 +316,7 @@ static int build_tree(nbx_tree *t, const double *pos, const double *vel, const d
         g.org = t->org;
         g.ext = t->ext;
         for (int d = 0; d < nd; ++d) g.xf[d] = t->xf[d];
+        g.xq = t->xq;
         hipLaunchKernelGGL(k_gather_leaf, dim3(nblk(npad)), dim3(256), 0, st, g);
     }
     {
@@ -406,7 +411,6 @@ extern "C" int nbx_tree_create(const nbx_params *p, const double *pos, const dou
 
 extern "C" int nbx_tree_destroy(nbx_tree *t) {
     if (!t) return NBX_OK;
-    hipDeviceSynchronize();
     free_tree(t);
     return NBX_OK;
 }

@@ -90,9 +90,16 @@ __global__ __launch_bounds__(256) void k_gather_leaf(GatherArgs a) {
         if (a.ext) a.ext[L] = sqrt(e2) * (1.0 + 1e-12);
     }
     if (a.org) {
+        float f[6];
         for (int d = 0; d < a.nd; ++d) {
             const double o = 0.5 * (mn[d] + mx[d]);  // the same in every lane of the leaf
-            a.xf[d][s] = real ? (float)(v[d] - o) : NBX_PAD_COORD_F;
+            f[d] = real ? (float)(v[d] - o) : NBX_PAD_COORD_F;
+        }
+        if (a.nd == 3) {
+            const float f2 = __builtin_fmaf(f[2], f[2], __builtin_fmaf(f[1], f[1], f[0] * f[0]));
+            a.xq[s] = make_float4(f[0], f[1], f[2], real ? f2 : 1.0f / 0.0f);
+        } else {
+            for (int d = 0; d < a.nd; ++d) a.xf[d][s] = f[d];
         }
     }
 }

@@ -43,7 +43,8 @@ struct DevTree {
     const double *box;     // node boxes, [node][lo[nd], hi[nd]]
     const float *plane;    // internal nodes: child-order plane [node][n[nd], off] (4 or 8 floats)
     const double *org;     // mixed form: per-leaf FP64 origin [leaf][nd]
-    const float *xf[6];    // mixed form: FP32 offsets from the leaf origin
+    const float *xf[6];    // mixed form, 6-D: FP32 offsets from the leaf origin
+    const float4 *xq;      // mixed form, 3-D: (x, y, z, |.|^2) FP32 offsets per slot (padding: 1e30, +inf)
     const double *kern;    // kernel table
     const float *kernf;    // kernel table in FP32 (mixed form)
     int32_t kernres;

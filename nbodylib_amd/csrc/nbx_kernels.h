@@ -23,7 +23,8 @@ struct GatherArgs {
     double *box;
     double *org;  // nullable: leaf centre (precision-form origin)
     double *ext;  // nullable: leaf half-diagonal (precision-form error bound)
-    float *xf[6];  // precision form: FP32 offsets from the leaf origin
+    float *xf[6];  // precision form, 6-D: FP32 offsets from the leaf origin
+    float4 *xq;    // precision form, 3-D: the offsets and fl(fl(fl(x^2) + y^2) + z^2) interleaved
 };
 
 __global__ void k_bbox(const double *c, int64_t n, double *part);

@@ -56,7 +56,7 @@ typedef const __attribute__((address_space(4))) fx8 *cf8ptr;
 #define HK(p) hk[(p) * NBX_WAVE + lane]   // kept keys (LDS)
 #define SLA(p) sl[(p) * NBX_WAVE + lane]  // their slots (scratch, by position; KMAX > 33)
 #define SLH(p) ((uint16_t *)sl)[(p) * NBX_WAVE + lane]  // their handles (scratch, by position; KMAX <= 33)
-#define MX_V 2048  // leaf visits a wave can record: handle = visit << 5 | index in leaf
+#define MX_V 2048 // leaf visits a wave can record: handle = visit << 5 | index in leaf
 
 // Kept keys.  The kh = k+1 kept candidates of a lane sit at positions
 // p < kh of KMAX = G * S (G groups of S positions; positions p >= kh hold
@@ -176,8 +176,15 @@ __device__ __forceinline__ void mix_leaf(const DevTree &T, const double *ext, in
     // candidate (lane mod 32) of this leaf: the tested point in phase 1 and
     // this lane's part of the LDS copy used in phase 2
     float st[ND];
+    if constexpr (ND == 3) {
+        const float4 v = T.xq[base + (lane & (NBX_LEAF - 1))];
+        st[0] = v.x;
+        st[1] = v.y;
+        st[2] = v.z;
+    } else {
 #pragma unroll
-    for (int d = 0; d < ND; ++d) st[d] = T.xf[d][base + (lane & (NBX_LEAF - 1))];
+        for (int d = 0; d < ND; ++d) st[d] = T.xf[d][base + (lane & (NBX_LEAF - 1))];
+    }
     // query relative to this leaf's centre, rounded once to FP32
     const double *o = T.org + (size_t)leaf * ND;
     float qv[ND];
@@ -186,28 +193,46 @@ __device__ __forceinline__ void mix_leaf(const DevTree &T, const double *ext, in
     const float th = __uint_as_float(c.root);
     uint32_t mask = 0u;
     if constexpr (ND == 3) {
-        // Phase 1: every lane tests the 32 candidates, read in 32-byte scalar
-        // loads per dimension (cheaper than the transposed form below at 3
-        // flops per dimension)
-        typedef float fxc __attribute__((ext_vector_type(8)));
+        // Phase 1: every lane tests the 32 candidates, read in 64-byte scalar
+        // loads, in the expanded form
+        //     w + 2 qv.p >= |p|^2,  w = th + m - |qv|^2
+        // (|p|^2 stored per slot): three FMAs and a compare per candidate,
+        // the accept bit shifted in with a carry.  In exact arithmetic this
+        // is |qv - p|^2 <= th + m.  Phase 2 accepts only candidates whose
+        // difference-form d2 (3 roundings, relative error < 6u, u = 2^-24)
+        // is <= th, i.e. with |qv - p|^2 <= th (1 + 6u); the rounding of the
+        // expanded form is below 11u th + 5u m + 7u (|qv| + |p|)^2, so the
+        // margin m = 2^-20 (th + (|qv| + e)^2) (e >= |p|: the leaf's
+        // half-diagonal) lets every such candidate through.  The few extra
+        // ones fail phase 2's test, so the kept keys are those of the
+        // difference form alone.
+        typedef float fxc __attribute__((ext_vector_type(16)));
         typedef const __attribute__((address_space(4))) fxc *cfcptr;
+        const float q2 = __builtin_fmaf(qv[2], qv[2], __builtin_fmaf(qv[1], qv[1], qv[0] * qv[0]));
+        const float r = sqrtf(q2) + (float)(((cdptr)ext)[leaf] * (1.0 + 0x1p-20));
+        const float m = 0x1p-20f * __builtin_fmaf(r, r, th);
+        // FLT_MAX (no bound yet): every real candidate passes
+        const float w = fminf((th + m) - q2, 3.402823466e38f);
+        const float a0 = 2.0f * qv[0], a1 = 2.0f * qv[1], a2 = 2.0f * qv[2];
+        // two halves of 16 candidates (x, y, z, |p|^2), 64 scalar registers
+        // each (the whole leaf at once would not fit the scalar register file)
+        const cfcptr lp = (cfcptr)(T.xq + base);
+#pragma unroll 1
+        for (int h = 0; h < 2; ++h) {
+            fxc c4[4];
 #pragma unroll
-        for (int jj = 0; jj < NBX_LEAF; jj += 8) {
-            fxc pv[ND];
+            for (int i = 0; i < 4; ++i) c4[i] = lp[4 * h + i];
 #pragma unroll
-            for (int d = 0; d < ND; ++d) pv[d] = *(cfcptr)(T.xf[d] + base + jj);
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                float t = qv[0] - pv[0][u];
-                float dd = t * t;
-#pragma unroll
-                for (int d = 1; d < ND; ++d) {
-                    t = qv[d] - pv[d][u];
-                    dd = __builtin_fmaf(t, t, dd);
-                }
-                mask |= (dd <= th) ? (1u << (jj + u)) : 0u;
+            for (int u = 0; u < 16; ++u) {
+                const int i = u >> 2, o = (u & 3) * 4;
+                float v = __builtin_fmaf(a0, c4[i][o], w);
+                v = __builtin_fmaf(a1, c4[i][o + 1], v);
+                v = __builtin_fmaf(a2, c4[i][o + 2], v);
+                mask = mask + mask + (v >= c4[i][o + 3] ? 1u : 0u);
             }
         }
+        // candidate j was shifted in at bit 31 - j
+        mask = __builtin_bitreverse32(mask);
     } else {
         // Phase 1, transposed: the lanes that want this leaf put their query
         // offsets and thresholds in LDS (qb, tb, which alias cx), and each

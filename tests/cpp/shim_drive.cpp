@@ -12,7 +12,8 @@
 //          order); n int32 FOF labels; n int32 FOF labels with the check
 //          (every 5th PID left out); n int32 ball-row counts (by ID), then the
 //          rows as IDs (by ID order); 8 points x k neighbour IDs of
-//          FindNearest(x); int32 status flags[8]
+//          FindNearest(x); int32 status flags[8]; n int32 FOF list heads
+//          (the ID at the head of each ID's list)
 // It returns 0, or a nonzero code after printing what failed; errors of the
 // tree are thrown (SetThrowOnError), so a harness in another program survives.
 #include <cstdio>
@@ -46,7 +47,7 @@ static int drive(int argc, char **argv) {
         parts[i] = Particle(r[6], r[0], r[1], r[2], r[3], r[4], r[5], 12345, 1, 0.0, 0.0, i);
     }
     double period[3] = {L, L, L};
-    std::vector<int> ids((size_t)n * k), flags(8, 0), lab1(n), lab2(n), ballcnt(n);
+    std::vector<int> ids((size_t)n * k), flags(8, 0), lab1(n), lab2(n), ballcnt(n), lhead(n);
     std::vector<std::vector<int>> ballrow(n);
     std::vector<int> ptnn(8 * (size_t)k);
     std::vector<double> rho(n);
@@ -110,25 +111,29 @@ static int drive(int argc, char **argv) {
         int *l1 = tree.FOF(b, ngroup, 8, 0, head.data(), next.data(), tail.data(), len.data());
         for (int i = 0; i < n; ++i) lab1[i] = l1[i];
         delete[] l1;
-        // the lists: each group once from its head, in tree-index order, of its length
+        // the lists: every linked component once from its head, in tree-index
+        // order; a group's list holds its label and pLen members, a pruned
+        // component's (label 0) fewer than minnum = 8
         {
             std::vector<int> seen(ngroup + 1, 0);
-            int lbad = 0;
+            int lbad = 0, listed = 0;
             for (int s = 0; s < n; ++s) {
-                const int g = lab1[parts[s].GetID()];
-                if (g == 0) { lbad += head[s] != s || tail[s] != s || next[s] != -1; continue; }
+                lhead[parts[s].GetID()] = parts[head[s]].GetID();
                 if (head[s] != s) continue;
-                if (seen[g]++) ++lbad;
+                const int g = lab1[parts[s].GetID()];
+                if (g > 0 && seen[g]++) ++lbad;
                 int c = 0, last = -1;
                 for (int e = s; e != -1; e = next[e]) {
-                    lbad += lab1[parts[e].GetID()] != g || e <= last;
+                    lbad += lab1[parts[e].GetID()] != g || e <= last || head[e] != s;
                     last = e;
                     ++c;
                 }
-                lbad += c != len[g] || tail[s] != last;
+                listed += c;
+                lbad += tail[s] != last || (g > 0 ? c != len[g] : c >= 8);
             }
             for (int g = 1; g <= ngroup; ++g) lbad += seen[g] != 1;
-            flags[5] = lbad;
+            for (int g = ngroup + 1; g < n; ++g) lbad += len[g] != 0;
+            flags[5] = lbad + (listed != n);
         }
         fprintf(stderr, "stage: FOF with a check function\n");
         int *l2 = tree.FOF(b, ngroup2, 8, 0, nullptr, nullptr, nullptr, nullptr, 1, skip_fifth, nullptr);
@@ -164,6 +169,7 @@ static int drive(int argc, char **argv) {
         if (!ballrow[i].empty()) fwrite(ballrow[i].data(), 4, ballrow[i].size(), o);
     fwrite(ptnn.data(), 4, ptnn.size(), o);
     fwrite(flags.data(), 4, flags.size(), o);
+    fwrite(lhead.data(), 4, n, o);
     if (fclose(o) != 0) return 4;
     fprintf(stderr, "shim driver ok n=%d k=%d groups=%d flags=%d %d %d %d %d\n", n, k, ngroup, flags[0], flags[1],
             flags[2], flags[4], flags[5]);

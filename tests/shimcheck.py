@@ -38,6 +38,7 @@ def read_output(path):
         out["ball"] = [np.sort(flat[off[i]:off[i + 1]]) for i in range(n)]
         out["ptnn"] = np.fromfile(f, np.int32, 8 * k).reshape(8, k)
         out["flags"] = np.fromfile(f, np.int32, 8)
+        out["lhead"] = np.fromfile(f, np.int32, n)
     return out
 
 
@@ -57,6 +58,14 @@ def check(out, pos, m, k, b, periodic, rball):
     assert np.max(np.abs(out["rho"] / ro - 1)) < 1e-12, "CalcDensity"
     ol, og = oracle.fof_brute(pos, b, 8, period=per)
     assert fl[3] == og and np.array_equal(out["lab"], ol), "FOF labels"
+    # the linked lists partition the particles into every linked component,
+    # the groups pruned below minnum included (KDFOF.cxx:98-104)
+    al, ag = oracle.fof_brute(pos, b, 1, period=per)
+    hl = out["lhead"]
+    assert np.array_equal(al[hl], al), "FOF list heads outside their component"
+    assert len(np.unique(hl)) == ag, "FOF lists: one per linked component"
+    sz = np.bincount(al)[1:]
+    assert ((sz > 1) & (sz < 8)).any(), "the case has pruned components of several members"
     # FOF with every fifth particle left out (check = -1 -> label 0)
     keep = np.arange(n) % 5 != 0
     sl, sg = oracle.fof_brute(pos[keep], b, 8, period=per)
