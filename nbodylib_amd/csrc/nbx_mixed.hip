@@ -82,6 +82,46 @@ struct GShape {
 
 __device__ __forceinline__ uint32_t umax3(uint32_t a, uint32_t b, uint32_t c) { return max(max(a, b), c); }
 
+// Batcher's odd-even merge sorting network over 64 inputs, keeping only the
+// comparators whose inputs are both below lim (the inputs from lim on are
+// +inf, which those comparators would leave in place).  Built at compile
+// time; a fully unrolled pass over it sorts keys held in registers.
+struct SortNet {
+    int n;
+    uint8_t a[256], b[256];
+};
+constexpr void sn_add(SortNet &s, int i, int j, int lim) {
+    if (j < lim) {
+        s.a[s.n] = (uint8_t)i;
+        s.b[s.n] = (uint8_t)j;
+        s.n += 1;
+    }
+}
+constexpr void sn_merge(SortNet &s, int lo, int hi, int r, int lim) {
+    const int step = 2 * r;
+    if (step < hi - lo) {
+        sn_merge(s, lo, hi, step, lim);
+        sn_merge(s, lo + r, hi, step, lim);
+        for (int i = lo + r; i < hi - r; i += step) sn_add(s, i, i + r, lim);
+    } else {
+        sn_add(s, lo, lo + r, lim);
+    }
+}
+constexpr void sn_sort(SortNet &s, int lo, int hi, int lim) {
+    if (hi - lo >= 1) {
+        const int mid = lo + (hi - lo) / 2;
+        sn_sort(s, lo, mid, lim);
+        sn_sort(s, mid + 1, hi, lim);
+        sn_merge(s, lo, hi, 1, lim);
+    }
+}
+constexpr SortNet sort_net(int lim) {
+    SortNet s{};
+    sn_sort(s, 0, 63, lim);
+    return s;
+}
+constexpr SortNet MX_NET33 = sort_net(33);  // 246 comparators
+
 // Largest of S keys (a max tree).
 template <int S>
 __device__ __forceinline__ uint32_t gmaxu(const uint32_t (&x)[S]) {
@@ -157,6 +197,7 @@ struct MixCtx {
     uint32_t root;  // largest kept key
     double emax;    // largest leaf half-diagonal that matters for this lane
     int nvis;       // leaves visited (wave-uniform), the next handle's visit number
+    int fillkh;     // kh while the heap is still empty and kh >= 32 (wave-uniform), else 0
 };
 
 template <int ND, int KMAX>
@@ -283,6 +324,47 @@ __device__ __forceinline__ void mix_leaf(const DevTree &T, const double *ext, in
         for (int d = 0; d < ND; ++d) cx[d * NBX_LEAF + lane] = st[d];
     }
     __syncthreads();
+    if constexpr (GShape<KMAX>::GT) {
+        if (c.fillkh) {
+            // First leaf of the walk, every heap still empty and at least 32
+            // positions: the replacement would put the m-th accepted
+            // candidate at position kh - 1 - m (the largest empty key's), so
+            // the keys go straight there in one pass over the leaf, and the
+            // group maxima are taken once at the end.
+            const int kh = c.fillkh;
+            c.fillkh = 0;
+            constexpr uint32_t TM = GShape<KMAX>::TM;
+            constexpr int G = GShape<KMAX>::G, S = GShape<KMAX>::S;
+            int cnt = 0;
+            for (int j = 0; j < NBX_LEAF; ++j) {
+                float t = qv[0] - cx[j];
+                float dd = t * t;
+#pragma unroll
+                for (int d = 1; d < ND; ++d) {
+                    t = qv[d] - cx[d * NBX_LEAF + j];
+                    dd = __builtin_fmaf(t, t, dd);
+                }
+                const int p = kh - 1 - cnt;
+                const uint32_t key = (__float_as_uint(dd) & ~TM) | (uint32_t)p;
+                const bool ins = ((mask >> j) & 1u) && key < ((MX_EMPTY & ~TM) | (uint32_t)p);
+                if (ins) {
+                    HK(p) = key;
+                    SLH(p) = (uint16_t)(rbase + j);
+                }
+                cnt += ins ? 1 : 0;
+            }
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                uint32_t x[S];
+#pragma unroll
+                for (int u = 0; u < S; ++u) x[u] = HK(g * S + u);
+                gm[g] = gmaxu<S>(x);
+            }
+            c.root = gmaxu<G>(gm);
+            __syncthreads();
+            return;
+        }
+    }
     // as many steps as the lane with the most candidates needs (a counted
     // loop keeps the kept-key state in place across steps)
     int nit = __popc(mask);
@@ -510,6 +592,7 @@ __device__ __forceinline__ void mix_task(const DevTree &T, const KnnArgs &A, con
         else SLA(p) = 0u;
     }
     c.nvis = 0;
+    c.fillkh = (GT && kh >= NBX_LEAF) ? kh : 0;
     float qc[ND];
 #pragma unroll
     for (int d = 0; d < ND; ++d) qc[d] = (float)readfirst_d(c.q[d]);
@@ -521,19 +604,21 @@ __device__ __forceinline__ void mix_task(const DevTree &T, const KnnArgs &A, con
     if (valid) {
         // rank sort of the kh kept keys: keys to SKC, slots to SLB
         if constexpr (GT) {
-            // unique keys, all in registers first: sorted in place in LDS
+            // unique keys (unused positions +inf) sorted in registers by the
+            // network, then stored back in order
+            static_assert(KMAX == 33, "the sorting network is built for 33 keys");
             uint32_t kq[KMAX];
 #pragma unroll
             for (int p = 0; p < KMAX; ++p) kq[p] = p < kh ? HK(p) : 0xffffffffu;
 #pragma unroll
-            for (int i = 0; i < KMAX; ++i) {
-                if (i < kh) {
-                    int r = 0;
-#pragma unroll
-                    for (int j = 0; j < KMAX; ++j) r += kq[j] < kq[i] ? 1 : 0;
-                    HK(r) = kq[i];
-                }
+            for (int e = 0; e < MX_NET33.n; ++e) {
+                const uint32_t x = kq[MX_NET33.a[e]], y = kq[MX_NET33.b[e]];
+                kq[MX_NET33.a[e]] = min(x, y);
+                kq[MX_NET33.b[e]] = max(x, y);
             }
+#pragma unroll
+            for (int p = 0; p < KMAX; ++p)
+                if (p < kh) HK(p) = kq[p];
         } else {
             // ties (possible only between groups) by position
             uint32_t kq[KMAX];
