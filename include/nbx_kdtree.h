@@ -401,8 +401,10 @@ class KDTree {
     void SetResetOrder(bool);
 
     static void SetThrowOnError(bool on);
-    // Precision form for the batched calls (NBX_FORM_MIXED): exact neighbour
-    // sets, densities within 2e-6 of FP64 (see nbx.h).  Off by default.
+    // Precision form for the batched calls (NBX_FORM_MIXED): the FP64 form's
+    // neighbour sets (certified, else re-run in FP64) and distances, with the
+    // densities summed from FP64 distances (tests hold them within 2e-6 of
+    // the FP64 form; about 1e-15 is measured).  Off by default.
     static void SetPrecisionForm(bool mixed);
 
   private:
