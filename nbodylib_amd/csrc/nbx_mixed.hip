@@ -54,9 +54,8 @@ typedef const __attribute__((address_space(4))) fx8 *cf8ptr;
 #define MX_EMPTY 0x7F7FFFFFu  // empty kept entry: the bits of FLT_MAX
 
 #define HK(p) hk[(p) * NBX_WAVE + lane]   // kept keys (LDS)
-#define SLA(p) sl[(p) * NBX_WAVE + lane]  // their slots (scratch, by position; KMAX > 33)
-#define SLH(p) ((uint16_t *)sl)[(p) * NBX_WAVE + lane]  // their handles (scratch, by position; KMAX <= 33)
-#define MX_V 2048 // leaf visits a wave can record: handle = visit << 5 | index in leaf
+#define SLH(p) ((uint16_t *)sl)[(p) * NBX_WAVE + lane]  // their handles (scratch, by position)
+#define MX_V 2048  // leaves a wave can record: handle = leaf's visit number << 5 | index in leaf
 
 // Kept keys.  The kh = k+1 kept candidates of a lane sit at positions
 // p < kh of KMAX = G * S (G groups of S positions; positions p >= kh hold
@@ -155,8 +154,8 @@ __device__ __forceinline__ uint32_t gtop_init(uint32_t *hk, int lane, int kh, ui
 
 // Replace the largest kept key (root) by key in the lanes with ins set (key
 // carries the root's position); the others store the root back, so nothing
-// but the slot store branches.  slot is the candidate's handle when
-// KMAX <= 33, else its slot.  Returns the new largest key.
+// but the handle store branches.  slot is the candidate's 16-bit handle.
+// Returns the new largest key.
 template <int KMAX>
 __device__ __forceinline__ uint32_t gtop_replace(uint32_t *hk, uint32_t *sl, int lane,
                                                  uint32_t (&gm)[GShape<KMAX>::G], bool ins, uint32_t key,
@@ -175,11 +174,7 @@ __device__ __forceinline__ uint32_t gtop_replace(uint32_t *hk, uint32_t *sl, int
     }
     const int b = g0 * S;
     HK(p) = ins ? key : root;
-    if constexpr (GShape<KMAX>::GT) {
-        if (ins) SLH(p) = (uint16_t)slot;
-    } else {
-        if (ins) SLA(p) = (uint32_t)slot;
-    }
+    if (ins) SLH(p) = (uint16_t)slot;
     uint32_t x[S];
 #pragma unroll
     for (int u = 0; u < S; ++u) x[u] = HK(b + u);
@@ -196,7 +191,7 @@ struct MixCtx {
     int32_t self;
     uint32_t root;  // largest kept key
     double emax;    // largest leaf half-diagonal that matters for this lane
-    int nvis;       // leaves visited (wave-uniform), the next handle's visit number
+    int nvis;       // leaves recorded (wave-uniform), the next handle's visit number
     int fillkh;     // kh while the heap is still empty and kh >= 32 (wave-uniform), else 0
 };
 
@@ -205,13 +200,15 @@ __device__ __forceinline__ void mix_leaf(const DevTree &T, const double *ext, in
                                          uint32_t *hk, uint32_t *sl, float *cx, int lane,
                                          uint32_t (&gm)[GShape<KMAX>::G]) {
     const int base = leaf * NBX_LEAF;
-    // what a kept candidate stores: its slot, or (KMAX <= 33) a 16-bit handle
-    // into the wave's visit table (vt, right after the handles in scratch)
-    int rbase = base;
-    if constexpr (GShape<KMAX>::GT) {
-        uint32_t *vt = sl + KMAX * (NBX_WAVE / 2);
+    // A kept candidate is stored as a 16-bit handle, its leaf's visit number
+    // << 5 | its index in the leaf; the wave's visit table (vt, right after
+    // the handles in scratch) maps visit numbers to leaves.  A 3-D wave
+    // numbers every leaf it visits (about 36); a 6-D wave visits hundreds,
+    // so it numbers only the leaves where some lane accepts a candidate.
+    uint32_t *vt = sl + KMAX * (NBX_WAVE / 2);
+    int rbase = c.nvis << 5;
+    if constexpr (ND == 3) {
         if (lane == 0) vt[c.nvis] = (uint32_t)leaf;
-        rbase = c.nvis << 5;
         c.nvis += 1;
     }
     // candidate (lane mod 32) of this leaf: the tested point in phase 1 and
@@ -319,6 +316,12 @@ __device__ __forceinline__ void mix_leaf(const DevTree &T, const double *ext, in
     if (c.self >= base && c.self < base + NBX_LEAF) mask &= ~(1u << (c.self - base));
     if (!c.act) mask = 0;
     if (c.act && (want || mask)) c.emax = fmax(c.emax, ((cdptr)ext)[leaf]);
+    if constexpr (ND == 6) {
+        // no lane accepts anything: nothing to insert, no visit number
+        if (!ballot(mask != 0)) return;
+        if (lane == 0) vt[c.nvis] = (uint32_t)leaf;
+        c.nvis += 1;
+    }
     if (lane < NBX_LEAF) {
 #pragma unroll
         for (int d = 0; d < ND; ++d) cx[d * NBX_LEAF + lane] = st[d];
@@ -408,7 +411,7 @@ __device__ __forceinline__ void mix_walk(const DevTree &T, const double *ext, Mi
             const bool want = c.act && box_lb<ND>(c.q, bx) <= (double)__uint_as_float(c.root);
             if (ballot(want)) {
                 if (dep == D) {
-                    if (GShape<KMAX>::GT && c.nvis >= MX_V) break;  // visit table full: lanes fall back
+                    if (c.nvis >= MX_V) break;  // visit table full: lanes fall back
                     mix_leaf<ND, KMAX>(T, ext, node - T.nleafp, c, want, hk, sl, cx, lane, gm);
                 } else {
                     int first;
@@ -435,7 +438,7 @@ __device__ __forceinline__ void mix_walk(const DevTree &T, const double *ext, Mi
                 const bool want = c.act && box_lb<ND>(c.q, bx) <= (double)__uint_as_float(c.root);
                 if (ballot(want)) {
                     if (dep == D) {
-                        if (GShape<KMAX>::GT && c.nvis >= MX_V) break;  // visit table full: lanes fall back
+                        if (c.nvis >= MX_V) break;  // visit table full: lanes fall back
                         mix_leaf<ND, KMAX>(T, ext, node - T.nleafp, c, want, hk, sl, cx, lane, gm);
                         down = false;
                     } else {
@@ -555,7 +558,7 @@ __device__ __forceinline__ void mix_task(const DevTree &T, const KnnArgs &A, con
     constexpr uint32_t TM = GShape<KMAX>::TM;
     uint32_t *const kb = GT ? hk : skc;
     // slot of sorted entry p; move sorted entry src to dst
-    uint32_t *const vt = sl + KMAX * (NBX_WAVE / 2);  // visit table (KMAX <= 33)
+    uint32_t *const vt = sl + KMAX * (NBX_WAVE / 2);  // visit table
     auto slot_at = [&](int p) -> int32_t {
         if constexpr (GT) {
             const uint32_t h = SLH(SKC(p) & TM);
@@ -585,12 +588,11 @@ __device__ __forceinline__ void mix_task(const DevTree &T, const KnnArgs &A, con
     c.emax = 0.0;
     uint32_t gm[GShape<KMAX>::G];
     c.root = gtop_init<KMAX>(hk, lane, kh, gm);
-    // every slot read later is a valid index (handle 0 names the first leaf
-    // visited, which every valid lane visits)
-    for (int p = 0; p < kh; ++p) {
-        if constexpr (GT) SLH(p) = 0;
-        else SLA(p) = 0u;
-    }
+    // every slot read later is a valid index: handle 0 names the leaf in
+    // visit table entry 0, first the wave's own leaf, then the first leaf
+    // with an accepted candidate
+    for (int p = 0; p < kh; ++p) SLH(p) = 0;
+    if (lane == 0) vt[0] = (uint32_t)(2 * g);
     c.nvis = 0;
     c.fillkh = (GT && kh >= NBX_LEAF) ? kh : 0;
     float qc[ND];
@@ -599,7 +601,7 @@ __device__ __forceinline__ void mix_task(const DevTree &T, const KnnArgs &A, con
     mix_walk<ND, KMAX>(T, ext, c, qc, T.nleafp + 2 * g, hk, sl, cx, lane, gm);
 
     // ---------------------------------------------------------------- band
-    const bool ovf = GT && c.nvis >= MX_V;  // walk stopped early: no certificate
+    const bool ovf = c.nvis >= MX_V;  // walk stopped early: no certificate
     bool ok = valid && !ovf;
     if (valid) {
         // rank sort of the kh kept keys: keys to SKC, slots to SLB
@@ -629,8 +631,9 @@ __device__ __forceinline__ void mix_task(const DevTree &T, const KnnArgs &A, con
                 int r = 0;
 #pragma unroll
                 for (int j = 0; j < KMAX; ++j) r += (kq[j] < ki || (kq[j] == ki && j < i)) ? 1 : 0;
+                const uint32_t h = SLH(i);
                 SKC(r) = ki;
-                SLB(r) = SLA(i);
+                SLB(r) = vt[h >> 5] * NBX_LEAF + (h & 31u);
             }
         }
         if ((SKC(kh - 1) & ~GShape<KMAX>::TM) == (MX_EMPTY & ~GShape<KMAX>::TM)) ok = false;  // < k+1 candidates
@@ -832,16 +835,17 @@ __device__ __forceinline__ void mix_task(const DevTree &T, const KnnArgs &A, con
 // waves of queries from that XCD's contiguous share of the tree order (the
 // xcd_wave split) through the XCD's counter, and owns one scratch block of
 // MX_BLKW(KMAX) words (16-bit handles by position and the visit table; for
-// KMAX > 33 slots by position and slots and keys in ascending order) for its
-// whole life, so the scratch is sized by the
-// resident workgroups and only ever touched from one XCD.  Once its own
+// KMAX > 33 also slots and keys in ascending order) for its whole life, so
+// the scratch is sized by the resident workgroups and only ever touched from
+// one XCD.  Once its own
 // share is drained a workgroup takes waves from the other XCDs' counters in
 // turn (each ticket names one wave, so every wave runs once), which evens out
 // shares of unequal cost; it exits when all eight counters have passed their
 // shares' ends.
 #define MX_CTR_STRIDE 16  // XCD counters 64 bytes apart
-// scratch words per workgroup: 16-bit handles + visit table, or 3 regions of slots/keys
-#define MX_BLKW(KM) ((KM) <= 33 ? (KM) * (NBX_WAVE / 2) + MX_V : 3 * (KM) * NBX_WAVE)
+// scratch words per workgroup: 16-bit handles and the visit table, and for
+// KMAX > 33 the slots and keys in ascending order
+#define MX_BLKW(KM) ((KM) * (NBX_WAVE / 2) + MX_V + ((KM) <= 33 ? 0 : 2 * (KM) * NBX_WAVE))
 template <int ND, int KMAX>
 __global__ __launch_bounds__(64) void k_knn_mixed(DevTree T, KnnArgs A, const double *ext, int32_t *fb_list,
                                                    int32_t *fb_count, uint32_t *scratch, int32_t *ctr) {
@@ -856,7 +860,7 @@ __global__ __launch_bounds__(64) void k_knn_mixed(DevTree T, KnnArgs A, const do
     double *racc = (double *)(ub + ND * NBX_LEAF);
     const int lane = threadIdx.x;
     uint32_t *sl = scratch + (size_t)blockIdx.x * MX_BLKW(KMAX);
-    uint32_t *slb = sl + KMAX * NBX_WAVE, *skc = slb + KMAX * NBX_WAVE;
+    uint32_t *slb = sl + KMAX * (NBX_WAVE / 2) + MX_V, *skc = slb + KMAX * NBX_WAVE;
     const int nw = A.wcnt > 0 ? A.wcnt : (T.n + NBX_WAVE - 1) / NBX_WAVE;
     const int x = blockIdx.x & 7, q8 = nw >> 3, r8 = nw & 7;
     for (int s = 0; s < 8; ++s) {
