@@ -231,15 +231,29 @@ __device__ __forceinline__ void fof_leaf(const DevTree &T, int leaf, const doubl
         }
     }
     if (!pair) mask = 0;
+    if (!ballot(mask != 0)) return;
+    // The leaf's parents, read once with ordinary loads.  A candidate whose
+    // parent is the lane's root hint rq (a former root of its set) is
+    // already in that set, because sets only ever merge, so its link needs
+    // no union.  A stale parent costs only the skip.
+    __shared__ int32_t par[NBX_LEAF];
+    const int ln = threadIdx.x;  // one wave per workgroup
+    if (ln < NBX_LEAF) par[ln] = base + ln < T.n ? P[base + ln] : -1;
+    __syncthreads();
     if (lconn) {
-        if (mask) rq = uf_union_r(P, rq, base + __builtin_ctz(mask));
-        return;
+        if (mask) {
+            const int j = __builtin_ctz(mask);
+            if (par[j] != rq) rq = uf_union_r(P, rq, base + j);
+        }
+    } else {
+        while (mask) {
+            const int j = __builtin_ctz(mask);
+            mask &= mask - 1;
+            if (par[j] == rq) continue;
+            rq = uf_union_r(P, rq, base + j);
+        }
     }
-    while (mask) {
-        const int j = __builtin_ctz(mask);
-        mask &= mask - 1;
-        rq = uf_union_r(P, rq, base + j);
-    }
+    __syncthreads();
 }
 
 template <int PASS0, int ND, int MT>
