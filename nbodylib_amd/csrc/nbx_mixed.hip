@@ -54,7 +54,8 @@ typedef const __attribute__((address_space(4))) fx8 *cf8ptr;
 #define MX_EMPTY 0x7F7FFFFFu  // empty kept entry: the bits of FLT_MAX
 
 #define HK(p) hk[(p) * NBX_WAVE + lane]   // kept keys (LDS)
-#define SLH(p) ((uint16_t *)sl)[(p) * NBX_WAVE + lane]  // their handles (scratch, by position)
+typedef __attribute__((address_space(1))) uint16_t gu16;
+#define SLH(p) ((gu16 *)sl + lane)[(p) * NBX_WAVE]  // their handles (scratch, by position)
 #define MX_V 2048  // leaves a wave can record: handle = leaf's visit number << 5 | index in leaf
 
 // Kept keys.  The kh = k+1 kept candidates of a lane sit at positions
@@ -860,6 +861,9 @@ __global__ __launch_bounds__(64) void k_knn_mixed(DevTree T, KnnArgs A, const do
     double *racc = (double *)(ub + ND * NBX_LEAF);
     const int lane = threadIdx.x;
     uint32_t *sl = scratch + (size_t)blockIdx.x * MX_BLKW(KMAX);
+    // kept in a VGPR pair: the scalar registers are oversubscribed, and a
+    // scalar base would be reloaded from its spill slot at every handle store
+    asm volatile("" : "+v"(sl));
     uint32_t *slb = sl + KMAX * (NBX_WAVE / 2) + MX_V, *skc = slb + KMAX * NBX_WAVE;
     const int nw = A.wcnt > 0 ? A.wcnt : (T.n + NBX_WAVE - 1) / NBX_WAVE;
     const int x = blockIdx.x & 7, q8 = nw >> 3, r8 = nw & 7;
