@@ -219,9 +219,9 @@ int nbx_knn_density_dev(nbx_tree *t, int32_t k, int32_t *nn_by_slot_dev, double 
  * counterpart in the reference.  Same arguments and results; the call makes
  * no hipMalloc / hipFree or other allocation: its temporaries come from a
  * 256 MiB device array that each GPU the library runs on holds until the
- * process ends.  n particles need 8n bytes (16n with h) plus, for the
- * precision form, 4n bytes and the search scratch (about 50 MB); larger
- * trees return NBX_ERR_UNSUPPORTED before anything runs.  Calls that overlap
+ * process ends.  n particles need 8n bytes plus, for the precision form,
+ * 4n bytes and the search scratch (about 50 MB); larger trees return
+ * NBX_ERR_UNSUPPORTED before anything runs.  Calls that overlap
  * on one GPU share that array and race. */
 int nbx_knn_density_noalloc(nbx_tree *t, int32_t k, int32_t *nn_by_slot_dev, double *rho_by_id_dev,
                             double *h_by_id_dev, void *stream);

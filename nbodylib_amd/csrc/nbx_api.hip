@@ -729,18 +729,16 @@ static int density_impl(nbx_tree *t, int32_t k, int32_t variant, int32_t nsearch
     if (A.kh > heap_cap(t)) return fail(NBX_ERR_UNSUPPORTED, "k too large (k <= 129 supported)");
     const bool mixed = mixed_ok(t, k) && A.dens != NBX_DO_VEL && A.dens != NBX_DO_VELPHYS;
     const int64_t n = t->n;
-    double *rho = nullptr, *h = nullptr;
+    double *rho = nullptr;
     int32_t *err = nullptr;
     DevBufs bufs(st);
     if (ws) {
         if (A.dens == NBX_DO_VELPHYS) return fail(NBX_ERR_UNSUPPORTED, "no workspace form of this density");
         rho = ws->carve<double>(sizeof(double) * n);
-        if (h_id) h = ws->carve<double>(sizeof(double) * n);
         err = ws->carve<int32_t>(sizeof(int32_t));
-        if (!rho || (h_id && !h) || !err) return fail(NBX_ERR_ARG, "scratch too small");
+        if (!rho || !err) return fail(NBX_ERR_ARG, "scratch too small");
     } else {
         HIPCK(bufs.alloc(&rho, sizeof(double) * n));
-        if (h_id) HIPCK(bufs.alloc(&h, sizeof(double) * n));
         HIPCK(bufs.alloc(&err, sizeof(int32_t)));
     }
     hipLaunchKernelGGL(k_fill_f64, dim3(nblk(n)), dim3(256), 0, st, rho, 0.0, n);
@@ -752,13 +750,13 @@ static int density_impl(nbx_tree *t, int32_t k, int32_t variant, int32_t nsearch
         A.rho_in = rin;
     }
     A.rho = rho;
-    A.hout = h;
+    A.hout = h_id;  // written by ID in the search
+    A.hout_by_id = 1;
     A.err = err;
     A.nn = nn_slot;
     hipError_t e = mixed ? run_mixed(t, A, k, st, ws) : nbx_launch_knn(t->view(), A, (int)n, st);
     if (e == hipSuccess) {
         hipLaunchKernelGGL(k_scatter_by_id, dim3(nblk(n)), dim3(256), 0, st, rho, t->perm, rho_id, n);
-        if (h_id) hipLaunchKernelGGL(k_scatter_by_id, dim3(nblk(n)), dim3(256), 0, st, h, t->perm, h_id, n);
         e = hipGetLastError();
     }
     int32_t herr = 0;
@@ -782,8 +780,8 @@ extern "C" int nbx_knn_density_dev(nbx_tree *t, int32_t k, int32_t *nn_by_slot_d
 }
 
 // ------------------------------------------------------------------ entry point that allocates nothing
-// nbx_knn_density_noalloc: nbx_knn_density_dev with every temporary (rho and
-// h by slot, the error flag, and for the precision form the fallback list,
+// nbx_knn_density_noalloc: nbx_knn_density_dev with every temporary (rho by
+// slot, the error flag, and for the precision form the fallback list,
 // the per-workgroup search scratch and the queue counters) carved from one
 // file-scope device array instead of allocated.  Each GPU holds its own copy
 // of the array from the library's load until the process ends; host code
@@ -819,7 +817,6 @@ extern "C" int nbx_knn_density_noalloc(nbx_tree *t, int32_t k, int32_t *nn_by_sl
     const bool mixed = mixed_ok(t, k);
     Workspace dry = {nullptr, ~(size_t)0, 0};
     dry.carve<char>(sizeof(double) * n);
-    if (h_by_id_dev) dry.carve<char>(sizeof(double) * n);
     dry.carve<char>(sizeof(int32_t));
     if (mixed) {
         dry.carve<char>(sizeof(int32_t) * (n + 1));

@@ -72,7 +72,8 @@ struct KnnArgs {
     int32_t *nn;        // nullable
     double *d2;         // nullable
     double *rho;        // by slot (density); by query row for position queries
-    double *hout;       // as rho (nullable)
+    double *hout;       // as rho, or by ID with hout_by_id (nullable)
+    int32_t hout_by_id;  // particle queries: hout[ID] rather than hout[slot]
     int32_t *err;       // set to 1 if a row has fewer than kh neighbours
     const int32_t *qlist;  // nullable: query slots (FP64 re-run of mixed-form fallbacks)
     // particle queries of a run of whole waves only: slots [64 wbeg, 64 (wbeg + wcnt)) (wcnt 0: every slot)

@@ -431,7 +431,7 @@ __global__ __launch_bounds__(64) void k_knn(DevTree T, KnnArgs A) {
                     }
                 }
                 A.rho[qi] = den;
-                if (A.hout) A.hout[qi] = hv;
+                if (A.hout) A.hout[A.hout_by_id ? T.perm[qi] : qi] = hv;
             }
         } else {
             double hi = 0.5 * sqrt(kth);
@@ -474,7 +474,7 @@ __global__ __launch_bounds__(64) void k_knn(DevTree T, KnnArgs A) {
             }
             if (A.dens == NBX_DO_SYM) racc_own = acc;
             else A.rho[qi] = acc;
-            if (A.hout) A.hout[qi] = hi;
+            if (A.hout) A.hout[A.hout_by_id ? T.perm[qi] : qi] = hi;
         }
         if (bad) atomicOr(A.err, 1);
     }

@@ -805,7 +805,7 @@ __device__ __forceinline__ void mix_task(const DevTree &T, const KnnArgs &A, con
         }
         if (sym) racc_own = acc;
         else A.rho[qi] = acc;
-        if (A.hout) A.hout[qi] = hi;
+        if (A.hout) A.hout[A.hout_by_id ? T.perm[qi] : qi] = hi;
     }
     __syncthreads();
     if (sym && valid) unsafeAtomicAdd(A.rho + qi, racc_own + racc[lane]);

@@ -58,7 +58,7 @@ def test_noalloc_rejects_oversize(gpu):
     import nbodylib_amd as nb
     from nbodylib_amd.device import DeviceTree
 
-    n = 14_000_000  # 16 bytes each with h plus the precision form's 4-byte fallback list: 267 MiB > 256 MiB
+    n = 24_000_000  # 8 bytes each plus the precision form's 4-byte fallback list: 275 MiB > 256 MiB
     print(f"noalloc oversize n={n}", flush=True)
     g = torch.Generator(device="cuda").manual_seed(63)
     pt = torch.rand((3, n), dtype=torch.float64, device="cuda", generator=g)
