@@ -164,7 +164,7 @@ __global__ __launch_bounds__(256) void k_fof_leafinit(DevTree T, FofLink L, int3
 
 // Internal node whose diagonal is < b: both children are connected (their
 // diagonals are smaller) and every cross pair links, so one union connects
-// it.  Nodes [first, first+count) of one depth.  An empty box (padding) has
+// it.  Nodes [first, first+count).  An empty box (padding) has
 // an infinite diagonal and is never connected.
 template <int ND, int MT>
 __global__ __launch_bounds__(256) void k_fof_nodeinit(DevTree T, FofLink L, int32_t *P, uint8_t *conn, int32_t first,
@@ -595,9 +595,11 @@ static void launch_fof_t(const DevTree &T, const FofArgs &A, uint8_t *conn, hipS
     {
         ProfScope ps("k_fof_init", st);
         hipLaunchKernelGGL((k_fof_leafinit<ND, MT>), dim3(nblk(T.nleaf)), dim3(256), 0, st, T, A.link, A.parent, conn);
-        for (int dep = T.depth - 1; dep >= 0; --dep)
-            hipLaunchKernelGGL((k_fof_nodeinit<ND, MT>), dim3(nblk((int64_t)1 << dep)), dim3(256), 0, st, T, A.link,
-                               A.parent, conn, 1 << dep, 1 << dep);
+        // every internal node at once: a node's union depends only on its own
+        // box, and unions commute
+        if (T.nleafp > 1)
+            hipLaunchKernelGGL((k_fof_nodeinit<ND, MT>), dim3(nblk((int64_t)T.nleafp - 1)), dim3(256), 0, st, T, A.link,
+                               A.parent, conn, 1, T.nleafp - 1);
     }
     ProfScope ps("k_fof_link", st);
     if (ND == 3) {
