@@ -219,6 +219,9 @@ class KDTree {
     // Rdistadapt and AdaptiveMedianFac shape the reference's tree only; the
     // results they give do not depend on it, so they are accepted and unused,
     // as are metric, iBuildInParallel, iKeepInputOrder and min_bucket_size.
+    // A TPHS tree needs Aniso = -1 (metric factor 1 on coordinates the caller
+    // has scaled); the reference's Aniso = 0 and 1 use a local metric tensor,
+    // which is not provided, and are reported as unsupported.
     KDTree(Particle *p, Int_t numparts, Int_t bucket_size = 16, int TreeType = TPHYS, int KernType = KEPAN,
            int KernRes = 1000, int SplittingCriterion = KDTREE_SPLIT_SPREAD, int Aniso = 0, int ScaleSpace = 0,
            Double_t *Period = nullptr, Double_t **metric = nullptr, bool iBuildInParallel = true,

@@ -60,8 +60,11 @@ void KDTree::init(Particle *p, Int_t n, Int_t bucket_size, int TreeType, int Ker
     prm.kerntype = KernType;
     prm.kernres = KernRes;
     prm.splitting = Splitting;
-    // the phase-space tree is supported with the metric factor taken as 1
-    prm.aniso = (TreeType == TPHS && Aniso == 0) ? -1 : Aniso;
+    // TPHS trees: Aniso = -1 only (metric factor 1 on coordinates the caller
+    // has scaled); the reference's Aniso = 0 and 1 search with a local metric
+    // tensor (KDCalcSmoothQuantities.cxx:236-262), which is not provided, so
+    // nbx_tree_create reports them as unsupported
+    prm.aniso = Aniso;
     prm.scalespace = ScaleSpace;
     prm.periodic = periodic ? 1 : 0;
     for (int k = 0; k < 3; ++k) prm.period[k] = period[k];
