@@ -16,6 +16,16 @@
  * gives the message (thread-local).  The reference prints and exit()s in
  * these cases (KDCalcSmoothQuantities.cxx:205-207, :273-276,
  * PriorityQueue.h:49-53, KDTree.cxx:1096-1105).
+ *
+ * Scratch kept by a tree: the density passes, the precision-form searches
+ * and FOF take their device temporaries from one block that the tree keeps
+ * from call to call, so repeated calls allocate nothing after the first.
+ * The block grows to the largest need seen (densities: 8 bytes per particle,
+ * plus 4 bytes per particle and about 50 MB of search scratch in the
+ * precision form; FOF: 16 bytes per particle and 24 per possible group of
+ * minnum members) and is freed by
+ * nbx_tree_destroy.  Calls on one tree from several threads or streams take
+ * turns with the block; each waits on the device for the previous one's use.
  */
 #ifndef NBX_H
 #define NBX_H

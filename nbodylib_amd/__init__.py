@@ -177,11 +177,15 @@ class KDTree:
         h = ctypes.c_void_p()
         _check(lib().nbx_tree_create(ctypes.byref(self._params), _p(spos), _p(svel), _p(m), self.n, ctypes.byref(h)))
         self._h = h
+        self._destroy = lib().nbx_tree_destroy
 
     def __del__(self):
+        # the destroy function is bound at creation: at interpreter exit the
+        # module's globals may already be gone
         h = getattr(self, "_h", None)
-        if h:
-            lib().nbx_tree_destroy(h)
+        destroy = getattr(self, "_destroy", None)
+        if h and destroy is not None:
+            destroy(h)
             self._h = None
 
     # -------------------------------------------------------------- info

@@ -7,6 +7,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -14,8 +15,9 @@
 #include "../../include/nbx.h"
 #include "nbx_kernels.h"
 
+size_t nbx_fof_canonical_bytes(int64_t n, int32_t minnum);
 hipError_t nbx_fof_canonical(const DevTree &T, int32_t *P, int32_t minnum, int32_t *labels_by_id, int64_t *ngroups,
-                             hipStream_t st);
+                             void *work, hipStream_t st);
 __global__ void k_ball(DevTree T, double qx, double qy, double qz, double r2, int32_t *out, double *dout,
                        int64_t cap, unsigned long long *count);
 
@@ -166,6 +168,17 @@ struct nbx_tree {
     std::vector<double> hkern;
     int32_t nleaf, nleafp, depth;
     int64_t nnodes;
+    // Device scratch kept from one call to the next (densities by slot, the
+    // fallback list, the search scratch, FOF's work arrays), so a repeated
+    // call allocates nothing: grown when a call needs more, freed with the
+    // tree.  One call holds it at a time (scr_mu); scr_done is recorded on
+    // the holder's stream after its last use, and the next holder's stream
+    // waits for it.
+    std::mutex scr_mu;
+    char *scr = nullptr;
+    size_t scr_cap = 0;
+    hipEvent_t scr_done = nullptr;
+    bool scr_recorded = false;
     DevTree view() const {
         DevTree T;
         memset(&T, 0, sizeof(T));
@@ -241,6 +254,8 @@ static void free_tree(nbx_tree *t) {
     if (t->kernf) hipFree(t->kernf);
     if (t->ext) hipFree(t->ext);
     if (t->xq) hipFree(t->xq);
+    if (t->scr) hipFree(t->scr);
+    if (t->scr_done) hipEventDestroy(t->scr_done);
     delete t;
 }
 
@@ -427,27 +442,83 @@ extern "C" int nbx_tree_kernel(const nbx_tree *t, double *kernnorm, double *tabl
     return t->kernres;
 }
 
+// The tree's kept scratch, held for one call as a Workspace of at least need
+// bytes.  A larger need replaces the block (after the work that used it has
+// finished).  On destruction the holder's stream records that its work on
+// the block is queued.  A call that already holds the lease passes its
+// Workspace down instead of taking another (the lock is not recursive).
+struct ScratchLease {
+    nbx_tree *t;
+    hipStream_t st;
+    std::unique_lock<std::mutex> lk;
+    Workspace ws = {nullptr, 0, 0};
+    hipError_t err = hipSuccess;
+    ScratchLease(nbx_tree *tr, size_t need, hipStream_t s) : t(tr), st(s), lk(tr->scr_mu) {
+        if (!t->scr_done) {
+            err = hipEventCreateWithFlags(&t->scr_done, hipEventDisableTiming);
+            if (err != hipSuccess) { t->scr_done = nullptr; return; }
+        }
+        if (need > t->scr_cap) {
+            if (t->scr) {
+                if (t->scr_recorded) err = hipEventSynchronize(t->scr_done);
+                if (err != hipSuccess) return;
+                hipFree(t->scr);
+                t->scr = nullptr;
+                t->scr_cap = 0;
+                t->scr_recorded = false;
+            }
+            err = hipMalloc((void **)&t->scr, need);
+            if (err != hipSuccess) { t->scr = nullptr; return; }
+            t->scr_cap = need;
+        }
+        if (t->scr_recorded) err = hipStreamWaitEvent(st, t->scr_done, 0);
+        if (err == hipSuccess) ws = {t->scr, t->scr_cap, 0};
+    }
+    ~ScratchLease() {
+        if (err == hipSuccess && hipEventRecord(t->scr_done, st) == hipSuccess) t->scr_recorded = true;
+    }
+    ScratchLease(const ScratchLease &) = delete;
+    ScratchLease &operator=(const ScratchLease &) = delete;
+};
+
+// Bytes the fused kNN + density pass carves, in its order: densities by slot,
+// the error flag and, for the precision form, the fallback list and the
+// search scratch.
+static size_t density_ws_bytes(const nbx_tree *t, int32_t k, bool mixed) {
+    Workspace dry = {nullptr, ~(size_t)0, 0};
+    dry.carve<char>(sizeof(double) * t->n);
+    dry.carve<char>(sizeof(int32_t));
+    if (mixed) {
+        dry.carve<char>(sizeof(int32_t) * (t->n + 1));
+        dry.carve<char>(nbx_mixed_scratch_bytes(t->nd, k + 1));
+    }
+    return dry.used;
+}
+
 // ------------------------------------------------------------------ precision form
 // FP32 search with the FP64 certificate; queries whose certificate fails are
 // re-run by the FP64 kernel from a list, so results equal the plain form's
 // neighbour sets.  A.dens / A.rho / A.nn / A.layout as for nbx_launch_knn.
+// The fallback list and the search scratch come from ws, or from the tree's
+// kept scratch when ws is null.
 static hipError_t run_mixed(nbx_tree *t, KnnArgs A, int k, hipStream_t st, Workspace *ws = nullptr) {
     const int64_t n = t->n;
-    int32_t *fb = nullptr;
-    void *mwork = nullptr;
-    size_t mbytes = 0;
-    hipError_t e = hipSuccess;
     // the fallback list holds at most the queries of the launch
     const int64_t nfb = A.wcnt > 0 ? std::min<int64_t>(n, (int64_t)A.wcnt * NBX_WAVE) : n;
-    if (ws) {
-        mbytes = nbx_mixed_scratch_bytes(t->nd, k + 1);
-        fb = ws->carve<int32_t>(sizeof(int32_t) * (nfb + 1));
-        mwork = ws->carve<void>(mbytes);
-        if (!fb || !mwork) return hipErrorInvalidValue;
-    } else {
-        e = hipMallocAsync((void **)&fb, sizeof(int32_t) * (nfb + 1), st);
-        if (e != hipSuccess) return e;
+    const size_t mbytes = nbx_mixed_scratch_bytes(t->nd, k + 1);
+    std::unique_ptr<ScratchLease> lease;
+    if (!ws) {
+        Workspace dry = {nullptr, ~(size_t)0, 0};
+        dry.carve<char>(sizeof(int32_t) * (nfb + 1));
+        dry.carve<char>(mbytes);
+        lease.reset(new ScratchLease(t, dry.used, st));
+        if (lease->err != hipSuccess) return lease->err;
+        ws = &lease->ws;
     }
+    int32_t *fb = ws->carve<int32_t>(sizeof(int32_t) * (nfb + 1));
+    void *mwork = ws->carve<void>(mbytes);
+    if (!fb || !mwork) return hipErrorInvalidValue;
+    hipError_t e = hipSuccess;
     int32_t *cnt = fb + nfb;
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, st, cnt, 0, (int64_t)1);
     KnnArgs M = A;
@@ -471,8 +542,7 @@ static hipError_t run_mixed(nbx_tree *t, KnnArgs A, int k, hipStream_t st, Works
         e = nbx_launch_knn(T, F, hc, st);
     }
     t->last_fallbacks = hc;
-    hipError_t e2 = ws ? hipSuccess : hipFreeAsync(fb, st);
-    return e != hipSuccess ? e : e2;
+    return e;
 }
 
 extern "C" int nbx_tree_last_fallbacks(const nbx_tree *t, int64_t *fallbacks) {
@@ -729,18 +799,19 @@ static int density_impl(nbx_tree *t, int32_t k, int32_t variant, int32_t nsearch
     if (A.kh > heap_cap(t)) return fail(NBX_ERR_UNSUPPORTED, "k too large (k <= 129 supported)");
     const bool mixed = mixed_ok(t, k) && A.dens != NBX_DO_VEL && A.dens != NBX_DO_VELPHYS;
     const int64_t n = t->n;
-    double *rho = nullptr;
-    int32_t *err = nullptr;
     DevBufs bufs(st);
+    // temporaries from the caller's workspace, else from the tree's kept scratch
+    std::unique_ptr<ScratchLease> lease;
     if (ws) {
         if (A.dens == NBX_DO_VELPHYS) return fail(NBX_ERR_UNSUPPORTED, "no workspace form of this density");
-        rho = ws->carve<double>(sizeof(double) * n);
-        err = ws->carve<int32_t>(sizeof(int32_t));
-        if (!rho || !err) return fail(NBX_ERR_ARG, "scratch too small");
     } else {
-        HIPCK(bufs.alloc(&rho, sizeof(double) * n));
-        HIPCK(bufs.alloc(&err, sizeof(int32_t)));
+        lease.reset(new ScratchLease(t, density_ws_bytes(t, k, mixed), st));
+        if (lease->err != hipSuccess) return hipfail(lease->err, "density scratch");
+        ws = &lease->ws;
     }
+    double *rho = ws->carve<double>(sizeof(double) * n);
+    int32_t *err = ws->carve<int32_t>(sizeof(int32_t));
+    if (!rho || !err) return fail(NBX_ERR_ARG, "scratch too small");
     hipLaunchKernelGGL(k_fill_f64, dim3(nblk(n)), dim3(256), 0, st, rho, 0.0, n);
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, st, err, 0, (int64_t)1);
     if (A.dens == NBX_DO_VELPHYS) {
@@ -812,19 +883,12 @@ extern "C" int nbx_knn_density_noalloc(nbx_tree *t, int32_t k, int32_t *nn_by_sl
                                        double *h_by_id_dev, void *stream) {
     if (!t || !rho_by_id_dev) return fail(NBX_ERR_ARG, "NULL argument");
     if (k < 1 || k > heap_cap(t)) return fail(NBX_ERR_UNSUPPORTED, "k out of range");
-    // the carves density_impl and run_mixed will make, in their order
+    // the carves density_impl and run_mixed will make
     const int64_t n = t->n;
-    const bool mixed = mixed_ok(t, k);
-    Workspace dry = {nullptr, ~(size_t)0, 0};
-    dry.carve<char>(sizeof(double) * n);
-    dry.carve<char>(sizeof(int32_t));
-    if (mixed) {
-        dry.carve<char>(sizeof(int32_t) * (n + 1));
-        dry.carve<char>(nbx_mixed_scratch_bytes(t->nd, k + 1));
-    }
-    if (dry.used > NBX_NOALLOC_BYTES)
+    const size_t need = density_ws_bytes(t, k, mixed_ok(t, k));
+    if (need > NBX_NOALLOC_BYTES)
         return fail(NBX_ERR_UNSUPPORTED, "nbx_knn_density_noalloc: " + std::to_string(n) + " particles need " +
-                                             std::to_string(dry.used >> 20) + " MiB of scratch; the maximum is " +
+                                             std::to_string(need >> 20) + " MiB of scratch; the maximum is " +
                                              std::to_string(NBX_NOALLOC_BYTES >> 20) + " MiB");
     void *base = nullptr;
     int rc = noalloc_scratch(&base);
@@ -1056,10 +1120,18 @@ extern "C" int nbx_veldensity_phys(nbx_tree *t, int32_t nsmooth, int32_t nsearch
 static int fof_impl(nbx_tree *t, int metric, const FofLink &link, int32_t minnum, int32_t *labels_by_id_dev,
                     int64_t *ngroups, hipStream_t st) {
     const int64_t n = t->n;
-    int32_t *P = nullptr;
     DevTree T = t->view();
-    DevBufs bufs(st);
-    HIPCK(bufs.alloc(&P, nbx_fof_work_bytes(T)));
+    // parents, node flags and queue counters, then the labelling's arrays,
+    // from the tree's kept scratch
+    const size_t wb = nbx_fof_work_bytes(T), cb = nbx_fof_canonical_bytes(n, minnum);
+    Workspace dry = {nullptr, ~(size_t)0, 0};
+    dry.carve<char>(wb);
+    dry.carve<char>(cb);
+    ScratchLease lease(t, dry.used, st);
+    if (lease.err != hipSuccess) return hipfail(lease.err, "fof scratch");
+    int32_t *P = lease.ws.carve<int32_t>(wb);
+    void *cw = lease.ws.carve<void>(cb);
+    if (!P || !cw) return fail(NBX_ERR_ARG, "fof scratch too small");
     FofArgs A;
     memset(&A, 0, sizeof(A));
     A.link = link;
@@ -1068,7 +1140,7 @@ static int fof_impl(nbx_tree *t, int metric, const FofLink &link, int32_t minnum
     for (int d = 0; d < 3; ++d) A.period[d] = t->p.period[d];
     A.parent = P;
     hipError_t e = nbx_launch_fof(T, A, st);
-    if (e == hipSuccess) e = nbx_fof_canonical(T, P, minnum, labels_by_id_dev, ngroups, st);
+    if (e == hipSuccess) e = nbx_fof_canonical(T, P, minnum, labels_by_id_dev, ngroups, cw, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return hipfail(e, "fof");
     return NBX_OK;

@@ -632,20 +632,47 @@ hipError_t nbx_launch_fof(const DevTree &T, const FofArgs &A, hipStream_t st) {
     return hipGetLastError();
 }
 
+// The canonical labelling's work arrays, carved in order from one block:
+// per-particle counts, minimum IDs and labels, the group counter, and the
+// (key, group) pairs of at most n / minnum + 1 groups with their sort's
+// temporary storage.
+struct FofCanonWork {
+    int64_t cap;
+    size_t off[9], sort_bytes, total;
+    FofCanonWork(int64_t n, int32_t minnum) {
+        cap = n / (minnum > 1 ? minnum : 1) + 1;
+        sort_bytes = 0;
+        nbx_sort_pairs_u64(nullptr, sort_bytes, nullptr, nullptr, nullptr, nullptr, cap, nullptr);
+        const size_t sz[9] = {sizeof(int32_t) * (size_t)n, sizeof(int32_t) * (size_t)n,
+                              sizeof(int32_t) * (size_t)n, sizeof(int32_t),
+                              sizeof(uint64_t) * (size_t)cap, sizeof(uint64_t) * (size_t)cap,
+                              sizeof(int32_t) * (size_t)cap, sizeof(int32_t) * (size_t)cap, sort_bytes};
+        size_t at = 0;
+        for (int i = 0; i < 9; ++i) {
+            off[i] = at;
+            at = (at + sz[i] + 255) & ~(size_t)255;
+        }
+        total = at;
+    }
+};
+
+size_t nbx_fof_canonical_bytes(int64_t n, int32_t minnum) { return FofCanonWork(n, minnum).total; }
+
 // Canonical labels: groups of >= minnum members numbered 1..G by size
-// descending, ties by smaller minimum member ID; everything else 0.
+// descending, ties by smaller minimum member ID; everything else 0.  work:
+// device memory of nbx_fof_canonical_bytes(n, minnum), 256-byte aligned.
 hipError_t nbx_fof_canonical(const DevTree &T, int32_t *P, int32_t minnum, int32_t *labels_by_id, int64_t *ngroups,
-                             hipStream_t st) {
+                             void *work, hipStream_t st) {
     int64_t n = T.n;
-    int32_t *cnt = nullptr, *minid = nullptr, *vals = nullptr, *vals2 = nullptr, *ng = nullptr, *lab = nullptr;
-    uint64_t *keys = nullptr, *keys2 = nullptr;
-    void *tmp = nullptr;
+    const FofCanonWork W(n, minnum);
+    char *wb = (char *)work;
+    int32_t *cnt = (int32_t *)(wb + W.off[0]), *minid = (int32_t *)(wb + W.off[1]), *lab = (int32_t *)(wb + W.off[2]);
+    int32_t *ng = (int32_t *)(wb + W.off[3]);
+    uint64_t *keys = (uint64_t *)(wb + W.off[4]), *keys2 = (uint64_t *)(wb + W.off[5]);
+    int32_t *vals = (int32_t *)(wb + W.off[6]), *vals2 = (int32_t *)(wb + W.off[7]);
+    void *tmp = wb + W.off[8];
     hipError_t e = hipSuccess;
-#define CK(x) do { e = (x); if (e != hipSuccess) goto done; } while (0)
-    CK(hipMallocAsync((void **)&cnt, sizeof(int32_t) * n, st));
-    CK(hipMallocAsync((void **)&minid, sizeof(int32_t) * n, st));
-    CK(hipMallocAsync((void **)&lab, sizeof(int32_t) * n, st));
-    CK(hipMallocAsync((void **)&ng, sizeof(int32_t), st));
+#define CK(x) do { e = (x); if (e != hipSuccess) return e; } while (0)
     hipLaunchKernelGGL(k_fill_i32, dim3(nblk(n)), dim3(256), 0, st, cnt, 0, n);
     hipLaunchKernelGGL(k_fill_i32, dim3(nblk(n)), dim3(256), 0, st, minid, 0x7fffffff, n);
     hipLaunchKernelGGL(k_fill_i32, dim3(nblk(n)), dim3(256), 0, st, lab, 0, n);
@@ -656,11 +683,6 @@ hipError_t nbx_fof_canonical(const DevTree &T, int32_t *P, int32_t minnum, int32
         hipLaunchKernelGGL(k_fof_store_root, dim3(nblk(n)), dim3(256), 0, st, P, n);
     }
     {
-        int64_t cap = n / (minnum > 1 ? minnum : 1) + 1;
-        CK(hipMallocAsync((void **)&keys, sizeof(uint64_t) * cap, st));
-        CK(hipMallocAsync((void **)&keys2, sizeof(uint64_t) * cap, st));
-        CK(hipMallocAsync((void **)&vals, sizeof(int32_t) * cap, st));
-        CK(hipMallocAsync((void **)&vals2, sizeof(int32_t) * cap, st));
         hipLaunchKernelGGL(k_fof_collect, dim3(nblk(n)), dim3(256), 0, st, P, cnt, minid, n, minnum > 1 ? minnum : 1,
                            keys, vals, ng);
         int32_t hng = 0;
@@ -668,25 +690,20 @@ hipError_t nbx_fof_canonical(const DevTree &T, int32_t *P, int32_t minnum, int32
         CK(hipStreamSynchronize(st));
         *ngroups = hng;
         if (hng > 0) {
+            // the sort's storage was sized for cap pairs; a sort of fewer
+            // that asks for more gets its own block
             size_t tb = 0;
             CK(nbx_sort_pairs_u64(nullptr, tb, keys, keys2, vals, vals2, hng, st));
-            CK(hipMallocAsync(&tmp, tb, st));
-            CK(nbx_sort_pairs_u64(tmp, tb, keys, keys2, vals, vals2, hng, st));
+            void *own = nullptr;
+            if (tb > W.sort_bytes) CK(hipMallocAsync(&own, tb, st));
+            e = nbx_sort_pairs_u64(own ? own : tmp, tb, keys, keys2, vals, vals2, hng, st);
+            if (own) hipFreeAsync(own, st);
+            CK(e);
             hipLaunchKernelGGL(k_fof_rank, dim3(nblk(hng)), dim3(256), 0, st, vals2, (int64_t)hng, lab);
         }
         hipLaunchKernelGGL(k_fof_out, dim3(nblk(n)), dim3(256), 0, st, P, lab, T.perm, n, labels_by_id);
         CK(hipGetLastError());
     }
-done:
-    if (cnt) hipFreeAsync(cnt, st);
-    if (minid) hipFreeAsync(minid, st);
-    if (lab) hipFreeAsync(lab, st);
-    if (ng) hipFreeAsync(ng, st);
-    if (keys) hipFreeAsync(keys, st);
-    if (keys2) hipFreeAsync(keys2, st);
-    if (vals) hipFreeAsync(vals, st);
-    if (vals2) hipFreeAsync(vals2, st);
-    if (tmp) hipFreeAsync(tmp, st);
 #undef CK
     return e;
 }

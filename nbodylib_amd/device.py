@@ -48,12 +48,16 @@ class DeviceTree:
         _check(lib().nbx_tree_create_dev(ctypes.byref(self._params), _ptr(pos), _ptr(vel), _ptr(mass), self.n,
                                          _stream(), ctypes.byref(h)))
         self._h = h
+        self._destroy = lib().nbx_tree_destroy
         self._keep = None  # the tree holds its own sorted copies
 
     def __del__(self):
+        # the destroy function is bound at creation: at interpreter exit the
+        # module's globals may already be gone
         h = getattr(self, "_h", None)
-        if h:
-            lib().nbx_tree_destroy(h)
+        destroy = getattr(self, "_destroy", None)
+        if h and destroy is not None:
+            destroy(h)
             self._h = None
 
     def order(self):
