@@ -127,12 +127,16 @@ __device__ __forceinline__ double boxbox_lb(cdptr a, cdptr b) {
 // point qc: the one whose box centre is nearer (a heuristic for the visiting
 // order only; pruning never depends on it).
 template <int ND>
-__device__ __forceinline__ bool first_is_right(const DevTree &T, int p, const float *qc) {
-    const cfptr pl = (cfptr)(T.plane + (size_t)p * (ND == 6 ? 8 : 4));
+__device__ __forceinline__ bool first_is_right(const float *plane, int p, const float *qc) {
+    const cfptr pl = (cfptr)(plane + (size_t)p * (ND == 6 ? 8 : 4));
     float s = 0.0f;
 #pragma unroll
     for (int d = 0; d < ND; ++d) s = __builtin_fmaf(pl[d], qc[d], s);
     return s > pl[ND];
+}
+template <int ND>
+__device__ __forceinline__ bool first_is_right(const DevTree &T, int p, const float *qc) {
+    return first_is_right<ND>(T.plane, p, qc);
 }
 
 // Workgroups of a launch are dealt round-robin to the 8 XCDs (workgroup b

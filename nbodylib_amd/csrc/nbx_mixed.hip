@@ -403,12 +403,20 @@ template <int ND, int KMAX>
 __device__ __forceinline__ void mix_walk(const DevTree &T, const double *ext, MixCtx<ND> &c, const float *qc, int anc,
                                          uint32_t *hk, uint32_t *sl, float *cx, int lane,
                                          uint32_t (&gm)[GShape<KMAX>::G]) {
-    const int D = T.depth;
+    // The depth and the box and plane bases as scalar values of their own.
+    // The tree view arrives as wide blocks of scalar registers, and the
+    // register allocator spills and restores such a block whole: a field
+    // read from it in the loop brought back 16 registers (16 v_readlane)
+    // per node in 6-D.
+    int D = T.depth;
+    const double *boxp = T.box;
+    const float *planep = T.plane;
+    asm volatile("" : "+s"(D), "+s"(boxp), "+s"(planep));
     int node = 1, dep = 0;
     uint64_t firstright = 0;
     if constexpr (ND == 3) {
         for (;;) {
-            cdptr bx = (cdptr)node_box(T, node, ND);
+            cdptr bx = (cdptr)(boxp + (size_t)node * 2 * ND);
             const bool want = c.act && box_lb<ND>(c.q, bx) <= (double)__uint_as_float(c.root);
             if (ballot(want)) {
                 if (dep == D) {
@@ -417,7 +425,7 @@ __device__ __forceinline__ void mix_walk(const DevTree &T, const double *ext, Mi
                 } else {
                     int first;
                     if ((anc >> (D - dep)) == node) first = anc >> (D - dep - 1);
-                    else first = 2 * node + (first_is_right<ND>(T, node, qc) ? 1 : 0);
+                    else first = 2 * node + (first_is_right<ND>(planep, node, qc) ? 1 : 0);
                     dep += 1;
                     firstright = (firstright & ~(1ULL << dep)) | ((uint64_t)(first & 1) << dep);
                     node = first;
@@ -435,7 +443,7 @@ __device__ __forceinline__ void mix_walk(const DevTree &T, const double *ext, Mi
         bool down = true;
         for (;;) {
             if (down) {
-                cdptr bx = (cdptr)node_box(T, node, ND);
+                cdptr bx = (cdptr)(boxp + (size_t)node * 2 * ND);
                 const bool want = c.act && box_lb<ND>(c.q, bx) <= (double)__uint_as_float(c.root);
                 if (ballot(want)) {
                     if (dep == D) {
@@ -447,7 +455,7 @@ __device__ __forceinline__ void mix_walk(const DevTree &T, const double *ext, Mi
                         int first = c0;
                         if (anc >= 0 && (anc >> (D - dep)) == node) {
                             first = anc >> (D - dep - 1);
-                        } else if (first_is_right<ND>(T, node, qc)) {
+                        } else if (first_is_right<ND>(planep, node, qc)) {
                             first = c0 + 1;
                         }
                         dep += 1;
