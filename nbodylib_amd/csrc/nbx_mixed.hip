@@ -196,8 +196,19 @@ struct MixCtx {
     int fillkh;     // kh while the heap is still empty and kh >= 32 (wave-uniform), else 0
 };
 
+// The leaf data a visit reads, each base a value of its own (see mix_walk):
+// origins and half-diagonals (scalar loads), and the FP32 offsets, 3-D
+// interleaved (scalar and vector loads) or 6-D per dimension (vector loads
+// only, so those bases are held in vector registers).
+template <int ND>
+struct MixSrc {
+    const double *org, *ext;
+    const float4 *xq;
+    const float *xf[ND == 6 ? 6 : 1];
+};
+
 template <int ND, int KMAX>
-__device__ __forceinline__ void mix_leaf(const DevTree &T, const double *ext, int leaf, MixCtx<ND> &c, bool want,
+__device__ __forceinline__ void mix_leaf(const MixSrc<ND> &S, int leaf, MixCtx<ND> &c, bool want,
                                          uint32_t *hk, uint32_t *sl, float *cx, int lane,
                                          uint32_t (&gm)[GShape<KMAX>::G]) {
     const int base = leaf * NBX_LEAF;
@@ -216,16 +227,16 @@ __device__ __forceinline__ void mix_leaf(const DevTree &T, const double *ext, in
     // this lane's part of the LDS copy used in phase 2
     float st[ND];
     if constexpr (ND == 3) {
-        const float4 v = T.xq[base + (lane & (NBX_LEAF - 1))];
+        const float4 v = S.xq[base + (lane & (NBX_LEAF - 1))];
         st[0] = v.x;
         st[1] = v.y;
         st[2] = v.z;
     } else {
 #pragma unroll
-        for (int d = 0; d < ND; ++d) st[d] = T.xf[d][base + (lane & (NBX_LEAF - 1))];
+        for (int d = 0; d < ND; ++d) st[d] = S.xf[d][base + (lane & (NBX_LEAF - 1))];
     }
     // query relative to this leaf's centre, rounded once to FP32
-    const double *o = T.org + (size_t)leaf * ND;
+    const double *o = S.org + (size_t)leaf * ND;
     float qv[ND];
 #pragma unroll
     for (int d = 0; d < ND; ++d) qv[d] = (float)(c.q[d] - ((cdptr)o)[d]);
@@ -248,14 +259,14 @@ __device__ __forceinline__ void mix_leaf(const DevTree &T, const double *ext, in
         typedef float fxc __attribute__((ext_vector_type(16)));
         typedef const __attribute__((address_space(4))) fxc *cfcptr;
         const float q2 = __builtin_fmaf(qv[2], qv[2], __builtin_fmaf(qv[1], qv[1], qv[0] * qv[0]));
-        const float r = sqrtf(q2) + (float)(((cdptr)ext)[leaf] * (1.0 + 0x1p-20));
+        const float r = sqrtf(q2) + (float)(((cdptr)S.ext)[leaf] * (1.0 + 0x1p-20));
         const float m = 0x1p-20f * __builtin_fmaf(r, r, th);
         // FLT_MAX (no bound yet): every real candidate passes
         const float w = fminf((th + m) - q2, 3.402823466e38f);
         const float a0 = 2.0f * qv[0], a1 = 2.0f * qv[1], a2 = 2.0f * qv[2];
         // two halves of 16 candidates (x, y, z, |p|^2), 64 scalar registers
         // each (the whole leaf at once would not fit the scalar register file)
-        const cfcptr lp = (cfcptr)(T.xq + base);
+        const cfcptr lp = (cfcptr)(S.xq + base);
 #pragma unroll 1
         for (int h = 0; h < 2; ++h) {
             fxc c4[4];
@@ -316,7 +327,7 @@ __device__ __forceinline__ void mix_leaf(const DevTree &T, const double *ext, in
     }
     if (c.self >= base && c.self < base + NBX_LEAF) mask &= ~(1u << (c.self - base));
     if (!c.act) mask = 0;
-    if (c.act && (want || mask)) c.emax = fmax(c.emax, ((cdptr)ext)[leaf]);
+    if (c.act && (want || mask)) c.emax = fmax(c.emax, ((cdptr)S.ext)[leaf]);
     if constexpr (ND == 6) {
         // no lane accepts anything: nothing to insert, no visit number
         if (!ballot(mask != 0)) return;
@@ -408,20 +419,34 @@ __device__ __forceinline__ void mix_walk(const DevTree &T, const double *ext, Mi
     // register allocator spills and restores such a block whole: a field
     // read from it in the loop brought back 16 registers (16 v_readlane)
     // per node in 6-D.
-    int D = T.depth;
+    int D = T.depth, nleafp = T.nleafp;
     const double *boxp = T.box;
     const float *planep = T.plane;
-    asm volatile("" : "+s"(D), "+s"(boxp), "+s"(planep));
+    asm volatile("" : "+s"(D), "+s"(nleafp), "+s"(boxp), "+s"(planep));
+    MixSrc<ND> S;
+    S.org = T.org;
+    S.ext = ext;
+    asm volatile("" : "+s"(S.org), "+s"(S.ext));
+    if constexpr (ND == 3) {
+        S.xq = T.xq;
+        asm volatile("" : "+s"(S.xq));
+    } else {
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+            S.xf[d] = T.xf[d];
+            asm volatile("" : "+v"(S.xf[d]));
+        }
+    }
     int node = 1, dep = 0;
     uint64_t firstright = 0;
     if constexpr (ND == 3) {
         for (;;) {
             cdptr bx = (cdptr)(boxp + (size_t)node * 2 * ND);
-            const bool want = c.act && box_lb<ND>(c.q, bx) <= (double)__uint_as_float(c.root);
+            const bool want = box_lb<ND>(c.q, bx) <= (double)__uint_as_float(c.root);
             if (ballot(want)) {
                 if (dep == D) {
                     if (c.nvis >= MX_V) break;  // visit table full: lanes fall back
-                    mix_leaf<ND, KMAX>(T, ext, node - T.nleafp, c, want, hk, sl, cx, lane, gm);
+                    mix_leaf<ND, KMAX>(S, node - nleafp, c, want, hk, sl, cx, lane, gm);
                 } else {
                     int first;
                     if ((anc >> (D - dep)) == node) first = anc >> (D - dep - 1);
@@ -444,11 +469,11 @@ __device__ __forceinline__ void mix_walk(const DevTree &T, const double *ext, Mi
         for (;;) {
             if (down) {
                 cdptr bx = (cdptr)(boxp + (size_t)node * 2 * ND);
-                const bool want = c.act && box_lb<ND>(c.q, bx) <= (double)__uint_as_float(c.root);
+                const bool want = box_lb<ND>(c.q, bx) <= (double)__uint_as_float(c.root);
                 if (ballot(want)) {
                     if (dep == D) {
                         if (c.nvis >= MX_V) break;  // visit table full: lanes fall back
-                        mix_leaf<ND, KMAX>(T, ext, node - T.nleafp, c, want, hk, sl, cx, lane, gm);
+                        mix_leaf<ND, KMAX>(S, node - nleafp, c, want, hk, sl, cx, lane, gm);
                         down = false;
                     } else {
                         const int c0 = 2 * node;
@@ -589,8 +614,11 @@ __device__ __forceinline__ void mix_task(const DevTree &T, const KnnArgs &A, con
     const bool valid = qi < T.n;
     const int kres = T.kernres;
     MixCtx<ND> c;
+    // A lane past the end queries from the padding sentinel: node boxes
+    // bound real particles only, so every box bound from there is +inf and
+    // the walk needs no activity test (the leaf code still masks the lane).
 #pragma unroll
-    for (int d = 0; d < ND; ++d) c.q[d] = valid ? T.x[d][qi] : 0.0;
+    for (int d = 0; d < ND; ++d) c.q[d] = valid ? T.x[d][qi] : NBX_PAD_COORD;
     c.act = valid;
     c.self = qi;
     c.root = 0u;
