@@ -55,6 +55,7 @@ typedef const __attribute__((address_space(4))) fx8 *cf8ptr;
 
 #define HK(p) hk[(p) * NBX_WAVE + lane]   // kept keys (LDS)
 typedef __attribute__((address_space(1))) uint16_t gu16;
+typedef __attribute__((address_space(1))) uint32_t gu32;
 #define SLH(p) ((gu16 *)sl + lane)[(p) * NBX_WAVE]  // their handles (scratch, by position)
 #define MX_V 2048  // leaves a wave can record: handle = leaf's visit number << 5 | index in leaf
 
@@ -796,6 +797,26 @@ __device__ __forceinline__ void mix_task(const DevTree &T, const KnnArgs &A, con
         if (valid && !ok) fb_list[base + __popcll(fbm & ((1ULL << lane) - 1))] = qi;
     }
 
+    // k <= 32: the slots of the k kept entries, decoded in one batch (every
+    // handle load, then every visit-table load, instead of a dependent pair
+    // per entry) and stored over their keys, which nothing reads after this;
+    // the density and the output rows take them from there
+    if constexpr (GT) {
+        if (ok) {
+            const gu32 *vtg = (const gu32 *)vt;
+            uint32_t hv[KMAX];
+#pragma unroll
+            for (int p = 0; p < KMAX; ++p) hv[p] = p < k ? (uint32_t)SLH(SKC(p) & TM) : 0u;
+#pragma unroll
+            for (int p = 0; p < KMAX; ++p)
+                if (p < k) SKC(p) = vtg[hv[p] >> 5] * NBX_LEAF + (hv[p] & 31u);
+        }
+    }
+    auto slot_final = [&](int p) -> int32_t {
+        if constexpr (GT) return (int32_t)SKC(p);
+        else return (int32_t)SLB(p);
+    };
+
     // ---------------------------------------------------------------- density
     // Symmetric scatter: contributions to the wave's own 64 queries (about
     // half of all neighbour entries) are summed in LDS and flushed with one
@@ -819,7 +840,7 @@ __device__ __forceinline__ void mix_task(const DevTree &T, const KnnArgs &A, con
             double d2v[MX_DB], mv[MX_DB];
 #pragma unroll
             for (int u = 0; u < MX_DB; ++u) {
-                sv[u] = p0 + u < k ? slot_at(p0 + u) : qi;
+                sv[u] = p0 + u < k ? slot_final(p0 + u) : qi;
                 d2v[u] = d64<ND>(T, c.q, sv[u]);
                 mv[u] = T.mass[sv[u]];
             }
@@ -857,10 +878,7 @@ __device__ __forceinline__ void mix_task(const DevTree &T, const KnnArgs &A, con
         if (qr >= T.n) break;
         if (!((okm >> r) & 1ULL)) continue;
         int32_t s;
-        if constexpr (GT) {
-            const uint32_t h = ((const uint16_t *)sl)[(hk[t * NBX_WAVE + r] & TM) * NBX_WAVE + r];
-            s = (int32_t)(vt[h >> 5] * NBX_LEAF + (h & 31u));
-        }
+        if constexpr (GT) s = (int32_t)hk[t * NBX_WAVE + r];  // decoded above
         else s = (int32_t)slb[t * NBX_WAVE + r];
         int64_t row = A.layout == 1 ? qr - A.row0 : T.perm[qr];
         A.nn[row * k + t] = A.layout == 1 ? s : T.perm[s];
