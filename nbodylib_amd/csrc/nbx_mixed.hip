@@ -832,6 +832,11 @@ __device__ __forceinline__ void mix_task(const DevTree &T, const KnnArgs &A, con
         const double hi = 0.5 * sqrt(Dk);
         const double norm = 1.0 / pow(hi, (double)ND);
         const double mi = T.mass[qi];
+        // the table lookup of Wsm (KDCalcSmoothQuantities.cxx:12-15) with
+        // its divisions by h and by the table step turned into products:
+        // index floor(r/h * c), c = (kernres - 1) / 2 = 1 / step, and the
+        // linear weight of the next entry the index's fraction
+        const double ih = 1.0 / hi, ck = 0.5 * (kres - 1);
         double acc = 0.0;
         // neighbours in groups of MX_DB so their coordinate/mass gathers are in
         // flight together
@@ -848,7 +853,14 @@ __device__ __forceinline__ void mix_task(const DevTree &T, const KnnArgs &A, con
             for (int u = 0; u < MX_DB; ++u) {
                 if (p0 + u >= k) break;
                 const int32_t s = sv[u];
-                double W = wsm_at(sqrt(d2v[u]), hi, T.kern, kres);
+                const double fi = sqrt(d2v[u]) * ih * ck;
+                double W;
+                if (!(fi < (double)(kres - 1))) {
+                    W = T.kern[kres - 1];
+                } else {
+                    const int i = (int)fi;
+                    W = T.kern[i] + (T.kern[i + 1] - T.kern[i]) * (fi - (double)i);
+                }
                 if (sym) {
                     double Wij = 0.5 * W * norm;
                     acc = acc + Wij * mv[u];
@@ -871,9 +883,22 @@ __device__ __forceinline__ void mix_task(const DevTree &T, const KnnArgs &A, con
     if (A.nn == nullptr) return;
     const uint64_t okm = ballot(ok);
     __syncthreads();
-    const int total = NBX_WAVE * k;
-    for (int e = lane; e < total; e += NBX_WAVE) {
-        int r = e / k, t = e - r * k;
+    // entry e = lane + 64 i of the wave's rows is row r = e / k, column
+    // t = e % k: stepped without a division in 3-D, divided in 6-D (each the
+    // measured faster form of its instance)
+    const int q64 = NBX_WAVE / k, m64 = NBX_WAVE - q64 * k;
+    int r = lane / k, t = lane - (lane / k) * k;
+    for (int i = 0; i < k; ++i, t += m64, r += q64) {
+        if constexpr (ND == 3) {
+            if (t >= k) {
+                t -= k;
+                r += 1;
+            }
+        } else {
+            const int e = lane + NBX_WAVE * i;
+            r = e / k;
+            t = e - r * k;
+        }
         int qr = g * NBX_WAVE + r;
         if (qr >= T.n) break;
         if (!((okm >> r) & 1ULL)) continue;
