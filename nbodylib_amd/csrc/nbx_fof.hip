@@ -198,7 +198,7 @@ __device__ __forceinline__ void fof_leaf(const DevTree &T, int leaf, const doubl
     if (encl) rq = uf_union_r(P, rq, base);
     const bool pair = act && !encl;
     if (!ballot(pair)) return;
-    uint32_t mask = 0;
+    uint32_t mask = 0, unsure = 0;
     // 64-byte (3-D) or 32-byte (6-D) scalar-load chunks per dimension
     constexpr int CH = ND == 6 ? 4 : 8;
     typedef double dxc __attribute__((ext_vector_type(CH)));
@@ -216,20 +216,53 @@ __device__ __forceinline__ void fof_leaf(const DevTree &T, int leaf, const doubl
                 const double t = q[d] - cv[d][u];
                 t2[d] = t * t;
             }
+            // decisions are shifted in as they are made (a bit per candidate
+            // held in scalar masks until the end would spill them all); the
+            // comparators' values inside the reciprocal band are decided
+            // with the divisions after the scan, or at once in the 6-D
+            // persistent kernel (each the measured better form)
             bool near;
             if constexpr (MT == 0) {
                 near = fof_val<ND, MT>(t2, L) < L.lim;
+            } else if constexpr (ND == 3) {
+                const double v = fof_val_fast<ND, MT>(t2, L);
+                near = v < NBX_FOF_BAND_LO;
+                unsure = unsure + unsure + ((v >= NBX_FOF_BAND_LO && v < NBX_FOF_BAND_HI) ? 1u : 0u);
             } else {
                 const double v = fof_val_fast<ND, MT>(t2, L);
                 near = v < NBX_FOF_BAND_LO;
-                const bool unsure = v >= NBX_FOF_BAND_LO && v < NBX_FOF_BAND_HI;
-                if (ballot(unsure) && unsure) near = fof_val<ND, MT>(t2, L) < 1.0;
+                const bool uns = v >= NBX_FOF_BAND_LO && v < NBX_FOF_BAND_HI;
+                if (ballot(uns) && uns) near = fof_val<ND, MT>(t2, L) < 1.0;
             }
-            const int32_t cs = base + jj + u;
-            const bool l = near && (PASS0 ? (lconn ? cs != qi : cs > qi) : cs != qi);
-            mask |= l ? (1u << (jj + u)) : 0u;
+            mask = mask + mask + (near ? 1u : 0u);
         }
     }
+    if constexpr (MT != 0 && ND == 3) {
+        // unsure holds candidate j at bit 31 - j, as mask does here
+        if (ballot(unsure != 0)) {
+            uint32_t um = unsure;
+            while (um) {
+                const int b = 31 - __builtin_clz(um);
+                um &= ~(1u << b);
+                const int32_t cs = base + (31 - b);
+                double t2[ND];
+#pragma unroll
+                for (int d = 0; d < ND; ++d) {
+                    const double t = q[d] - T.x[d][cs];
+                    t2[d] = t * t;
+                }
+                if (fof_val<ND, MT>(t2, L) < 1.0) mask |= 1u << b;
+            }
+        }
+    }
+    // candidate j was shifted in at bit 31 - j.  Never the query itself; in
+    // the first pass, in a leaf not internally connected, only candidates at
+    // larger slots (the pair is met from both sides).
+    mask = __builtin_bitreverse32(mask);
+    const int qrel = qi - base;
+    const uint32_t self = (qrel >= 0 && qrel < NBX_LEAF) ? (1u << qrel) : 0u;
+    const uint32_t above = qrel < 0 ? ~0u : (qrel >= NBX_LEAF - 1 ? 0u : (~0u << (qrel + 1)));
+    mask &= (PASS0 && !lconn) ? above : ~self;
     if (!pair) mask = 0;
     if (!ballot(mask != 0)) return;
     // The leaf's parents, read once with ordinary loads.  A candidate whose
