@@ -155,6 +155,7 @@ struct nbx_tree {
     int32_t *perm = nullptr;
     int32_t *inv = nullptr;
     double *box = nullptr;
+    float *boxf = nullptr;  // precision form: outward FP32 node boxes (null when coordinates exceed 1e30)
     float *plane = nullptr;
     double *org = nullptr;
     float *xf[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -191,6 +192,7 @@ struct nbx_tree {
         T.perm = perm;
         T.inv = inv;
         T.box = box;
+        T.boxf = boxf;
         T.plane = plane;
         T.org = org;
         T.kern = kern;
@@ -248,6 +250,7 @@ static void free_tree(nbx_tree *t) {
     if (t->perm) hipFree(t->perm);
     if (t->inv) hipFree(t->inv);
     if (t->box) hipFree(t->box);
+    if (t->boxf) hipFree(t->boxf);
     if (t->plane) hipFree(t->plane);
     if (t->org) hipFree(t->org);
     if (t->kern) hipFree(t->kern);
@@ -344,6 +347,22 @@ static int build_tree(nbx_tree *t, const double *pos, const double *vel, const d
                                1 << dep, 1 << dep, nd);
     }
     HIPCK(hipGetLastError());
+    if (t->p.form == NBX_FORM_MIXED) {
+        // FP32 node boxes for the precision form's walk, when every
+        // coordinate lies within 1e30 (the bound its FP32 node test assumes;
+        // otherwise the precision-form calls run the FP64 kernel)
+        double rb[12];
+        HIPCK(hipMemcpyAsync(rb, t->box + 2 * nd, sizeof(double) * 2 * nd, hipMemcpyDeviceToHost, st));
+        HIPCK(hipStreamSynchronize(st));
+        bool fits = true;
+        for (int d = 0; d < 2 * nd; ++d) fits = fits && fabs(rb[d]) <= 1e30;
+        if (fits) {
+            const int64_t cnt = 2 * nd * (int64_t)t->nnodes;
+            HIPCK(hipMalloc((void **)&t->boxf, sizeof(float) * cnt));
+            hipLaunchKernelGGL(k_boxf, dim3(nblk(cnt)), dim3(256), 0, st, t->box, t->boxf, cnt, nd);
+            HIPCK(hipGetLastError());
+        }
+    }
     // kernel table
     make_kernel(nd, t->p.kerntype, t->kernres, t->kernnorm, t->hkern);
     HIPCK(hipMalloc((void **)&t->kern, sizeof(double) * t->kernres));
@@ -557,7 +576,9 @@ extern "C" int nbx_tree_last_fallbacks(const nbx_tree *t, int64_t *fallbacks) {
 // precision-form kernel keeps k + 1 entries up to 65 (k <= 64); larger k
 // run on the FP64 kernel, whose neighbour sets the precision form equals.
 static int heap_cap(const nbx_tree *) { return 129; }
-static bool mixed_ok(const nbx_tree *t, int32_t k) { return t->p.form == NBX_FORM_MIXED && k + 1 <= 65; }
+static bool mixed_ok(const nbx_tree *t, int32_t k) {
+    return t->p.form == NBX_FORM_MIXED && k + 1 <= 65 && t->boxf != nullptr;
+}
 
 static int setup_knn(nbx_tree *t, int32_t k, int32_t variant, KnnArgs &A, int &nq) {
     memset(&A, 0, sizeof(A));

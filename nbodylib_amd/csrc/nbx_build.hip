@@ -145,6 +145,28 @@ __global__ __launch_bounds__(256) void k_levelbox(double *__restrict__ box, floa
     for (int d = nd + 1; d < plw; ++d) pl[d] = 0.0f;
 }
 
+// the next FP32 value above f (f finite or -inf; +inf and NaN unchanged)
+__device__ __forceinline__ float f32_next_up(float f) {
+    if (!(f < 1.0f / 0.0f)) return f;
+    if (f == 0.0f) return __uint_as_float(1u);
+    const uint32_t b = __float_as_uint(f);
+    return __uint_as_float(f > 0.0f ? b + 1u : b - 1u);
+}
+
+// FP32 copy of the node boxes rounded outward (lower corners down, upper
+// corners up), for the precision form's node tests; count = 2 nd per node.
+__global__ __launch_bounds__(256) void k_boxf(const double *__restrict__ box, float *__restrict__ boxf, int64_t count,
+                                              int nd) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= count) return;
+    const double v = box[i];
+    const bool upper = (int)(i % (2 * nd)) >= nd;
+    float f = (float)v;
+    if (upper && (double)f < v) f = f32_next_up(f);
+    if (!upper && (double)f > v) f = -f32_next_up(-f);
+    boxf[i] = f;
+}
+
 // ---------------------------------------------------------------- sort
 // sort of 64-bit keys with 32-bit values (FOF group ranking)
 hipError_t nbx_sort_pairs_u64(void *temp, size_t &temp_bytes, uint64_t *kin, uint64_t *kout, int32_t *vin,

@@ -41,6 +41,7 @@ struct DevTree {
     const int32_t *perm;   // slot -> ID
     const int32_t *inv;    // ID -> slot
     const double *box;     // node boxes, [node][lo[nd], hi[nd]]
+    const float *boxf;     // precision form: the node boxes rounded outward to FP32 (same layout)
     const float *plane;    // internal nodes: child-order plane [node][n[nd], off] (4 or 8 floats)
     const double *org;     // mixed form: per-leaf FP64 origin [leaf][nd]
     const float *xf[6];    // mixed form, 6-D: FP32 offsets from the leaf origin
@@ -81,6 +82,29 @@ __device__ __forceinline__ double box_lb(const double *q, cdptr b) {
         // at most one of (lo - q), (q - hi) is positive; max() is exact
         double t = fmax(fmax(b[d] - q[d], q[d] - b[ND + d]), 0.0);
         s += t * t;
+    }
+    return s;
+}
+// A lower bound on the squared distance from q to any point of the FP32
+// box b (rounded outward from the FP64 box), never above the FP64 box_lb of
+// the same node, so a walk that opens a node when this bound reaches its
+// threshold opens every node the FP64 test would.  q32 = fl32(q), with
+// |q| <= 1e30 (the tree's coordinates are checked when the FP32 boxes are
+// made).  Per dimension the FP32 gap t over-states the exact gap g by at most
+// (g + u|q|)(1 + u), u = 2^-24, because the outward box only shrinks the gap
+// and q32 and the subtraction add one rounding each; t (1 - 2^-20) -
+// 2^-20 |q32|, rounded once by the FMA, is then at most g (1 - 14u), and the
+// squares and their sum (at most 2 ND roundings) stay below the exact bound
+// times (1 - 2^-20), itself below the FP64 value, which is within 2^-50
+// relative of exact.  An empty box (+inf, -inf) gives +inf.
+template <int ND>
+__device__ __forceinline__ float box_lb32(const float *q32, cfptr b) {
+    float s = 0.0f;
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        const float t = fmaxf(b[d] - q32[d], q32[d] - b[ND + d]);
+        const float tl = fmaxf(__builtin_fmaf(t, 1.0f - 0x1p-20f, -0x1p-20f * fabsf(q32[d])), 0.0f);
+        s = __builtin_fmaf(tl, tl, s);
     }
     return s;
 }

@@ -31,6 +31,7 @@ __global__ void k_bbox(const double *c, int64_t n, double *part);
 __global__ void k_gather_leaf(GatherArgs a);
 __global__ void k_leafbox_pad(double *box, int32_t nleaf, int32_t nleafp, int nd);
 __global__ void k_levelbox(double *box, float *plane, int32_t first, int32_t count, int nd);
+__global__ void k_boxf(const double *box, float *boxf, int64_t count, int nd);
 
 // order (slot -> ID) of the implicit kd tree: midpoint splits along the
 // widest dimension of every node (nbx_kdbuild.hip)

@@ -421,9 +421,15 @@ __device__ __forceinline__ void mix_walk(const DevTree &T, const double *ext, Mi
     // read from it in the loop brought back 16 registers (16 v_readlane)
     // per node in 6-D.
     int D = T.depth, nleafp = T.nleafp;
-    const double *boxp = T.box;
+    const float *boxp = T.boxf;
     const float *planep = T.plane;
     asm volatile("" : "+s"(D), "+s"(nleafp), "+s"(boxp), "+s"(planep));
+    // node tests in FP32 on the outward-rounded boxes: never above the FP64
+    // bound, so every node the FP64 test opens is opened (a lane past the end
+    // sits at 1e30, beyond every box)
+    float q32[ND];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) q32[d] = c.act ? (float)c.q[d] : NBX_PAD_COORD_F;
     MixSrc<ND> S;
     S.org = T.org;
     S.ext = ext;
@@ -442,8 +448,8 @@ __device__ __forceinline__ void mix_walk(const DevTree &T, const double *ext, Mi
     uint64_t firstright = 0;
     if constexpr (ND == 3) {
         for (;;) {
-            cdptr bx = (cdptr)(boxp + (size_t)node * 2 * ND);
-            const bool want = box_lb<ND>(c.q, bx) <= (double)__uint_as_float(c.root);
+            cfptr bx = (cfptr)(boxp + (size_t)node * 2 * ND);
+            const bool want = box_lb32<ND>(q32, bx) <= __uint_as_float(c.root);
             if (ballot(want)) {
                 if (dep == D) {
                     if (c.nvis >= MX_V) break;  // visit table full: lanes fall back
@@ -469,8 +475,8 @@ __device__ __forceinline__ void mix_walk(const DevTree &T, const double *ext, Mi
         bool down = true;
         for (;;) {
             if (down) {
-                cdptr bx = (cdptr)(boxp + (size_t)node * 2 * ND);
-                const bool want = box_lb<ND>(c.q, bx) <= (double)__uint_as_float(c.root);
+                cfptr bx = (cfptr)(boxp + (size_t)node * 2 * ND);
+                const bool want = box_lb32<ND>(q32, bx) <= __uint_as_float(c.root);
                 if (ballot(want)) {
                     if (dep == D) {
                         if (c.nvis >= MX_V) break;  // visit table full: lanes fall back
@@ -996,7 +1002,8 @@ hipError_t nbx_launch_knn_mixed(const DevTree &T, const KnnArgs &A, const double
                                 int32_t *fallback_count, hipStream_t st, void *work, size_t work_bytes) {
     const int nw = A.wcnt > 0 ? A.wcnt : (T.n + NBX_WAVE - 1) / NBX_WAVE;
     if (nw == 0) return hipSuccess;
-    if ((T.nd != 3 && T.nd != 6) || A.kh > 65 || A.kh < 2 || !T.kernf || !T.org || !ext) return hipErrorInvalidValue;
+    if ((T.nd != 3 && T.nd != 6) || A.kh > 65 || A.kh < 2 || !T.kernf || !T.org || !T.boxf || !ext)
+        return hipErrorInvalidValue;
     const int km = A.kh <= 33 ? 33 : 65;
     const int grid = mx_launch_grid(T.nd, km);
     const size_t nscr = (size_t)grid * MX_BLKW(km);

@@ -87,11 +87,13 @@ def test_lattice_ties(gpu, form, k):
     assert z.all() or np.max(np.abs(rho[~z] / ro[~z] - 1.0)) < 1e-12
 
 
-@pytest.mark.parametrize("offset", [1.0e4, 1.0e6])
+@pytest.mark.parametrize("offset", [1.0e4, 1.0e6, 1.0e8])
 def test_precision_form_far_box(gpu, offset):
     """A unit box placed far from the origin: the precision form works on
     FP32 offsets from each leaf's FP64 centre, so its neighbour sets stay
-    exact and its densities within 1e-12 of the FP64 form."""
+    exact and its densities within 1e-12 of the FP64 form.  At 1e8 the FP32
+    node boxes, rounded outward, are wider than the whole cloud, so the walk
+    prunes little but still finds the same sets."""
     nb = _nb()
     n, k = 20000, 32
     say(f"far box offset={offset:g} n={n} k={k}")
@@ -106,6 +108,31 @@ def test_precision_form_far_box(gpu, offset):
     assert np.array_equal(np.sort(npl, 1), np.sort(bi, 1))
     rm, rp = tm.CalcDensity(k), tp.CalcDensity(k)
     assert np.max(np.abs(rm / rp - 1.0)) < 1e-12
+
+
+def test_precision_form_huge_coordinate(gpu):
+    """One particle at 1e31 puts the tree beyond the range the precision
+    form's FP32 node test assumes (1e30): its calls then run the FP64 kernel,
+    with the FP64 form's results and errors."""
+    nb = _nb()
+    n, k = 5000, 16
+    say(f"huge coordinate n={n} k={k}")
+    pos = datagen.uniform(n, 1.0, 63)
+    pos[17] = (1.0e31, 0.5, 0.5)
+    tm = nb.KDTree(pos, kerntype=nb.KSPH, form=nb.FORM_MIXED)
+    tp = nb.KDTree(pos, kerntype=nb.KSPH, form=nb.FORM_PLAIN)
+    nm, dm = tm.FindNearest(k)
+    npl, dp = tp.FindNearest(k)
+    assert np.array_equal(nm, npl)
+    assert np.array_equal(dm, dp)
+    bi, bd = oracle.knn_brute(pos, k, mode=0)
+    assert np.array_equal(np.sort(npl, 1), np.sort(bi, 1))
+    # the far particle has no neighbour within the queue's initial bound
+    # (MAXVALUE = 1e32 in d2, Precision.h:48), so CalcDensity fails in both
+    # forms, as the reference's does
+    for t in (tm, tp):
+        with pytest.raises(nb.NbxError):
+            t.CalcDensity(k)
 
 
 @pytest.mark.parametrize("periodic", [False, True])
