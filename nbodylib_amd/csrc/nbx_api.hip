@@ -7,6 +7,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -163,7 +164,7 @@ struct nbx_tree {
     double *kern = nullptr;
     float *kernf = nullptr;
     double *ext = nullptr;
-    int64_t last_fallbacks = 0;
+    std::atomic<int64_t> last_fallbacks{0};  // set by every precision-form search, from any thread
     int kernres;
     double kernnorm;
     std::vector<double> hkern;
@@ -560,13 +561,13 @@ static hipError_t run_mixed(nbx_tree *t, KnnArgs A, int k, hipStream_t st, Works
         F.wbeg = F.wcnt = 0;
         e = nbx_launch_knn(T, F, hc, st);
     }
-    t->last_fallbacks = hc;
+    t->last_fallbacks.store(hc);
     return e;
 }
 
 extern "C" int nbx_tree_last_fallbacks(const nbx_tree *t, int64_t *fallbacks) {
     if (!t || !fallbacks) return fail(NBX_ERR_ARG, "NULL argument");
-    *fallbacks = t->last_fallbacks;
+    *fallbacks = t->last_fallbacks.load();
     return NBX_OK;
 }
 
