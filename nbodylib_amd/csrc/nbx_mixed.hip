@@ -839,10 +839,12 @@ __device__ __forceinline__ void mix_task(const DevTree &T, const KnnArgs &A, con
         const double norm = 1.0 / pow(hi, (double)ND);
         const double mi = T.mass[qi];
         // the table lookup of Wsm (KDCalcSmoothQuantities.cxx:12-15) with
-        // its divisions by h and by the table step turned into products:
-        // index floor(r/h * c), c = (kernres - 1) / 2 = 1 / step, and the
-        // linear weight of the next entry the index's fraction
-        const double ih = 1.0 / hi, ck = 0.5 * (kres - 1);
+        // its division by the table step turned into a product: index
+        // floor(r/h * c), c = (kernres - 1) / 2 = 1 / step, and the linear
+        // weight of the next entry the index's fraction.  r/h stays a
+        // division, so the k-th neighbour's is exactly 2 and it takes the
+        // table's last entry, as in the reference.
+        const double ck = 0.5 * (kres - 1);
         double acc = 0.0;
         // neighbours in groups of MX_DB so their coordinate/mass gathers are in
         // flight together
@@ -859,7 +861,7 @@ __device__ __forceinline__ void mix_task(const DevTree &T, const KnnArgs &A, con
             for (int u = 0; u < MX_DB; ++u) {
                 if (p0 + u >= k) break;
                 const int32_t s = sv[u];
-                const double fi = sqrt(d2v[u]) * ih * ck;
+                const double fi = sqrt(d2v[u]) / hi * ck;
                 double W;
                 if (!(fi < (double)(kres - 1))) {
                     W = T.kern[kres - 1];

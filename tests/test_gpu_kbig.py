@@ -120,6 +120,30 @@ def test_mixed_fp64_distances_equal_plain(gpu, k, layout):
     assert np.array_equal(out[nb.FORM_MIXED][1], out[nb.FORM_PLAIN][1])
 
 
+@pytest.mark.parametrize("k", [1, 7, 20, 33, 50])
+def test_mixed_rows_odd_k(gpu, k):
+    """Precision-form rows for k that do not divide the 64 lanes of a wave
+    (the output rows of a wave wrap across lanes), against the FP64 form's,
+    by ID and by slot, with distances."""
+    import nbodylib_amd as nb
+
+    n = 20011
+    say(f"mixed rows odd k={k} n={n}")
+    pos = datagen.clustered(n, 100.0, nhalo=30, seed=88)[0]
+    for layout in (0, 1):
+        rows = {}
+        for form in (nb.FORM_PLAIN, nb.FORM_MIXED):
+            t = nb.KDTree(pos, kerntype=nb.KSPH, form=form)
+            rows[form] = t._knn(k, nb.KNN_FINDNEAREST, layout)
+        assert np.array_equal(rows[nb.FORM_MIXED][0], rows[nb.FORM_PLAIN][0])
+        assert np.array_equal(rows[nb.FORM_MIXED][1], rows[nb.FORM_PLAIN][1])
+    tm = nb.KDTree(pos, kerntype=nb.KSPH, form=nb.FORM_MIXED)
+    tp = nb.KDTree(pos, kerntype=nb.KSPH, form=nb.FORM_PLAIN)
+    rm, rp = tm.CalcDensity(k), tp.CalcDensity(k)
+    # (k = 1: the only neighbour sits at r = 2h, where the kernel is 0)
+    assert np.all(np.abs(rm - rp) <= 2e-6 * np.abs(rp))
+
+
 @pytest.mark.parametrize("band", [1e-12, 1e-5, 1e-3])
 def test_mixed_recheck_band(gpu, band):
     import nbodylib_amd as nb
