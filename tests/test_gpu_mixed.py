@@ -103,12 +103,15 @@ def test_mixed_phase_knn_sets(gpu, k):
     say(f"  fallbacks={t.last_fallbacks()}")
 
 
-def test_mixed_phase_density(gpu):
+@pytest.mark.parametrize("k", [20, 64])
+def test_mixed_phase_density(gpu, k):
     """6-D CalcDensity on a TPHS tree in the precision form against the FP64
-    oracle (6-D norm 3024/(762 pi^3), metric factor 1)."""
+    oracle (6-D norm 3024/(762 pi^3), metric factor 1), for both kept-key
+    widths (k <= 32 and k <= 64); and the FindNearestPhase rows with their
+    FP64 distances against brute force."""
     import nbodylib_amd as nb
 
-    n, k = 20000, 64
+    n = 20000
     say(f"mixed phase density n={n} k={k}")
     x, v = _phase_set(n, 35)
     m = datagen.lognormal_masses(n, 35)
@@ -118,6 +121,10 @@ def test_mixed_phase_density(gpu):
     rel = np.max(np.abs(rho / ro - 1.0))
     say(f"  max rel {rel:.3e} fallbacks={t.last_fallbacks()}")
     assert rel < MIXED_TOL
+    nn, d2 = t.FindNearestPhase(k)
+    bi, bd = oracle.knn_brute(x, k, mode=0, vel=v, nd=6)
+    assert np.array_equal(np.sort(nn, 1), np.sort(bi, 1))
+    assert np.array_equal(d2, bd)
 
 
 def test_mixed_sets_full_size():
