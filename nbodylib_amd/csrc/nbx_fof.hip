@@ -297,9 +297,24 @@ __device__ __forceinline__ void fof_walk(const DevTree &T, const double *q, bool
     // single leaf when the tree has one)
     const int gnode = D >= 1 ? (T.nleafp + 2 * g) >> 1 : 1;
     cdptr gb = (cdptr)node_box(T, gnode, ND);
-    int node = 1;
+    // In the first pass a node whose slots all lie below the wave's first
+    // slot need not be entered.  Each of its links with a wave particle is
+    // also met from the lower particle's wave, whose walk reaches the wave's
+    // leaves (a node holding a slot above that wave's own is never skipped
+    // there): the pair's computed distance is the same from both sides,
+    // fl(a - b) = -fl(b - a), and each route that side can take (scan,
+    // enclosure of a connected leaf, wave-node shortcut) puts the pair in
+    // one set.  On a 6-D tree every such node is skipped; on a 3-D tree
+    // only such leaves, because there the wave-node shortcuts met in the
+    // lower subtrees join dense regions in fewer unions than their pairs'
+    // links do (the measured better form for each).  sh is the node's
+    // height, so its leaves are [node << sh, (node + 1) << sh) in heap
+    // order.
+    const int lfirst = T.nleafp + 2 * g;
+    int node = 1, sh = D;
     bool down = true;
     for (;;) {
+        if (down && PASS0 && (ND == 6 || sh == 0) && ((node + 1) << sh) <= lfirst) down = false;
         if (down) {
             cdptr bx = (cdptr)node_box(T, node, ND);
             bool want = act && fof_maybe<MT>(fof_lb<ND, MT>(q, bx, L), L);
@@ -320,6 +335,7 @@ __device__ __forceinline__ void fof_walk(const DevTree &T, const double *q, bool
                     down = false;
                 } else {
                     node = 2 * node;
+                    --sh;
                     continue;
                 }
             } else {
@@ -332,6 +348,7 @@ __device__ __forceinline__ void fof_walk(const DevTree &T, const double *q, bool
             down = true;
         } else {
             node >>= 1;
+            ++sh;
         }
     }
 }
