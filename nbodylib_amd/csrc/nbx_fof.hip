@@ -195,7 +195,16 @@ __device__ __forceinline__ void fof_leaf(const DevTree &T, int leaf, const doubl
     const bool lconn = conn[T.nleafp + leaf];
     cdptr bx = (cdptr)node_box(T, T.nleafp + leaf, ND);
     const bool encl = act && lconn && fof_surely<MT>(fof_ub<ND, MT>(q, bx, L), L);
-    if (encl) rq = uf_union_r(P, rq, base);
+    // The members of a connected leaf are one set (from the pre-pass, or
+    // joined by the shortcut that flagged it before the pass ends), so once
+    // its first slot's parent is rq, or it is rq, the lane is joined to all
+    // of it.  The check saves the finds, device-scope loads that each make a
+    // round trip past the XCD's L2; in dense regions most such links are
+    // already joined.
+    if (encl) {
+        const int32_t pb = P[base];
+        if (pb != rq && base != rq) rq = uf_union_r(P, rq, base);
+    }
     const bool pair = act && !encl;
     if (!ballot(pair)) return;
     uint32_t mask = 0, unsure = 0;
@@ -274,9 +283,11 @@ __device__ __forceinline__ void fof_leaf(const DevTree &T, int leaf, const doubl
     if (ln < NBX_LEAF) par[ln] = base + ln < T.n ? P[base + ln] : -1;
     __syncthreads();
     if (lconn) {
+        // one link joins the whole leaf; the union goes through the first
+        // slot, whose chain to the root is the shortest
         if (mask) {
             const int j = __builtin_ctz(mask);
-            if (par[j] != rq) rq = uf_union_r(P, rq, base + j);
+            if (par[j] != rq && par[0] != rq && base != rq) rq = uf_union_r(P, rq, base);
         }
     } else {
         while (mask) {
