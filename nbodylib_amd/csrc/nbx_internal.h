@@ -199,43 +199,68 @@ __device__ __forceinline__ double wsm_at(double rij, double hi, const double *ke
 }
 
 // Lock-free union-find over slots (hook the larger root under the smaller).
+//
+// A parent pointer only ever points to a smaller slot of the same set, and
+// only ever changes to another such slot (a root is hooked under a smaller
+// slot; path splitting skips to a grandparent).  So a parent value read
+// late, from a cache that another XCD's writes have not reached, still
+// names a slot of the same set, and a chain of such values still descends
+// and ends.  The links made while walking (uf_union_r) therefore take
+// ordinary cached loads first (the wavefront-scope load hits the CU's and
+// the XCD's caches) and leave device-scope loads, each a round trip past
+// the XCD's L2, for the cases that need the present state:
+//  * two chains that meet prove the sets joined, since sets only merge;
+//  * a hook is a device-scope compare-and-swap that succeeds only on a
+//    true root, and hangs it under a smaller slot, so no cycle can form;
+//    when it fails the union runs again on device-scope finds.
+// uf_union, which joins whole nodes slot by slot, and the finds after the
+// linking, which need the final root, use device-scope loads throughout
+// (measured the better form for both).
 __device__ __forceinline__ int32_t uf_load(int32_t *P, int32_t x) {
     return __hip_atomic_load(P + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ int32_t uf_find(int32_t *P, int32_t x) {
-    int32_t p = uf_load(P, x);
+__device__ __forceinline__ int32_t uf_load_cached(int32_t *P, int32_t x) {
+    return __hip_atomic_load(P + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+}
+template <bool CACHED>
+__device__ __forceinline__ int32_t uf_find_t(int32_t *P, int32_t x) {
+    int32_t p = CACHED ? uf_load_cached(P, x) : uf_load(P, x);
     while (p != x) {
-        int32_t gp = uf_load(P, p);
+        int32_t gp = CACHED ? uf_load_cached(P, p) : uf_load(P, p);
         if (gp != p) __hip_atomic_store(P + x, gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         x = p;
         p = gp;
     }
     return x;
 }
-__device__ __forceinline__ void uf_union(int32_t *P, int32_t a, int32_t b) {
-    for (;;) {
-        a = uf_find(P, a);
-        b = uf_find(P, b);
-        if (a == b) return;
-        if (a < b) { int32_t t = a; a = b; b = t; }
-        int32_t expected = a;
-        if (__hip_atomic_compare_exchange_strong(P + a, &expected, b, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                                 __HIP_MEMORY_SCOPE_AGENT))
-            return;
-    }
+__device__ __forceinline__ int32_t uf_find(int32_t *P, int32_t x) { return uf_find_t<false>(P, x); }
+
+// One attempt to hook the larger of two roots under the smaller.
+__device__ __forceinline__ bool uf_hook(int32_t *P, int32_t &a, int32_t &b) {
+    if (a < b) { int32_t t = a; a = b; b = t; }
+    int32_t expected = a;
+    return __hip_atomic_compare_exchange_strong(P + a, &expected, b, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Union of the sets of a and b; returns the root the joined set had when
-// the link was made (a valid starting point for later finds from a).
+// Union of the sets of a and b; returns a root the joined set had when the
+// link was made or found (a valid starting point for later finds from a).
 __device__ __forceinline__ int32_t uf_union_r(int32_t *P, int32_t a, int32_t b) {
+    a = uf_find_t<true>(P, a);
+    b = uf_find_t<true>(P, b);
+    if (a == b) return a;
+    if (uf_hook(P, a, b)) return b;
     for (;;) {
         a = uf_find(P, a);
         b = uf_find(P, b);
         if (a == b) return a;
-        if (a < b) { int32_t t = a; a = b; b = t; }
-        int32_t expected = a;
-        if (__hip_atomic_compare_exchange_strong(P + a, &expected, b, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                                 __HIP_MEMORY_SCOPE_AGENT))
-            return b;
+        if (uf_hook(P, a, b)) return b;
+    }
+}
+__device__ __forceinline__ void uf_union(int32_t *P, int32_t a, int32_t b) {
+    for (;;) {
+        a = uf_find(P, a);
+        b = uf_find(P, b);
+        if (a == b || uf_hook(P, a, b)) return;
     }
 }
