@@ -181,13 +181,69 @@ __global__ __launch_bounds__(256) void k_fof_nodeinit(DevTree T, FofLink L, int3
     if (br[0] <= br[ND]) uf_union(P, node_first(T, node), node_first(T, right));
 }
 
+// Leaf scan for a wave in which few lanes want the leaf: their positions are
+// compacted into LDS and each step tests two of them against the leaf's 32
+// candidates, one candidate per lane (lanes 0-31 the first position, 32-63
+// the second), so the wave takes ceil(few / 2) steps instead of 32.  Each
+// pair is computed as the scan by lanes computes it (q - x, squared, summed
+// in the same order; band values decided by the divisions), so the link
+// decisions are the same.  Returns the lane's mask, candidate j at bit j.
+template <int ND, int MT>
+__device__ __forceinline__ uint32_t fof_scan_few(const DevTree &T, int base, const double *q, bool pair,
+                                                 uint64_t pm, const FofLink &L) {
+    __shared__ double qs[ND][NBX_WAVE];
+    const int lane = threadIdx.x;  // one wave per workgroup
+    const int npair = __builtin_popcountll(pm);
+    const int rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(pm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)pm, 0u));
+    if (pair) {
+#pragma unroll
+        for (int d = 0; d < ND; ++d) qs[d][rank] = q[d];
+    }
+    __syncthreads();
+    const int c = lane & (NBX_LEAF - 1), h = lane >> 5;
+    double cx[ND];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) cx[d] = T.x[d][base + c];
+    uint32_t own = 0;
+    for (int s = 0; s < npair; s += 2) {
+        const int r = s + h;
+        const int rr = r < npair ? r : npair - 1;  // the second half idles on an odd tail
+        double t2[ND];
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+            const double t = qs[d][rr] - cx[d];
+            t2[d] = t * t;
+        }
+        bool near;
+        if constexpr (MT == 0) {
+            near = fof_val<ND, MT>(t2, L) < L.lim;
+        } else {
+            const double v = fof_val_fast<ND, MT>(t2, L);
+            near = v < NBX_FOF_BAND_LO;
+            const bool uns = v >= NBX_FOF_BAND_LO && v < NBX_FOF_BAND_HI;
+            if (ballot(uns) && uns) near = fof_val<ND, MT>(t2, L) < 1.0;
+        }
+        const uint64_t b = ballot(near && r < npair);
+        if (rank == s) own = (uint32_t)b;
+        if (rank == s + 1) own = (uint32_t)(b >> 32);
+    }
+    __syncthreads();  // qs is written again at the next leaf
+    return pair ? own : 0u;
+}
+// Largest count of wanting lanes for which the leaf is scanned by
+// fof_scan_few: the measured crossover with the scan by lanes on 3-D trees;
+// on 6-D trees, where a leaf is wanted by 6 lanes on average, every count
+// (measured faster throughout).
+template <int ND>
+__device__ __forceinline__ constexpr int fof_few_max() { return ND == 6 ? NBX_WAVE : 24; }
+
 // Links from the (possibly image-shifted) position q to the particles of one
-// leaf.  Candidates arrive as 64-byte scalar-load chunks (padding slots are
-// 1e300 away and never link).  A leaf that is internally connected needs one
-// union with any particle it links to; otherwise every link is a union (in
-// the first pass only toward larger slots: the pair is met from both sides).
-// rq is a hint for the root of qi (any former root of its set), kept across
-// unions so each find starts near the top.
+// leaf.  act is the lane's own bound test for the leaf: a lane whose bound
+// rules out every link takes no part.  A leaf that is internally connected
+// needs one union with any particle it links to; otherwise every link is a
+// union (in the first pass only toward larger slots: the pair is met from
+// both sides).  rq is a hint for the root of qi (any former root of its
+// set), kept across unions so each find starts near the top.
 template <int PASS0, int ND, int MT>
 __device__ __forceinline__ void fof_leaf(const DevTree &T, int leaf, const double *q, bool act, int32_t qi,
                                          const FofLink &L, int32_t *P, const uint8_t *conn, int32_t &rq) {
@@ -206,68 +262,77 @@ __device__ __forceinline__ void fof_leaf(const DevTree &T, int leaf, const doubl
         if (pb != rq && base != rq) rq = uf_union_r(P, rq, base);
     }
     const bool pair = act && !encl;
-    if (!ballot(pair)) return;
-    uint32_t mask = 0, unsure = 0;
-    // 64-byte (3-D) or 32-byte (6-D) scalar-load chunks per dimension
-    constexpr int CH = ND == 6 ? 4 : 8;
-    typedef double dxc __attribute__((ext_vector_type(CH)));
-    typedef const __attribute__((address_space(4))) dxc *cdcptr;
+    const uint64_t pm = ballot(pair);
+    if (!pm) return;
+    uint32_t mask = 0;
+    if (__builtin_popcountll(pm) <= fof_few_max<ND>()) {
+        mask = fof_scan_few<ND, MT>(T, base, q, pair, pm, L);
+    } else {
+        uint32_t unsure = 0;
+        // the scan by lanes: candidates arrive as 64-byte (3-D) or 32-byte
+        // (6-D) scalar-load chunks per dimension (padding slots are 1e300
+        // away and never link)
+        constexpr int CH = ND == 6 ? 4 : 8;
+        typedef double dxc __attribute__((ext_vector_type(CH)));
+        typedef const __attribute__((address_space(4))) dxc *cdcptr;
 #pragma unroll
-    for (int jj = 0; jj < NBX_LEAF; jj += CH) {
-        dxc cv[ND];
+        for (int jj = 0; jj < NBX_LEAF; jj += CH) {
+            dxc cv[ND];
 #pragma unroll
-        for (int d = 0; d < ND; ++d) cv[d] = *(cdcptr)(T.x[d] + base + jj);
+            for (int d = 0; d < ND; ++d) cv[d] = *(cdcptr)(T.x[d] + base + jj);
 #pragma unroll
-        for (int u = 0; u < CH; ++u) {
-            double t2[ND];
-#pragma unroll
-            for (int d = 0; d < ND; ++d) {
-                const double t = q[d] - cv[d][u];
-                t2[d] = t * t;
-            }
-            // decisions are shifted in as they are made (a bit per candidate
-            // held in scalar masks until the end would spill them all); the
-            // comparators' values inside the reciprocal band are decided
-            // with the divisions after the scan, or at once in the 6-D
-            // persistent kernel (each the measured better form)
-            bool near;
-            if constexpr (MT == 0) {
-                near = fof_val<ND, MT>(t2, L) < L.lim;
-            } else if constexpr (ND == 3) {
-                const double v = fof_val_fast<ND, MT>(t2, L);
-                near = v < NBX_FOF_BAND_LO;
-                unsure = unsure + unsure + ((v >= NBX_FOF_BAND_LO && v < NBX_FOF_BAND_HI) ? 1u : 0u);
-            } else {
-                const double v = fof_val_fast<ND, MT>(t2, L);
-                near = v < NBX_FOF_BAND_LO;
-                const bool uns = v >= NBX_FOF_BAND_LO && v < NBX_FOF_BAND_HI;
-                if (ballot(uns) && uns) near = fof_val<ND, MT>(t2, L) < 1.0;
-            }
-            mask = mask + mask + (near ? 1u : 0u);
-        }
-    }
-    if constexpr (MT != 0 && ND == 3) {
-        // unsure holds candidate j at bit 31 - j, as mask does here
-        if (ballot(unsure != 0)) {
-            uint32_t um = unsure;
-            while (um) {
-                const int b = 31 - __builtin_clz(um);
-                um &= ~(1u << b);
-                const int32_t cs = base + (31 - b);
+            for (int u = 0; u < CH; ++u) {
                 double t2[ND];
 #pragma unroll
                 for (int d = 0; d < ND; ++d) {
-                    const double t = q[d] - T.x[d][cs];
+                    const double t = q[d] - cv[d][u];
                     t2[d] = t * t;
                 }
-                if (fof_val<ND, MT>(t2, L) < 1.0) mask |= 1u << b;
+                // decisions are shifted in as they are made (a bit per candidate
+                // held in scalar masks until the end would spill them all); the
+                // comparators' values inside the reciprocal band are decided
+                // with the divisions after the scan, or at once in the 6-D
+                // persistent kernel (each the measured better form)
+                bool near;
+                if constexpr (MT == 0) {
+                    near = fof_val<ND, MT>(t2, L) < L.lim;
+                } else if constexpr (ND == 3) {
+                    const double v = fof_val_fast<ND, MT>(t2, L);
+                    near = v < NBX_FOF_BAND_LO;
+                    unsure = unsure + unsure + ((v >= NBX_FOF_BAND_LO && v < NBX_FOF_BAND_HI) ? 1u : 0u);
+                } else {
+                    const double v = fof_val_fast<ND, MT>(t2, L);
+                    near = v < NBX_FOF_BAND_LO;
+                    const bool uns = v >= NBX_FOF_BAND_LO && v < NBX_FOF_BAND_HI;
+                    if (ballot(uns) && uns) near = fof_val<ND, MT>(t2, L) < 1.0;
+                }
+                mask = mask + mask + (near ? 1u : 0u);
             }
         }
+        if constexpr (MT != 0 && ND == 3) {
+            // unsure holds candidate j at bit 31 - j, as mask does here
+            if (ballot(unsure != 0)) {
+                uint32_t um = unsure;
+                while (um) {
+                    const int b = 31 - __builtin_clz(um);
+                    um &= ~(1u << b);
+                    const int32_t cs = base + (31 - b);
+                    double t2[ND];
+#pragma unroll
+                    for (int d = 0; d < ND; ++d) {
+                        const double t = q[d] - T.x[d][cs];
+                        t2[d] = t * t;
+                    }
+                    if (fof_val<ND, MT>(t2, L) < 1.0) mask |= 1u << b;
+                }
+            }
+        }
+        // candidate j was shifted in at bit 31 - j
+        mask = __builtin_bitreverse32(mask);
     }
-    // candidate j was shifted in at bit 31 - j.  Never the query itself; in
-    // the first pass, in a leaf not internally connected, only candidates at
-    // larger slots (the pair is met from both sides).
-    mask = __builtin_bitreverse32(mask);
+    // Never the query itself; in the first pass, in a leaf not internally
+    // connected, only candidates at larger slots (the pair is met from both
+    // sides).
     const int qrel = qi - base;
     const uint32_t self = (qrel >= 0 && qrel < NBX_LEAF) ? (1u << qrel) : 0u;
     const uint32_t above = qrel < 0 ? ~0u : (qrel >= NBX_LEAF - 1 ? 0u : (~0u << (qrel + 1)));
@@ -342,7 +407,7 @@ __device__ __forceinline__ void fof_walk(const DevTree &T, const double *q, bool
                     }
                     down = false;
                 } else if (node >= T.nleafp) {
-                    fof_leaf<PASS0, ND, MT>(T, node - T.nleafp, q, act, qi, L, P, conn, rq);
+                    fof_leaf<PASS0, ND, MT>(T, node - T.nleafp, q, want, qi, L, P, conn, rq);
                     down = false;
                 } else {
                     node = 2 * node;
