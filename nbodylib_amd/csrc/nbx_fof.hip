@@ -729,8 +729,9 @@ static void launch_fof_t(const DevTree &T, const FofArgs &A, uint8_t *conn, hipS
     }
     ProfScope ps("k_fof_link", st);
     if (ND == 3) {
-        // measured on 16.8M clustered particles: 15.8 ms per launch in this
-        // order, 20 ms with the XCD shares below
+        // measured on 16.8M clustered particles: 10.2 ms per call in this
+        // order, 11.1 ms with the XCD shares below; on the 6-D set the other
+        // way round (FOF6d 15.3 ms with the shares, 17.0 ms in this order)
         hipLaunchKernelGGL((k_fof_waves<ND, MT>), dim3((T.n + NBX_WAVE - 1) / NBX_WAVE), dim3(NBX_WAVE), 0, st, T, A,
                            conn);
         return;
