@@ -4,7 +4,11 @@ particles, periodic L = 100), on the GPU:
     labels (the same closure through two link metrics);
   * with unit velocities and rho = 1 given, CalcSmoothVel adds exactly
     CalcDensity's terms, so each component equals the density up to the
-    order of the atomic sums.
+    order of the atomic sums;
+  * on the 6-D phase-space set, FOFCriterion with FOF6d at scales (s6, sv)
+    and plain FOF at b^2 = s6 on a tree whose velocities are scaled by
+    sqrt(s6 / sv) give the same labels (the same closure through two link
+    metrics and two trees).
 Each case prints its name before the GPU call."""
 import numpy as np
 import pytest
@@ -46,3 +50,21 @@ def test_smooth_vel_unit_fields_is_density(gpu, clustered16m):
     rho = t.CalcDensity(k)
     mean = t.CalcSmoothVel(k, rho=np.ones(N))
     assert np.max(np.abs(mean / rho[:, None] - 1.0)) < 1e-12
+
+
+def test_fof6d_criterion_equals_scaled_plain_fof(gpu):
+    import nbodylib_amd as nb
+
+    pos, vel, _ = datagen.clustered(N, L, nhalo=2000, seed=20261203 + 2, with_vel=True)
+    x = pos / pos.std(axis=0)
+    v = vel / vel.std(axis=0)
+    s6 = (0.2 * datagen.mean_spacing(N, L) / float(np.mean(pos.std(axis=0)))) ** 2
+    sv = 0.01
+    say(f"FOF6d vs scaled plain 6-D FOF n={N} s6={s6:.3e} sv={sv}")
+    t = nb.KDTree(x, vel=v, treetype=nb.TPHS, aniso=-1, kerntype=nb.KSPH)
+    lc, nc = t.FOFCriterion(nb.FOF_CMP_6D, [nb.TPHS, s6, sv, 0, 0, 0, s6, sv], 8)
+    del t
+    ts = nb.KDTree(x, vel=v * np.sqrt(s6 / sv), treetype=nb.TPHS, aniso=-1, kerntype=nb.KSPH)
+    lab, ng = ts.FOF(np.sqrt(s6), 8)
+    assert nc > 1000 and ng == nc
+    assert np.array_equal(lab, lc)
