@@ -254,14 +254,13 @@ __device__ __forceinline__ void fof_leaf(const DevTree &T, int leaf, const doubl
     // The members of a connected leaf are one set (from the pre-pass, or
     // joined by the shortcut that flagged it before the pass ends), so once
     // its first slot's parent is rq, or it is rq, the lane is joined to all
-    // of it.  The check saves the finds, device-scope loads that each make a
-    // round trip past the XCD's L2; in dense regions most such links are
-    // already joined.
-    if (encl) {
-        const int32_t pb = P[base];
-        if (pb != rq && base != rq) rq = uf_union_r(P, rq, base);
-    }
-    const bool pair = act && !encl;
+    // of it and neither scans the leaf nor makes a union; in dense regions
+    // most visits to connected leaves are such.  A stale parent only misses
+    // the skip, since sets only merge.
+    const int32_t pb = lconn ? P[base] : -1;
+    const bool joined = lconn && (pb == rq || base == rq);
+    if (encl && !joined) rq = uf_union_r(P, rq, base);
+    const bool pair = act && !encl && !joined;
     const uint64_t pm = ballot(pair);
     if (!pm) return;
     uint32_t mask = 0;
@@ -730,8 +729,9 @@ static void launch_fof_t(const DevTree &T, const FofArgs &A, uint8_t *conn, hipS
     ProfScope ps("k_fof_link", st);
     if (ND == 3) {
         // measured on 16.8M clustered particles: 10.2 ms per call in this
-        // order, 11.1 ms with the XCD shares below; on the 6-D set the other
-        // way round (FOF6d 15.3 ms with the shares, 17.0 ms in this order)
+        // order, 11.1 ms with the XCD shares below (before the skip of
+        // joined leaves); on the 6-D set the other way round (FOF6d 15.3 ms
+        // with the shares, 17.0 ms in this order)
         hipLaunchKernelGGL((k_fof_waves<ND, MT>), dim3((T.n + NBX_WAVE - 1) / NBX_WAVE), dim3(NBX_WAVE), 0, st, T, A,
                            conn);
         return;
